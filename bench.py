@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Particle-steps/s of the HIP particle step on one MI355X.
+
+Workload (default): 400x300x30 tanh-shelf grid with a masked island, 16
+drifting Gaussian eddies plus an M2 tide, RK4 advection + vertical random
+displacement (idt 120 s, 10 sub-steps of 12 s) + horizontal random walk
+(Kh 1 m2/s), 1,000,000 particles, FP32 fields with FP64 particle state.
+
+A "step" is one internal time step of all particles (one rp_step).  The
+timed region covers K steps including the hydro-slot uploads the schedule
+asks for; record generation (the analogue of the reference's netCDF reads)
+happens before it.  value = active particle-steps / wall seconds.
+
+The FP64 form is run over the same steps with the same counter-based
+deviates, giving its rate and the FP32-field position error against it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from romspath_amd import synth  # noqa: E402
+from romspath_amd.runner import GpuRun, RecordSource  # noqa: E402
+from romspath_amd.schedule import schedule  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+# minimal stencil bytes per active particle-step, no reuse between particles:
+# per RK stage 88f+12 (h 4f, zeta 12f, u/v/w 72f, masks 12 B) x4, pm/pn 8f,
+# post-step depth+zeta 16f + 4 B per grid tested, VTurb (16+12(N+1))f + 4,
+# particle state 88 B.  f = field bytes (4 FP32, 8 FP64).
+def b_alg(f, N, vturb=True, ngrid=1):
+    rk = 4 * (88 * f + 12)
+    post = 16 * f + 4 * ngrid
+    vt = (16 + 12 * (N + 1)) * f + 4 if vturb else 0
+    return rk + 8 * f + post + vt + 88
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_form(cfg, records, steps, warmup, precision):
+    prm = cfg.params
+    sched = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, warmup + steps))
+    run = GpuRun(cfg, precision=precision, records=records)
+    ctx = run.ctx
+
+    def do(st):
+        for ld in st.loads:
+            ctx.set_hydro_slot(1, ld.slot, records.get(1, ld.record))
+        ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index, want_diag=st.output)
+
+    for st in sched[:warmup]:
+        do(st)
+    ctx.synchronize()
+    a0 = ctx.active_steps()
+    ctx.kernel_timing()
+    t0 = time.perf_counter()
+    for k, st in enumerate(sched[warmup:]):
+        do(st)
+        if (k + 1) % 60 == 0:
+            log(f"  {precision}: {k + 1}/{steps} steps")
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    a1 = ctx.active_steps()
+    avg_ms, launches = ctx.kernel_timing()
+    capped = ctx.secant_capped()
+    out = ctx.get_particles()
+    run.close()
+    return dict(active=a1 - a0, seconds=t1 - t0, avg_kernel_ms=avg_ms, launches=launches,
+                capped=capped, state=out)
+
+
+def cpu_baseline(cfg, n_sample, steps):
+    from oracle import oracle as O
+    sub = synth.Particles(cfg.particles.x[:n_sample].copy(), cfg.particles.y[:n_sample].copy(),
+                          cfg.particles.z[:n_sample].copy(), cfg.particles.dob[:n_sample].copy())
+    scfg = synth.Config(cfg.name, cfg.grids, cfg.hydro, cfg.params, sub, cfg.refine)
+    t0 = time.perf_counter()
+    st = O.run_oracle(scfg, max_steps=steps, inject=True)
+    dt = time.perf_counter() - t0
+    return st.active / dt, st.active, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--no-fp64-form", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-particles", type=int, default=4000)
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(every GPU box here has a single MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    log("building workload C2 (400x300x30, 16 eddies + M2, RK4 + VRW + HRW)")
+    cfg = synth.config2(n_particles=args.particles, days=3.0)
+    prm = cfg.params
+    total = args.warmup + args.steps
+    need = set()
+    for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, total):
+        for ld in st.loads:
+            need.add(ld.record)
+    records = RecordSource(cfg)
+    for r in sorted(need):
+        records.get(1, r)
+    log(f"  {len(need)} hydro records generated")
+
+    res = run_form(cfg, records, args.steps, args.warmup, args.precision)
+    value = res["active"] / res["seconds"]
+    log(f"  {args.precision}: {value:.4g} particle-steps/s, kernel {res['avg_kernel_ms']:.3f} ms")
+    f = 4 if args.precision == "fp32" else 8
+    N = cfg.grids[0].N
+    balg = b_alg(f, N)
+    per_launch = res["active"] / max(res["launches"], 1)
+    achieved = balg * per_launch / (res["avg_kernel_ms"] * 1e-3) / 1e9 if res["avg_kernel_ms"] else 0.0
+
+    extra = {}
+    if not args.no_fp64_form and args.precision == "fp32":
+        r64 = run_form(cfg, records, args.steps, args.warmup, "fp64")
+        v64 = r64["active"] / r64["seconds"]
+        a, b = res["state"], r64["state"]
+        same = a["status"] == b["status"]
+        layer = np.maximum(b["bath"], 1.0) / N
+        ex = np.abs(a["x"] - b["x"])[same]
+        ey = np.abs(a["y"] - b["y"])[same]
+        ez = (np.abs(a["z"] - b["z"]) / layer)[same]
+        exy = np.maximum(ex, ey)
+        extra["fp64_form"] = {"value": v64, "avg_kernel_ms": r64["avg_kernel_ms"]}
+        extra["fp32_vs_fp64_error_cells"] = {
+            "horizontal_max": float(exy.max()), "horizontal_p95": float(np.percentile(exy, 95)),
+            "vertical_max": float(ez.max()), "vertical_p95": float(np.percentile(ez, 95)),
+            "status_agreement": float(same.mean())}
+        log(f"  fp64: {v64:.4g} particle-steps/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline: oracle, {args.cpu_particles} particles x {args.cpu_steps} steps, 1 thread")
+        rate, act, secs = cpu_baseline(cfg, args.cpu_particles, args.cpu_steps)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"first {args.cpu_particles} particles of the same workload, "
+                         f"{args.cpu_steps} internal steps, FP64 C restatement, {secs:.1f} s"}
+
+    line = {
+        "metric": "particle-steps/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["seconds"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "C2: 400x300x30 tanh shelf + island, 16 drifting eddies + M2 tide, "
+                               "RK4 + vertical random walk (10 x 12 s) + horizontal random walk, "
+                               f"{args.particles} particles, "
+                               f"{'FP32' if f == 4 else 'FP64'} fields / FP64 state",
+                   "particles": args.particles, "grid": [400, 300, 30], "idt_s": prm.idt,
+                   "dt_turb_s": prm.deltat, "field_precision": args.precision,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "b_alg_bytes_per_particle_step": balg,
+                     "avg_kernel_ms": res["avg_kernel_ms"],
+                     "active_particles_per_launch": per_launch},
+        "cpu_baseline": cpu,
+        "secant_capped": res["capped"],
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

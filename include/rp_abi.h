@@ -1,0 +1,150 @@
+/*
+ * rp_abi.h -- C-ABI of librp.so, the MI355X implementation of ROMSPath's
+ * per-particle time step.
+ *
+ * The reference has no plugin or FFI layer: the replaced region is the body
+ * of update_particles (ROMSPath.f90:608-1008), called once per internal step
+ * from run_Internal_Timestep (ROMSPath.f90:529), which communicates through
+ * module globals (par, isIn, GRIDS, HYDRODATA, PARAM_MOD, t_b/t_c/t_f,
+ * ex/ix/ets).  Each entry point below replaces one of those hand-over points;
+ * its comment cites the reference code it stands in for.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative RP_E* code on
+ *     failure; rp_last_error() gives the message.  Nothing aborts the host.
+ *   - Arrays are passed exactly as the reference holds them: FP64, Fortran
+ *     column-major (xi fastest), positions as fractional 1-based rho indices.
+ *   - The caller owns all host arrays (copied in and out); the library owns
+ *     device memory.  One context = one HIP stream = one host thread.
+ *   - Grid numbers ng and hydro slots are 1-based, as in the reference.
+ */
+#ifndef RP_ABI_H
+#define RP_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RP_MAXGRID 5
+
+enum {
+    RP_OK = 0,
+    RP_E_ARG = -1,        /* invalid argument */
+    RP_E_STATE = -2,      /* call out of order (grid/slot/particles missing) */
+    RP_E_HIP = -3,        /* HIP runtime error */
+    RP_E_ALLOC = -4,      /* device allocation failed */
+    RP_E_RANGE = -5,      /* size beyond a limit */
+    RP_E_NODEV = -6       /* no usable GPU */
+};
+
+enum { RP_PREC_FP32_FIELDS = 0, RP_PREC_FP64 = 1 };
+
+/* Namelist scalars on the hot path (ROMSPath.h:77-258) plus build knobs.
+ * Stand-in for PARAM_MOD after getParams (parameter_module.f90:22-152). */
+typedef struct {
+    int32_t idt;                 /* timeparam: internal step (s) */
+    int32_t dt;                  /* timeparam: external step (s) */
+    double deltat;               /* turbparam: RDM sub-step (s) */
+    double serr, smth, sub;      /* turbparam: spline error, smoothing, resolution */
+    double aksback;              /* turbparam: AKs background */
+    int32_t vturb_on, hturb_on;  /* turbparam */
+    double kh[RP_MAXGRID];       /* turbparam: ConstantHTurb(ng) */
+    int32_t scheme, nsb;         /* advectparam */
+    double vertdist;             /* advectparam */
+    int32_t open_boundary;       /* behavparam: OpenOceanBoundary */
+    int32_t ngrid;               /* romsgrid: Ngrid */
+    uint64_t seed;               /* seed of the counter-based RNG */
+    /* build knobs (no namelist counterpart) */
+    int32_t precision;           /* RP_PREC_FP32_FIELDS or RP_PREC_FP64 */
+    int32_t block;               /* threads per block, 0 = default */
+    int32_t sort_interval;       /* re-sort particles by cell every k steps, 0 = never */
+    int32_t reserved[5];
+} rp_config;
+
+/* One grid, as InitGrid leaves GRIDDATA (grid_module.f90:14-36, 41-481). */
+typedef struct {
+    int32_t L, M, N;             /* xi_rho, eta_rho, s_rho */
+    int32_t vtransform;
+    double hc;
+    const double *s_rho, *cs_r;  /* N   */
+    const double *s_w, *cs_w;    /* N+1 */
+    const double *h, *pm, *pn;   /* (xi_rho, eta_rho) */
+    const int32_t *mask_rho;     /* (xi_rho, eta_rho) */
+    const int32_t *mask_u;       /* (xi_u, eta_u) */
+    const int32_t *mask_v;       /* (xi_v, eta_v) */
+    double scl[RP_MAXGRID][2];   /* scl(i,:), off(i,:): affine map to grid i */
+    double off[RP_MAXGRID][2];
+} rp_grid;
+
+/* after getParams: replaces PARAM_MOD's hot-path globals */
+int rp_create(void **ctx, const rp_config *cfg);
+
+/* after InitGrid (grid_module.f90:41-499): one call per grid ng=1..Ngrid */
+int rp_set_grid(void *ctx, int32_t ng, const rp_grid *g);
+
+/* end of updateHydro (hydrodynamic_module.f90:112-760), which fills time
+ * slot tind=1..3 of HYDRODATA(ng).  zeta (xi_rho,eta_rho); u (xi_u,eta_u,s_rho);
+ * v (xi_v,eta_v,s_rho); w and aks (xi_rho,eta_rho,s_w).  su/sv (Stokes drift,
+ * may be NULL) are added to u/v in FP64 before storage, as under -DSTOKES
+ * (hydrodynamic_module.f90:608-657).  Asynchronous on the context stream; the
+ * host arrays may be reused when the call returns. */
+int rp_set_hydro_slot(void *ctx, int32_t ng, int32_t slot, const double *zeta, const double *u,
+                      const double *v, const double *w, const double *aks, const double *su,
+                      const double *sv);
+
+/* end of ini_ROMSPath (ROMSPath.f90:319-433): the par(:,k) columns for
+ * X, Y, Z, Age, DOB, Status, GID and isIn of n particles. */
+int rp_set_particles(void *ctx, int64_t n, const double *x, const double *y, const double *z,
+                     const double *age, const double *dob, const double *status,
+                     const int32_t *gid, const uint8_t *isin);
+
+/* replaces call update_particles() (ROMSPath.f90:529): enqueues one internal
+ * step for all particles.  ex/ix and the slot numbers are the host schedule's
+ * values (ROMSPath.f90:488-524); step_index counts internal steps from 0 and
+ * keys the random deviates.  want_diag != 0 also stores the water depth,
+ * zeta and bathymetry columns (pWD, pZeta, pBath; ROMSPath.f90:845-847),
+ * which only the output reads.  Asynchronous. */
+int rp_step(void *ctx, const double ex[3], const double ix[3], int32_t ets, int32_t t_b,
+            int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag);
+
+/* dataOutput (ROMSPath.f90:1012-1072): copies the particle columns back in
+ * release order.  Any pointer may be NULL.  Synchronises the stream. */
+int rp_get_particles(void *ctx, int64_t n, double *x, double *y, double *z, double *age,
+                     double *status, int32_t *gid, uint8_t *isin, double *wd, double *zeta,
+                     double *bath);
+
+/* Test hook: use these normal deviates instead of the RNG for the following
+ * steps.  normals is [n][ndev] in release order, ndev = 2 + idt/int(deltat):
+ * columns 0,1 feed HTurb (x, y), the rest VTurb's sub-steps in order.
+ * normals == NULL returns to the RNG. */
+int rp_set_deviates(void *ctx, const double *normals, int64_t n, int32_t ndev);
+
+/* Test hook: the RNG's normal deviates for particles [0,n) at step
+ * step_index, in the layout of rp_set_deviates. */
+int rp_sample_deviates(void *ctx, int64_t step_index, int64_t n, int32_t ndev, double *out);
+
+/* Sum over all steps so far of the particles that took the step (those that
+ * pass the skip tests of ROMSPath.f90:672-690).  Synchronises the stream. */
+int rp_active_steps(void *ctx, int64_t *count);
+
+/* Number of smoothing-spline secant searches stopped at the iteration cap
+ * (the reference's search, pppack.f90:5270-5292, has none; a GPU lane must
+ * end).  Synchronises the stream. */
+int rp_secant_capped(void *ctx, int64_t *count);
+
+/* Wait for all work queued on the context stream. */
+int rp_synchronize(void *ctx);
+
+/* Average device time per rp_step launch (ms) of the step kernel since the
+ * last call, measured with HIP events on the context stream. */
+int rp_kernel_timing(void *ctx, double *avg_ms, int64_t *launches);
+
+const char *rp_last_error(void *ctx);
+int rp_destroy(void *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,280 @@
+"""ctypes access to the oracle (liboracle.so) and the reference PPPACK build.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's CPU-baseline leg, always as the checker or the CPU baseline,
+never as part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+REF_PPPACK = HERE / "_ref" / "libpppack.so"
+
+PD = C.POINTER(C.c_double)
+PI32 = C.POINTER(C.c_int32)
+PU8 = C.POINTER(C.c_uint8)
+MAXGRID = 5
+
+
+class OgGrid(C.Structure):
+    _fields_ = [("L", C.c_int), ("M", C.c_int), ("N", C.c_int), ("vtransform", C.c_int),
+                ("hc", C.c_double),
+                ("s_rho", PD), ("cs_r", PD), ("s_w", PD), ("cs_w", PD),
+                ("h", PD), ("pm", PD), ("pn", PD),
+                ("mask_rho", PI32), ("mask_u", PI32), ("mask_v", PI32),
+                ("scl", (C.c_double * 2) * MAXGRID), ("off", (C.c_double * 2) * MAXGRID),
+                ("zeta", PD * 3), ("u", PD * 3), ("v", PD * 3), ("w", PD * 3), ("aks", PD * 3)]
+
+
+class OgParams(C.Structure):
+    _fields_ = [("idt", C.c_int), ("deltat", C.c_double), ("serr", C.c_double),
+                ("smth", C.c_double), ("sub", C.c_double), ("aksback", C.c_double),
+                ("vturb_on", C.c_int), ("hturb_on", C.c_int), ("kh", C.c_double * MAXGRID),
+                ("scheme", C.c_int), ("nsb", C.c_int), ("vertdist", C.c_double),
+                ("open_boundary", C.c_int), ("ngrid", C.c_int)]
+
+
+class OgTime(C.Structure):
+    _fields_ = [("ex", C.c_double * 3), ("ix", C.c_double * 3), ("ets", C.c_int),
+                ("tb", C.c_int), ("tc", C.c_int), ("tf", C.c_int)]
+
+
+class OgParticles(C.Structure):
+    _fields_ = [("x", PD), ("y", PD), ("z", PD), ("age", PD), ("dob", PD), ("status", PD),
+                ("wd", PD), ("zeta", PD), ("bath", PD), ("gid", PI32), ("isin", PU8)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(str(LIB_PATH))
+        L.og_polintd.restype = C.c_double
+        L.og_polintd.argtypes = [PD, PD, C.c_double]
+        L.og_linint.argtypes = [PD, PD, C.c_int, C.c_double, PD, PD]
+        for f in ("og_getslevel", "og_getwlevel"):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [C.POINTER(OgGrid), C.c_double, C.c_double, C.c_int]
+        L.og_getcoeff.argtypes = [C.c_double, C.c_double, PD, PD]
+        L.og_interp2d.restype = C.c_double
+        L.og_interp2d.argtypes = [C.POINTER(OgGrid), C.c_int, C.c_int, C.c_double, C.c_double]
+        L.og_interp3d.restype = C.c_double
+        L.og_interp3d.argtypes = [C.POINTER(OgGrid), C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_int, C.c_int, C.c_double, C.c_double]
+        L.og_interp_aks.restype = C.c_double
+        L.og_interp_aks.argtypes = [C.POINTER(OgGrid), C.c_double, C.c_double, C.c_int, C.c_int]
+        L.og_inside.restype = C.c_int
+        L.og_inside.argtypes = [C.c_int, PD, PD, C.c_double, C.c_double]
+        L.og_bounds.argtypes = [C.POINTER(OgGrid), C.c_double, C.c_double,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.og_find_currents.argtypes = [C.POINTER(OgGrid), C.POINTER(OgTime), C.c_double, C.c_double,
+                                       PD, PD, PD, PD, PD, PD]
+        L.og_rkadvect.argtypes = [C.POINTER(OgGrid), C.POINTER(OgParams), C.POINTER(OgTime),
+                                  C.c_double, C.c_double, PD, C.c_double, C.c_double, PD, PD, PD]
+        L.og_smooth.restype = C.c_double
+        L.og_smooth.argtypes = [PD, PD, PD, C.c_int, C.c_double, PD, PD]
+        L.og_ppvalu.restype = C.c_double
+        L.og_ppvalu.argtypes = [PD, PD, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.og_interv.restype = C.c_int
+        L.og_interv.argtypes = [PD, C.c_int, C.c_double, C.POINTER(C.c_int)]
+        L.og_vturb.restype = C.c_double
+        L.og_vturb.argtypes = [C.POINTER(OgGrid), C.POINTER(OgParams), C.POINTER(OgTime),
+                               C.c_double, C.c_double, C.c_double, PD]
+        L.og_update_particles.restype = C.c_int64
+        L.og_update_particles.argtypes = [C.POINTER(OgGrid), C.POINTER(OgParams), C.POINTER(OgTime),
+                                          C.POINTER(OgParticles), C.c_int64, PD, C.c_int]
+        _lib = L
+    return _lib
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(PD)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(PI32)
+
+
+class OracleGrid:
+    """Holds the numpy arrays behind one og_grid and its three hydro slots."""
+
+    def __init__(self, grid, scl=None, off=None):
+        self.grid = grid
+        self._keep = []
+        s = OgGrid()
+        s.L, s.M, s.N = grid.L, grid.M, grid.N
+        s.vtransform = grid.vtransform
+        s.hc = grid.hc
+        for name in ("s_rho", "cs_r", "s_w", "cs_w"):
+            a = np.ascontiguousarray(getattr(grid, name), dtype=np.float64)
+            self._keep.append(a)
+            setattr(s, name, dptr(a))
+        for name in ("h", "pm", "pn"):
+            a = np.ascontiguousarray(getattr(grid, name), dtype=np.float64)
+            self._keep.append(a)
+            setattr(s, name, dptr(a))
+        for name in ("mask_rho", "mask_u", "mask_v"):
+            a = np.ascontiguousarray(getattr(grid, name), dtype=np.int32)
+            self._keep.append(a)
+            setattr(s, name, iptr(a))
+        for i in range(MAXGRID):
+            for c in range(2):
+                s.scl[i][c] = 1.0 if scl is None else scl[i][c]
+                s.off[i][c] = 0.0 if off is None else off[i][c]
+        self.s = s
+        self.slots = [None, None, None]
+
+    def set_slot(self, slot: int, rec: dict):
+        """slot is 1-based like the reference's t_b/t_c/t_f.  Stokes drift, when
+        present, is summed into u and v in FP64 (hydrodynamic_module.f90:608-657)."""
+        keep = {}
+        for name in ("zeta", "u", "v", "w", "aks"):
+            a = np.ascontiguousarray(rec[name], dtype=np.float64)
+            if name == "u" and "su" in rec:
+                a = np.ascontiguousarray(a + rec["su"])
+            if name == "v" and "sv" in rec:
+                a = np.ascontiguousarray(a + rec["sv"])
+            keep[name] = a
+            getattr(self.s, name)[slot - 1] = dptr(a)
+        self.slots[slot - 1] = keep
+
+
+def make_params(prm, ngrid=1) -> OgParams:
+    P = OgParams()
+    P.idt = prm.idt
+    P.deltat = prm.deltat
+    P.serr = prm.serr
+    P.smth = prm.smth
+    P.sub = prm.sub
+    P.aksback = prm.AKSback
+    P.vturb_on = int(prm.VTurbOn)
+    P.hturb_on = int(prm.HTurbOn)
+    for i in range(MAXGRID):
+        P.kh[i] = prm.ConstantHTurb[i] if i < len(prm.ConstantHTurb) else 0.0
+    P.scheme = prm.scheme
+    P.nsb = prm.nsb
+    P.vertdist = prm.vertdist
+    P.open_boundary = int(prm.OpenOceanBoundary)
+    P.ngrid = ngrid
+    return P
+
+
+def make_time(ex, ix, ets, tb, tc, tf) -> OgTime:
+    T = OgTime()
+    for k in range(3):
+        T.ex[k] = ex[k]
+        T.ix[k] = ix[k]
+    T.ets = ets
+    T.tb, T.tc, T.tf = tb, tc, tf
+    return T
+
+
+class OracleState:
+    """Particle SoA for the oracle (float64 status like par(:,pStatus))."""
+
+    def __init__(self, parts):
+        n = parts.n
+        self.x = np.ascontiguousarray(parts.x, dtype=np.float64).copy()
+        self.y = np.ascontiguousarray(parts.y, dtype=np.float64).copy()
+        self.z = np.ascontiguousarray(parts.z, dtype=np.float64).copy()
+        self.age = np.ascontiguousarray(parts.age, dtype=np.float64).copy()
+        self.dob = np.ascontiguousarray(parts.dob, dtype=np.float64).copy()
+        self.status = np.ascontiguousarray(parts.status, dtype=np.float64).copy()
+        self.wd = np.full(n, 9999.0)
+        self.zeta = np.full(n, 9999.0)
+        self.bath = np.full(n, 9999.0)
+        self.gid = np.ascontiguousarray(parts.gid, dtype=np.int32).copy()
+        self.isin = np.ascontiguousarray(parts.isin, dtype=np.uint8).copy()
+        s = OgParticles()
+        for name in ("x", "y", "z", "age", "dob", "status", "wd", "zeta", "bath"):
+            setattr(s, name, dptr(getattr(self, name)))
+        s.gid = iptr(self.gid)
+        s.isin = self.isin.ctypes.data_as(PU8)
+        self.s = s
+
+    @property
+    def n(self):
+        return self.x.size
+
+
+def update_particles(grids, P, T, state, dev, ndev):
+    arr = (OgGrid * len(grids))(*[g.s for g in grids])
+    dev = np.ascontiguousarray(dev, dtype=np.float64)
+    return lib().og_update_particles(arr, C.byref(P), C.byref(T), C.byref(state.s),
+                                     state.n, dptr(dev), ndev)
+
+
+# ---- the reference's own PPPACK (oracle/_ref), Fortran calling convention ----
+_ref = None
+
+
+def ref_pppack():
+    """The reference pppack.f90 compiled as is; None when it was not built."""
+    global _ref
+    if _ref is None:
+        if not REF_PPPACK.exists():
+            return None
+        L = C.CDLL(str(REF_PPPACK))
+        L.smooth_.restype = C.c_double
+        L.smooth_.argtypes = [PD, PD, PD, C.POINTER(C.c_int), C.POINTER(C.c_double), PD, PD]
+        L.ppvalu_.restype = C.c_double
+        L.ppvalu_.argtypes = [PD, PD, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                              C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.interv_.restype = None
+        L.interv_.argtypes = [PD, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                              C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _ref = L
+    return _ref
+
+
+# ---- whole runs on the oracle (same schedule as the product driver) ----
+
+def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None):
+    """Run cfg on the oracle; deviates are the same seeded per-step blocks the
+    GPU tests inject.  Returns the OracleState."""
+    import sys
+    root = str(HERE.parent)
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    from romspath_amd.nesting import scale_offsets
+    from romspath_amd.runner import RecordSource, injected_deviates, ndev_of
+    from romspath_amd.schedule import schedule
+
+    prm = cfg.params
+    ng = cfg.ngrid
+    if ng > 1:
+        scl, off = scale_offsets(cfg.grids, cfg.refine)
+        grids = [OracleGrid(g, scl[i], off[i]) for i, g in enumerate(cfg.grids)]
+    else:
+        grids = [OracleGrid(cfg.grids[0])]
+    P = make_params(prm, ng)
+    state = OracleState(particles if particles is not None else cfg.particles)
+    recs = RecordSource(cfg)
+    ndev = ndev_of(prm)
+    active = 0
+    for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps):
+        for ld in st.loads:
+            for i, g in enumerate(grids):
+                g.set_slot(ld.slot, recs.get(i + 1, ld.record))
+        T = make_time(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf)
+        if inject:
+            dev = injected_deviates(prm.seed, st.step_index, state.n, ndev)
+        else:
+            dev = np.zeros((state.n, ndev))
+        active += update_particles(grids, P, T, state, dev, ndev)
+        if on_step is not None:
+            on_step(st, state)
+    state.active = active
+    return state

@@ -1,0 +1,240 @@
+"""ctypes binding of librp.so (include/rp_abi.h).
+
+The library must be built (romspath_amd/build.py); there is no fallback:
+importing this module without the library, or creating a context on a host
+without a GPU, raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "librp.so"
+MAXGRID = 5
+
+RP_PREC_FP32_FIELDS = 0
+RP_PREC_FP64 = 1
+
+ERRORS = {-1: "RP_E_ARG", -2: "RP_E_STATE", -3: "RP_E_HIP", -4: "RP_E_ALLOC",
+          -5: "RP_E_RANGE", -6: "RP_E_NODEV"}
+
+PD = C.POINTER(C.c_double)
+PI32 = C.POINTER(C.c_int32)
+PU8 = C.POINTER(C.c_uint8)
+PI64 = C.POINTER(C.c_int64)
+
+EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", "rp_step",
+           "rp_get_particles", "rp_set_deviates", "rp_sample_deviates", "rp_active_steps",
+           "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_last_error",
+           "rp_destroy"]
+
+
+class RpConfig(C.Structure):
+    _fields_ = [("idt", C.c_int32), ("dt", C.c_int32), ("deltat", C.c_double),
+                ("serr", C.c_double), ("smth", C.c_double), ("sub", C.c_double),
+                ("aksback", C.c_double), ("vturb_on", C.c_int32), ("hturb_on", C.c_int32),
+                ("kh", C.c_double * MAXGRID), ("scheme", C.c_int32), ("nsb", C.c_int32),
+                ("vertdist", C.c_double), ("open_boundary", C.c_int32), ("ngrid", C.c_int32),
+                ("seed", C.c_uint64), ("precision", C.c_int32), ("block", C.c_int32),
+                ("sort_interval", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class RpGrid(C.Structure):
+    _fields_ = [("L", C.c_int32), ("M", C.c_int32), ("N", C.c_int32), ("vtransform", C.c_int32),
+                ("hc", C.c_double), ("s_rho", PD), ("cs_r", PD), ("s_w", PD), ("cs_w", PD),
+                ("h", PD), ("pm", PD), ("pn", PD), ("mask_rho", PI32), ("mask_u", PI32),
+                ("mask_v", PI32), ("scl", (C.c_double * 2) * MAXGRID),
+                ("off", (C.c_double * 2) * MAXGRID)]
+
+
+class RpError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} -> {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"librp.so not built: {LIB_PATH} (python -m romspath_amd.build)")
+        L = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        L.rp_create.argtypes = [C.POINTER(vp), C.POINTER(RpConfig)]
+        L.rp_set_grid.argtypes = [vp, C.c_int32, C.POINTER(RpGrid)]
+        L.rp_set_hydro_slot.argtypes = [vp, C.c_int32, C.c_int32, PD, PD, PD, PD, PD, PD, PD]
+        L.rp_set_particles.argtypes = [vp, C.c_int64, PD, PD, PD, PD, PD, PD, PI32, PU8]
+        L.rp_step.argtypes = [vp, PD, PD, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
+                              C.c_int32]
+        L.rp_get_particles.argtypes = [vp, C.c_int64, PD, PD, PD, PD, PD, PI32, PU8, PD, PD, PD]
+        L.rp_set_deviates.argtypes = [vp, PD, C.c_int64, C.c_int32]
+        L.rp_sample_deviates.argtypes = [vp, C.c_int64, C.c_int64, C.c_int32, PD]
+        L.rp_active_steps.argtypes = [vp, PI64]
+        L.rp_secant_capped.argtypes = [vp, PI64]
+        L.rp_synchronize.argtypes = [vp]
+        L.rp_kernel_timing.argtypes = [vp, PD, PI64]
+        L.rp_last_error.argtypes = [vp]
+        L.rp_last_error.restype = C.c_char_p
+        L.rp_destroy.argtypes = [vp]
+        for name in EXPORTS:
+            if name != "rp_last_error":
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(PD)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def make_config(prm, ngrid=1, precision="fp32", block=0, sort_interval=0) -> RpConfig:
+    c = RpConfig()
+    c.idt = prm.idt
+    c.dt = prm.dt
+    c.deltat = prm.deltat
+    c.serr = prm.serr
+    c.smth = prm.smth
+    c.sub = prm.sub
+    c.aksback = prm.AKSback
+    c.vturb_on = int(prm.VTurbOn)
+    c.hturb_on = int(prm.HTurbOn)
+    for i in range(MAXGRID):
+        c.kh[i] = prm.ConstantHTurb[i] if i < len(prm.ConstantHTurb) else 0.0
+    c.scheme = prm.scheme
+    c.nsb = prm.nsb
+    c.vertdist = prm.vertdist
+    c.open_boundary = int(prm.OpenOceanBoundary)
+    c.ngrid = ngrid
+    c.seed = prm.seed
+    c.precision = RP_PREC_FP64 if precision == "fp64" else RP_PREC_FP32_FIELDS
+    c.block = block
+    c.sort_interval = sort_interval
+    return c
+
+
+class Context:
+    """One librp context (one HIP stream)."""
+
+    def __init__(self, cfg: RpConfig):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.rp_create(C.byref(h), C.byref(cfg))
+        if rc != 0:
+            raise RpError("rp_create", rc, "cannot create context")
+        self.h = h
+        self.cfg = cfg
+        self.n = 0
+        self._keep = []
+
+    def _chk(self, fn, rc):
+        if rc != 0:
+            msg = self._L.rp_last_error(self.h).decode(errors="replace")
+            raise RpError(fn, rc, msg)
+
+    def close(self):
+        if self.h:
+            self._L.rp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_grid(self, ng, grid, scl=None, off=None):
+        g = RpGrid()
+        g.L, g.M, g.N = grid.L, grid.M, grid.N
+        g.vtransform = grid.vtransform
+        g.hc = grid.hc
+        keep = {}
+        for name in ("s_rho", "cs_r", "s_w", "cs_w", "h", "pm", "pn"):
+            keep[name] = _f64(getattr(grid, name))
+            setattr(g, name, keep[name].ctypes.data_as(PD))
+        for name in ("mask_rho", "mask_u", "mask_v"):
+            keep[name] = np.ascontiguousarray(getattr(grid, name), dtype=np.int32)
+            setattr(g, name, keep[name].ctypes.data_as(PI32))
+        for i in range(MAXGRID):
+            for k in range(2):
+                g.scl[i][k] = 1.0 if scl is None else scl[i][k]
+                g.off[i][k] = 0.0 if off is None else off[i][k]
+        self._chk("rp_set_grid", self._L.rp_set_grid(self.h, ng, C.byref(g)))
+
+    def set_hydro_slot(self, ng, slot, rec):
+        a = {k: _f64(rec[k]) for k in ("zeta", "u", "v", "w", "aks")}
+        su = _f64(rec["su"]) if "su" in rec else None
+        sv = _f64(rec["sv"]) if "sv" in rec else None
+        self._chk("rp_set_hydro_slot", self._L.rp_set_hydro_slot(
+            self.h, ng, slot, _d(a["zeta"]), _d(a["u"]), _d(a["v"]), _d(a["w"]), _d(a["aks"]),
+            _d(su), _d(sv)))
+
+    def set_particles(self, parts):
+        n = parts.n
+        cols = [_f64(parts.x), _f64(parts.y), _f64(parts.z), _f64(parts.age), _f64(parts.dob),
+                _f64(parts.status)]
+        gid = np.ascontiguousarray(parts.gid, dtype=np.int32)
+        isin = np.ascontiguousarray(parts.isin, dtype=np.uint8)
+        self._chk("rp_set_particles", self._L.rp_set_particles(
+            self.h, n, *[_d(c) for c in cols], gid.ctypes.data_as(PI32),
+            isin.ctypes.data_as(PU8)))
+        self.n = n
+
+    def step(self, ex, ix, ets, tb, tc, tf, step_index, want_diag=False):
+        exa = (C.c_double * 3)(*ex)
+        ixa = (C.c_double * 3)(*ix)
+        self._chk("rp_step", self._L.rp_step(self.h, exa, ixa, ets, tb, tc, tf, step_index,
+                                             int(want_diag)))
+
+    def get_particles(self):
+        n = self.n
+        out = {k: np.empty(n) for k in ("x", "y", "z", "age", "status", "wd", "zeta", "bath")}
+        gid = np.empty(n, dtype=np.int32)
+        isin = np.empty(n, dtype=np.uint8)
+        self._chk("rp_get_particles", self._L.rp_get_particles(
+            self.h, n, _d(out["x"]), _d(out["y"]), _d(out["z"]), _d(out["age"]),
+            _d(out["status"]), gid.ctypes.data_as(PI32), isin.ctypes.data_as(PU8),
+            _d(out["wd"]), _d(out["zeta"]), _d(out["bath"])))
+        out["gid"] = gid
+        out["isin"] = isin
+        return out
+
+    def set_deviates(self, normals):
+        if normals is None:
+            self._chk("rp_set_deviates", self._L.rp_set_deviates(self.h, None, 0, 0))
+            return
+        a = _f64(normals)
+        self._chk("rp_set_deviates", self._L.rp_set_deviates(self.h, _d(a), a.shape[0], a.shape[1]))
+
+    def sample_deviates(self, step_index, n, ndev):
+        out = np.empty((n, ndev))
+        self._chk("rp_sample_deviates",
+                  self._L.rp_sample_deviates(self.h, step_index, n, ndev, _d(out)))
+        return out
+
+    def active_steps(self):
+        v = C.c_int64()
+        self._chk("rp_active_steps", self._L.rp_active_steps(self.h, C.byref(v)))
+        return v.value
+
+    def secant_capped(self):
+        v = C.c_int64()
+        self._chk("rp_secant_capped", self._L.rp_secant_capped(self.h, C.byref(v)))
+        return v.value
+
+    def synchronize(self):
+        self._chk("rp_synchronize", self._L.rp_synchronize(self.h))
+
+    def kernel_timing(self):
+        ms = C.c_double()
+        k = C.c_int64()
+        self._chk("rp_kernel_timing", self._L.rp_kernel_timing(self.h, C.byref(ms), C.byref(k)))
+        return ms.value, k.value

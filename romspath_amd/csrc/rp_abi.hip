@@ -1,0 +1,626 @@
+// rp_abi.hip -- host side of librp.so: context, device memory, uploads and
+// the extern "C" entry points declared in include/rp_abi.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/rp_abi.h"
+#include "rp_kernels.h"
+
+using namespace rp;
+
+namespace {
+
+struct GridDev {
+    int L = 0, M = 0, N = 0, vtransform = 0;
+    double hc = 0;
+    double *s_rho = nullptr, *cs_r = nullptr, *s_w = nullptr, *cs_w = nullptr;
+    double *h = nullptr, *pm = nullptr, *pn = nullptr;
+    int8_t *mrho = nullptr, *mu = nullptr, *mv = nullptr;
+    void *zeta = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *aks = nullptr;
+    double scl[kMaxGrid][2], off[kMaxGrid][2];
+    bool set = false;
+    bool slot_set[3] = {false, false, false};
+};
+
+struct Ctx {
+    rp_config cfg;
+    int fbytes = 4;
+    int device = 0;
+    int ncu = 256;
+    hipStream_t stream = nullptr;
+    GridDev grids[kMaxGrid];
+    // particles (stored order; id[] gives release order)
+    int64_t n = 0;
+    double *x = nullptr, *y = nullptr, *z = nullptr, *age = nullptr, *dob = nullptr;
+    double *wd = nullptr, *zeta = nullptr, *bath = nullptr;
+    int8_t *status = nullptr, *gid = nullptr;
+    uint8_t* isin = nullptr;
+    uint32_t* id = nullptr;
+    // injected deviates
+    double* dev = nullptr;
+    int ndev = 0;
+    bool use_dev = false;
+    // uploads
+    double* staging = nullptr;
+    size_t staging_elems = 0;
+    // VTurb scratch
+    double* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // device copy of the grid table read by the step kernel
+    DGrid* dgrids = nullptr;
+    DGrid tab_host[kMaxGrid];
+    // counters
+    unsigned long long* active = nullptr;
+    unsigned* capped = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+    std::string err;
+};
+
+int fail(Ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((c), RP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+template <typename T>
+int dalloc(Ctx* c, T** p, size_t count)
+{
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, RP_E_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                       " bytes failed: " + hipGetErrorString(e));
+    }
+    return RP_OK;
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+int put(Ctx* c, void* dst, const void* src, size_t bytes)
+{
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return RP_OK;
+}
+
+int status_code(double s, int8_t* out)
+{
+    if (s == 9.0) *out = ST_UNSET;
+    else if (s == 0.0) *out = ST_NEW;
+    else if (s == 1.0) *out = ST_MOVED;
+    else if (s == 1.1) *out = ST_TOP;
+    else if (s == 1.2) *out = ST_BOT;
+    else if (s == -8.0) *out = ST_EXIT;
+    else if (s == -9.0) *out = ST_LAND;
+    else return -1;
+    return 0;
+}
+
+double status_value(int8_t c)
+{
+    switch (c) {
+    case ST_UNSET: return 9.0;
+    case ST_NEW: return 0.0;
+    case ST_MOVED: return 1.0;
+    case ST_TOP: return 1.1;
+    case ST_BOT: return 1.2;
+    case ST_EXIT: return -8.0;
+    case ST_LAND: return -9.0;
+    default: return NAN;
+    }
+}
+
+int ensure_staging(Ctx* c, size_t elems)
+{
+    if (c->staging_elems >= elems) return RP_OK;
+    int rc = dalloc(c, &c->staging, 2 * elems);
+    if (rc) return rc;
+    c->staging_elems = elems;
+    return RP_OK;
+}
+
+// Upload one FP64 Fortran (nx, ny, nk) field into physical slot `slot` of
+// the interleaved device array, adding `add` (may be null) in FP64 first.
+int upload_field(Ctx* c, void* dst, const double* src, const double* add, int nx, int ny, int nk,
+                 int slot)
+{
+    const size_t elems = (size_t)nx * ny * nk;
+    int rc = ensure_staging(c, elems);
+    if (rc) return rc;
+    rc = put(c, c->staging, src, elems * sizeof(double));
+    if (rc) return rc;
+    const double* dadd = nullptr;
+    if (add) {
+        rc = put(c, c->staging + elems, add, elems * sizeof(double));
+        if (rc) return rc;
+        dadd = c->staging + elems;
+    }
+    const int block = 256;
+    const int64_t want = ((int64_t)elems + block - 1) / block;
+    const int grid = (int)std::min<int64_t>(want, (int64_t)c->ncu * 16);
+    if (c->fbytes == 4)
+        hipLaunchKernelGGL(pack_field<float>, dim3(grid), dim3(block), 0, c->stream, c->staging,
+                           dadd, (float*)dst, nx, ny, nk, slot);
+    else
+        hipLaunchKernelGGL(pack_field<double>, dim3(grid), dim3(block), 0, c->stream, c->staging,
+                           dadd, (double*)dst, nx, ny, nk, slot);
+    HIPCHK(c, hipGetLastError());
+    // the staging buffer is reused by the next upload: finish this one first
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RP_OK;
+}
+
+int np_of(const Ctx* c, int N) { return (int)((double)(N + 1) * c->cfg.sub); }
+
+void free_grid(GridDev& g)
+{
+    dfree(g.s_rho); dfree(g.cs_r); dfree(g.s_w); dfree(g.cs_w);
+    dfree(g.h); dfree(g.pm); dfree(g.pn);
+    dfree(g.mrho); dfree(g.mu); dfree(g.mv);
+    dfree(g.zeta); dfree(g.u); dfree(g.v); dfree(g.w); dfree(g.aks);
+    g.set = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rp_create(void** out, const rp_config* cfg)
+{
+    if (!out || !cfg) return RP_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return RP_E_NODEV;
+    Ctx* c = new Ctx();
+    c->cfg = *cfg;
+    std::memset(c->tab_host, 0, sizeof(c->tab_host));
+    if (cfg->precision != RP_PREC_FP32_FIELDS && cfg->precision != RP_PREC_FP64) {
+        delete c;
+        return RP_E_ARG;
+    }
+    if (cfg->ngrid < 1 || cfg->ngrid > RP_MAXGRID || cfg->idt <= 0 || cfg->dt <= 0 ||
+        (cfg->vturb_on && (cfg->deltat < 1.0 || cfg->sub <= 0.0))) {
+        delete c;
+        return RP_E_ARG;
+    }
+    c->fbytes = cfg->precision == RP_PREC_FP64 ? 8 : 4;
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) !=
+            hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RP_E_HIP;
+    }
+    if (dalloc(c, &c->active, 1) || dalloc(c, &c->capped, 1) || dalloc(c, &c->dgrids, kMaxGrid)) {
+        rp_destroy(c);
+        return RP_E_ALLOC;
+    }
+    if (hipMemsetAsync(c->active, 0, sizeof(unsigned long long), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->capped, 0, sizeof(unsigned), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        rp_destroy(c);
+        return RP_E_HIP;
+    }
+    *out = c;
+    return RP_OK;
+}
+
+int rp_set_grid(void* ctx, int32_t ng, const rp_grid* gh)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !gh) return RP_E_ARG;
+    if (ng < 1 || ng > c->cfg.ngrid) return fail(c, RP_E_ARG, "grid number out of range");
+    if (gh->L < 4 || gh->M < 4 || gh->N < 1 || gh->vtransform < 1 || gh->vtransform > 3 ||
+        !gh->s_rho || !gh->cs_r || !gh->s_w || !gh->cs_w || !gh->h || !gh->pm || !gh->pn ||
+        !gh->mask_rho || !gh->mask_u || !gh->mask_v)
+        return fail(c, RP_E_ARG, "invalid grid description");
+    GridDev& g = c->grids[ng - 1];
+    free_grid(g);
+    g.L = gh->L; g.M = gh->M; g.N = gh->N;
+    g.vtransform = gh->vtransform;
+    g.hc = gh->hc;
+    const size_t nr = (size_t)g.L * g.M, nu = (size_t)(g.L - 1) * g.M, nv = (size_t)g.L * (g.M - 1);
+    int rc = 0;
+    rc |= dalloc(c, &g.s_rho, g.N); rc |= dalloc(c, &g.cs_r, g.N);
+    rc |= dalloc(c, &g.s_w, g.N + 1); rc |= dalloc(c, &g.cs_w, g.N + 1);
+    rc |= dalloc(c, &g.h, nr); rc |= dalloc(c, &g.pm, nr); rc |= dalloc(c, &g.pn, nr);
+    rc |= dalloc(c, &g.mrho, nr); rc |= dalloc(c, &g.mu, nu); rc |= dalloc(c, &g.mv, nv);
+    const size_t fb = c->fbytes;
+    rc |= dalloc(c, (char**)&g.zeta, nr * 4 * fb);
+    rc |= dalloc(c, (char**)&g.u, nu * g.N * 4 * fb);
+    rc |= dalloc(c, (char**)&g.v, nv * g.N * 4 * fb);
+    rc |= dalloc(c, (char**)&g.w, nr * (g.N + 1) * 4 * fb);
+    rc |= dalloc(c, (char**)&g.aks, nr * (g.N + 1) * 4 * fb);
+    if (rc) {
+        free_grid(g);
+        return RP_E_ALLOC;
+    }
+    // zero the fields so unused slot lanes hold finite values
+    HIPCHK(c, hipMemsetAsync(g.zeta, 0, nr * 4 * fb, c->stream));
+    HIPCHK(c, hipMemsetAsync(g.u, 0, nu * g.N * 4 * fb, c->stream));
+    HIPCHK(c, hipMemsetAsync(g.v, 0, nv * g.N * 4 * fb, c->stream));
+    HIPCHK(c, hipMemsetAsync(g.w, 0, nr * (g.N + 1) * 4 * fb, c->stream));
+    HIPCHK(c, hipMemsetAsync(g.aks, 0, nr * (g.N + 1) * 4 * fb, c->stream));
+    if ((rc = put(c, g.s_rho, gh->s_rho, g.N * 8)) || (rc = put(c, g.cs_r, gh->cs_r, g.N * 8)) ||
+        (rc = put(c, g.s_w, gh->s_w, (g.N + 1) * 8)) || (rc = put(c, g.cs_w, gh->cs_w, (g.N + 1) * 8)) ||
+        (rc = put(c, g.h, gh->h, nr * 8)) || (rc = put(c, g.pm, gh->pm, nr * 8)) ||
+        (rc = put(c, g.pn, gh->pn, nr * 8)))
+        return rc;
+    std::vector<int8_t> m8(nr);
+    auto conv = [&](const int32_t* src, size_t cnt, int8_t* dst) -> int {
+        for (size_t i = 0; i < cnt; i++) {
+            if (src[i] != 0 && src[i] != 1) return -1;
+            m8[i] = (int8_t)src[i];
+        }
+        hipError_t e = hipMemcpy(dst, m8.data(), cnt, hipMemcpyHostToDevice);
+        return e == hipSuccess ? 0 : -2;
+    };
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (conv(gh->mask_rho, nr, g.mrho) || conv(gh->mask_u, nu, g.mu) || conv(gh->mask_v, nv, g.mv))
+        return fail(c, RP_E_ARG, "masks must be 0/1");
+    for (int i = 0; i < kMaxGrid; i++)
+        for (int k = 0; k < 2; k++) {
+            g.scl[i][k] = gh->scl[i][k];
+            g.off[i][k] = gh->off[i][k];
+        }
+    for (int s = 0; s < 3; s++) g.slot_set[s] = false;
+    g.set = true;
+    // size the VTurb scratch for the largest grid
+    return RP_OK;
+}
+
+int rp_set_hydro_slot(void* ctx, int32_t ng, int32_t slot, const double* zeta, const double* u,
+                      const double* v, const double* w, const double* aks, const double* su,
+                      const double* sv)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (ng < 1 || ng > c->cfg.ngrid || !c->grids[ng - 1].set)
+        return fail(c, RP_E_STATE, "rp_set_hydro_slot: grid not set");
+    if (slot < 1 || slot > 3) return fail(c, RP_E_ARG, "slot must be 1..3");
+    if (!zeta || !u || !v || !w || !aks) return fail(c, RP_E_ARG, "null field");
+    GridDev& g = c->grids[ng - 1];
+    const int s = slot - 1;
+    int rc;
+    if ((rc = upload_field(c, g.zeta, zeta, nullptr, g.L, g.M, 1, s)) ||
+        (rc = upload_field(c, g.u, u, su, g.L - 1, g.M, g.N, s)) ||
+        (rc = upload_field(c, g.v, v, sv, g.L, g.M - 1, g.N, s)) ||
+        (rc = upload_field(c, g.w, w, nullptr, g.L, g.M, g.N + 1, s)) ||
+        (rc = upload_field(c, g.aks, aks, nullptr, g.L, g.M, g.N + 1, s)))
+        return rc;
+    g.slot_set[s] = true;
+    return RP_OK;
+}
+
+int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, const double* z,
+                     const double* age, const double* dob, const double* status, const int32_t* gid,
+                     const uint8_t* isin)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (n < 0 || n > (int64_t)0xFFFFFFFFLL) return fail(c, RP_E_RANGE, "particle count out of range");
+    if (n > 0 && (!x || !y || !z || !age || !dob || !status || !gid || !isin))
+        return fail(c, RP_E_ARG, "null particle column");
+    std::vector<int8_t> st(n), gd(n);
+    std::vector<uint32_t> ids(n);
+    for (int64_t i = 0; i < n; i++) {
+        if (status_code(status[i], &st[i])) return fail(c, RP_E_ARG, "unknown status value");
+        if (gid[i] < 1 || gid[i] > c->cfg.ngrid) return fail(c, RP_E_ARG, "grid id out of range");
+        gd[i] = (int8_t)gid[i];
+        ids[i] = (uint32_t)i;
+    }
+    int rc = 0;
+    rc |= dalloc(c, &c->x, n); rc |= dalloc(c, &c->y, n); rc |= dalloc(c, &c->z, n);
+    rc |= dalloc(c, &c->age, n); rc |= dalloc(c, &c->dob, n);
+    rc |= dalloc(c, &c->wd, n); rc |= dalloc(c, &c->zeta, n); rc |= dalloc(c, &c->bath, n);
+    rc |= dalloc(c, &c->status, n); rc |= dalloc(c, &c->gid, n); rc |= dalloc(c, &c->isin, n);
+    rc |= dalloc(c, &c->id, n);
+    if (rc) return RP_E_ALLOC;
+    c->n = n;
+    if (n == 0) return RP_OK;
+    if ((rc = put(c, c->x, x, n * 8)) || (rc = put(c, c->y, y, n * 8)) || (rc = put(c, c->z, z, n * 8)) ||
+        (rc = put(c, c->age, age, n * 8)) || (rc = put(c, c->dob, dob, n * 8)) ||
+        (rc = put(c, c->status, st.data(), n)) || (rc = put(c, c->gid, gd.data(), n)) ||
+        (rc = put(c, c->isin, isin, n)) || (rc = put(c, c->id, ids.data(), n * 4)))
+        return rc;
+    std::vector<double> fill(n, 9999.0);
+    if ((rc = put(c, c->wd, fill.data(), n * 8)) || (rc = put(c, c->zeta, fill.data(), n * 8)) ||
+        (rc = put(c, c->bath, fill.data(), n * 8)))
+        return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RP_OK;
+}
+
+int rp_set_deviates(void* ctx, const double* normals, int64_t n, int32_t ndev)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (!normals) {
+        c->use_dev = false;
+        return RP_OK;
+    }
+    const int need = 2 + (c->cfg.vturb_on ? c->cfg.idt / (int)c->cfg.deltat : 0);
+    if (n != c->n || ndev < need) return fail(c, RP_E_ARG, "deviate block has the wrong shape");
+    int rc = dalloc(c, &c->dev, (size_t)n * ndev);
+    if (rc) return rc;
+    if ((rc = put(c, c->dev, normals, (size_t)n * ndev * 8))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ndev = ndev;
+    c->use_dev = true;
+    return RP_OK;
+}
+
+int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int32_t t_b,
+            int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !ex || !ix) return RP_E_ARG;
+    if (t_b < 1 || t_b > 3 || t_c < 1 || t_c > 3 || t_f < 1 || t_f > 3)
+        return fail(c, RP_E_ARG, "slot numbers must be 1..3");
+    if (step_index < 0) return fail(c, RP_E_ARG, "negative step index");
+    int maxN = 0;
+    for (int i = 0; i < c->cfg.ngrid; i++) {
+        const GridDev& g = c->grids[i];
+        if (!g.set) return fail(c, RP_E_STATE, "rp_step: grid " + std::to_string(i + 1) + " not set");
+        if (!g.slot_set[t_b - 1] || !g.slot_set[t_c - 1] || !g.slot_set[t_f - 1])
+            return fail(c, RP_E_STATE, "rp_step: hydro slot not loaded");
+        maxN = std::max(maxN, g.N);
+    }
+    if (c->n == 0) return RP_OK;
+    const bool vt = c->cfg.vturb_on != 0;
+    StepArgs A;
+    for (int k = 0; k < 3; k++) {
+        A.ex[k] = ex[k];
+        A.ix[k] = ix[k];
+    }
+    A.ets = ets;
+    A.slots.s[0] = t_b - 1;
+    A.slots.s[1] = t_c - 1;
+    A.slots.s[2] = t_f - 1;
+    A.idt = c->cfg.idt;
+    A.deltat = c->cfg.deltat;
+    A.serr = c->cfg.serr;
+    A.smth = c->cfg.smth;
+    A.aksback = c->cfg.aksback;
+    A.vturb_on = vt;
+    A.hturb_on = c->cfg.hturb_on != 0;
+    A.scheme = c->cfg.scheme;
+    A.nsb = c->cfg.nsb;
+    A.vertdist = c->cfg.vertdist;
+    A.open_boundary = c->cfg.open_boundary != 0;
+    A.ngrid = c->cfg.ngrid;
+    A.np = np_of(c, maxN);
+    A.loop = vt ? c->cfg.idt / (int)c->cfg.deltat : 0;
+    A.n = c->n;
+    A.step_index = step_index;
+    A.seed = c->cfg.seed;
+    A.x = c->x; A.y = c->y; A.z = c->z; A.age = c->age; A.dob = c->dob;
+    A.status = c->status; A.gid = c->gid; A.isin = c->isin;
+    A.wd = c->wd; A.zeta = c->zeta; A.bath = c->bath;
+    A.want_diag = want_diag != 0;
+    A.id = c->id;
+    A.dev = c->use_dev ? c->dev : nullptr;
+    A.ndev = c->ndev;
+    A.active = c->active;
+    A.capped = c->capped;
+    if (vt && A.np < 3) return fail(c, RP_E_ARG, "smoothing spline needs sub*(N+1) >= 3");
+
+    const int block = c->cfg.block > 0 ? c->cfg.block : 256;
+    if (block % 64 || block > 256) return fail(c, RP_E_ARG, "block must be a multiple of 64, <= 256");
+    int64_t grid = (c->n + block - 1) / block;
+    A.scratch = nullptr;
+    if (vt) {
+        // persistent grid: bounded scratch, lanes loop over particles
+        const int64_t cap = (int64_t)c->ncu * 1024 / block;
+        grid = std::min<int64_t>(grid, cap);
+        const int64_t T = grid * block;
+        const size_t need = (size_t)vturb_rows(A.np, maxN + 1) * T * sizeof(double);
+        if (c->scratch_bytes < need) {
+            int rc = dalloc(c, (char**)&c->scratch, need);
+            if (rc) return rc;
+            c->scratch_bytes = need;
+        }
+        A.scratch = c->scratch;
+    }
+    {
+        DGrid tab[kMaxGrid];
+        std::memset(tab, 0, sizeof(tab));
+        for (int i = 0; i < c->cfg.ngrid; i++) {
+            const GridDev& g = c->grids[i];
+            DGrid& d = tab[i];
+            d.L = g.L; d.M = g.M; d.N = g.N; d.vtransform = g.vtransform; d.hc = g.hc;
+            d.s_rho = g.s_rho; d.cs_r = g.cs_r; d.s_w = g.s_w; d.cs_w = g.cs_w;
+            d.h = g.h; d.pm = g.pm; d.pn = g.pn;
+            d.mrho = g.mrho; d.mu = g.mu; d.mv = g.mv;
+            d.zeta = g.zeta; d.u = g.u; d.v = g.v; d.w = g.w; d.aks = g.aks;
+            d.kh = c->cfg.kh[i];
+            for (int a = 0; a < kMaxGrid; a++)
+                for (int k = 0; k < 2; k++) {
+                    d.scl[a][k] = g.scl[a][k];
+                    d.off[a][k] = g.off[a][k];
+                }
+        }
+        if (std::memcmp(tab, c->tab_host, sizeof(tab)) != 0) {
+            std::memcpy(c->tab_host, tab, sizeof(tab));
+            HIPCHK(c, hipMemcpyAsync(c->dgrids, c->tab_host, sizeof(tab), hipMemcpyHostToDevice,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+    }
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!c->ev_free.empty()) {
+        ev = c->ev_free.back();
+        c->ev_free.pop_back();
+    } else {
+        HIPCHK(c, hipEventCreate(&ev.first));
+        HIPCHK(c, hipEventCreate(&ev.second));
+    }
+    HIPCHK(c, hipEventRecord(ev.first, c->stream));
+    const bool nested = c->cfg.ngrid > 1;
+    const dim3 gd((unsigned)grid), bd(block);
+    if (c->fbytes == 4 && !nested)
+        hipLaunchKernelGGL((step_kernel<float, false>), gd, bd, 0, c->stream, c->dgrids, A);
+    else if (c->fbytes == 4)
+        hipLaunchKernelGGL((step_kernel<float, true>), gd, bd, 0, c->stream, c->dgrids, A);
+    else if (!nested)
+        hipLaunchKernelGGL((step_kernel<double, false>), gd, bd, 0, c->stream, c->dgrids, A);
+    else
+        hipLaunchKernelGGL((step_kernel<double, true>), gd, bd, 0, c->stream, c->dgrids, A);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(ev.second, c->stream));
+    c->ev_used.push_back(ev);
+    return RP_OK;
+}
+
+int rp_get_particles(void* ctx, int64_t n, double* x, double* y, double* z, double* age,
+                     double* status, int32_t* gid, uint8_t* isin, double* wd, double* zeta,
+                     double* bath)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (n != c->n) return fail(c, RP_E_ARG, "particle count differs from rp_set_particles");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n == 0) return RP_OK;
+    std::vector<uint32_t> ids(n);
+    HIPCHK(c, hipMemcpy(ids.data(), c->id, n * 4, hipMemcpyDeviceToHost));
+    std::vector<double> tmp(n);
+    auto getd = [&](double* dst, const double* src) -> int {
+        if (!dst) return 0;
+        if (hipMemcpy(tmp.data(), src, n * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        for (int64_t i = 0; i < n; i++) dst[ids[i]] = tmp[i];
+        return 0;
+    };
+    if (getd(x, c->x) || getd(y, c->y) || getd(z, c->z) || getd(age, c->age) || getd(wd, c->wd) ||
+        getd(zeta, c->zeta) || getd(bath, c->bath))
+        return fail(c, RP_E_HIP, "device-to-host copy failed");
+    std::vector<int8_t> b8(n);
+    if (status) {
+        HIPCHK(c, hipMemcpy(b8.data(), c->status, n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++) status[ids[i]] = status_value(b8[i]);
+    }
+    if (gid) {
+        HIPCHK(c, hipMemcpy(b8.data(), c->gid, n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++) gid[ids[i]] = b8[i];
+    }
+    if (isin) {
+        std::vector<uint8_t> u8(n);
+        HIPCHK(c, hipMemcpy(u8.data(), c->isin, n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++) isin[ids[i]] = u8[i];
+    }
+    return RP_OK;
+}
+
+int rp_sample_deviates(void* ctx, int64_t step_index, int64_t n, int32_t ndev, double* out)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !out || n < 0 || ndev < 1) return RP_E_ARG;
+    if (n == 0) return RP_OK;
+    double* d = nullptr;
+    int rc = dalloc(c, &d, (size_t)n * ndev);
+    if (rc) return rc;
+    hipLaunchKernelGGL(sample_deviates, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       (uint64_t)c->cfg.seed, step_index, n, ndev, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * ndev * 8, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(c, RP_E_HIP, hipGetErrorString(e));
+    return RP_OK;
+}
+
+int rp_active_steps(void* ctx, int64_t* count)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !count) return RP_E_ARG;
+    unsigned long long v = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&v, c->active, sizeof(v), hipMemcpyDeviceToHost));
+    *count = (int64_t)v;
+    return RP_OK;
+}
+
+int rp_secant_capped(void* ctx, int64_t* count)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !count) return RP_E_ARG;
+    unsigned v = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&v, c->capped, sizeof(v), hipMemcpyDeviceToHost));
+    *count = (int64_t)v;
+    return RP_OK;
+}
+
+int rp_synchronize(void* ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RP_OK;
+}
+
+int rp_kernel_timing(void* ctx, double* avg_ms, int64_t* launches)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !avg_ms || !launches) return RP_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (auto& e : c->ev_used) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e.first, e.second));
+        tot += ms;
+        c->ev_free.push_back(e);
+    }
+    *launches = (int64_t)c->ev_used.size();
+    *avg_ms = c->ev_used.empty() ? 0.0 : tot / (double)c->ev_used.size();
+    c->ev_used.clear();
+    return RP_OK;
+}
+
+const char* rp_last_error(void* ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return "null context";
+    return c->err.c_str();
+}
+
+int rp_destroy(void* ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& g : c->grids) free_grid(g);
+    dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->age); dfree(c->dob);
+    dfree(c->wd); dfree(c->zeta); dfree(c->bath);
+    dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id);
+    dfree(c->dev); dfree(c->staging); dfree(c->scratch);
+    dfree(c->active); dfree(c->capped); dfree(c->dgrids);
+    for (auto& e : c->ev_used) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto& e : c->ev_free) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+// rp_device.h -- device-side building blocks of the particle step (gfx950).
+//
+// Each function restates one reference routine; the citation is on the
+// function.  FP64 arithmetic follows the reference's operation order and the
+// translation unit is built with -ffp-contract=off, so with FP64 fields the
+// results equal the reference's to the last bit wherever the reference's
+// arithmetic is IEEE (no FMA, correctly rounded +-*/ and sqrt).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rp {
+
+constexpr int kMaxGrid = 5;
+
+// Status codes held on the device (int8); the host maps them to the
+// reference's FP64 par(:,pStatus) values {9, 0, 1, 1.1, 1.2, -8, -9}.
+enum : int8_t {
+    ST_NEW = 0,       // 0.0  released, not yet moved
+    ST_UNSET = 9,     // 9.0  below the bed at release: never moves
+    ST_MOVED = 10,    // 1.0
+    ST_TOP = 11,      // 1.1  reflected at the surface
+    ST_BOT = 12,      // 1.2  reflected at the bed
+    ST_EXIT = -8,     // -8.0 left through an open boundary
+    ST_LAND = -9      // -9.0 move rejected: landed on a dry cell
+};
+
+// Device view of one grid.  Hydro fields hold four physical time slots
+// interleaved innermost: element (i, j, k) slot s of a field with nx points
+// along xi and nk levels sits at (((j-1)*nx + (i-1))*nk + (k-1))*4 + s, so the
+// three slots the step reads (back, centre, forward) arrive in one 16-byte
+// (FP32) or 32-byte (FP64) load, and levels k and k+1 are adjacent.
+struct DGrid {
+    int L, M, N, vtransform;
+    double hc;
+    const double *s_rho, *cs_r, *s_w, *cs_w;
+    const double *h, *pm, *pn;           // (xi_rho, eta_rho) column-major
+    const int8_t *mrho, *mu, *mv;        // masks
+    const void *zeta, *u, *v, *w, *aks;  // FP32 or FP64, slot-interleaved
+    double kh;                           // ConstantHTurb(ng)
+    double scl[kMaxGrid][2], off[kMaxGrid][2];
+};
+
+// ---------------------------------------------------------------------------
+// loads
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ld4(const float* p, double o[4])
+{
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+__device__ __forceinline__ void ld4(const double* p, double o[4])
+{
+    const double2 a = reinterpret_cast<const double2*>(p)[0];
+    const double2 b = reinterpret_cast<const double2*>(p)[1];
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ __forceinline__ double pick(const double o[4], int s)
+{
+    return s == 0 ? o[0] : (s == 1 ? o[1] : (s == 2 ? o[2] : o[3]));
+}
+
+// ---------------------------------------------------------------------------
+// interpolation_module.f90:72-109  (exact 3-point quadratic, Newton form
+// based at the node nearest x)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double polintd(const double xa[3], double y0, double y1, double y2,
+                                          double x)
+{
+    int ns = 1;
+    double dif = fabs(x - xa[0]);
+    double dift = fabs(x - xa[1]);
+    if (dift < dif) { ns = 2; dif = dift; }
+    dift = fabs(x - xa[2]);
+    if (dift < dif) { ns = 3; dif = dift; }
+    double c = (xa[1] - x) * ((y2 - y1) / (xa[1] - xa[2]));
+    c = c - (xa[1] - x) * ((y1 - y0) / (xa[0] - xa[1]));
+    c = c / (xa[0] - xa[2]);
+    double a, b;
+    if (ns == 3) {
+        a = (y2 - y1) / (xa[1] - xa[2]);
+        b = xa[2] - x;
+    } else {
+        a = (y1 - y0) / (xa[0] - xa[1]);
+        b = xa[0] - x;
+    }
+    const double yn = ns == 1 ? y0 : (ns == 2 ? y1 : y2);
+    const double xn = ns == 1 ? xa[0] : (ns == 2 ? xa[1] : xa[2]);
+    return yn + (xn - x) * a + b * c;
+}
+
+// interpolation_module.f90:1230-1273 (masked inverse-distance weights)
+__device__ __forceinline__ void getcoeff(double X, double Y, const double m[4], double cff[4])
+{
+    const double d0 = sqrt(X * X + Y * Y);
+    const double d1 = sqrt((1 - X) * (1 - X) + Y * Y);
+    const double d2 = sqrt((1 - X) * (1 - X) + (1 - Y) * (1 - Y));
+    const double d3 = sqrt(X * X + (1 - Y) * (1 - Y));
+    const double w0 = m[0] / d0, w1 = m[1] / d1, w2 = m[2] / d2, w3 = m[3] / d3;
+    const double wt = w0 + w1 + w2 + w3;
+    if (wt == 0.0) {
+        cff[0] = cff[1] = cff[2] = cff[3] = 0.0;
+    } else {
+        cff[0] = w0 / wt; cff[1] = w1 / wt; cff[2] = w2 / wt; cff[3] = w3 / wt;
+    }
+}
+
+__device__ __forceinline__ void bilin(double x, double y, double c[4])
+{
+    c[0] = (1 - x) * (1 - y);
+    c[1] = x * (1 - y);
+    c[2] = x * y;
+    c[3] = (1 - x) * y;
+}
+
+// grid_module.f90:513-598.  S(k) does not depend on zeta, so callers that
+// need several slots at one level compute it once (same value bit for bit).
+__device__ __forceinline__ double level_S(const DGrid& g, const double* s, const double* cs,
+                                          int k, double h)
+{
+    if (g.vtransform == 2) return (g.hc * s[k - 1] + h * cs[k - 1]) / (g.hc + h);
+    if (g.vtransform == 1) return g.hc * s[k - 1] + (h - g.hc) * cs[k - 1];
+    return 0.0;
+}
+__device__ __forceinline__ double level_z(const DGrid& g, const double* s, const double* cs,
+                                          int k, double zeta, double h, double S)
+{
+    if (g.vtransform == 2) return zeta + (zeta + h) * S;
+    if (g.vtransform == 1) return S + zeta * (1.0 + S / h);
+    return zeta * (1.0 + s[k - 1]) + g.hc * s[k - 1] + (h - g.hc) * cs[k - 1];
+}
+__device__ __forceinline__ double level(const DGrid& g, const double* s, const double* cs, int k,
+                                        double zeta, double h)
+{
+    return level_z(g, s, cs, k, zeta, h, level_S(g, s, cs, k, h));
+}
+
+// First k in [1, nz] with level(k) > Z (nz+1 if none).  The reference scans
+// linearly (interpolation_module.f90:533-577); the levels are non-decreasing
+// in k, so the predicate is monotone and bisection returns the same index.
+__device__ __forceinline__ int first_above(const DGrid& g, const double* s, const double* cs,
+                                           int nz, double Z, double zeta, double h)
+{
+    int lo = 1, hi = nz + 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (level(g, s, cs, mid, zeta, h) > Z) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Vertical bracket of getInterp3d (interpolation_module.f90:528-589).
+struct VBracket {
+    int ibot, itop;
+    double c0, c1;
+};
+
+__device__ __forceinline__ VBracket bracket_rho(const DGrid& g, double Z, double zeta, double depth)
+{
+    const double h = -1.0 * depth;
+    const int nz = g.N;
+    const int i = first_above(g, g.s_rho, g.cs_r, nz, Z, zeta, h);
+    VBracket b;
+    if (i > nz) {
+        b.ibot = b.itop = nz; b.c0 = 0.5; b.c1 = 0.5;
+    } else if (i == 1) {
+        b.ibot = b.itop = 1; b.c0 = 0.5; b.c1 = 0.5;
+    } else {
+        const double z1 = level(g, g.s_rho, g.cs_r, i - 1, zeta, h);
+        const double z2 = level(g, g.s_rho, g.cs_r, i, zeta, h);
+        const double Zr = (Z - z1) / (z2 - z1);
+        b.ibot = i - 1; b.itop = i; b.c0 = 1.0 - Zr; b.c1 = Zr;
+    }
+    return b;
+}
+
+__device__ __forceinline__ VBracket bracket_w(const DGrid& g, double Z, double zeta, double depth)
+{
+    const double h = -1.0 * depth;
+    const int nz = g.N + 1;
+    const int i = first_above(g, g.s_w, g.cs_w, nz, Z, zeta, h);
+    VBracket b;
+    if (i > nz) {
+        // z at or above the top w level: two copies of level nz (the
+        // reference's weight is undefined there; the value is not).
+        b.ibot = b.itop = nz; b.c0 = 0.5; b.c1 = 0.5;
+    } else {
+        const int ib = i > 1 ? i - 1 : 1;
+        const double z1 = level(g, g.s_w, g.cs_w, ib, zeta, h);
+        const double z2 = level(g, g.s_w, g.cs_w, i, zeta, h);
+        const double Zr = (Z - z1) / (z2 - z1);
+        b.ibot = ib; b.itop = i; b.c0 = 1.0 - Zr; b.c1 = Zr;
+    }
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11), the counter-based generator that replaces the global
+// serial MT19937 stream of random_module.f90 / pdf_module.f90:26-40.
+// ---------------------------------------------------------------------------
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)M0 * c.x;
+        const uint64_t p1 = (uint64_t)M1 * c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        U4 n;
+        n.x = hi1 ^ c.y ^ k0;
+        n.y = lo1;
+        n.z = hi0 ^ c.w ^ k1;
+        n.w = lo0;
+        c = n;
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// Four standard normals from one Philox block: two Box-Muller pairs on
+// uniforms (x + 1/2) 2^-32 in (0,1), the resolution of genrand_real3
+// (random_module.f90:240-247).  The reference keeps only the cosine branch
+// (pdf_module.f90:38); both branches are independent normals, so both are used.
+__device__ __forceinline__ void normals4(uint64_t seed, uint64_t id, uint64_t step, uint32_t blk,
+                                         double out[4])
+{
+    U4 c;
+    c.x = blk;
+    c.y = (uint32_t)id;
+    c.z = (uint32_t)(id >> 32);
+    c.w = (uint32_t)step;
+    const U4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32) ^ (uint32_t)(step >> 32));
+    const double s = 2.3283064365386963e-10;  // 2^-32
+    const double u0 = ((double)r.x + 0.5) * s, u1 = ((double)r.y + 0.5) * s;
+    const double u2 = ((double)r.z + 0.5) * s, u3 = ((double)r.w + 0.5) * s;
+    const double twopi = 6.283185307179586;
+    const double ra = sqrt(-2.0 * log(u0)), rb = sqrt(-2.0 * log(u2));
+    double sa, ca, sb, cb;
+    sincos(twopi * u1, &sa, &ca);
+    sincos(twopi * u3, &sb, &cb);
+    out[0] = ra * ca; out[1] = ra * sa; out[2] = rb * cb; out[3] = rb * sb;
+}
+
+}  // namespace rp

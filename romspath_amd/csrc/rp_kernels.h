@@ -1,0 +1,59 @@
+// rp_kernels.h -- kernel argument blocks and kernel declarations shared by
+// the step kernels and the C-ABI host layer.
+#pragma once
+
+#include "rp_device.h"
+
+namespace rp {
+
+// The reference's secant search for the smoothing parameter has no iteration
+// cap (pppack.f90:5270-5292); a GPU lane must terminate, so it stops here and
+// the stop is counted (rp_secant_capped reports it).
+constexpr int kSecantCap = 64;
+
+struct SlotSel {
+    int s[3];   // physical slot (0..3) holding t_b, t_c, t_f
+};
+
+struct StepArgs {
+    double ex[3], ix[3];
+    int ets;
+    SlotSel slots;
+    int idt;
+    double deltat, serr, smth, aksback;
+    int vturb_on, hturb_on, scheme, nsb;
+    double vertdist;
+    int open_boundary, ngrid;
+    int np, loop;
+    int64_t n;
+    int64_t step_index;
+    uint64_t seed;
+    double *x, *y, *z, *age;
+    const double* dob;
+    int8_t* status;
+    int8_t* gid;
+    uint8_t* isin;
+    double *wd, *zeta, *bath;
+    int want_diag;
+    const uint32_t* id;          // release-order id of each stored particle
+    const double* dev;           // injected deviates [id][ndev] or null
+    int ndev;
+    double* scratch;             // VTurb work arrays, column per lane
+    unsigned long long* active;  // += particles that took the step
+    unsigned* capped;            // += secant searches stopped at the cap
+};
+
+// NESTED=false: one grid, all lanes read grid 0 (uniform, scalar loads)
+template <typename F, bool NESTED>
+__global__ void step_kernel(const DGrid* __restrict__ grids, StepArgs A);
+
+template <typename F>
+__global__ void pack_field(const double* __restrict__ src, const double* __restrict__ add, F* dst,
+                           int nx, int ny, int nk, int slot);
+
+__global__ void sample_deviates(uint64_t seed, int64_t step, int64_t n, int ndev, double* out);
+
+// rows of VTurb scratch per lane for np resampled points and nw w-levels
+inline int64_t vturb_rows(int np, int nw) { return 9LL * np + 2LL * nw; }
+
+}  // namespace rp

@@ -1,0 +1,851 @@
+// rp_step.hip -- the particle-step kernel (one lane per particle) and the
+// field-upload kernels.  Replaces update_particles (ROMSPath.f90:662-1003) and
+// everything it calls for Behavior=0.
+#include "rp_kernels.h"
+
+namespace rp {
+
+// ---------------------------------------------------------------------------
+// horizontal stencil of a rho-point field: floor cell, bilinear weights, the
+// four corner offsets (I,J) (I+1,J) (I+1,J+1) (I,J+1) in an (L, M) array.
+// ---------------------------------------------------------------------------
+struct RhoCell {
+    int I, J;
+    double x, y;
+    int64_t o[4];
+};
+
+__device__ __forceinline__ RhoCell rho_cell(const DGrid& g, double X, double Y)
+{
+    RhoCell c;
+    c.I = (int)floor(X);
+    c.J = (int)floor(Y);
+    c.x = X - c.I;
+    c.y = Y - c.J;
+    const int64_t b = (int64_t)(c.J - 1) * g.L + (c.I - 1);
+    c.o[0] = b;
+    c.o[1] = b + 1;
+    c.o[2] = b + g.L + 1;
+    c.o[3] = b + g.L;
+    return c;
+}
+
+// Same cell with indices clamped into the grid; used only for reads at the
+// post-step position, where the reference can index outside the arrays
+// after a failed nested hand-off (ROMSPath.f90:837-843).
+__device__ __forceinline__ RhoCell rho_cell_clamped(const DGrid& g, double X, double Y)
+{
+    RhoCell c;
+    c.I = (int)floor(X);
+    c.J = (int)floor(Y);
+    c.x = X - c.I;
+    c.y = Y - c.J;
+    const int i0 = min(max(c.I, 1), g.L), i1 = min(max(c.I + 1, 1), g.L);
+    const int j0 = min(max(c.J, 1), g.M), j1 = min(max(c.J + 1, 1), g.M);
+    c.o[0] = (int64_t)(j0 - 1) * g.L + (i0 - 1);
+    c.o[1] = (int64_t)(j0 - 1) * g.L + (i1 - 1);
+    c.o[2] = (int64_t)(j1 - 1) * g.L + (i1 - 1);
+    c.o[3] = (int64_t)(j1 - 1) * g.L + (i0 - 1);
+    return c;
+}
+
+// getInterp2D("depth"|"pm"|"pn") -- plain bilinear (interpolation_module.f90:421-507)
+__device__ __forceinline__ double bilinear2d(const double* f, const RhoCell& c)
+{
+    double w[4];
+    bilin(c.x, c.y, w);
+    return w[0] * f[c.o[0]] + w[1] * f[c.o[1]] + w[2] * f[c.o[2]] + w[3] * f[c.o[3]];
+}
+
+// getInterp2D("zeta") at slots b, c, f: always inverse-distance weighted
+// (getcoeff is called unconditionally, interpolation_module.f90:486).
+template <typename F>
+__device__ __forceinline__ void zeta3(const DGrid& g, const RhoCell& c, const SlotSel& S,
+                                      double out[3])
+{
+    double m[4], w[4];
+    for (int k = 0; k < 4; k++) m[k] = (double)g.mrho[c.o[k]];
+    getcoeff(c.x, c.y, m, w);
+    const F* z = (const F*)g.zeta;
+    double v[4][4];
+    for (int k = 0; k < 4; k++) ld4(z + c.o[k] * 4, v[k]);
+    for (int t = 0; t < 3; t++) {
+        const int s = S.s[t];
+        out[t] = w[0] * pick(v[0], s) + w[1] * pick(v[1], s) + w[2] * pick(v[2], s) +
+                 w[3] * pick(v[3], s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// getInterp3d for u, v, w at the slots b, c, f (interpolation_module.f90:509-962)
+// The vertical bracket is shared by every slot: the reference recomputes it
+// per call from the same (z, zeta, depth), so the values are identical.
+// ---------------------------------------------------------------------------
+template <typename F>
+__device__ __forceinline__ void blend3(const F* f, const int64_t col[4], int nk,
+                                       const double cff[4], const bool zero[4],
+                                       const VBracket& vb, const SlotSel& S, int nslot,
+                                       double out[3])
+{
+    double zb[3] = {0, 0, 0}, zt[3] = {0, 0, 0};
+    for (int k = 0; k < 4; k++) {
+        double lb[4], lt[4];
+        ld4(f + (col[k] * nk + (vb.ibot - 1)) * 4, lb);
+        ld4(f + (col[k] * nk + (vb.itop - 1)) * 4, lt);
+        for (int t = 0; t < nslot; t++) {
+            const int s = S.s[t];
+            const double b = zero[k] ? 0.0 : pick(lb, s);
+            const double a = zero[k] ? 0.0 : pick(lt, s);
+            if (k == 0) {
+                zb[t] = cff[0] * b;
+                zt[t] = cff[0] * a;
+            } else {
+                zb[t] = zb[t] + cff[k] * b;
+                zt[t] = zt[t] + cff[k] * a;
+            }
+        }
+    }
+    for (int t = 0; t < nslot; t++) out[t] = vb.c0 * zb[t] + vb.c1 * zt[t];
+}
+
+template <typename F>
+__device__ __forceinline__ void interp_u(const DGrid& g, double X, double Y, const VBracket& vb,
+                                        const SlotSel& S, int nslot, double out[3])
+{
+    const int Jnode = (int)floor(Y);
+    const double y = Y - Jnode;
+    const double uvI = X - 0.5;
+    const int Inode = (int)floor(uvI);
+    const double x = uvI - Inode;
+    double cff[4];
+    bilin(x, y, cff);
+    const int xiu = g.L - 1;
+    int ia, ib;
+    double m[4];
+    auto MU = [&](int i, int j) { return (double)g.mu[(int64_t)(j - 1) * xiu + (i - 1)]; };
+    if (Inode >= xiu) {
+        ia = ib = xiu;
+        m[0] = MU(xiu, Jnode); m[1] = 0; m[2] = 0; m[3] = MU(xiu, Jnode + 1);
+    } else if (Inode < 1) {
+        ia = ib = 1;
+        m[0] = 0; m[1] = MU(1, Jnode); m[2] = MU(1, Jnode + 1); m[3] = 0;
+    } else {
+        ia = Inode; ib = Inode + 1;
+        m[0] = MU(ia, Jnode); m[1] = MU(ib, Jnode); m[2] = MU(ib, Jnode + 1); m[3] = MU(ia, Jnode + 1);
+    }
+    const double nwater = m[0] + m[1] + m[2] + m[3];
+    bool zero[4];
+    if (nwater < 4.0 && nwater > 0.0) {
+        for (int k = 0; k < 4; k++) zero[k] = (m[k] == 0.0);
+    } else if (nwater == 0.0) {
+        for (int k = 0; k < 4; k++) { zero[k] = true; cff[k] = 0.0; }
+    } else {
+        for (int k = 0; k < 4; k++) zero[k] = false;
+    }
+    int64_t col[4];
+    col[0] = (int64_t)(Jnode - 1) * xiu + (ia - 1);
+    col[1] = (int64_t)(Jnode - 1) * xiu + (ib - 1);
+    col[2] = (int64_t)(Jnode) * xiu + (ib - 1);
+    col[3] = (int64_t)(Jnode) * xiu + (ia - 1);
+    blend3<F>((const F*)g.u, col, g.N, cff, zero, vb, S, nslot, out);
+}
+
+template <typename F>
+__device__ __forceinline__ void interp_v(const DGrid& g, double X, double Y, const VBracket& vb,
+                                        const SlotSel& S, int nslot, double out[3])
+{
+    int Inode = (int)floor(X);
+    const double x = X - Inode;
+    const double uvJ = Y - 0.5;
+    int Jnode = (int)floor(uvJ);
+    if (Jnode < 1) Jnode = 1;
+    const int etav = g.M - 1;
+    if (Jnode > etav) Inode = etav;   // interpolation_module.f90:696 assigns Inode
+    const double y = uvJ - Jnode;
+    double cff[4];
+    bilin(x, y, cff);
+    int ja, jb;
+    double m[4];
+    auto MV = [&](int i, int j) { return (double)g.mv[(int64_t)(j - 1) * g.L + (i - 1)]; };
+    if (Jnode >= etav) {
+        ja = jb = etav;
+        m[0] = MV(Inode, etav); m[1] = MV(Inode + 1, etav); m[2] = 0; m[3] = 0;
+    } else {
+        ja = Jnode; jb = Jnode + 1;
+        m[0] = MV(Inode, ja); m[1] = MV(Inode + 1, ja); m[2] = MV(Inode + 1, jb); m[3] = MV(Inode, jb);
+    }
+    const double nwater = m[0] + m[1] + m[2] + m[3];
+    bool zero[4];
+    if (nwater < 4.0 && nwater > 0.0) {
+        for (int k = 0; k < 4; k++) zero[k] = (m[k] == 0.0);
+    } else if (nwater == 0.0) {
+        for (int k = 0; k < 4; k++) { zero[k] = true; cff[k] = 0.0; }
+    } else {
+        for (int k = 0; k < 4; k++) zero[k] = false;
+    }
+    int64_t col[4];
+    col[0] = (int64_t)(ja - 1) * g.L + (Inode - 1);
+    col[1] = (int64_t)(ja - 1) * g.L + (Inode);
+    col[2] = (int64_t)(jb - 1) * g.L + (Inode);
+    col[3] = (int64_t)(jb - 1) * g.L + (Inode - 1);
+    blend3<F>((const F*)g.v, col, g.N, cff, zero, vb, S, nslot, out);
+}
+
+// masked rho-point weights for w and AKs: bilinear, or IDW when a corner is dry
+__device__ __forceinline__ void rho_weights(const DGrid& g, const RhoCell& c, double cff[4])
+{
+    double m[4];
+    for (int k = 0; k < 4; k++) m[k] = (double)g.mrho[c.o[k]];
+    bilin(c.x, c.y, cff);
+    if (m[0] + m[1] + m[2] + m[3] < 4.0) getcoeff(c.x, c.y, m, cff);
+}
+
+template <typename F>
+__device__ __forceinline__ void interp_w(const DGrid& g, const RhoCell& c, const VBracket& vb,
+                                        const SlotSel& S, int nslot, double out[3])
+{
+    double cff[4];
+    rho_weights(g, c, cff);
+    const bool zero[4] = {false, false, false, false};
+    blend3<F>((const F*)g.w, c.o, g.N + 1, cff, zero, vb, S, nslot, out);
+}
+
+// ---------------------------------------------------------------------------
+// find_currents (advection_module.f90:129-350), version=2: all stages use
+// time ix(2).  Z is clamped in place, as the reference's INTENT(INOUT) Zpar.
+// ---------------------------------------------------------------------------
+struct Surface {
+    double tdepth, zb, zc, zf, zeta;
+};
+
+template <typename F>
+__device__ __forceinline__ Surface surface_at(const DGrid& g, const StepArgs& A, const RhoCell& c)
+{
+    Surface s;
+    s.tdepth = -1.0 * bilinear2d(g.h, c);
+    double z3[3];
+    zeta3<F>(g, c, A.slots, z3);
+    s.zb = z3[0]; s.zc = z3[1]; s.zf = z3[2];
+    s.zeta = polintd(A.ex, s.zb, s.zc, s.zf, A.ix[1]);
+    return s;
+}
+
+template <typename F>
+__device__ __forceinline__ void find_currents(const DGrid& g, const StepArgs& A, double X,
+                                              double Y, double& Z, double& U, double& V,
+                                              double& W, Surface* surf_out)
+{
+    const RhoCell c = rho_cell(g, X, Y);
+    const Surface s = surface_at<F>(g, A, c);
+    if (surf_out) *surf_out = s;
+    if (Z < s.tdepth) Z = s.tdepth + (double)0.001f;
+    if (Z > s.zeta) Z = s.zeta - (double)0.001f;
+    const VBracket br = bracket_rho(g, Z, s.zeta, s.tdepth);
+    const VBracket bw = bracket_w(g, Z, s.zeta, s.tdepth);
+    // ets==1 uses (b,b,c) for u,v,w (advection_module.f90:277-334)
+    const int nslot = A.ets == 1 ? 2 : 3;
+    double q[3];
+    interp_u<F>(g, X, Y, br, A.slots, nslot, q);
+    U = A.ets == 1 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+    interp_v<F>(g, X, Y, br, A.slots, nslot, q);
+    V = A.ets == 1 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+    interp_w<F>(g, c, bw, A.slots, nslot, q);
+    W = A.ets == 1 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+}
+
+// ---------------------------------------------------------------------------
+// deviates: injected (test hook) or Philox, in the fixed per-particle layout
+// [HTurb x, HTurb y, VTurb sub-step 1..loop]
+// ---------------------------------------------------------------------------
+struct DevSrc {
+    const double* inj;
+    uint64_t seed, id, step;
+    int blk;
+    double buf[4];
+    __device__ __forceinline__ double get(int j)
+    {
+        if (inj) return inj[j];
+        const int b = j >> 2;
+        if (b != blk) {
+            normals4(seed, id, step, (uint32_t)b, buf);
+            blk = b;
+        }
+        const int r = j & 3;
+        return r == 0 ? buf[0] : (r == 1 ? buf[1] : (r == 2 ? buf[2] : buf[3]));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// ver_turb_module.f90:31-329 with pppack.f90 smooth/setupq/chol1d/ppvalu.
+// Per-lane work arrays live in a global scratch area, one column per lane
+// (row r of lane t at r*T + t), so every access of a wavefront is one
+// contiguous 512-byte segment.
+// ---------------------------------------------------------------------------
+struct Scr {
+    double* p;
+    int64_t T;
+    __device__ __forceinline__ double& operator()(int row) const { return p[(int64_t)row * T]; }
+};
+
+struct VtLayout {
+    int np, nw;
+    int fitz, y, h, r, qty, d, l1, l2, u, z, pkh;
+    __device__ __forceinline__ VtLayout(int np_, int nw_) : np(np_), nw(nw_)
+    {
+        fitz = 0; y = np; h = 2 * np; r = 3 * np; qty = 4 * np; d = 5 * np; l1 = 6 * np;
+        l2 = 7 * np; u = 8 * np; z = 9 * np; pkh = 9 * np + nw;
+    }
+};
+
+// chol1d (pppack.f90:1428-1566) for dy(:)=serr: the setupq products are
+// rebuilt row by row from r(i)=serr/h(i) (bit-identical to setupq's
+// v(:,1..3,5..7)), elimination and forward substitution share one sweep.
+// Returns sum_i qu(i)^2 dy(i)^2 (the dot_product of smooth) and, when asked,
+// utru of the p=0 solve.
+__device__ double chol1d_dev(double p, const VtLayout& Lo, const Scr& S, double dy2,
+                             bool want_utru, double* utru)
+{
+    const int np = Lo.np;
+    const double six1mp = 6.0 * (1.0 - p);
+    const double twop = 2.0 * p;
+    auto R = [&](int i) { return S(Lo.r + i - 1); };
+    auto H = [&](int i) { return S(Lo.h + i - 1); };
+    auto row = [&](int i, double& d0, double& e0, double& f0) {
+        // setupq (pppack.f90:4793-4813) then chol1d's matrix (pppack.f90:1509-1512)
+        const double ri = R(i), rim1 = R(i - 1);
+        const double v1 = rim1, v3 = ri;
+        const double v2 = (-ri) - rim1;
+        const double v5 = v1 * v1 + v2 * v2 + v3 * v3;
+        double v6 = 0.0, v7 = 0.0;
+        if (i <= np - 2) {
+            const double rip1 = R(i + 1);
+            const double v1n = ri, v2n = (-rip1) - ri;
+            v6 = v2 * v1n + v3 * v2n;
+            if (i <= np - 3) v7 = v3 * rip1;
+        }
+        d0 = six1mp * v5 + twop * (H(i - 1) + H(i));
+        e0 = six1mp * v6 + p * H(i);
+        f0 = six1mp * v7;
+    };
+    if (np < 4) {
+        double d0, e0, f0;
+        row(2, d0, e0, f0);
+        S(Lo.u + 0) = 0.0;
+        S(Lo.u + 1) = S(Lo.qty + 1) / d0;
+        S(Lo.u + 2) = 0.0;
+    } else {
+        double di, ei, fi, dn, en, fn, dnn, enn, fnn;
+        row(2, di, ei, fi);
+        row(3, dn, en, fn);
+        double u_im1 = 0.0, u_i = S(Lo.qty + 1), l2_im1 = 0.0;
+        S(Lo.u + 0) = 0.0;
+        S(Lo.u + 1) = u_i;
+        for (int i = 2; i <= np - 2; i++) {
+            double rat = ei / di;
+            dn = dn - rat * ei;
+            en = en - rat * fi;
+            const double l1 = rat;
+            rat = fi / di;
+            if (i + 2 <= np - 1) {
+                row(i + 2, dnn, enn, fnn);
+            } else {
+                dnn = 0.0; enn = 0.0; fnn = 0.0;
+            }
+            dnn = dnn - rat * fi;
+            const double l2 = rat;
+            S(Lo.d + i - 1) = di;
+            S(Lo.l1 + i - 1) = l1;
+            S(Lo.l2 + i - 1) = l2;
+            const double u_ip1 = S(Lo.qty + i) - l1 * u_i - l2_im1 * u_im1;
+            S(Lo.u + i) = u_ip1;
+            di = dn; ei = en; fi = fn;
+            dn = dnn; en = enn; fn = fnn;
+            u_im1 = u_i; u_i = u_ip1; l2_im1 = l2;
+        }
+        // back substitution (pppack.f90:1545-1550); di is v(np-1,1)
+        S(Lo.u + np - 1) = 0.0;
+        double u_p2 = 0.0;
+        double u_p1 = u_i / di;
+        S(Lo.u + np - 2) = u_p1;
+        for (int i = np - 2; i >= 2; i--) {
+            const double ui = S(Lo.u + i - 1) / S(Lo.d + i - 1) - u_p1 * S(Lo.l1 + i - 1) -
+                              u_p2 * S(Lo.l2 + i - 1);
+            S(Lo.u + i - 1) = ui;
+            u_p2 = u_p1;
+            u_p1 = ui;
+        }
+    }
+    // qu = Q*u (pppack.f90:1556-1563) fused with smooth's dot_product and utru
+    double sum = 0.0, ut = 0.0, prev = 0.0;
+    double um1 = S(Lo.u + 0);
+    for (int i = 2; i <= np; i++) {
+        const double ui = S(Lo.u + i - 1);
+        const double hi = H(i - 1);
+        const double raw = (ui - um1) / hi;
+        const double qu = raw - prev;
+        sum = sum + (qu * qu) * dy2;
+        if (want_utru) ut = ut + hi * (um1 * (um1 + ui) + ui * ui);
+        prev = raw;
+        um1 = ui;
+    }
+    const double qn = -prev;
+    sum = sum + (qn * qn) * dy2;
+    if (want_utru) *utru = ut;
+    return sum;
+}
+
+// interv (pppack.f90:3363-3576) on the nearly uniform breakpoints fitz: a
+// guess from the mean spacing, corrected against the stored breakpoints, so
+// the result is the reference's: the largest i<=np-1 with fitz(i)<=x, else 1.
+__device__ __forceinline__ int interv_dev(const VtLayout& Lo, const Scr& S, double x, double f1,
+                                          double dz)
+{
+    const int np = Lo.np;
+    const double gss = (x - f1) / dz;
+    int i;
+    if (!(gss >= 0.0)) i = 1;
+    else if (gss >= (double)(np - 1)) i = np - 1;
+    else i = (int)gss + 1;
+    while (i > 1 && S(Lo.fitz + i - 1) > x) i--;
+    while (i < np - 1 && S(Lo.fitz + i) <= x) i++;
+    return i;
+}
+
+struct PieceCache {
+    int i;
+    double c[4];
+    double brk;
+};
+
+// Coefficients of piece i of the smoothing spline (smooth's final loop,
+// pppack.f90:5305-5314), formed on demand from u, h and y.
+__device__ void spline_piece(const VtLayout& Lo, const Scr& S, int i, double p, double k2,
+                             PieceCache& pc)
+{
+    const int np = Lo.np;
+    auto U = [&](int j) { return S(Lo.u + j - 1); };
+    auto H = [&](int j) { return S(Lo.h + j - 1); };
+    auto raw = [&](int j) { return (U(j) - U(j - 1)) / H(j - 1); };   // j >= 2
+    const double r_i = (i >= 2) ? raw(i) : 0.0;
+    const double r_i1 = raw(i + 1);
+    const double qu_i = r_i1 - r_i;
+    double qu_i1;
+    if (i + 1 == np) qu_i1 = -r_i1;
+    else qu_i1 = raw(i + 2) - r_i1;
+    const double a1 = S(Lo.y + i - 1) - k2 * qu_i;
+    const double a1n = S(Lo.y + i) - k2 * qu_i1;
+    const double a3 = U(i) * 6.0 * p;
+    const double a3n = U(i + 1) * 6.0 * p;
+    const double hi = H(i);
+    const double a4 = (a3n - a3) / hi;
+    const double a2 = (a1n - a1) / hi - (a3 + a4 / 3.0 * hi) / 2.0 * hi;
+    pc.i = i;
+    pc.c[0] = a1; pc.c[1] = a2; pc.c[2] = a3; pc.c[3] = a4;
+    pc.brk = S(Lo.fitz + i - 1);
+}
+
+// ppvalu (pppack.f90:4246-4362), k=4
+__device__ __forceinline__ double ppvalu_dev(const VtLayout& Lo, const Scr& S, PieceCache& pc,
+                                             double x, int jderiv, double p, double k2,
+                                             double f1, double dz)
+{
+    const int i = interv_dev(Lo, S, x, f1, dz);
+    if (i != pc.i) spline_piece(Lo, S, i, p, k2, pc);
+    const double h = x - pc.brk;
+    double value = 0.0;
+    double fmmjdr = 4 - jderiv;
+    int m = 4;
+    for (;;) {
+        value = (value / fmmjdr) * h + pc.c[m - 1];
+        m = m - 1;
+        fmmjdr = fmmjdr - 1.0;
+        if (fmmjdr <= 0.0) break;
+    }
+    return value;
+}
+
+template <typename F>
+__device__ double vturb(const DGrid& g, const StepArgs& A, const RhoCell& c, const Surface& sf,
+                        double Zpar, DevSrc& dv, const Scr& S, unsigned* capped)
+{
+    const int nw = g.N + 1;
+    const int np = A.np;
+    const VtLayout Lo(np, nw);
+    const double tdepth = sf.tdepth, zeta = sf.zeta;
+    const double h = -1.0 * tdepth;
+    // AKs column at the particle (getInterpAKs, interpolation_module.f90:1175-1229)
+    double cff[4];
+    rho_weights(g, c, cff);
+    const F* aks = (const F*)g.aks;
+    for (int k = 1; k <= nw; k++) {
+        const double Sk = level_S(g, g.s_w, g.cs_w, k, h);
+        const double zb = level_z(g, g.s_w, g.cs_w, k, sf.zb, h, Sk);
+        const double zc = level_z(g, g.s_w, g.cs_w, k, sf.zc, h, Sk);
+        const double zf = level_z(g, g.s_w, g.cs_w, k, sf.zf, h, Sk);
+        S(Lo.z + k - 1) = polintd(A.ex, zb, zc, zf, A.ix[1]);
+        double v[4][4];
+        for (int q = 0; q < 4; q++) ld4(aks + (c.o[q] * nw + (k - 1)) * 4, v[q]);
+        double kh[3];
+        for (int t = 0; t < 3; t++) {
+            const int s = A.slots.s[t];
+            kh[t] = cff[0] * pick(v[0], s) + cff[1] * pick(v[1], s) + cff[2] * pick(v[2], s) +
+                    cff[3] * pick(v[3], s);
+        }
+        S(Lo.pkh + k - 1) = polintd(A.ex, kh[0], kh[1], kh[2], A.ix[1]);
+    }
+    // resample to np uniform points with linint (ver_turb_module.f90:126-136;
+    // interpolation_module.f90:27-61).  fitz increases, so the bracket is
+    // found by a forward merge; it equals linint's bisection result.
+    const double z1 = S(Lo.z + 0), zn = S(Lo.z + nw - 1);
+    const double dz = (zn - z1) / (double)(np - 1);
+    S(Lo.fitz + 0) = z1;
+    S(Lo.fitz + np - 1) = zn;
+    S(Lo.y + 0) = S(Lo.pkh + 0);
+    S(Lo.y + np - 1) = S(Lo.pkh + nw - 1);
+    {
+        double fz = z1;
+        int jlo = 1;
+        double zlo = z1, zhi = S(Lo.z + 1);
+        for (int i = 2; i <= np - 1; i++) {
+            fz = fz + dz;
+            S(Lo.fitz + i - 1) = fz;
+            while (jlo < nw - 1 && zhi <= fz) {
+                jlo++;
+                zlo = zhi;
+                zhi = S(Lo.z + jlo);
+            }
+            const double ylo = S(Lo.pkh + jlo - 1), yhi = S(Lo.pkh + jlo);
+            const double m = (ylo - yhi) / (zlo - zhi);
+            const double b = ylo - m * zlo;
+            S(Lo.y + i - 1) = m * fz + b;
+        }
+    }
+    // setupq (pppack.f90:4726-4825) with dy(:)=serr
+    const double serr = A.serr;
+    {
+        double prev = 0.0;
+        for (int i = 1; i <= np - 1; i++) {
+            const double hi = S(Lo.fitz + i) - S(Lo.fitz + i - 1);
+            S(Lo.h + i - 1) = hi;
+            S(Lo.r + i - 1) = serr / hi;
+            const double diff = (S(Lo.y + i) - S(Lo.y + i - 1)) / hi;
+            if (i >= 2) S(Lo.qty + i - 1) = diff - prev;
+            prev = diff;
+        }
+    }
+    // smooth (pppack.f90:5236-5298)
+    const double dy2 = serr * serr;
+    const double s = A.smth;
+    double p, q, sfq;
+    if (s <= 0.0) {
+        p = 1.0;
+        chol1d_dev(p, Lo, S, dy2, false, nullptr);
+        sfq = 0.0;
+    } else {
+        p = 0.0;
+        double utru = 0.0;
+        sfq = 36.0 * chol1d_dev(p, Lo, S, dy2, true, &utru);
+        if (s < sfq) {
+            const double ooss = 1.0 / sqrt(s);
+            double oosf = 1.0 / sqrt(sfq);
+            q = -(oosf - ooss) * sfq / (6.0 * utru * oosf);
+            double prevq = 0.0, prevsf = oosf;
+            int it = 0;
+            for (;;) {
+                sfq = 36.0 * chol1d_dev(q / (1.0 + q), Lo, S, dy2, false, nullptr) /
+                      ((1.0 + q) * (1.0 + q));
+                if (fabs(sfq - s) <= 0.01 * s) break;
+                if (++it >= kSecantCap) {   // the reference has no cap
+                    atomicAdd(capped, 1u);
+                    break;
+                }
+                oosf = 1.0 / sqrt(sfq);
+                const double change = (q - prevq) / (oosf - prevsf) * (oosf - ooss);
+                prevq = q;
+                q = q - change;
+                prevsf = oosf;
+            }
+            p = q / (1.0 + q);
+        }
+    }
+    const double k2 = 6.0 * (1.0 - p) * dy2;
+    // RDM sub-steps (ver_turb_module.f90:231-293)
+    PieceCache pc;
+    pc.i = -1;
+    const double deltat = A.deltat;
+    double ParZc = Zpar;
+    for (int i = 1; i <= A.loop; i++) {
+        double Kprimec;
+        if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
+        else Kprimec = ppvalu_dev(Lo, S, pc, ParZc, 1, p, k2, z1, dz);
+        const double KprimeZc = Kprimec * deltat;
+        const double Z3rdc = ParZc + 0.5 * KprimeZc;
+        double KH3rdc;
+        if (Z3rdc < tdepth || Z3rdc > zeta) {
+            KH3rdc = A.aksback;
+        } else {
+            KH3rdc = ppvalu_dev(Lo, S, pc, Z3rdc, 0, p, k2, z1, dz);
+            if (KH3rdc < A.aksback) KH3rdc = A.aksback;
+        }
+        const double DEV = dv.get(1 + i);
+        const double r = 1.0;
+        const double dTz = KprimeZc + DEV * sqrt(2.0 / r * KH3rdc * deltat);
+        ParZc = ParZc + dTz;
+        if (ParZc <= tdepth) ParZc = tdepth + fabs(ParZc - tdepth);
+        if (ParZc >= zeta) ParZc = zeta - fabs(ParZc - zeta);
+    }
+    return ParZc - Zpar;
+}
+
+// boundary_module.f90:43-143 with the 3-point inside() of
+// interpolation_module.f90:269-420 specialised to the unit cell.
+__device__ bool inside3(const double xb[3], const double yb[3], double xo, double yo)
+{
+    double X[4] = {xb[0], xb[1], xb[2], xb[0]};
+    double Y[4] = {yb[0], yb[1], yb[2], yb[0]};
+    int crossings = 0;
+    bool sel[3];
+    for (int k = 0; k < 3; k++)
+        sel[k] = ((X[k + 1] - xo) * (xo - X[k]) >= 0.0) && (X[k] != X[k + 1]);
+    for (int k = 0; k < 3; k++) {
+        if (!sel[k]) continue;
+        const double dx1 = xo - X[k];
+        const double dx2 = X[k + 1] - xo;
+        const double dxy = dx2 * (yo - Y[k]) - dx1 * (Y[k + 1] - yo);
+        int inc = 0;
+        if ((X[k] == xo) && (Y[k] == yo)) {
+            return true;
+        } else if (((dx1 == 0.0) && (yo >= Y[k])) || ((dx2 == 0.0) && (yo >= Y[k + 1]))) {
+            inc = 1;
+        } else if ((dx1 * dx2 > 0.0) && ((X[k + 1] - X[k]) * dxy >= 0.0)) {
+            inc = 2;
+        }
+        if (X[k + 1] > X[k]) crossings += inc;
+        else crossings -= inc;
+    }
+    return crossings != 0;
+}
+
+__device__ void bounds(const DGrid& g, double Xp, double Yp, bool& ingrid, bool& obound)
+{
+    const double ecut = (double)0.1f, fcut = 1.0 - ecut;
+    ingrid = false;
+    obound = false;
+    const int I = (int)floor(Xp), J = (int)floor(Yp);
+    const double X = Xp - (double)I, Y = Yp - (double)J;
+    if ((I > 1) && (J > 1) && (I < g.L - 1) && (J < g.M - 1)) {
+        const int64_t b = (int64_t)(J - 1) * g.L + (I - 1);
+        const double m11 = g.mrho[b], m21 = g.mrho[b + 1];
+        const double m22 = g.mrho[b + g.L + 1], m12 = g.mrho[b + g.L];
+        const int nm = (int)(m11 + m21 + m22 + m12);
+        if (nm == 4) ingrid = true;
+        if (nm == 3) {
+            double tX[3] = {0, 0, 0}, tY[3] = {0, 0, 0};
+            const double mm[4] = {m11, m12, m21, m22};   // (it,jt) = (1,1) (1,2) (2,1) (2,2)
+            int n = 0;
+            for (int q = 0; q < 4; q++) {
+                if (mm[q] == 1.0 && n < 3) {
+                    tX[n] = (double)(q >> 1);
+                    tY[n] = (double)(q & 1);
+                    n++;
+                }
+            }
+            if (inside3(tX, tY, X, Y)) ingrid = true;
+        }
+        if (nm == 2) {
+            if (m11 == 1.0 && (X < ecut) && (Y < ecut)) ingrid = true;
+            if (m21 == 1.0 && (X > fcut) && (Y < ecut)) ingrid = true;
+            if (m22 == 1.0 && (X > fcut) && (Y > fcut)) ingrid = true;
+            if (m12 == 1.0 && (X < ecut) && (Y > fcut)) ingrid = true;
+        }
+        if ((I <= 1) || (J <= 1) || (I >= g.L - 1) || (J >= g.M - 1)) obound = true;
+    } else {
+        obound = true;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the step: update_particles (ROMSPath.f90:662-1003), one lane per particle,
+// grid-stride over the particles; lane t uses scratch column t.
+// ---------------------------------------------------------------------------
+template <typename F, bool NESTED>
+__global__ void __launch_bounds__(256) step_kernel(const DGrid* __restrict__ grids, StepArgs A)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * blockDim.x;
+    unsigned long long nact = 0;
+    for (int64_t q = tid; q < A.n; q += T) {
+        if (A.ix[2] <= A.dob[q]) continue;
+        int8_t st = A.status[q];
+        if (st == ST_UNSET) continue;
+        if (A.open_boundary && !A.isin[q]) continue;
+        nact++;
+        A.age[q] = A.age[q] + (double)(float)A.idt;
+        const double X0 = A.x[q], Y0 = A.y[q], Z0 = A.z[q];
+        int gid = NESTED ? (int)A.gid[q] : 1;
+        const DGrid& g = grids[NESTED ? gid - 1 : 0];
+        const RhoCell c0 = rho_cell(g, X0, Y0);
+        const double pm = bilinear2d(g.pm, c0);
+        const double pn = bilinear2d(g.pn, c0);
+        double Zpar = Z0;
+        double AdvectX = 0.0, AdvectY = 0.0, AdvectZ = 0.0;
+        double TurbHx = 0.0, TurbHy = 0.0, TurbV = 0.0;
+        Surface s0;
+        bool have_s0 = false;
+        if (A.scheme == 1) {
+            // RKAdvect (advection_module.f90:20-125): every stage starts from
+            // the step's start point with half a step (idt/2).
+            const double idt = (double)A.idt;
+            double Uad, Vad, Wad;
+            find_currents<F>(g, A, X0, Y0, Zpar, Uad, Vad, Wad, &s0);
+            have_s0 = true;
+            const double k1u = Uad * pm, k1v = Vad * pn, k1w = Wad;
+            double x1 = X0 + Uad * pm * idt / 2.0;
+            double y1 = Y0 + Vad * pn * idt / 2.0;
+            double z1 = Zpar + Wad * idt / 2.0;
+            find_currents<F>(g, A, x1, y1, z1, Uad, Vad, Wad, nullptr);
+            const double k2u = Uad * pm, k2v = Vad * pn, k2w = Wad;
+            double x2 = X0 + Uad * pm * idt / 2.0;
+            double y2 = Y0 + Vad * pn * idt / 2.0;
+            double z2 = Zpar + Wad * idt / 2.0;
+            find_currents<F>(g, A, x2, y2, z2, Uad, Vad, Wad, nullptr);
+            const double k3u = Uad * pm, k3v = Vad * pn, k3w = Wad;
+            double x3 = X0 + Uad * pm * idt / 2.0;
+            double y3 = Y0 + Vad * pn * idt / 2.0;
+            double z3 = Zpar + Wad * idt / 2.0;
+            find_currents<F>(g, A, x3, y3, z3, Uad, Vad, Wad, nullptr);
+            const double k4u = Uad * pm, k4v = Vad * pn, k4w = Wad;
+            const double PU = (k1u + 2.0 * k2u + 2.0 * k3u + k4u) / 6.0;
+            const double PV = (k1v + 2.0 * k2v + 2.0 * k3v + k4v) / 6.0;
+            const double PW = (k1w + 2.0 * k2w + 2.0 * k3w + k4w) / 6.0;
+            AdvectX = idt * PU;
+            AdvectY = idt * PV;
+            AdvectZ = idt * PW;
+        }
+        DevSrc dv;
+        dv.inj = A.dev ? A.dev + A.id[q] * (int64_t)A.ndev : nullptr;
+        dv.seed = A.seed;
+        dv.id = A.id[q];
+        dv.step = A.step_index;
+        dv.blk = -1;
+        if (A.hturb_on) {
+            // hor_turb_module.f90:30-51
+            const double KM = g.kh, r = 1.0;
+            TurbHx = dv.get(0) * sqrt(2.0 / r * KM * (double)A.idt);
+            TurbHy = dv.get(1) * sqrt(2.0 / r * KM * (double)A.idt);
+        }
+        TurbHx = TurbHx * pm;
+        TurbHy = TurbHy * pn;
+        if (A.vturb_on) {
+            if (!have_s0) s0 = surface_at<F>(g, A, c0);
+            const Scr S{A.scratch + tid, T};
+            TurbV = vturb<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
+        }
+        const double Zbehav = 0.0;
+        const double newX = X0 + AdvectX + TurbHx;
+        const double newY = Y0 + AdvectY + TurbHy;
+        double tempX = newX, tempY = newY;
+        bool ingrid = false, obound = false;
+        const int ngrid = NESTED ? A.ngrid : 1;
+        for (int i = 1; i <= ngrid; i++) {
+            const DGrid& gc = grids[NESTED ? gid - 1 : 0];
+            tempX = gc.scl[i - 1][0] * newX + gc.off[i - 1][0];
+            tempY = gc.scl[i - 1][1] * newY + gc.off[i - 1][1];
+            bounds(grids[i - 1], tempX, tempY, ingrid, obound);
+            if (ingrid) {
+                gid = i;
+                break;
+            }
+        }
+        const DGrid& gn = grids[NESTED ? gid - 1 : 0];
+        const RhoCell cn = rho_cell_clamped(gn, tempX, tempY);
+        const Surface sn = surface_at<F>(gn, A, cn);
+        const double tdepth = sn.tdepth, Pzeta = sn.zeta;
+        if (A.want_diag) {
+            A.wd[q] = (-1.0 * tdepth) + Pzeta;
+            A.zeta[q] = Pzeta;
+            A.bath[q] = -1.0 * tdepth;
+        }
+        bool hbot = false, htop = false;
+        double newZ;
+        if (A.nsb == 1) {
+            newZ = Pzeta - A.vertdist;
+        } else if (A.nsb == 2) {
+            newZ = tdepth + A.vertdist;
+        } else {
+            newZ = Z0 + AdvectZ + TurbV + Zbehav;
+            if (newZ < tdepth) {
+                newZ = tdepth + fabs(newZ - tdepth);
+                hbot = true;
+            }
+            if (newZ > Pzeta) {
+                newZ = Pzeta - fabs(newZ - Pzeta);
+                htop = true;
+            }
+        }
+        if (ingrid) {
+            A.isin[q] = 1;
+            A.x[q] = tempX;
+            A.y[q] = tempY;
+            A.z[q] = newZ;
+            st = hbot ? ST_BOT : (htop ? ST_TOP : ST_MOVED);
+        } else if (obound) {
+            A.isin[q] = 0;
+            gid = A.ngrid;
+            st = ST_EXIT;
+        } else {
+            A.isin[q] = 1;
+            st = ST_LAND;
+        }
+        A.status[q] = st;
+        A.gid[q] = (int8_t)gid;
+    }
+    // active-particle count: wave reduction, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) nact += __shfl_down(nact, off, 64);
+    if ((threadIdx.x & 63) == 0 && nact) atomicAdd(A.active, nact);
+}
+
+template __global__ void step_kernel<float, false>(const DGrid*, StepArgs);
+template __global__ void step_kernel<double, false>(const DGrid*, StepArgs);
+template __global__ void step_kernel<float, true>(const DGrid*, StepArgs);
+template __global__ void step_kernel<double, true>(const DGrid*, StepArgs);
+
+// ---------------------------------------------------------------------------
+// upload: Fortran (nx, ny, nk) FP64 column-major -> slot-interleaved layout
+// dst[(((j*nx + i)*nk + k)*4 + slot], optional FP64 addend (Stokes drift)
+// ---------------------------------------------------------------------------
+template <typename F>
+__global__ void __launch_bounds__(256) pack_field(const double* __restrict__ src,
+                                                  const double* __restrict__ add, F* dst,
+                                                  int nx, int ny, int nk, int slot)
+{
+    const int64_t total = (int64_t)nx * ny * nk;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        // e enumerates the destination order (k fastest) so stores coalesce
+        const int k = (int)(e % nk);
+        const int64_t col = e / nk;            // j*nx + i
+        const int64_t s = col + (int64_t)k * nx * ny;
+        double v = src[s];
+        if (add) v = v + add[s];
+        dst[e * 4 + slot] = (F)v;
+    }
+}
+
+template __global__ void pack_field<float>(const double*, const double*, float*, int, int, int, int);
+template __global__ void pack_field<double>(const double*, const double*, double*, int, int, int, int);
+
+__global__ void sample_deviates(uint64_t seed, int64_t step, int64_t n, int ndev, double* out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    DevSrc dv;
+    dv.inj = nullptr;
+    dv.seed = seed;
+    dv.id = (uint64_t)q;
+    dv.step = (uint64_t)step;
+    dv.blk = -1;
+    for (int j = 0; j < ndev; j++) out[q * ndev + j] = dv.get(j);
+}
+
+}  // namespace rp

@@ -1,0 +1,68 @@
+"""Drive librp through a run: the host loop of ROMSPath.f90:151-546 around
+rp_step, with hydro records produced by the synthetic models."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+from .nesting import scale_offsets
+from .schedule import record_time, schedule
+
+
+class RecordSource:
+    """Hydro records by (grid, record number), cached."""
+
+    def __init__(self, cfg, cache=True):
+        self.cfg = cfg
+        self.cache = {} if cache else None
+
+    def get(self, ng, record):
+        key = (ng, record)
+        if self.cache is not None and key in self.cache:
+            return self.cache[key]
+        rec = self.cfg.hydro[ng - 1].record(record_time(record, self.cfg.params.dt))
+        if self.cache is not None:
+            self.cache[key] = rec
+        return rec
+
+
+def ndev_of(prm):
+    return 2 + (prm.loop if prm.VTurbOn else 0)
+
+
+def injected_deviates(seed, step_index, n, ndev):
+    """Deterministic N(0,1) deviates for one step, [n][ndev] (test hook input)."""
+    rng = np.random.default_rng([seed, step_index])
+    return rng.standard_normal((n, ndev))
+
+
+class GpuRun:
+    def __init__(self, cfg, precision="fp32", block=0, records=None):
+        self.cfg = cfg
+        ng = cfg.ngrid
+        self.ctx = abi.Context(abi.make_config(cfg.params, ngrid=ng, precision=precision,
+                                               block=block))
+        scl, off = scale_offsets(cfg.grids, cfg.refine) if ng > 1 else (None, None)
+        for i, g in enumerate(cfg.grids):
+            self.ctx.set_grid(i + 1, g, None if scl is None else scl[i], None if off is None else off[i])
+        self.ctx.set_particles(cfg.particles)
+        self.records = records or RecordSource(cfg)
+
+    def run(self, max_steps=None, inject=False, on_step=None, diag_every_step=False):
+        prm = self.cfg.params
+        n = self.cfg.particles.n
+        ndev = ndev_of(prm)
+        for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps):
+            for ld in st.loads:
+                for g in range(1, self.cfg.ngrid + 1):
+                    self.ctx.set_hydro_slot(g, ld.slot, self.records.get(g, ld.record))
+            if inject:
+                self.ctx.set_deviates(injected_deviates(prm.seed, st.step_index, n, ndev))
+            self.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index,
+                          want_diag=diag_every_step or st.output)
+            if on_step is not None:
+                on_step(st, self)
+        return self.ctx.get_particles()
+
+    def close(self):
+        self.ctx.close()

@@ -1,0 +1,69 @@
+"""Host-side time schedule around the step (ROMSPath.f90:151-546).
+
+The reference keeps three hydro slots t_b, t_c, t_f.  Records 1, 2, 3 fill
+slots 1, 2, 3 at start-up (ROMSPath.f90:210-213, 307-309).  From the third
+external step on, the slots rotate and record `tstep` is read into the new
+forward slot (ROMSPath.f90:488-493); `tstep` advances after every external
+step (ROMSPath.f90:510).  Record r holds model time (r-1)*dt.  The step itself
+only sees the slot numbers and the times ex(3), ix(3).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+
+@dataclass
+class Load:
+    slot: int       # 1-based slot to fill
+    record: int     # 1-based record number
+
+
+@dataclass
+class InternalStep:
+    ets: int
+    its: int
+    ex: tuple
+    ix: tuple
+    tb: int
+    tc: int
+    tf: int
+    step_index: int           # 0-based count of internal steps
+    loads: list               # records to read before this step
+    output: bool              # dataOutput after this step (printdt >= iprint)
+
+
+def schedule(dt: int, idt: int, days: float, iprint: int | None = None, max_steps=None):
+    """Yield the internal steps of a run in order (run_ROMSPath and below)."""
+    t_b, t_c, t_f = 1, 2, 3
+    tstep = 1
+    pending = [Load(1, 1), Load(2, 2), Load(3, 3)]
+    seconds = int(days * 86400.0)
+    stepT = seconds // dt
+    stepIT = int(dt / idt)
+    printdt = 0
+    iprint = iprint if iprint is not None else dt
+    k = 0
+    for ets in range(1, stepT + 1):
+        if ets > 2:
+            t_b = t_b % 3 + 1
+            t_c = t_c % 3 + 1
+            t_f = t_f % 3 + 1
+            pending.append(Load(t_f, tstep))
+        ex = (float((ets - 2) * dt), float((ets - 1) * dt), float(ets * dt))
+        for its in range(1, stepIT + 1):
+            ix = (ex[1] + float((its - 2) * idt), ex[1] + float((its - 1) * idt),
+                  ex[1] + float(its * idt))
+            printdt += idt
+            out = printdt >= iprint
+            if out:
+                printdt = 0
+            yield InternalStep(ets, its, ex, ix, t_b, t_c, t_f, k, pending, out)
+            pending = []
+            k += 1
+            if max_steps is not None and k >= max_steps:
+                return
+        tstep += 1
+
+
+def record_time(record: int, dt: int) -> float:
+    return float((record - 1) * dt)

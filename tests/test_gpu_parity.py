@@ -1,0 +1,112 @@
+"""Parity of the HIP step (librp.so) with the oracle on the same seeded inputs.
+
+FP64 mode: status, grid id and isIn must match exactly and positions to
+1e-9 of a cell (x, y in cell units; z in units of the local mean layer
+thickness h/N).  FP32-field mode: position error against the FP64 oracle
+within 1e-4 of a cell.  Random walks use identical injected deviates.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from romspath_amd import synth
+from romspath_amd.runner import GpuRun
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(cfg, steps, precision, inject):
+    print(f"case {cfg.name} n={cfg.particles.n} steps={steps} {precision}", flush=True)
+    run = GpuRun(cfg, precision=precision)
+    g = run.run(max_steps=steps, inject=inject, diag_every_step=True)
+    act = run.ctx.active_steps()
+    capped = run.ctx.secant_capped()
+    run.close()
+    o = O.run_oracle(cfg, max_steps=steps, inject=inject)
+    return g, o, act, capped
+
+
+def _layer(cfg, o):
+    return np.maximum(o.bath, 1.0) / cfg.grids[0].N
+
+
+def _check_exact_discrete(g, o):
+    assert np.array_equal(g["status"], o.status)
+    assert np.array_equal(g["gid"], o.gid)
+    assert np.array_equal(g["isin"], o.isin)
+
+
+def _err(g, o, cfg):
+    ex = np.abs(g["x"] - o.x)
+    ey = np.abs(g["y"] - o.y)
+    ez = np.abs(g["z"] - o.z) / _layer(cfg, o)
+    return ex, ey, ez
+
+
+def test_c1_fp64_matches_oracle():
+    cfg = synth.config1(n_particles=3000)
+    g, o, act, _ = _run_both(cfg, 240, "fp64", inject=False)
+    _check_exact_discrete(g, o)
+    ex, ey, ez = _err(g, o, cfg)
+    print("c1 fp64 max err cells", ex.max(), ey.max(), ez.max(), flush=True)
+    assert ex.max() <= 1e-9 and ey.max() <= 1e-9 and ez.max() <= 1e-9
+    assert np.array_equal(g["age"], o.age)
+    assert act == o.active
+
+
+def test_mini_fp64_turbulence_injected_matches_oracle():
+    cfg = synth.config_mini(n_particles=400, days=2 / 24.0)
+    g, o, act, capped = _run_both(cfg, 60, "fp64", inject=True)
+    _check_exact_discrete(g, o)
+    ex, ey, ez = _err(g, o, cfg)
+    print("mini fp64 max err", ex.max(), ey.max(), ez.max(), "capped", capped, flush=True)
+    assert ex.max() <= 1e-9 and ey.max() <= 1e-9 and ez.max() <= 1e-9
+    m = np.isin(o.status, (1.0, 1.1, 1.2, -8.0, -9.0))
+    for k in ("wd", "zeta", "bath"):
+        assert np.max(np.abs(g[k][m] - getattr(o, k)[m])) <= 1e-9
+    assert act == o.active
+    assert capped == 0
+
+
+def test_c1_fp32_fields_error_bound():
+    cfg = synth.config1(n_particles=3000)
+    g, o, _, _ = _run_both(cfg, 240, "fp32", inject=False)
+    ex, ey, ez = _err(g, o, cfg)
+    print("c1 fp32 err max/p95", ex.max(), np.percentile(ex, 95), ez.max(), flush=True)
+    same = g["status"] == o.status
+    assert same.mean() > 0.999
+    assert ex[same].max() <= 1e-4 and ey[same].max() <= 1e-4 and ez[same].max() <= 1e-4
+
+
+def test_mini_fp32_turbulence_error():
+    cfg = synth.config_mini(n_particles=400, days=2 / 24.0)
+    g, o, _, _ = _run_both(cfg, 60, "fp32", inject=True)
+    ex, ey, ez = _err(g, o, cfg)
+    same = g["status"] == o.status
+    print("mini fp32 err p95", np.percentile(ex, 95), np.percentile(ez, 95), "max", ex.max(), ez.max(),
+          "status agree", same.mean(), flush=True)
+    assert same.mean() > 0.99
+    assert np.percentile(ex[same], 95) <= 1e-4 and np.percentile(ez[same], 95) <= 1e-4
+
+
+def test_rng_matches_philox_reference():
+    from rng_ref import normals
+    cfg = synth.config_mini(n_particles=64)
+    run = GpuRun(cfg, precision="fp64")
+    got = run.ctx.sample_deviates(5, 64, 12)
+    run.close()
+    want = np.stack([normals(cfg.params.seed, p, 5, 12) for p in range(64)])
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_error_codes_not_crashes():
+    from romspath_amd.abi import RpError
+    cfg = synth.config1(n_particles=10)
+    run = GpuRun(cfg, precision="fp64")
+    with pytest.raises(RpError) as e:
+        run.ctx.step((0.0, 1.0, 2.0), (0.0, 1.0, 2.0), 1, 1, 2, 3, 0)
+    assert e.value.code == -2            # slots not loaded yet
+    with pytest.raises(RpError) as e:
+        run.ctx.step((0.0, 1.0, 2.0), (0.0, 1.0, 2.0), 1, 0, 2, 3, 0)
+    assert e.value.code == -1
+    run.close()

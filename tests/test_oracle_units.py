@@ -1,0 +1,345 @@
+"""Pins of the oracle's unit routines.
+
+The reference ships no tests or golden vectors, so each routine is pinned by
+what the mathematics fixes (exactness on polynomials, closed-form special
+cases, an independent geometric predicate) or, for PPPACK, by the
+reference's own pppack.f90 compiled as is (oracle/_ref).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+from oracle import oracle as O
+from romspath_amd import synth
+
+L = O.lib()
+
+
+def arr(*v):
+    return np.ascontiguousarray(np.array(v, dtype=np.float64))
+
+
+# ---------------------------------------------------------------- polintd
+def test_polintd_exact_on_quadratics():
+    rng = np.random.default_rng(1)
+    for _ in range(500):
+        xa = np.sort(rng.uniform(-5e5, 5e5, 3))
+        a, b, c = rng.normal(size=3)
+        f = lambda x: a + b * x + c * x * x
+        ya = f(xa)
+        x = rng.uniform(xa[0] - 1e5, xa[2] + 1e5)
+        got = L.og_polintd(O.dptr(np.ascontiguousarray(xa)), O.dptr(np.ascontiguousarray(ya)), x)
+        scale = abs(a) + abs(b * x) + abs(c * x * x) + abs(c) * 1e12
+        assert abs(got - f(x)) <= 1e-12 * scale
+
+
+def test_polintd_equals_lagrange_on_hourly_nodes():
+    # the configuration it is used in: ex=((ets-2)dt, (ets-1)dt, ets dt), x in [ex2, ex3]
+    rng = np.random.default_rng(2)
+    for ets in range(1, 6):
+        xa = np.array([(ets - 2) * 3600.0, (ets - 1) * 3600.0, ets * 3600.0])
+        for _ in range(50):
+            ya = rng.normal(size=3)
+            x = xa[1] + rng.uniform(0, 3600.0)
+            want = np.polyval(np.polyfit(xa, ya, 2), x)
+            got = L.og_polintd(O.dptr(xa), O.dptr(np.ascontiguousarray(ya)), x)
+            assert abs(got - want) < 1e-9
+
+
+# ---------------------------------------------------------------- s-levels
+def _grid(N=12, vt=2, theta_s=5.0, theta_b=0.4, hc=10.0, h=50.0):
+    g = synth.Grid(8, 8, N, 1000.0, 1000.0, 0.0, 0.0, h=np.full((8, 8), h),
+                   mask_rho=np.ones((8, 8), np.int32), vtransform=vt, hc=hc,
+                   theta_s=theta_s, theta_b=theta_b)
+    return g, O.OracleGrid(g)
+
+
+@pytest.mark.parametrize("vt", [1, 2, 3])
+def test_wlevels_end_at_bed_and_surface(vt):
+    g, og = _grid(vt=vt, hc=5.0)
+    for h in (7.0, 50.0, 200.0):
+        for zeta in (-0.4, 0.0, 0.7):
+            zb = L.og_getwlevel(C.byref(og.s), zeta, -h, 1)
+            zt = L.og_getwlevel(C.byref(og.s), zeta, -h, g.N + 1)
+            assert abs(zb + h) < 1e-12 * h
+            assert abs(zt - zeta) < 1e-12 * h
+
+
+def test_uniform_sigma_levels_when_unstretched():
+    # Cs = s: Vtransform 2 reduces to evenly spaced sigma levels between bed and surface
+    g, _ = _grid(N=10)
+    g.cs_r = g.s_rho.copy()
+    g.cs_w = g.s_w.copy()
+    og = O.OracleGrid(g)
+    h, zeta = 37.0, 0.3
+    for k in range(1, g.N + 2):
+        z = L.og_getwlevel(C.byref(og.s), zeta, -h, k)
+        assert abs(z - (zeta + (zeta + h) * (k - 1 - g.N) / g.N)) < 1e-12
+    for k in range(1, g.N + 1):
+        z = L.og_getslevel(C.byref(og.s), zeta, -h, k)
+        assert abs(z - (zeta + (zeta + h) * (k - 0.5 - g.N) / g.N)) < 1e-12
+
+
+def test_vstretching4_published_endpoints():
+    s_rho, cs_r, s_w, cs_w = synth.vstretching4(30, 5.0, 0.4)
+    assert cs_w[0] == -1.0 and cs_w[-1] == 0.0
+    assert np.all(np.diff(cs_w) > 0) and np.all(np.diff(cs_r) > 0)
+    assert np.all(cs_r > -1) and np.all(cs_r < 0)
+
+
+# ---------------------------------------------------------------- 2-D interpolation
+def test_bilinear_exact_on_affine_depth():
+    g, _ = _grid()
+    i = np.arange(1, 9)[None, :]
+    j = np.arange(1, 9)[:, None]
+    g.h = np.ascontiguousarray(3.0 + 0.7 * i + 1.9 * j + 0.0 * i * j)
+    og = O.OracleGrid(g)
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        X, Y = rng.uniform(1.01, 7.99, 2)
+        got = L.og_interp2d(C.byref(og.s), 0, 1, X, Y)
+        assert abs(got - (3.0 + 0.7 * X + 1.9 * Y)) < 1e-12
+
+
+def _with_zeta(g, zeta):
+    og = O.OracleGrid(g)
+    rec = dict(zeta=zeta, u=np.zeros((g.N, g.M, g.L - 1)), v=np.zeros((g.N, g.M - 1, g.L)),
+               w=np.zeros((g.N + 1, g.M, g.L)), aks=np.zeros((g.N + 1, g.M, g.L)))
+    for s in (1, 2, 3):
+        og.set_slot(s, rec)
+    return og
+
+
+def test_zeta_is_inverse_distance_even_when_wet():
+    g, _ = _grid()
+    zeta = np.zeros((8, 8))
+    zeta[2, 2], zeta[2, 3], zeta[3, 3], zeta[3, 2] = 1.0, 2.0, 3.0, 4.0   # corners of cell (3,3)
+    og = _with_zeta(g, zeta)
+    # cell centre: equal distances -> arithmetic mean
+    assert abs(L.og_interp2d(C.byref(og.s), 3, 1, 3.5, 3.5) - 2.5) < 1e-15
+    # off-centre: weights 1/d_k normalised (not bilinear)
+    X, Y = 3.2, 3.7
+    x, y = 0.2, 0.7
+    d = np.array([np.hypot(x, y), np.hypot(1 - x, y), np.hypot(1 - x, 1 - y), np.hypot(x, 1 - y)])
+    w = (1 / d) / (1 / d).sum()
+    want = w @ np.array([1.0, 2.0, 3.0, 4.0])
+    bil = (1 - x) * (1 - y) * 1 + x * (1 - y) * 2 + x * y * 3 + (1 - x) * y * 4
+    got = L.og_interp2d(C.byref(og.s), 3, 1, X, Y)
+    assert abs(got - want) < 1e-14 and abs(got - bil) > 1e-3
+
+
+def test_getcoeff_dry_corners_renormalise():
+    cff = np.zeros(4)
+    for m in ([1, 1, 1, 0], [0, 1, 0, 1], [1, 0, 0, 0], [0, 0, 0, 0]):
+        L.og_getcoeff(0.3, 0.6, O.dptr(arr(*m)), O.dptr(cff))
+        if sum(m):
+            assert abs(cff.sum() - 1.0) < 1e-15
+            assert np.all(cff[np.array(m) == 0] == 0.0)
+        else:
+            assert np.all(cff == 0.0)
+
+
+# ---------------------------------------------------------------- 3-D interpolation
+def _field_grid(N=6, L_=10, M=9):
+    g = synth.Grid(L_, M, N, 1000.0, 1000.0, 0.0, 0.0, h=np.full((M, L_), 30.0),
+                   mask_rho=np.ones((M, L_), np.int32))
+    return g
+
+
+def test_u_affine_exact_and_vertical_bracket():
+    g = _field_grid()
+    # u(i_u, j, k) = 1 + 0.5*(i_u + 0.5) + 0.25*j + 3*k  (affine in the grid index of each point)
+    iu = np.arange(1, g.L)[None, None, :]
+    j = np.arange(1, g.M + 1)[None, :, None]
+    k = np.arange(1, g.N + 1)[:, None, None]
+    u = 1 + 0.5 * (iu + 0.5) + 0.25 * j + 3.0 * k + 0 * iu * j
+    og = O.OracleGrid(g)
+    rec = dict(zeta=np.zeros((g.M, g.L)), u=u, v=np.zeros((g.N, g.M - 1, g.L)),
+               w=np.zeros((g.N + 1, g.M, g.L)), aks=np.zeros((g.N + 1, g.M, g.L)))
+    og.set_slot(1, rec)
+    rng = np.random.default_rng(4)
+    zr = (g.hc * g.s_rho + 30.0 * g.cs_r) / (g.hc + 30.0) * 30.0    # rho depths, zeta=0
+    for _ in range(200):
+        X = rng.uniform(2.0, g.L - 2.0)
+        Y = rng.uniform(2.0, g.M - 2.0)
+        Z = rng.uniform(-29.9, -0.1)
+        got = L.og_interp3d(C.byref(og.s), 0, X, Y, Z, 1, 1, 0.0, -30.0)
+        kk = np.searchsorted(zr, Z, side="right")          # first level above Z (0-based)
+        if kk == 0:
+            kz = 1.0
+        elif kk == g.N:
+            kz = float(g.N)
+        else:
+            kz = kk + (Z - zr[kk - 1]) / (zr[kk] - zr[kk - 1])
+        want = 1 + 0.5 * X + 0.25 * Y + 3.0 * kz
+        assert abs(got - want) < 1e-11, (X, Y, Z, got, want)
+
+
+def test_u_partial_mask_zeroes_without_renormalising():
+    g = _field_grid()
+    m = np.ones((g.M, g.L), np.int32)
+    m[4, 5] = 0          # rho (6,5) dry -> u points (5,5) and (6,5) dry
+    g.mask_rho = m
+    u = np.full((g.N, g.M, g.L - 1), 2.0)
+    og = O.OracleGrid(g)
+    og.set_slot(1, dict(zeta=np.zeros((g.M, g.L)), u=u, v=np.zeros((g.N, g.M - 1, g.L)),
+                        w=np.zeros((g.N + 1, g.M, g.L)), aks=np.zeros((g.N + 1, g.M, g.L))))
+    # u cell with corners (5,4) (6,4) (6,5) (5,5): (6,5) and (5,5) dry
+    X, Y = 5.5 + 0.3 + 0.5, 4.6      # uvI = X-0.5 = 5.3 -> Inode 5, x=0.3; Jnode 4, y=0.6
+    got = L.og_interp3d(C.byref(og.s), 0, X, Y, -15.0, 1, 1, 0.0, -30.0)
+    x, y = 0.3, 0.6
+    want = 2.0 * ((1 - x) * (1 - y) + x * (1 - y))
+    assert abs(got - want) < 1e-14
+
+
+# ---------------------------------------------------------------- bounds / inside
+def _in_triangle(p, a, b, c):
+    def s(p1, p2, p3):
+        return (p1[0] - p3[0]) * (p2[1] - p3[1]) - (p2[0] - p3[0]) * (p1[1] - p3[1])
+    d1, d2, d3 = s(p, a, b), s(p, b, c), s(p, c, a)
+    neg = (d1 < 0) or (d2 < 0) or (d3 < 0)
+    pos = (d1 > 0) or (d2 > 0) or (d3 > 0)
+    return not (neg and pos)
+
+
+def test_inside_matches_barycentric_triangle():
+    rng = np.random.default_rng(5)
+    for _ in range(2000):
+        tx, ty = rng.uniform(0, 1, 3), rng.uniform(0, 1, 3)
+        x, y = rng.uniform(0, 1, 2)
+        got = L.og_inside(3, O.dptr(np.ascontiguousarray(tx)), O.dptr(np.ascontiguousarray(ty)), x, y)
+        assert bool(got) == _in_triangle((x, y), (tx[0], ty[0]), (tx[1], ty[1]), (tx[2], ty[2]))
+
+
+def test_bounds_mask_patterns():
+    Lg = Mg = 8
+    ecut = float(np.float32(0.1))
+    rng = np.random.default_rng(6)
+    for pattern in range(16):
+        m = np.ones((Mg, Lg), np.int32)
+        bits = [(pattern >> b) & 1 for b in range(4)]     # corners (I,J) (I+1,J) (I+1,J+1) (I,J+1)
+        I, J = 4, 4
+        m[J - 1, I - 1], m[J - 1, I], m[J, I], m[J, I - 1] = bits
+        g = synth.Grid(Lg, Mg, 4, 1000.0, 1000.0, 0.0, 0.0, h=np.full((Mg, Lg), 10.0), mask_rho=m)
+        og = O.OracleGrid(g)
+        corners = [(0, 0), (1, 0), (1, 1), (0, 1)]
+        for _ in range(300):
+            x, y = rng.uniform(0.001, 0.999, 2)
+            nm, ing, ob = C.c_int(), C.c_int(), C.c_int()
+            L.og_bounds(C.byref(og.s), I + x, J + y, C.byref(nm), C.byref(ing), C.byref(ob))
+            n = sum(bits)
+            if n == 4:
+                want = True
+            elif n == 3:
+                wet = [c for c, b in zip(corners, bits) if b]
+                want = _in_triangle((x, y), *wet)
+            elif n == 2:
+                want = any(b and abs(x - cx) < ecut and abs(y - cy) < ecut
+                           for (cx, cy), b in zip(corners, bits))
+            else:
+                want = False
+            assert bool(ing.value) == want, (pattern, x, y)
+            assert ob.value == 0
+
+
+def test_bounds_outer_ring_is_open_boundary():
+    g, og = _grid()
+    for X, Y in ((1.5, 4.0), (4.0, 1.2), (7.5, 4.0), (4.0, 7.9), (7.2, 7.2)):
+        nm, ing, ob = C.c_int(), C.c_int(), C.c_int()
+        L.og_bounds(C.byref(og.s), X, Y, C.byref(nm), C.byref(ing), C.byref(ob))
+        assert ing.value == 0 and ob.value == 1
+
+
+# ---------------------------------------------------------------- PPPACK
+def _profile(np_, rng, noise=1e-4):
+    z0 = -rng.uniform(5, 200)
+    dz = -z0 / (np_ - 1)
+    x = np.empty(np_)
+    x[0] = z0
+    for i in range(1, np_):
+        x[i] = x[i - 1] + dz
+    sig = (x - z0) / (-z0)
+    y = 4e-2 * sig * (1 - sig) + rng.normal(0, noise, np_)
+    return x, np.ascontiguousarray(y)
+
+
+def _og_smooth(x, y, dy, s):
+    n = x.size
+    v = np.zeros(n * 7)
+    a = np.zeros(n * 4)
+    sf = L.og_smooth(O.dptr(x), O.dptr(y), O.dptr(dy), n, s, O.dptr(v), O.dptr(a))
+    return sf, a.reshape(4, n).T.copy()
+
+
+@pytest.mark.skipif(O.ref_pppack() is None, reason="reference PPPACK not built (oracle/_ref)")
+def test_smooth_bit_exact_vs_reference_pppack():
+    R = O.ref_pppack()
+    rng = np.random.default_rng(7)
+    for trial in range(150):
+        n = int(rng.integers(6, 130))
+        x, y = _profile(n, rng)
+        dy = np.full(n, 1e-4)
+        s = float(rng.choice([0.6, 0.0, 1e3, 1e-3, 0.05]))
+        sf1, a1 = _og_smooth(x, y, dy, s)
+        v2 = np.zeros(n * 7)
+        a2 = np.zeros(n * 4)
+        nn, ss = C.c_int(n), C.c_double(s)
+        sf2 = R.smooth_(O.dptr(x), O.dptr(y), O.dptr(dy), C.byref(nn), C.byref(ss), O.dptr(v2), O.dptr(a2))
+        assert sf1 == sf2
+        assert np.array_equal(a1, a2.reshape(4, n).T)
+
+
+@pytest.mark.skipif(O.ref_pppack() is None, reason="reference PPPACK not built (oracle/_ref)")
+def test_ppvalu_bit_exact_vs_reference_pppack():
+    R = O.ref_pppack()
+    rng = np.random.default_rng(8)
+    n = 40
+    x, y = _profile(n, rng)
+    _, a = _og_smooth(x, y, np.full(n, 1e-4), 0.6)
+    coef = np.ascontiguousarray(a)           # coef(4, n) column-major == a(n,4) row-major
+    for _ in range(300):
+        xx = rng.uniform(x[0] - 1, x[-1] + 1)
+        for jd in (0, 1):
+            g1 = L.og_ppvalu(O.dptr(x), O.dptr(coef), n - 1, 4, xx, jd)
+            l, k, xv, j = C.c_int(n - 1), C.c_int(4), C.c_double(xx), C.c_int(jd)
+            g2 = R.ppvalu_(O.dptr(x), O.dptr(coef), C.byref(l), C.byref(k), C.byref(xv), C.byref(j))
+            assert g1 == g2
+
+
+def test_smooth_s0_is_natural_interpolating_spline():
+    rng = np.random.default_rng(9)
+    x, y = _profile(25, rng, noise=1e-3)
+    _, a = _og_smooth(x, y, np.full(25, 1e-4), 0.0)
+    cs = CubicSpline(x, y, bc_type="natural")
+    assert np.allclose(a[:, 0], y, atol=1e-12)
+    assert np.allclose(a[:-1, 1], cs(x[:-1], 1), rtol=1e-8, atol=1e-10)
+    assert np.allclose(a[:-1, 2], cs(x[:-1], 2), rtol=1e-7, atol=1e-10)
+
+
+def test_smooth_large_s_is_least_squares_line():
+    rng = np.random.default_rng(10)
+    x, y = _profile(30, rng, noise=1e-3)
+    _, a = _og_smooth(x, y, np.full(30, 1e-4), 1e12)
+    fit = np.polyval(np.polyfit(x, y, 1), x)
+    assert np.allclose(a[:, 0], fit, rtol=1e-9, atol=1e-12)
+
+
+def test_ppvalu_horner_on_known_cubic():
+    x = arr(0.0, 1.0, 3.0)
+    c = np.ascontiguousarray(np.array([[2.0, -1.0, 0.5, 3.0], [1.0, 1.0, 1.0, 1.0], [0, 0, 0, 0]]))
+    for xx in (0.0, 0.4, 0.99):
+        f = 2.0 - 1.0 * xx + 0.5 * xx ** 2 / 2 + 3.0 * xx ** 3 / 6
+        df = -1.0 + 0.5 * xx + 3.0 * xx ** 2 / 2
+        assert abs(L.og_ppvalu(O.dptr(x), O.dptr(c), 2, 4, xx, 0) - f) < 1e-15
+        assert abs(L.og_ppvalu(O.dptr(x), O.dptr(c), 2, 4, xx, 1) - df) < 1e-15
+
+
+def test_interv_matches_searchsorted():
+    rng = np.random.default_rng(11)
+    xt = np.ascontiguousarray(np.sort(rng.uniform(-50, 0, 40)))
+    mf = C.c_int()
+    for xx in np.concatenate([rng.uniform(-60, 10, 500), xt]):
+        want = int(np.clip(np.searchsorted(xt, xx, side="right"), 1, xt.size - 1))
+        assert L.og_interv(O.dptr(xt), xt.size, xx, C.byref(mf)) == want
