@@ -28,6 +28,32 @@ FLAGS = [
 ]
 
 
+FORTRAN = PKG / "fortran"
+DRIVER = PKG / "lib" / "rp_driver"
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = True) -> Path:
+    """The ISO_C_BINDING module and the Fortran driver, linked to librp.so."""
+    out = PKG / "lib"
+    mods = out / "fmod"
+    mods.mkdir(parents=True, exist_ok=True)
+    steps = [
+        [FC, "-O2", "-J", str(mods), "-c", str(FORTRAN / "romspath_hip.f90"), "-o", str(mods / "romspath_hip.o")],
+        [FC, "-O2", "-I", str(mods), "-c", str(FORTRAN / "rp_driver.f90"), "-o", str(mods / "rp_driver.o")],
+        [FC, "-o", str(DRIVER), str(mods / "rp_driver.o"), str(mods / "romspath_hip.o"),
+         "-L", str(out), "-lrp", "-Wl,-rpath,$ORIGIN"],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"Fortran build failed ({r.returncode})")
+    return DRIVER
+
+
 def build(verbose: bool = True) -> Path:
     LIB.parent.mkdir(parents=True, exist_ok=True)
     tmp = LIB.with_suffix(".so.tmp")
@@ -45,6 +71,7 @@ def build(verbose: bool = True) -> Path:
         if lines:
             sys.stderr.write("\n".join(lines) + "\n")
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -15,25 +15,11 @@ struct RhoCell {
     int64_t o[4];
 };
 
+// Corner indices are clamped into the grid.  For positions inside the grid
+// this changes nothing; it keeps every read in bounds where the reference
+// indexes outside its arrays (after a failed nested hand-off,
+// ROMSPath.f90:837-843, or for a non-finite position).
 __device__ __forceinline__ RhoCell rho_cell(const DGrid& g, double X, double Y)
-{
-    RhoCell c;
-    c.I = (int)floor(X);
-    c.J = (int)floor(Y);
-    c.x = X - c.I;
-    c.y = Y - c.J;
-    const int64_t b = (int64_t)(c.J - 1) * g.L + (c.I - 1);
-    c.o[0] = b;
-    c.o[1] = b + 1;
-    c.o[2] = b + g.L + 1;
-    c.o[3] = b + g.L;
-    return c;
-}
-
-// Same cell with indices clamped into the grid; used only for reads at the
-// post-step position, where the reference can index outside the arrays
-// after a failed nested hand-off (ROMSPath.f90:837-843).
-__device__ __forceinline__ RhoCell rho_cell_clamped(const DGrid& g, double X, double Y)
 {
     RhoCell c;
     c.I = (int)floor(X);
@@ -112,8 +98,9 @@ template <typename F>
 __device__ __forceinline__ void interp_u(const DGrid& g, double X, double Y, const VBracket& vb,
                                         const SlotSel& S, int nslot, double out[3])
 {
-    const int Jnode = (int)floor(Y);
-    const double y = Y - Jnode;
+    const int Jf = (int)floor(Y);
+    const double y = Y - Jf;
+    const int Jnode = min(max(Jf, 1), g.M - 1);   // in-range positions: unchanged
     const double uvI = X - 0.5;
     const int Inode = (int)floor(uvI);
     const double x = uvI - Inode;
@@ -161,6 +148,7 @@ __device__ __forceinline__ void interp_v(const DGrid& g, double X, double Y, con
     if (Jnode < 1) Jnode = 1;
     const int etav = g.M - 1;
     if (Jnode > etav) Inode = etav;   // interpolation_module.f90:696 assigns Inode
+    Inode = min(max(Inode, 1), g.L - 1);   // in-range positions: unchanged
     const double y = uvJ - Jnode;
     double cff[4];
     bilin(x, y, cff);
@@ -758,7 +746,7 @@ __global__ void __launch_bounds__(256) step_kernel(const DGrid* __restrict__ gri
             }
         }
         const DGrid& gn = grids[NESTED ? gid - 1 : 0];
-        const RhoCell cn = rho_cell_clamped(gn, tempX, tempY);
+        const RhoCell cn = rho_cell(gn, tempX, tempY);
         const Surface sn = surface_at<F>(gn, A, cn);
         const double tdepth = sn.tdepth, Pzeta = sn.zeta;
         if (A.want_diag) {

@@ -1,0 +1,172 @@
+! romspath_hip.f90 -- ISO_C_BINDING interface to librp.so (include/rp_abi.h).
+!
+! The reference driver (ROMSPath.f90) calls update_particles() once per
+! internal step (ROMSPath.f90:529).  With this module it instead hands its
+! state to the GPU once and calls rp_step per internal step; see the
+! rp_driver program next to this file for the call sequence.
+module romspath_hip
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: RP_MAXGRID = 5
+  integer(c_int32_t), parameter, public :: RP_PREC_FP32_FIELDS = 0, RP_PREC_FP64 = 1
+
+  ! rp_config: namelist scalars of PARAM_MOD on the hot path plus build knobs
+  type, bind(C), public :: rp_config
+    integer(c_int32_t) :: idt, dt
+    real(c_double) :: deltat, serr, smth, sub, aksback
+    integer(c_int32_t) :: vturb_on, hturb_on
+    real(c_double) :: kh(RP_MAXGRID)
+    integer(c_int32_t) :: scheme, nsb
+    real(c_double) :: vertdist
+    integer(c_int32_t) :: open_boundary, ngrid
+    integer(c_int64_t) :: seed
+    integer(c_int32_t) :: precision, block, sort_interval
+    integer(c_int32_t) :: reserved(5)
+  end type rp_config
+
+  ! rp_grid: GRIDDATA after InitGrid.  C's scl[5][2] is Fortran scl(2,5):
+  ! scl(:,i) is the reference's GRIDS(ng)%scl(i,:).
+  type, bind(C), public :: rp_grid
+    integer(c_int32_t) :: L, M, N, vtransform
+    real(c_double) :: hc
+    type(c_ptr) :: s_rho, cs_r, s_w, cs_w
+    type(c_ptr) :: h, pm, pn
+    type(c_ptr) :: mask_rho, mask_u, mask_v
+    real(c_double) :: scl(2, RP_MAXGRID), off(2, RP_MAXGRID)
+  end type rp_grid
+
+  public :: rp_create, rp_set_grid, rp_set_hydro_slot, rp_set_particles, rp_step, &
+            rp_get_particles, rp_set_deviates, rp_sample_deviates, rp_active_steps, &
+            rp_secant_capped, rp_synchronize, rp_kernel_timing, rp_last_error, rp_destroy, &
+            rp_error_text
+
+  interface
+    integer(c_int) function rp_create(ctx, cfg) bind(C, name="rp_create")
+      import :: c_int, c_ptr, rp_config
+      type(c_ptr), intent(out) :: ctx
+      type(rp_config), intent(in) :: cfg
+    end function
+
+    integer(c_int) function rp_set_grid(ctx, ng, g) bind(C, name="rp_set_grid")
+      import :: c_int, c_int32_t, c_ptr, rp_grid
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ng
+      type(rp_grid), intent(in) :: g
+    end function
+
+    integer(c_int) function rp_set_hydro_slot(ctx, ng, slot, zeta, u, v, w, aks, su, sv) &
+        bind(C, name="rp_set_hydro_slot")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ng, slot
+      real(c_double), intent(in) :: zeta(*), u(*), v(*), w(*), aks(*)
+      type(c_ptr), value :: su, sv
+    end function
+
+    integer(c_int) function rp_set_particles(ctx, n, x, y, z, age, dob, status, gid, isin) &
+        bind(C, name="rp_set_particles")
+      import :: c_int, c_int32_t, c_int64_t, c_int8_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: x(*), y(*), z(*), age(*), dob(*), status(*)
+      integer(c_int32_t), intent(in) :: gid(*)
+      integer(c_int8_t), intent(in) :: isin(*)
+    end function
+
+    integer(c_int) function rp_step(ctx, ex, ix, ets, t_b, t_c, t_f, step_index, want_diag) &
+        bind(C, name="rp_step")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ex(3), ix(3)
+      integer(c_int32_t), value :: ets, t_b, t_c, t_f
+      integer(c_int64_t), value :: step_index
+      integer(c_int32_t), value :: want_diag
+    end function
+
+    integer(c_int) function rp_get_particles(ctx, n, x, y, z, age, status, gid, isin, wd, zeta, &
+        bath) bind(C, name="rp_get_particles")
+      import :: c_int, c_int32_t, c_int64_t, c_int8_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(out) :: x(*), y(*), z(*), age(*), status(*), wd(*), zeta(*), bath(*)
+      integer(c_int32_t), intent(out) :: gid(*)
+      integer(c_int8_t), intent(out) :: isin(*)
+    end function
+
+    integer(c_int) function rp_set_deviates(ctx, normals, n, ndev) bind(C, name="rp_set_deviates")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx, normals
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), value :: ndev
+    end function
+
+    integer(c_int) function rp_sample_deviates(ctx, step_index, n, ndev, out) &
+        bind(C, name="rp_sample_deviates")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: step_index, n
+      integer(c_int32_t), value :: ndev
+      real(c_double), intent(out) :: out(*)
+    end function
+
+    integer(c_int) function rp_active_steps(ctx, count) bind(C, name="rp_active_steps")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: count
+    end function
+
+    integer(c_int) function rp_secant_capped(ctx, count) bind(C, name="rp_secant_capped")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: count
+    end function
+
+    integer(c_int) function rp_synchronize(ctx) bind(C, name="rp_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function rp_kernel_timing(ctx, avg_ms, launches) bind(C, name="rp_kernel_timing")
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: avg_ms
+      integer(c_int64_t), intent(out) :: launches
+    end function
+
+    type(c_ptr) function rp_last_error(ctx) bind(C, name="rp_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function rp_destroy(ctx) bind(C, name="rp_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function
+  end interface
+
+contains
+
+  ! rp_last_error as a Fortran string, for errorHandler-style reporting
+  function rp_error_text(ctx) result(msg)
+    type(c_ptr), value :: ctx
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: n, i
+    p = rp_last_error(ctx)
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+      msg(i:i) = s(i)
+    end do
+  end function rp_error_text
+
+end module romspath_hip
