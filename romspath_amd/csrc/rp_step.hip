@@ -290,7 +290,7 @@ struct VtLayout {
 // v(:,1..3,5..7)), elimination and forward substitution share one sweep.
 // Returns sum_i qu(i)^2 dy(i)^2 (the dot_product of smooth) and, when asked,
 // utru of the p=0 solve.
-__device__ double chol1d_dev(double p, const VtLayout& Lo, const Scr& S, double dy2,
+__device__ __forceinline__ double chol1d_dev(double p, const VtLayout& Lo, const Scr& S, double dy2,
                              bool want_utru, double* utru)
 {
     const int np = Lo.np;
@@ -407,7 +407,7 @@ struct PieceCache {
 
 // Coefficients of piece i of the smoothing spline (smooth's final loop,
 // pppack.f90:5305-5314), formed on demand from u, h and y.
-__device__ void spline_piece(const VtLayout& Lo, const Scr& S, int i, double p, double k2,
+__device__ __forceinline__ void spline_piece(const VtLayout& Lo, const Scr& S, int i, double p, double k2,
                              PieceCache& pc)
 {
     const int np = Lo.np;
@@ -432,6 +432,15 @@ __device__ void spline_piece(const VtLayout& Lo, const Scr& S, int i, double p, 
     pc.brk = S(Lo.fitz + i - 1);
 }
 
+// The Horner loop of ppvalu (pppack.f90:4349-4357) for k=4, written out:
+// value=(value/fmmjdr)*h+coef(m,i) for fmmjdr=4-jderiv..1; its first step
+// yields coef(4,i) exactly and the last divides by 1.
+__device__ __forceinline__ double horner(const PieceCache& pc, double h, int jderiv)
+{
+    if (jderiv == 0) return ((pc.c[3] / 3.0 * h + pc.c[2]) / 2.0 * h + pc.c[1]) * h + pc.c[0];
+    return (pc.c[3] / 2.0 * h + pc.c[2]) * h + pc.c[1];
+}
+
 // ppvalu (pppack.f90:4246-4362), k=4
 __device__ __forceinline__ double ppvalu_dev(const VtLayout& Lo, const Scr& S, PieceCache& pc,
                                              double x, int jderiv, double p, double k2,
@@ -440,20 +449,11 @@ __device__ __forceinline__ double ppvalu_dev(const VtLayout& Lo, const Scr& S, P
     const int i = interv_dev(Lo, S, x, f1, dz);
     if (i != pc.i) spline_piece(Lo, S, i, p, k2, pc);
     const double h = x - pc.brk;
-    double value = 0.0;
-    double fmmjdr = 4 - jderiv;
-    int m = 4;
-    for (;;) {
-        value = (value / fmmjdr) * h + pc.c[m - 1];
-        m = m - 1;
-        fmmjdr = fmmjdr - 1.0;
-        if (fmmjdr <= 0.0) break;
-    }
-    return value;
+    return horner(pc, h, jderiv);
 }
 
 template <typename F>
-__device__ double vturb(const DGrid& g, const StepArgs& A, const RhoCell& c, const Surface& sf,
+__device__ __forceinline__ double vturb(const DGrid& g, const StepArgs& A, const RhoCell& c, const Surface& sf,
                         double Zpar, DevSrc& dv, const Scr& S, unsigned* capped)
 {
     const int nw = g.N + 1;
@@ -585,9 +585,283 @@ __device__ double vturb(const DGrid& g, const StepArgs& A, const RhoCell& c, con
     return ParZc - Zpar;
 }
 
+// ---------------------------------------------------------------------------
+// VTurb, FP32-field form.  Same model and the same secant search for the
+// smoothing parameter as above, evaluated without per-iteration work arrays:
+//
+//   * the resampled points are taken as exactly uniform (spacing dz; the
+//     reference accumulates fitz(i)=fitz(i-1)+dz, which differs in the last
+//     bits), so chol1d's matrix T(p) = 6(1-p) V + p W is pentadiagonal
+//     Toeplitz, V = r^2 (Q'Q), W = h tridiag(1,4,1), r = serr/dz;
+//   * smooth's sum_i (a(i,1) dy(i))^2 = u'V u for the solution of T u = qty.
+//     With G(p) = qty' T(p)^-1 qty (= sum w_i^2/d_i of an LDL' sweep) and
+//     u'T u = G, dG/dp = -u'(dT/dp)u = 6 u'Vu - u'Wu, it follows that
+//     u'Vu = (G + p G')/6 and, at p=0, utru = u'Wu/2 = (G - G')/2.  One
+//     forward LDL' sweep carrying derivatives in p therefore gives every
+//     quantity the secant search needs, with no back substitution;
+//   * only the accepted p gets a full solve, whose u builds the spline
+//     pieces the random-displacement sub-steps visit.
+// ---------------------------------------------------------------------------
+struct FastLayout {
+    int np, nw;
+    int qty, y, rd, l1, l2, w, u, z, pkh;
+    __device__ __forceinline__ FastLayout(int np_, int nw_) : np(np_), nw(nw_)
+    {
+        qty = 0; y = np; rd = 2 * np; l1 = 3 * np; l2 = 4 * np; w = 5 * np; u = 6 * np;
+        z = 7 * np; pkh = 7 * np + nw;
+    }
+};
+
+// Toeplitz entries of chol1d's matrix (pppack.f90:1509-1512) for uniform h
+struct Band {
+    double a, b, c, da, db, dc;
+};
+
+__device__ __forceinline__ Band band_of(double p, double h, double serr)
+{
+    const double r = serr / h;
+    const double v5 = 6.0 * r * r, v6 = -4.0 * r * r, v7 = r * r;
+    const double six1mp = 6.0 * (1.0 - p), twop = 2.0 * p;
+    Band B;
+    B.a = six1mp * v5 + twop * (h + h);
+    B.b = six1mp * v6 + p * h;
+    B.c = six1mp * v7;
+    B.da = -6.0 * v5 + 4.0 * h;
+    B.db = -6.0 * v6 + h;
+    B.dc = -6.0 * v7;
+    return B;
+}
+
+// G(p) and dG/dp by one LDL' sweep over the n interior unknowns
+__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int oq, double& G, double& Gp)
+{
+    double d = B.a, dd = B.da, e = B.b, de = B.db;
+    double dn = B.a, ddn = B.da, en = B.b, den = B.db;
+    double w = S(oq), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
+    double g = 0.0, gp = 0.0;
+    for (int m = 0; m < n; m++) {
+        const double rd = 1.0 / d;
+        const double wrd = w * rd;
+        g += w * wrd;
+        gp += (2.0 * w * dw - wrd * w * dd) * rd;
+        if (m == n - 1) break;
+        const double rat = e * rd, drat = (de - rat * dd) * rd;
+        const double rat2 = B.c * rd, drat2 = (B.dc - rat2 * dd) * rd;
+        const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
+        const double e1 = en - rat * B.c, de1 = den - drat * B.c - rat * B.dc;
+        const double d2 = B.a - rat2 * B.c, dd2 = B.da - drat2 * B.c - rat2 * B.dc;
+        const double wn = S(oq + m + 1) - rat * w - l2p * wp;
+        const double dwn = -drat * w - rat * dw - dl2p * wp - l2p * dwp;
+        wp = w; dwp = dw; w = wn; dw = dwn;
+        l2p = rat2; dl2p = drat2;
+        d = d1; dd = dd1; e = e1; de = de1;
+        dn = d2; ddn = dd2; en = B.b; den = B.db;
+    }
+    G = g;
+    Gp = gp;
+}
+
+// full solve at the accepted p: u(2..np-1) into Lo.u (u(1)=u(np)=0)
+__device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr& S)
+{
+    const int n = Lo.np - 2;
+    double d = B.a, e = B.b, dn = B.a, en = B.b;
+    double w = S(Lo.qty), wp = 0.0, l2p = 0.0;
+    for (int m = 0; m < n; m++) {
+        const double rd = 1.0 / d;
+        S(Lo.rd + m) = rd;
+        S(Lo.w + m) = w;
+        if (m == n - 1) break;
+        const double rat = e * rd, rat2 = B.c * rd;
+        S(Lo.l1 + m) = rat;
+        S(Lo.l2 + m) = rat2;
+        const double d1 = dn - rat * e;
+        const double e1 = en - rat * B.c;
+        const double d2 = B.a - rat2 * B.c;
+        const double wn = S(Lo.qty + m + 1) - rat * w - l2p * wp;
+        wp = w; w = wn; l2p = rat2;
+        d = d1; e = e1; dn = d2; en = B.b;
+    }
+    double up1 = 0.0, up2 = 0.0;
+    for (int m = n - 1; m >= 0; m--) {
+        double um = S(Lo.w + m) * S(Lo.rd + m);
+        if (m < n - 1) um = um - up1 * S(Lo.l1 + m) - up2 * S(Lo.l2 + m);
+        S(Lo.u + m + 1) = um;
+        up2 = up1;
+        up1 = um;
+    }
+    S(Lo.u + 0) = 0.0;
+    S(Lo.u + Lo.np - 1) = 0.0;
+}
+
+__device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr& S, int i, double p, double k2,
+                           double h, double z1, PieceCache& pc)
+{
+    const int np = Lo.np;
+    auto U = [&](int j) { return S(Lo.u + j - 1); };
+    const double rh = 1.0 / h;
+    auto raw = [&](int j) { return (U(j) - U(j - 1)) * rh; };
+    const double r_i = (i >= 2) ? raw(i) : 0.0;
+    const double r_i1 = raw(i + 1);
+    const double qu_i = r_i1 - r_i;
+    const double qu_i1 = (i + 1 == np) ? -r_i1 : raw(i + 2) - r_i1;
+    const double a1 = S(Lo.y + i - 1) - k2 * qu_i;
+    const double a1n = S(Lo.y + i) - k2 * qu_i1;
+    const double a3 = U(i) * 6.0 * p;
+    const double a3n = U(i + 1) * 6.0 * p;
+    const double a4 = (a3n - a3) * rh;
+    const double a2 = (a1n - a1) * rh - (a3 + a4 / 3.0 * h) / 2.0 * h;
+    pc.i = i;
+    pc.c[0] = a1; pc.c[1] = a2; pc.c[2] = a3; pc.c[3] = a4;
+    pc.brk = z1 + (double)(i - 1) * h;
+}
+
+__device__ __forceinline__ double ppvalu_fast(const FastLayout& Lo, const Scr& S, PieceCache& pc,
+                                              double x, int jderiv, double p, double k2, double h,
+                                              double z1)
+{
+    const int np = Lo.np;
+    const double gss = (x - z1) / h;
+    int i;
+    if (!(gss >= 0.0)) i = 1;
+    else if (gss >= (double)(np - 1)) i = np - 1;
+    else i = (int)gss + 1;
+    if (i != pc.i) piece_fast(Lo, S, i, p, k2, h, z1, pc);
+    return horner(pc, x - pc.brk, jderiv);
+}
+
+template <typename F>
+__device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, const RhoCell& c,
+                             const Surface& sf, double Zpar, DevSrc& dv, const Scr& S,
+                             unsigned* capped)
+{
+    const int nw = g.N + 1;
+    const int np = A.np;
+    const FastLayout Lo(np, nw);
+    const double tdepth = sf.tdepth, zeta = sf.zeta;
+    const double hh = -1.0 * tdepth;
+    double cff[4];
+    rho_weights(g, c, cff);
+    const F* aks = (const F*)g.aks;
+    for (int k = 1; k <= nw; k++) {
+        const double Sk = level_S(g, g.s_w, g.cs_w, k, hh);
+        const double zb = level_z(g, g.s_w, g.cs_w, k, sf.zb, hh, Sk);
+        const double zc = level_z(g, g.s_w, g.cs_w, k, sf.zc, hh, Sk);
+        const double zf = level_z(g, g.s_w, g.cs_w, k, sf.zf, hh, Sk);
+        S(Lo.z + k - 1) = polintd(A.ex, zb, zc, zf, A.ix[1]);
+        double v[4][4];
+        for (int q = 0; q < 4; q++) ld4(aks + (c.o[q] * nw + (k - 1)) * 4, v[q]);
+        double kh[3];
+        for (int t = 0; t < 3; t++) {
+            const int s = A.slots.s[t];
+            kh[t] = cff[0] * pick(v[0], s) + cff[1] * pick(v[1], s) + cff[2] * pick(v[2], s) +
+                    cff[3] * pick(v[3], s);
+        }
+        S(Lo.pkh + k - 1) = polintd(A.ex, kh[0], kh[1], kh[2], A.ix[1]);
+    }
+    // resample to np uniform points (linint) and form qty = second differences
+    const double z1 = S(Lo.z + 0), zn = S(Lo.z + nw - 1);
+    const double h = (zn - z1) / (double)(np - 1);
+    {
+        int jlo = 1;
+        double zlo = z1, zhi = S(Lo.z + 1);
+        double ylo = S(Lo.pkh + 0), yhi = S(Lo.pkh + 1);
+        double m = (ylo - yhi) / (zlo - zhi);
+        double ym1 = 0.0, y0 = ylo;
+        S(Lo.y + 0) = y0;
+        const double rh = 1.0 / h;
+        for (int i = 2; i <= np; i++) {
+            double yi;
+            if (i == np) {
+                yi = S(Lo.pkh + nw - 1);
+            } else {
+                const double fz = z1 + (double)(i - 1) * h;
+                bool moved = false;
+                while (jlo < nw - 1 && zhi <= fz) {
+                    jlo++;
+                    zlo = zhi;
+                    zhi = S(Lo.z + jlo);
+                    moved = true;
+                }
+                if (moved) {
+                    ylo = S(Lo.pkh + jlo - 1);
+                    yhi = S(Lo.pkh + jlo);
+                    m = (ylo - yhi) / (zlo - zhi);
+                }
+                yi = m * fz + (ylo - m * zlo);
+            }
+            S(Lo.y + i - 1) = yi;
+            if (i >= 3) S(Lo.qty + i - 3) = ((yi - y0) - (y0 - ym1)) * rh;   // qty(i-1)
+            ym1 = y0;
+            y0 = yi;
+        }
+    }
+    // secant search for p (pppack.f90:5236-5298), sfq from G and G'
+    const double serr = A.serr, s = A.smth;
+    const int n = np - 2;
+    double p = 0.0, q = 0.0, sfq;
+    if (s <= 0.0) {
+        p = 1.0;
+    } else {
+        double G, Gp;
+        stream_G(band_of(0.0, h, serr), n, S, Lo.qty, G, Gp);
+        sfq = 6.0 * G;
+        if (s < sfq) {
+            const double utru = 0.5 * (G - Gp);
+            const double ooss = 1.0 / sqrt(s);
+            double oosf = 1.0 / sqrt(sfq);
+            q = -(oosf - ooss) * sfq / (6.0 * utru * oosf);
+            double prevq = 0.0, prevsf = oosf;
+            int it = 0;
+            for (;;) {
+                const double pq = q / (1.0 + q);
+                stream_G(band_of(pq, h, serr), n, S, Lo.qty, G, Gp);
+                sfq = 6.0 * (G + pq * Gp) / ((1.0 + q) * (1.0 + q));
+                if (fabs(sfq - s) <= 0.01 * s) break;
+                if (++it >= kSecantCap) {
+                    atomicAdd(capped, 1u);
+                    break;
+                }
+                oosf = 1.0 / sqrt(sfq);
+                const double change = (q - prevq) / (oosf - prevsf) * (oosf - ooss);
+                prevq = q;
+                q = q - change;
+                prevsf = oosf;
+            }
+            p = q / (1.0 + q);
+        }
+    }
+    solve_u(band_of(p, h, serr), Lo, S);
+    const double k2 = 6.0 * (1.0 - p) * (serr * serr);
+    PieceCache pc;
+    pc.i = -1;
+    const double deltat = A.deltat;
+    double ParZc = Zpar;
+    for (int i = 1; i <= A.loop; i++) {
+        double Kprimec;
+        if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
+        else Kprimec = ppvalu_fast(Lo, S, pc, ParZc, 1, p, k2, h, z1);
+        const double KprimeZc = Kprimec * deltat;
+        const double Z3rdc = ParZc + 0.5 * KprimeZc;
+        double KH3rdc;
+        if (Z3rdc < tdepth || Z3rdc > zeta) {
+            KH3rdc = A.aksback;
+        } else {
+            KH3rdc = ppvalu_fast(Lo, S, pc, Z3rdc, 0, p, k2, h, z1);
+            if (KH3rdc < A.aksback) KH3rdc = A.aksback;
+        }
+        const double DEV = dv.get(1 + i);
+        const double dTz = KprimeZc + DEV * sqrt(2.0 * KH3rdc * deltat);
+        ParZc = ParZc + dTz;
+        if (ParZc <= tdepth) ParZc = tdepth + fabs(ParZc - tdepth);
+        if (ParZc >= zeta) ParZc = zeta - fabs(ParZc - zeta);
+    }
+    return ParZc - Zpar;
+}
+
 // boundary_module.f90:43-143 with the 3-point inside() of
 // interpolation_module.f90:269-420 specialised to the unit cell.
-__device__ bool inside3(const double xb[3], const double yb[3], double xo, double yo)
+__device__ __forceinline__ bool inside3(const double xb[3], const double yb[3], double xo, double yo)
 {
     double X[4] = {xb[0], xb[1], xb[2], xb[0]};
     double Y[4] = {yb[0], yb[1], yb[2], yb[0]};
@@ -614,7 +888,7 @@ __device__ bool inside3(const double xb[3], const double yb[3], double xo, doubl
     return crossings != 0;
 }
 
-__device__ void bounds(const DGrid& g, double Xp, double Yp, bool& ingrid, bool& obound)
+__device__ __forceinline__ void bounds(const DGrid& g, double Xp, double Yp, bool& ingrid, bool& obound)
 {
     const double ecut = (double)0.1f, fcut = 1.0 - ecut;
     ingrid = false;
@@ -727,7 +1001,10 @@ __global__ void __launch_bounds__(256) step_kernel(const DGrid* __restrict__ gri
         if (A.vturb_on) {
             if (!have_s0) s0 = surface_at<F>(g, A, c0);
             const Scr S{A.scratch + tid, T};
-            TurbV = vturb<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
+            if (sizeof(F) == 4)
+                TurbV = vturb_fast<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
+            else
+                TurbV = vturb<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
         }
         const double Zbehav = 0.0;
         const double newX = X0 + AdvectX + TurbHx;
