@@ -42,6 +42,7 @@ struct Ctx {
     int8_t *status = nullptr, *gid = nullptr;
     uint8_t* isin = nullptr;
     uint32_t* id = nullptr;
+    double* mid = nullptr;       // [4][n] hand-over between the step's two kernels
     // injected deviates
     double* dev = nullptr;
     int ndev = 0;
@@ -337,6 +338,7 @@ int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, con
     rc |= dalloc(c, &c->wd, n); rc |= dalloc(c, &c->zeta, n); rc |= dalloc(c, &c->bath, n);
     rc |= dalloc(c, &c->status, n); rc |= dalloc(c, &c->gid, n); rc |= dalloc(c, &c->isin, n);
     rc |= dalloc(c, &c->id, n);
+    rc |= dalloc(c, &c->mid, 4 * (size_t)n);
     if (rc) return RP_E_ALLOC;
     c->n = n;
     if (n == 0) return RP_OK;
@@ -425,17 +427,19 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     A.ndev = c->ndev;
     A.active = c->active;
     A.capped = c->capped;
+    A.mid = c->mid;
     if (vt && A.np < 3) return fail(c, RP_E_ARG, "smoothing spline needs sub*(N+1) >= 3");
 
     const int block = c->cfg.block > 0 ? c->cfg.block : 256;
     if (block % 64 || block > 256) return fail(c, RP_E_ARG, "block must be a multiple of 64, <= 256");
-    int64_t grid = (c->n + block - 1) / block;
+    const int64_t grid_all = (c->n + block - 1) / block;
+    int64_t grid_vt = grid_all;
     A.scratch = nullptr;
     if (vt) {
-        // persistent grid: bounded scratch, lanes loop over particles
+        // persistent grid for the vertical walk: bounded scratch, lanes loop
         const int64_t cap = (int64_t)c->ncu * 1024 / block;
-        grid = std::min<int64_t>(grid, cap);
-        const int64_t T = grid * block;
+        grid_vt = std::min<int64_t>(grid_all, cap);
+        const int64_t T = grid_vt * block;
         const size_t need = (size_t)vturb_rows(A.np, maxN + 1) * T * sizeof(double);
         if (c->scratch_bytes < need) {
             int rc = dalloc(c, (char**)&c->scratch, need);
@@ -479,15 +483,22 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     }
     HIPCHK(c, hipEventRecord(ev.first, c->stream));
     const bool nested = c->cfg.ngrid > 1;
-    const dim3 gd((unsigned)grid), bd(block);
-    if (c->fbytes == 4 && !nested)
-        hipLaunchKernelGGL((step_kernel<float, false>), gd, bd, 0, c->stream, c->dgrids, A);
-    else if (c->fbytes == 4)
-        hipLaunchKernelGGL((step_kernel<float, true>), gd, bd, 0, c->stream, c->dgrids, A);
-    else if (!nested)
-        hipLaunchKernelGGL((step_kernel<double, false>), gd, bd, 0, c->stream, c->dgrids, A);
-    else
-        hipLaunchKernelGGL((step_kernel<double, true>), gd, bd, 0, c->stream, c->dgrids, A);
+    const bool f32 = c->fbytes == 4;
+    const dim3 bd(block);
+    using KFn = void (*)(const DGrid*, StepArgs);
+    auto pick = [&](int phase) -> KFn {
+        if (f32 && !nested) return phase == 0 ? step_kernel<float, false, 0> : phase == 1 ? step_kernel<float, false, 1> : step_kernel<float, false, 2>;
+        if (f32) return phase == 0 ? step_kernel<float, true, 0> : phase == 1 ? step_kernel<float, true, 1> : step_kernel<float, true, 2>;
+        if (!nested) return phase == 0 ? step_kernel<double, false, 0> : phase == 1 ? step_kernel<double, false, 1> : step_kernel<double, false, 2>;
+        return phase == 0 ? step_kernel<double, true, 0> : phase == 1 ? step_kernel<double, true, 1> : step_kernel<double, true, 2>;
+    };
+    if (!vt) {
+        hipLaunchKernelGGL(pick(0), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
+    } else {
+        hipLaunchKernelGGL(pick(1), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(pick(2), dim3((unsigned)grid_vt), bd, 0, c->stream, c->dgrids, A);
+    }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev.second, c->stream));
     c->ev_used.push_back(ev);
@@ -613,7 +624,7 @@ int rp_destroy(void* ctx)
     for (auto& g : c->grids) free_grid(g);
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->age); dfree(c->dob);
     dfree(c->wd); dfree(c->zeta); dfree(c->bath);
-    dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id);
+    dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
     dfree(c->dev); dfree(c->staging); dfree(c->scratch);
     dfree(c->active); dfree(c->capped); dfree(c->dgrids);
     for (auto& e : c->ev_used) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }

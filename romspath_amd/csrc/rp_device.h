@@ -45,20 +45,30 @@ struct DGrid {
 // ---------------------------------------------------------------------------
 // loads
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void ld4(const float* p, double o[4])
+// The four slots of one point arrive in one load and stay in the storage
+// type until one of them is picked (widened to FP64 only then).
+template <typename F> struct Slots4;
+template <> struct Slots4<float> {
+    float v[4];
+};
+template <> struct Slots4<double> {
+    double v[4];
+};
+__device__ __forceinline__ void ld4(const float* p, Slots4<float>& o)
 {
     const float4 v = *reinterpret_cast<const float4*>(p);
-    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+    o.v[0] = v.x; o.v[1] = v.y; o.v[2] = v.z; o.v[3] = v.w;
 }
-__device__ __forceinline__ void ld4(const double* p, double o[4])
+__device__ __forceinline__ void ld4(const double* p, Slots4<double>& o)
 {
     const double2 a = reinterpret_cast<const double2*>(p)[0];
     const double2 b = reinterpret_cast<const double2*>(p)[1];
-    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    o.v[0] = a.x; o.v[1] = a.y; o.v[2] = b.x; o.v[3] = b.y;
 }
-__device__ __forceinline__ double pick(const double o[4], int s)
+template <typename F>
+__device__ __forceinline__ double pick(const Slots4<F>& o, int s)
 {
-    return s == 0 ? o[0] : (s == 1 ? o[1] : (s == 2 ? o[2] : o[3]));
+    return (double)(s == 0 ? o.v[0] : (s == 1 ? o.v[1] : (s == 2 ? o.v[2] : o.v[3])));
 }
 
 // ---------------------------------------------------------------------------

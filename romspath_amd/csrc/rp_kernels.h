@@ -39,12 +39,14 @@ struct StepArgs {
     const double* dev;           // injected deviates [id][ndev] or null
     int ndev;
     double* scratch;             // VTurb work arrays, column per lane
+    double* mid;                 // [4][n] newX, newY, AdvectZ, clamped z between phases
     unsigned long long* active;  // += particles that took the step
     unsigned* capped;            // += secant searches stopped at the cap
 };
 
-// NESTED=false: one grid, all lanes read grid 0 (uniform, scalar loads)
-template <typename F, bool NESTED>
+// NESTED=false: one grid, all lanes read grid 0 (uniform, scalar loads).
+// PHASE 0: whole step; 1: advection + horizontal walk; 2: the rest.
+template <typename F, bool NESTED, int PHASE>
 __global__ void step_kernel(const DGrid* __restrict__ grids, StepArgs A);
 
 template <typename F>

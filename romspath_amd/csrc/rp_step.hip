@@ -53,7 +53,7 @@ __device__ __forceinline__ void zeta3(const DGrid& g, const RhoCell& c, const Sl
     for (int k = 0; k < 4; k++) m[k] = (double)g.mrho[c.o[k]];
     getcoeff(c.x, c.y, m, w);
     const F* z = (const F*)g.zeta;
-    double v[4][4];
+    Slots4<F> v[4];
     for (int k = 0; k < 4; k++) ld4(z + c.o[k] * 4, v[k]);
     for (int t = 0; t < 3; t++) {
         const int s = S.s[t];
@@ -67,18 +67,18 @@ __device__ __forceinline__ void zeta3(const DGrid& g, const RhoCell& c, const Sl
 // The vertical bracket is shared by every slot: the reference recomputes it
 // per call from the same (z, zeta, depth), so the values are identical.
 // ---------------------------------------------------------------------------
-template <typename F>
+template <typename F, int NSLOT>
 __device__ __forceinline__ void blend3(const F* f, const int64_t col[4], int nk,
                                        const double cff[4], const bool zero[4],
-                                       const VBracket& vb, const SlotSel& S, int nslot,
-                                       double out[3])
+                                       const VBracket& vb, const SlotSel& S, double out[3])
 {
     double zb[3] = {0, 0, 0}, zt[3] = {0, 0, 0};
     for (int k = 0; k < 4; k++) {
-        double lb[4], lt[4];
+        Slots4<F> lb, lt;
         ld4(f + (col[k] * nk + (vb.ibot - 1)) * 4, lb);
         ld4(f + (col[k] * nk + (vb.itop - 1)) * 4, lt);
-        for (int t = 0; t < nslot; t++) {
+#pragma unroll
+        for (int t = 0; t < NSLOT; t++) {
             const int s = S.s[t];
             const double b = zero[k] ? 0.0 : pick(lb, s);
             const double a = zero[k] ? 0.0 : pick(lt, s);
@@ -91,12 +91,13 @@ __device__ __forceinline__ void blend3(const F* f, const int64_t col[4], int nk,
             }
         }
     }
-    for (int t = 0; t < nslot; t++) out[t] = vb.c0 * zb[t] + vb.c1 * zt[t];
+#pragma unroll
+    for (int t = 0; t < NSLOT; t++) out[t] = vb.c0 * zb[t] + vb.c1 * zt[t];
 }
 
-template <typename F>
+template <typename F, int NSLOT>
 __device__ __forceinline__ void interp_u(const DGrid& g, double X, double Y, const VBracket& vb,
-                                        const SlotSel& S, int nslot, double out[3])
+                                        const SlotSel& S, double out[3])
 {
     const int Jf = (int)floor(Y);
     const double y = Y - Jf;
@@ -134,12 +135,12 @@ __device__ __forceinline__ void interp_u(const DGrid& g, double X, double Y, con
     col[1] = (int64_t)(Jnode - 1) * xiu + (ib - 1);
     col[2] = (int64_t)(Jnode) * xiu + (ib - 1);
     col[3] = (int64_t)(Jnode) * xiu + (ia - 1);
-    blend3<F>((const F*)g.u, col, g.N, cff, zero, vb, S, nslot, out);
+    blend3<F, NSLOT>((const F*)g.u, col, g.N, cff, zero, vb, S, out);
 }
 
-template <typename F>
+template <typename F, int NSLOT>
 __device__ __forceinline__ void interp_v(const DGrid& g, double X, double Y, const VBracket& vb,
-                                        const SlotSel& S, int nslot, double out[3])
+                                        const SlotSel& S, double out[3])
 {
     int Inode = (int)floor(X);
     const double x = X - Inode;
@@ -176,7 +177,7 @@ __device__ __forceinline__ void interp_v(const DGrid& g, double X, double Y, con
     col[1] = (int64_t)(ja - 1) * g.L + (Inode);
     col[2] = (int64_t)(jb - 1) * g.L + (Inode);
     col[3] = (int64_t)(jb - 1) * g.L + (Inode - 1);
-    blend3<F>((const F*)g.v, col, g.N, cff, zero, vb, S, nslot, out);
+    blend3<F, NSLOT>((const F*)g.v, col, g.N, cff, zero, vb, S, out);
 }
 
 // masked rho-point weights for w and AKs: bilinear, or IDW when a corner is dry
@@ -188,14 +189,14 @@ __device__ __forceinline__ void rho_weights(const DGrid& g, const RhoCell& c, do
     if (m[0] + m[1] + m[2] + m[3] < 4.0) getcoeff(c.x, c.y, m, cff);
 }
 
-template <typename F>
+template <typename F, int NSLOT>
 __device__ __forceinline__ void interp_w(const DGrid& g, const RhoCell& c, const VBracket& vb,
-                                        const SlotSel& S, int nslot, double out[3])
+                                        const SlotSel& S, double out[3])
 {
     double cff[4];
     rho_weights(g, c, cff);
     const bool zero[4] = {false, false, false, false};
-    blend3<F>((const F*)g.w, c.o, g.N + 1, cff, zero, vb, S, nslot, out);
+    blend3<F, NSLOT>((const F*)g.w, c.o, g.N + 1, cff, zero, vb, S, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -218,7 +219,9 @@ __device__ __forceinline__ Surface surface_at(const DGrid& g, const StepArgs& A,
     return s;
 }
 
-template <typename F>
+// NSLOT=2 on the first external step, whose u, v, w use slots (b,b,c)
+// (advection_module.f90:277-334); 3 otherwise.
+template <typename F, int NSLOT>
 __device__ __forceinline__ void find_currents(const DGrid& g, const StepArgs& A, double X,
                                               double Y, double& Z, double& U, double& V,
                                               double& W, Surface* surf_out)
@@ -230,15 +233,13 @@ __device__ __forceinline__ void find_currents(const DGrid& g, const StepArgs& A,
     if (Z > s.zeta) Z = s.zeta - (double)0.001f;
     const VBracket br = bracket_rho(g, Z, s.zeta, s.tdepth);
     const VBracket bw = bracket_w(g, Z, s.zeta, s.tdepth);
-    // ets==1 uses (b,b,c) for u,v,w (advection_module.f90:277-334)
-    const int nslot = A.ets == 1 ? 2 : 3;
     double q[3];
-    interp_u<F>(g, X, Y, br, A.slots, nslot, q);
-    U = A.ets == 1 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
-    interp_v<F>(g, X, Y, br, A.slots, nslot, q);
-    V = A.ets == 1 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
-    interp_w<F>(g, c, bw, A.slots, nslot, q);
-    W = A.ets == 1 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+    interp_u<F, NSLOT>(g, X, Y, br, A.slots, q);
+    U = NSLOT == 2 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+    interp_v<F, NSLOT>(g, X, Y, br, A.slots, q);
+    V = NSLOT == 2 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+    interp_w<F, NSLOT>(g, c, bw, A.slots, q);
+    W = NSLOT == 2 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -471,7 +472,7 @@ __device__ __forceinline__ double vturb(const DGrid& g, const StepArgs& A, const
         const double zc = level_z(g, g.s_w, g.cs_w, k, sf.zc, h, Sk);
         const double zf = level_z(g, g.s_w, g.cs_w, k, sf.zf, h, Sk);
         S(Lo.z + k - 1) = polintd(A.ex, zb, zc, zf, A.ix[1]);
-        double v[4][4];
+        Slots4<F> v[4];
         for (int q = 0; q < 4; q++) ld4(aks + (c.o[q] * nw + (k - 1)) * 4, v[q]);
         double kh[3];
         for (int t = 0; t < 3; t++) {
@@ -601,16 +602,28 @@ __device__ __forceinline__ double vturb(const DGrid& g, const StepArgs& A, const
 //     quantity the secant search needs, with no back substitution;
 //   * only the accepted p gets a full solve, whose u builds the spline
 //     pieces the random-displacement sub-steps visit.
+// Work arrays are read in chunks of kCh rows so their loads overlap.
 // ---------------------------------------------------------------------------
+constexpr int kCh = 8;
+
 struct FastLayout {
-    int np, nw;
-    int qty, y, rd, l1, l2, w, u, z, pkh;
-    __device__ __forceinline__ FastLayout(int np_, int nw_) : np(np_), nw(nw_)
+    int np;
+    int qty, y, rd, l1, l2, w, u;
+    __device__ __forceinline__ FastLayout(int np_) : np(np_)
     {
         qty = 0; y = np; rd = 2 * np; l1 = 3 * np; l2 = 4 * np; w = 5 * np; u = 6 * np;
-        z = 7 * np; pkh = 7 * np + nw;
     }
 };
+
+// 1/d to within an ulp: hardware estimate and two Newton steps
+__device__ __forceinline__ double rcp64(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
 
 // Toeplitz entries of chol1d's matrix (pppack.f90:1509-1512) for uniform h
 struct Band {
@@ -633,29 +646,38 @@ __device__ __forceinline__ Band band_of(double p, double h, double serr)
 }
 
 // G(p) and dG/dp by one LDL' sweep over the n interior unknowns
-__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int oq, double& G, double& Gp)
+__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int oq, double& G,
+                                         double& Gp)
 {
     double d = B.a, dd = B.da, e = B.b, de = B.db;
     double dn = B.a, ddn = B.da, en = B.b, den = B.db;
     double w = S(oq), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
     double g = 0.0, gp = 0.0;
-    for (int m = 0; m < n; m++) {
-        const double rd = 1.0 / d;
-        const double wrd = w * rd;
-        g += w * wrd;
-        gp += (2.0 * w * dw - wrd * w * dd) * rd;
-        if (m == n - 1) break;
-        const double rat = e * rd, drat = (de - rat * dd) * rd;
-        const double rat2 = B.c * rd, drat2 = (B.dc - rat2 * dd) * rd;
-        const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
-        const double e1 = en - rat * B.c, de1 = den - drat * B.c - rat * B.dc;
-        const double d2 = B.a - rat2 * B.c, dd2 = B.da - drat2 * B.c - rat2 * B.dc;
-        const double wn = S(oq + m + 1) - rat * w - l2p * wp;
-        const double dwn = -drat * w - rat * dw - dl2p * wp - l2p * dwp;
-        wp = w; dwp = dw; w = wn; dw = dwn;
-        l2p = rat2; dl2p = drat2;
-        d = d1; dd = dd1; e = e1; de = de1;
-        dn = d2; ddn = dd2; en = B.b; den = B.db;
+    for (int m0 = 0; m0 < n; m0 += kCh) {
+        double qn[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(oq + m0 + k + 1) : 0.0;
+#pragma unroll
+        for (int k = 0; k < kCh; k++) {
+            const int m = m0 + k;
+            if (m < n) {
+                const double rd = rcp64(d);
+                const double wrd = w * rd;
+                g += w * wrd;
+                gp += (2.0 * w * dw - wrd * w * dd) * rd;
+                const double rat = e * rd, drat = (de - rat * dd) * rd;
+                const double rat2 = B.c * rd, drat2 = (B.dc - rat2 * dd) * rd;
+                const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
+                const double e1 = en - rat * B.c, de1 = den - drat * B.c - rat * B.dc;
+                const double d2 = B.a - rat2 * B.c, dd2 = B.da - drat2 * B.c - rat2 * B.dc;
+                const double wn = qn[k] - rat * w - l2p * wp;
+                const double dwn = -drat * w - rat * dw - dl2p * wp - l2p * dwp;
+                wp = w; dwp = dw; w = wn; dw = dwn;
+                l2p = rat2; dl2p = drat2;
+                d = d1; dd = dd1; e = e1; de = de1;
+                dn = d2; ddn = dd2; en = B.b; den = B.db;
+            }
+        }
     }
     G = g;
     Gp = gp;
@@ -667,48 +689,75 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
     const int n = Lo.np - 2;
     double d = B.a, e = B.b, dn = B.a, en = B.b;
     double w = S(Lo.qty), wp = 0.0, l2p = 0.0;
-    for (int m = 0; m < n; m++) {
-        const double rd = 1.0 / d;
-        S(Lo.rd + m) = rd;
-        S(Lo.w + m) = w;
-        if (m == n - 1) break;
-        const double rat = e * rd, rat2 = B.c * rd;
-        S(Lo.l1 + m) = rat;
-        S(Lo.l2 + m) = rat2;
-        const double d1 = dn - rat * e;
-        const double e1 = en - rat * B.c;
-        const double d2 = B.a - rat2 * B.c;
-        const double wn = S(Lo.qty + m + 1) - rat * w - l2p * wp;
-        wp = w; w = wn; l2p = rat2;
-        d = d1; e = e1; dn = d2; en = B.b;
+    for (int m0 = 0; m0 < n; m0 += kCh) {
+        double qn[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(Lo.qty + m0 + k + 1) : 0.0;
+#pragma unroll
+        for (int k = 0; k < kCh; k++) {
+            const int m = m0 + k;
+            if (m < n) {
+                const double rd = rcp64(d);
+                const double rat = e * rd, rat2 = B.c * rd;
+                S(Lo.rd + m) = rd;
+                S(Lo.w + m) = w;
+                S(Lo.l1 + m) = rat;
+                S(Lo.l2 + m) = rat2;
+                const double d1 = dn - rat * e;
+                const double e1 = en - rat * B.c;
+                const double d2 = B.a - rat2 * B.c;
+                const double wn = qn[k] - rat * w - l2p * wp;
+                wp = w; w = wn; l2p = rat2;
+                d = d1; e = e1; dn = d2; en = B.b;
+            }
+        }
     }
     double up1 = 0.0, up2 = 0.0;
-    for (int m = n - 1; m >= 0; m--) {
-        double um = S(Lo.w + m) * S(Lo.rd + m);
-        if (m < n - 1) um = um - up1 * S(Lo.l1 + m) - up2 * S(Lo.l2 + m);
-        S(Lo.u + m + 1) = um;
-        up2 = up1;
-        up1 = um;
+    for (int m1 = n - 1; m1 >= 0; m1 -= kCh) {
+        double wv[kCh], rv[kCh], a1[kCh], a2[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; k++) {
+            const int m = m1 - k;
+            const bool ok = m >= 0;
+            wv[k] = ok ? S(Lo.w + m) : 0.0;
+            rv[k] = ok ? S(Lo.rd + m) : 0.0;
+            a1[k] = ok ? S(Lo.l1 + m) : 0.0;
+            a2[k] = ok ? S(Lo.l2 + m) : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kCh; k++) {
+            const int m = m1 - k;
+            if (m >= 0) {
+                double um = wv[k] * rv[k];
+                if (m < n - 1) um = um - up1 * a1[k] - up2 * a2[k];
+                S(Lo.u + m + 1) = um;
+                up2 = up1;
+                up1 = um;
+            }
+        }
     }
     S(Lo.u + 0) = 0.0;
     S(Lo.u + Lo.np - 1) = 0.0;
 }
 
-__device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr& S, int i, double p, double k2,
-                           double h, double z1, PieceCache& pc)
+__device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr& S, int i, double p,
+                                           double k2, double h, double rh, double z1,
+                                           PieceCache& pc)
 {
     const int np = Lo.np;
-    auto U = [&](int j) { return S(Lo.u + j - 1); };
-    const double rh = 1.0 / h;
-    auto raw = [&](int j) { return (U(j) - U(j - 1)) * rh; };
-    const double r_i = (i >= 2) ? raw(i) : 0.0;
-    const double r_i1 = raw(i + 1);
+    // u(i-1..i+2), y(i..i+1), issued together
+    const double um1 = (i >= 2) ? S(Lo.u + i - 2) : 0.0;
+    const double u0 = S(Lo.u + i - 1), u1 = S(Lo.u + i);
+    const double u2 = (i + 2 <= np) ? S(Lo.u + i + 1) : 0.0;
+    const double y0 = S(Lo.y + i - 1), y1 = S(Lo.y + i);
+    const double r_i = (i >= 2) ? (u0 - um1) * rh : 0.0;
+    const double r_i1 = (u1 - u0) * rh;
     const double qu_i = r_i1 - r_i;
-    const double qu_i1 = (i + 1 == np) ? -r_i1 : raw(i + 2) - r_i1;
-    const double a1 = S(Lo.y + i - 1) - k2 * qu_i;
-    const double a1n = S(Lo.y + i) - k2 * qu_i1;
-    const double a3 = U(i) * 6.0 * p;
-    const double a3n = U(i + 1) * 6.0 * p;
+    const double qu_i1 = (i + 1 == np) ? -r_i1 : (u2 - u1) * rh - r_i1;
+    const double a1 = y0 - k2 * qu_i;
+    const double a1n = y1 - k2 * qu_i1;
+    const double a3 = u0 * 6.0 * p;
+    const double a3n = u1 * 6.0 * p;
     const double a4 = (a3n - a3) * rh;
     const double a2 = (a1n - a1) * rh - (a3 + a4 / 3.0 * h) / 2.0 * h;
     pc.i = i;
@@ -718,77 +767,92 @@ __device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr& S, i
 
 __device__ __forceinline__ double ppvalu_fast(const FastLayout& Lo, const Scr& S, PieceCache& pc,
                                               double x, int jderiv, double p, double k2, double h,
-                                              double z1)
+                                              double rh, double z1)
 {
     const int np = Lo.np;
-    const double gss = (x - z1) / h;
+    const double gss = (x - z1) * rh;
     int i;
     if (!(gss >= 0.0)) i = 1;
     else if (gss >= (double)(np - 1)) i = np - 1;
     else i = (int)gss + 1;
-    if (i != pc.i) piece_fast(Lo, S, i, p, k2, h, z1, pc);
+    if (i != pc.i) piece_fast(Lo, S, i, p, k2, h, rh, z1, pc);
     return horner(pc, x - pc.brk, jderiv);
+}
+
+// AKs at w level k of the particle's column, time-interpolated, and the
+// level's depth (getInterpAKs + getWlevel + polintd, ver_turb_module.f90:100-124)
+template <typename F>
+__device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, const RhoCell& c,
+                                             const double cff[4], const Surface& sf, double hh,
+                                             int k, double& z, double& kh)
+{
+    const int nw = g.N + 1;
+    const F* aks = (const F*)g.aks;
+    Slots4<F> v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) ld4(aks + (c.o[q] * nw + (k - 1)) * 4, v[q]);
+    const double Sk = level_S(g, g.s_w, g.cs_w, k, hh);
+    const double zb = level_z(g, g.s_w, g.cs_w, k, sf.zb, hh, Sk);
+    const double zc = level_z(g, g.s_w, g.cs_w, k, sf.zc, hh, Sk);
+    const double zf = level_z(g, g.s_w, g.cs_w, k, sf.zf, hh, Sk);
+    z = polintd(A.ex, zb, zc, zf, A.ix[1]);
+    double kk[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        const int s = A.slots.s[t];
+        kk[t] = cff[0] * pick(v[0], s) + cff[1] * pick(v[1], s) + cff[2] * pick(v[2], s) +
+                cff[3] * pick(v[3], s);
+    }
+    kh = polintd(A.ex, kk[0], kk[1], kk[2], A.ix[1]);
 }
 
 template <typename F>
 __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, const RhoCell& c,
-                             const Surface& sf, double Zpar, DevSrc& dv, const Scr& S,
-                             unsigned* capped)
+                                             const Surface& sf, double Zpar, DevSrc& dv,
+                                             const Scr& S, unsigned* capped)
 {
     const int nw = g.N + 1;
     const int np = A.np;
-    const FastLayout Lo(np, nw);
+    const FastLayout Lo(np);
     const double tdepth = sf.tdepth, zeta = sf.zeta;
     const double hh = -1.0 * tdepth;
     double cff[4];
     rho_weights(g, c, cff);
-    const F* aks = (const F*)g.aks;
-    for (int k = 1; k <= nw; k++) {
-        const double Sk = level_S(g, g.s_w, g.cs_w, k, hh);
-        const double zb = level_z(g, g.s_w, g.cs_w, k, sf.zb, hh, Sk);
-        const double zc = level_z(g, g.s_w, g.cs_w, k, sf.zc, hh, Sk);
-        const double zf = level_z(g, g.s_w, g.cs_w, k, sf.zf, hh, Sk);
-        S(Lo.z + k - 1) = polintd(A.ex, zb, zc, zf, A.ix[1]);
-        double v[4][4];
-        for (int q = 0; q < 4; q++) ld4(aks + (c.o[q] * nw + (k - 1)) * 4, v[q]);
-        double kh[3];
-        for (int t = 0; t < 3; t++) {
-            const int s = A.slots.s[t];
-            kh[t] = cff[0] * pick(v[0], s) + cff[1] * pick(v[1], s) + cff[2] * pick(v[2], s) +
-                    cff[3] * pick(v[3], s);
-        }
-        S(Lo.pkh + k - 1) = polintd(A.ex, kh[0], kh[1], kh[2], A.ix[1]);
-    }
-    // resample to np uniform points (linint) and form qty = second differences
-    const double z1 = S(Lo.z + 0), zn = S(Lo.z + nw - 1);
+    // the bottom and top levels fix the resampling grid; the column is then
+    // walked once, emitting y (linint) and qty (second differences of y)
+    double z1, k1, zn, kn;
+    column_level<F>(g, A, c, cff, sf, hh, 1, z1, k1);
+    column_level<F>(g, A, c, cff, sf, hh, nw, zn, kn);
     const double h = (zn - z1) / (double)(np - 1);
+    const double rh = 1.0 / h;
     {
-        int jlo = 1;
-        double zlo = z1, zhi = S(Lo.z + 1);
-        double ylo = S(Lo.pkh + 0), yhi = S(Lo.pkh + 1);
-        double m = (ylo - yhi) / (zlo - zhi);
-        double ym1 = 0.0, y0 = ylo;
+        double zlo = z1, klo = k1, zhi, khi;
+        column_level<F>(g, A, c, cff, sf, hh, 2, zhi, khi);
+        int j = 2;                        // current segment is (j-1, j)
+        double m = (klo - khi) / (zlo - zhi);
+        double b = klo - m * zlo;
+        double ym1 = 0.0, y0 = k1;
         S(Lo.y + 0) = y0;
-        const double rh = 1.0 / h;
         for (int i = 2; i <= np; i++) {
             double yi;
             if (i == np) {
-                yi = S(Lo.pkh + nw - 1);
+                yi = kn;
             } else {
                 const double fz = z1 + (double)(i - 1) * h;
-                bool moved = false;
-                while (jlo < nw - 1 && zhi <= fz) {
-                    jlo++;
+                while (j < nw && zhi <= fz) {
                     zlo = zhi;
-                    zhi = S(Lo.z + jlo);
-                    moved = true;
+                    klo = khi;
+                    j++;
+                    if (j == nw) {
+                        zhi = zn;
+                        khi = kn;
+                    } else {
+                        column_level<F>(g, A, c, cff, sf, hh, j, zhi, khi);
+                    }
+                    m = (klo - khi) / (zlo - zhi);
+                    b = klo - m * zlo;
                 }
-                if (moved) {
-                    ylo = S(Lo.pkh + jlo - 1);
-                    yhi = S(Lo.pkh + jlo);
-                    m = (ylo - yhi) / (zlo - zhi);
-                }
-                yi = m * fz + (ylo - m * zlo);
+                yi = m * fz + b;
             }
             S(Lo.y + i - 1) = yi;
             if (i >= 3) S(Lo.qty + i - 3) = ((yi - y0) - (y0 - ym1)) * rh;   // qty(i-1)
@@ -840,14 +904,14 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     for (int i = 1; i <= A.loop; i++) {
         double Kprimec;
         if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
-        else Kprimec = ppvalu_fast(Lo, S, pc, ParZc, 1, p, k2, h, z1);
+        else Kprimec = ppvalu_fast(Lo, S, pc, ParZc, 1, p, k2, h, rh, z1);
         const double KprimeZc = Kprimec * deltat;
         const double Z3rdc = ParZc + 0.5 * KprimeZc;
         double KH3rdc;
         if (Z3rdc < tdepth || Z3rdc > zeta) {
             KH3rdc = A.aksback;
         } else {
-            KH3rdc = ppvalu_fast(Lo, S, pc, Z3rdc, 0, p, k2, h, z1);
+            KH3rdc = ppvalu_fast(Lo, S, pc, Z3rdc, 0, p, k2, h, rh, z1);
             if (KH3rdc < A.aksback) KH3rdc = A.aksback;
         }
         const double DEV = dv.get(1 + i);
@@ -928,152 +992,203 @@ __device__ __forceinline__ void bounds(const DGrid& g, double Xp, double Yp, boo
 
 // ---------------------------------------------------------------------------
 // the step: update_particles (ROMSPath.f90:662-1003), one lane per particle,
-// grid-stride over the particles; lane t uses scratch column t.
+// grid-stride over the particles.  PHASE 0 does everything in one pass (used
+// without vertical turbulence); with it, PHASE 1 (advection + horizontal walk)
+// hands newX, newY, AdvectZ and the clamped z to PHASE 2 (vertical walk,
+// grid hand-off, reflection, status), so each kernel keeps a smaller
+// register footprint.  Lane t of PHASE 2 uses scratch column t.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool skips(const StepArgs& A, int64_t q)
+{
+    if (A.ix[2] <= A.dob[q]) return true;                    // ROMSPath.f90:672
+    if (A.status[q] == ST_UNSET) return true;                // ROMSPath.f90:676
+    if (A.open_boundary && !A.isin[q]) return true;          // ROMSPath.f90:688-690
+    return false;
+}
+
+__device__ __forceinline__ DevSrc dev_of(const StepArgs& A, int64_t q)
+{
+    DevSrc dv;
+    dv.inj = A.dev ? A.dev + A.id[q] * (int64_t)A.ndev : nullptr;
+    dv.seed = A.seed;
+    dv.id = A.id[q];
+    dv.step = A.step_index;
+    dv.blk = -1;
+    return dv;
+}
+
+// RKAdvect (advection_module.f90:20-125) + HTurb (hor_turb_module.f90:30-51,
+// ROMSPath.f90:766-768).  Returns newX/newY (ROMSPath.f90:821-822).
+template <typename F, int NSLOT>
+__device__ __forceinline__ void advect(const DGrid& g, const StepArgs& A, DevSrc& dv, double X0,
+                                       double Y0, double& Zpar, double& newX, double& newY,
+                                       double& AdvectZ)
+{
+    const RhoCell c0 = rho_cell(g, X0, Y0);
+    const double pm = bilinear2d(g.pm, c0);
+    const double pn = bilinear2d(g.pn, c0);
+    double AdvectX = 0.0, AdvectY = 0.0, TurbHx = 0.0, TurbHy = 0.0;
+    AdvectZ = 0.0;
+    if (A.scheme == 1) {
+        // every stage starts from the step's start point with half a step (idt/2)
+        const double idt = (double)A.idt;
+        double Uad, Vad, Wad;
+        find_currents<F, NSLOT>(g, A, X0, Y0, Zpar, Uad, Vad, Wad, nullptr);
+        const double k1u = Uad * pm, k1v = Vad * pn, k1w = Wad;
+        double x1 = X0 + Uad * pm * idt / 2.0;
+        double y1 = Y0 + Vad * pn * idt / 2.0;
+        double z1 = Zpar + Wad * idt / 2.0;
+        find_currents<F, NSLOT>(g, A, x1, y1, z1, Uad, Vad, Wad, nullptr);
+        const double k2u = Uad * pm, k2v = Vad * pn, k2w = Wad;
+        double x2 = X0 + Uad * pm * idt / 2.0;
+        double y2 = Y0 + Vad * pn * idt / 2.0;
+        double z2 = Zpar + Wad * idt / 2.0;
+        find_currents<F, NSLOT>(g, A, x2, y2, z2, Uad, Vad, Wad, nullptr);
+        const double k3u = Uad * pm, k3v = Vad * pn, k3w = Wad;
+        double x3 = X0 + Uad * pm * idt / 2.0;
+        double y3 = Y0 + Vad * pn * idt / 2.0;
+        double z3 = Zpar + Wad * idt / 2.0;
+        find_currents<F, NSLOT>(g, A, x3, y3, z3, Uad, Vad, Wad, nullptr);
+        const double k4u = Uad * pm, k4v = Vad * pn, k4w = Wad;
+        const double PU = (k1u + 2.0 * k2u + 2.0 * k3u + k4u) / 6.0;
+        const double PV = (k1v + 2.0 * k2v + 2.0 * k3v + k4v) / 6.0;
+        const double PW = (k1w + 2.0 * k2w + 2.0 * k3w + k4w) / 6.0;
+        AdvectX = idt * PU;
+        AdvectY = idt * PV;
+        AdvectZ = idt * PW;
+    }
+    if (A.hturb_on) {
+        const double KM = g.kh, r = 1.0;
+        TurbHx = dv.get(0) * sqrt(2.0 / r * KM * (double)A.idt);
+        TurbHy = dv.get(1) * sqrt(2.0 / r * KM * (double)A.idt);
+    }
+    TurbHx = TurbHx * pm;
+    TurbHy = TurbHy * pn;
+    newX = X0 + AdvectX + TurbHx;
+    newY = Y0 + AdvectY + TurbHy;
+}
+
+// grid hand-off, new-position depth and zeta, vertical reflection and status
+// (ROMSPath.f90:828-892)
 template <typename F, bool NESTED>
+__device__ __forceinline__ void finish(const DGrid* __restrict__ grids, const StepArgs& A,
+                                       int64_t q, int gid, double Z0, double newX, double newY,
+                                       double AdvectZ, double TurbV)
+{
+    const double Zbehav = 0.0;
+    double tempX = newX, tempY = newY;
+    bool ingrid = false, obound = false;
+    const int ngrid = NESTED ? A.ngrid : 1;
+    for (int i = 1; i <= ngrid; i++) {
+        const DGrid& gc = grids[NESTED ? gid - 1 : 0];
+        tempX = gc.scl[i - 1][0] * newX + gc.off[i - 1][0];
+        tempY = gc.scl[i - 1][1] * newY + gc.off[i - 1][1];
+        bounds(grids[NESTED ? i - 1 : 0], tempX, tempY, ingrid, obound);
+        if (ingrid) {
+            gid = i;
+            break;
+        }
+    }
+    const DGrid& gn = grids[NESTED ? gid - 1 : 0];
+    const RhoCell cn = rho_cell(gn, tempX, tempY);
+    const Surface sn = surface_at<F>(gn, A, cn);
+    const double tdepth = sn.tdepth, Pzeta = sn.zeta;
+    if (A.want_diag) {
+        A.wd[q] = (-1.0 * tdepth) + Pzeta;
+        A.zeta[q] = Pzeta;
+        A.bath[q] = -1.0 * tdepth;
+    }
+    bool hbot = false, htop = false;
+    double newZ;
+    if (A.nsb == 1) {
+        newZ = Pzeta - A.vertdist;
+    } else if (A.nsb == 2) {
+        newZ = tdepth + A.vertdist;
+    } else {
+        newZ = Z0 + AdvectZ + TurbV + Zbehav;
+        if (newZ < tdepth) {
+            newZ = tdepth + fabs(newZ - tdepth);
+            hbot = true;
+        }
+        if (newZ > Pzeta) {
+            newZ = Pzeta - fabs(newZ - Pzeta);
+            htop = true;
+        }
+    }
+    int8_t st;
+    if (ingrid) {
+        A.isin[q] = 1;
+        A.x[q] = tempX;
+        A.y[q] = tempY;
+        A.z[q] = newZ;
+        st = hbot ? ST_BOT : (htop ? ST_TOP : ST_MOVED);
+    } else if (obound) {
+        A.isin[q] = 0;
+        gid = A.ngrid;
+        st = ST_EXIT;
+    } else {
+        A.isin[q] = 1;
+        st = ST_LAND;
+    }
+    A.status[q] = st;
+    A.gid[q] = (int8_t)gid;
+}
+
+template <typename F, bool NESTED, int PHASE>
 __global__ void __launch_bounds__(256) step_kernel(const DGrid* __restrict__ grids, StepArgs A)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
     unsigned long long nact = 0;
     for (int64_t q = tid; q < A.n; q += T) {
-        if (A.ix[2] <= A.dob[q]) continue;
-        int8_t st = A.status[q];
-        if (st == ST_UNSET) continue;
-        if (A.open_boundary && !A.isin[q]) continue;
+        if (skips(A, q)) continue;
         nact++;
-        A.age[q] = A.age[q] + (double)(float)A.idt;
         const double X0 = A.x[q], Y0 = A.y[q], Z0 = A.z[q];
-        int gid = NESTED ? (int)A.gid[q] : 1;
+        const int gid = NESTED ? (int)A.gid[q] : 1;
         const DGrid& g = grids[NESTED ? gid - 1 : 0];
-        const RhoCell c0 = rho_cell(g, X0, Y0);
-        const double pm = bilinear2d(g.pm, c0);
-        const double pn = bilinear2d(g.pn, c0);
-        double Zpar = Z0;
-        double AdvectX = 0.0, AdvectY = 0.0, AdvectZ = 0.0;
-        double TurbHx = 0.0, TurbHy = 0.0, TurbV = 0.0;
-        Surface s0;
-        bool have_s0 = false;
-        if (A.scheme == 1) {
-            // RKAdvect (advection_module.f90:20-125): every stage starts from
-            // the step's start point with half a step (idt/2).
-            const double idt = (double)A.idt;
-            double Uad, Vad, Wad;
-            find_currents<F>(g, A, X0, Y0, Zpar, Uad, Vad, Wad, &s0);
-            have_s0 = true;
-            const double k1u = Uad * pm, k1v = Vad * pn, k1w = Wad;
-            double x1 = X0 + Uad * pm * idt / 2.0;
-            double y1 = Y0 + Vad * pn * idt / 2.0;
-            double z1 = Zpar + Wad * idt / 2.0;
-            find_currents<F>(g, A, x1, y1, z1, Uad, Vad, Wad, nullptr);
-            const double k2u = Uad * pm, k2v = Vad * pn, k2w = Wad;
-            double x2 = X0 + Uad * pm * idt / 2.0;
-            double y2 = Y0 + Vad * pn * idt / 2.0;
-            double z2 = Zpar + Wad * idt / 2.0;
-            find_currents<F>(g, A, x2, y2, z2, Uad, Vad, Wad, nullptr);
-            const double k3u = Uad * pm, k3v = Vad * pn, k3w = Wad;
-            double x3 = X0 + Uad * pm * idt / 2.0;
-            double y3 = Y0 + Vad * pn * idt / 2.0;
-            double z3 = Zpar + Wad * idt / 2.0;
-            find_currents<F>(g, A, x3, y3, z3, Uad, Vad, Wad, nullptr);
-            const double k4u = Uad * pm, k4v = Vad * pn, k4w = Wad;
-            const double PU = (k1u + 2.0 * k2u + 2.0 * k3u + k4u) / 6.0;
-            const double PV = (k1v + 2.0 * k2v + 2.0 * k3v + k4v) / 6.0;
-            const double PW = (k1w + 2.0 * k2w + 2.0 * k3w + k4w) / 6.0;
-            AdvectX = idt * PU;
-            AdvectY = idt * PV;
-            AdvectZ = idt * PW;
+        DevSrc dv = dev_of(A, q);
+        double Zpar = Z0, newX, newY, AdvectZ, TurbV = 0.0;
+        if (PHASE != 2) {
+            A.age[q] = A.age[q] + (double)(float)A.idt;
+            if (A.ets == 1) advect<F, 2>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
+            else advect<F, 3>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
+            if (PHASE == 1) {
+                A.mid[q] = newX;
+                A.mid[A.n + q] = newY;
+                A.mid[2 * A.n + q] = AdvectZ;
+                A.mid[3 * A.n + q] = Zpar;
+                continue;
+            }
+        } else {
+            newX = A.mid[q];
+            newY = A.mid[A.n + q];
+            AdvectZ = A.mid[2 * A.n + q];
+            Zpar = A.mid[3 * A.n + q];
         }
-        DevSrc dv;
-        dv.inj = A.dev ? A.dev + A.id[q] * (int64_t)A.ndev : nullptr;
-        dv.seed = A.seed;
-        dv.id = A.id[q];
-        dv.step = A.step_index;
-        dv.blk = -1;
-        if (A.hturb_on) {
-            // hor_turb_module.f90:30-51
-            const double KM = g.kh, r = 1.0;
-            TurbHx = dv.get(0) * sqrt(2.0 / r * KM * (double)A.idt);
-            TurbHy = dv.get(1) * sqrt(2.0 / r * KM * (double)A.idt);
-        }
-        TurbHx = TurbHx * pm;
-        TurbHy = TurbHy * pn;
         if (A.vturb_on) {
-            if (!have_s0) s0 = surface_at<F>(g, A, c0);
+            const RhoCell c0 = rho_cell(g, X0, Y0);
+            const Surface s0 = surface_at<F>(g, A, c0);
             const Scr S{A.scratch + tid, T};
             if (sizeof(F) == 4)
                 TurbV = vturb_fast<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
             else
                 TurbV = vturb<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
         }
-        const double Zbehav = 0.0;
-        const double newX = X0 + AdvectX + TurbHx;
-        const double newY = Y0 + AdvectY + TurbHy;
-        double tempX = newX, tempY = newY;
-        bool ingrid = false, obound = false;
-        const int ngrid = NESTED ? A.ngrid : 1;
-        for (int i = 1; i <= ngrid; i++) {
-            const DGrid& gc = grids[NESTED ? gid - 1 : 0];
-            tempX = gc.scl[i - 1][0] * newX + gc.off[i - 1][0];
-            tempY = gc.scl[i - 1][1] * newY + gc.off[i - 1][1];
-            bounds(grids[i - 1], tempX, tempY, ingrid, obound);
-            if (ingrid) {
-                gid = i;
-                break;
-            }
-        }
-        const DGrid& gn = grids[NESTED ? gid - 1 : 0];
-        const RhoCell cn = rho_cell(gn, tempX, tempY);
-        const Surface sn = surface_at<F>(gn, A, cn);
-        const double tdepth = sn.tdepth, Pzeta = sn.zeta;
-        if (A.want_diag) {
-            A.wd[q] = (-1.0 * tdepth) + Pzeta;
-            A.zeta[q] = Pzeta;
-            A.bath[q] = -1.0 * tdepth;
-        }
-        bool hbot = false, htop = false;
-        double newZ;
-        if (A.nsb == 1) {
-            newZ = Pzeta - A.vertdist;
-        } else if (A.nsb == 2) {
-            newZ = tdepth + A.vertdist;
-        } else {
-            newZ = Z0 + AdvectZ + TurbV + Zbehav;
-            if (newZ < tdepth) {
-                newZ = tdepth + fabs(newZ - tdepth);
-                hbot = true;
-            }
-            if (newZ > Pzeta) {
-                newZ = Pzeta - fabs(newZ - Pzeta);
-                htop = true;
-            }
-        }
-        if (ingrid) {
-            A.isin[q] = 1;
-            A.x[q] = tempX;
-            A.y[q] = tempY;
-            A.z[q] = newZ;
-            st = hbot ? ST_BOT : (htop ? ST_TOP : ST_MOVED);
-        } else if (obound) {
-            A.isin[q] = 0;
-            gid = A.ngrid;
-            st = ST_EXIT;
-        } else {
-            A.isin[q] = 1;
-            st = ST_LAND;
-        }
-        A.status[q] = st;
-        A.gid[q] = (int8_t)gid;
+        finish<F, NESTED>(grids, A, q, gid, Z0, newX, newY, AdvectZ, TurbV);
     }
-    // active-particle count: wave reduction, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) nact += __shfl_down(nact, off, 64);
-    if ((threadIdx.x & 63) == 0 && nact) atomicAdd(A.active, nact);
+    if (PHASE != 1) {
+        // active-particle count: wave reduction, one atomic per wave
+        for (int off = 32; off > 0; off >>= 1) nact += __shfl_down(nact, off, 64);
+        if ((threadIdx.x & 63) == 0 && nact) atomicAdd(A.active, nact);
+    }
 }
 
-template __global__ void step_kernel<float, false>(const DGrid*, StepArgs);
-template __global__ void step_kernel<double, false>(const DGrid*, StepArgs);
-template __global__ void step_kernel<float, true>(const DGrid*, StepArgs);
-template __global__ void step_kernel<double, true>(const DGrid*, StepArgs);
+#define RP_INST(F, N, P) template __global__ void step_kernel<F, N, P>(const DGrid*, StepArgs);
+RP_INST(float, false, 0) RP_INST(float, false, 1) RP_INST(float, false, 2)
+RP_INST(float, true, 0) RP_INST(float, true, 1) RP_INST(float, true, 2)
+RP_INST(double, false, 0) RP_INST(double, false, 1) RP_INST(double, false, 2)
+RP_INST(double, true, 0) RP_INST(double, true, 1) RP_INST(double, true, 2)
+#undef RP_INST
 
 // ---------------------------------------------------------------------------
 // upload: Fortran (nx, ny, nk) FP64 column-major -> slot-interleaved layout
