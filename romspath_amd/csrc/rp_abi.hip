@@ -398,6 +398,13 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
         A.ix[k] = ix[k];
     }
     A.ets = ets;
+    {
+        // Lagrange weights of the quadratic through (ex, .) at ix(2)
+        const double x = ix[1], x0 = ex[0], x1 = ex[1], x2 = ex[2];
+        A.pw[0] = (x - x1) * (x - x2) / ((x0 - x1) * (x0 - x2));
+        A.pw[1] = (x - x0) * (x - x2) / ((x1 - x0) * (x1 - x2));
+        A.pw[2] = (x - x0) * (x - x1) / ((x2 - x0) * (x2 - x1));
+    }
     A.slots.s[0] = t_b - 1;
     A.slots.s[1] = t_c - 1;
     A.slots.s[2] = t_f - 1;

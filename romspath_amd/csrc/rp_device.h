@@ -146,16 +146,36 @@ __device__ __forceinline__ double level(const DGrid& g, const double* s, const d
     return level_z(g, s, cs, k, zeta, h, level_S(g, s, cs, k, h));
 }
 
+// Level z(k) for one column: the FP32-field form multiplies by 1/(hc+h)
+// computed once instead of dividing at every level.
+template <bool FAST>
+struct Levels {
+    const DGrid& g;
+    const double *s, *cs;
+    double zeta, h, inv;
+    __device__ __forceinline__ Levels(const DGrid& g_, const double* s_, const double* cs_,
+                                      double zeta_, double h_)
+        : g(g_), s(s_), cs(cs_), zeta(zeta_), h(h_)
+    {
+        inv = (FAST && g.vtransform == 2) ? 1.0 / (g.hc + h) : 0.0;
+    }
+    __device__ __forceinline__ double operator()(int k) const
+    {
+        if (FAST && g.vtransform == 2) return zeta + (zeta + h) * ((g.hc * s[k - 1] + h * cs[k - 1]) * inv);
+        return level(g, s, cs, k, zeta, h);
+    }
+};
+
 // First k in [1, nz] with level(k) > Z (nz+1 if none).  The reference scans
 // linearly (interpolation_module.f90:533-577); the levels are non-decreasing
 // in k, so the predicate is monotone and bisection returns the same index.
-__device__ __forceinline__ int first_above(const DGrid& g, const double* s, const double* cs,
-                                           int nz, double Z, double zeta, double h)
+template <class Lev>
+__device__ __forceinline__ int first_above(const Lev& lev, int nz, double Z)
 {
     int lo = 1, hi = nz + 1;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (level(g, s, cs, mid, zeta, h) > Z) hi = mid;
+        if (lev(mid) > Z) hi = mid;
         else lo = mid + 1;
     }
     return lo;
@@ -167,30 +187,34 @@ struct VBracket {
     double c0, c1;
 };
 
+template <bool FAST>
 __device__ __forceinline__ VBracket bracket_rho(const DGrid& g, double Z, double zeta, double depth)
 {
     const double h = -1.0 * depth;
     const int nz = g.N;
-    const int i = first_above(g, g.s_rho, g.cs_r, nz, Z, zeta, h);
+    const Levels<FAST> lev(g, g.s_rho, g.cs_r, zeta, h);
+    const int i = first_above(lev, nz, Z);
     VBracket b;
     if (i > nz) {
         b.ibot = b.itop = nz; b.c0 = 0.5; b.c1 = 0.5;
     } else if (i == 1) {
         b.ibot = b.itop = 1; b.c0 = 0.5; b.c1 = 0.5;
     } else {
-        const double z1 = level(g, g.s_rho, g.cs_r, i - 1, zeta, h);
-        const double z2 = level(g, g.s_rho, g.cs_r, i, zeta, h);
+        const double z1 = lev(i - 1);
+        const double z2 = lev(i);
         const double Zr = (Z - z1) / (z2 - z1);
         b.ibot = i - 1; b.itop = i; b.c0 = 1.0 - Zr; b.c1 = Zr;
     }
     return b;
 }
 
+template <bool FAST>
 __device__ __forceinline__ VBracket bracket_w(const DGrid& g, double Z, double zeta, double depth)
 {
     const double h = -1.0 * depth;
     const int nz = g.N + 1;
-    const int i = first_above(g, g.s_w, g.cs_w, nz, Z, zeta, h);
+    const Levels<FAST> lev(g, g.s_w, g.cs_w, zeta, h);
+    const int i = first_above(lev, nz, Z);
     VBracket b;
     if (i > nz) {
         // z at or above the top w level: two copies of level nz (the
@@ -198,8 +222,8 @@ __device__ __forceinline__ VBracket bracket_w(const DGrid& g, double Z, double z
         b.ibot = b.itop = nz; b.c0 = 0.5; b.c1 = 0.5;
     } else {
         const int ib = i > 1 ? i - 1 : 1;
-        const double z1 = level(g, g.s_w, g.cs_w, ib, zeta, h);
-        const double z2 = level(g, g.s_w, g.cs_w, i, zeta, h);
+        const double z1 = lev(ib);
+        const double z2 = lev(i);
         const double Zr = (Z - z1) / (z2 - z1);
         b.ibot = ib; b.itop = i; b.c0 = 1.0 - Zr; b.c1 = Zr;
     }

@@ -17,6 +17,7 @@ struct SlotSel {
 
 struct StepArgs {
     double ex[3], ix[3];
+    double pw[3];                // Lagrange weights of polintd(ex, ., ix(2))
     int ets;
     SlotSel slots;
     int idt;

@@ -5,6 +5,16 @@
 
 namespace rp {
 
+// polintd(ex, y, ix(2)) (interpolation_module.f90:72-109).  The FP64 form
+// evaluates the reference's Newton formula; the FP32-field form uses the
+// Lagrange weights of the same quadratic, computed once per launch.
+template <typename F>
+__device__ __forceinline__ double tinterp(const StepArgs& A, double y0, double y1, double y2)
+{
+    if (sizeof(F) == 8) return polintd(A.ex, y0, y1, y2, A.ix[1]);
+    return A.pw[0] * y0 + A.pw[1] * y1 + A.pw[2] * y2;
+}
+
 // ---------------------------------------------------------------------------
 // horizontal stencil of a rho-point field: floor cell, bilinear weights, the
 // four corner offsets (I,J) (I+1,J) (I+1,J+1) (I,J+1) in an (L, M) array.
@@ -215,7 +225,7 @@ __device__ __forceinline__ Surface surface_at(const DGrid& g, const StepArgs& A,
     double z3[3];
     zeta3<F>(g, c, A.slots, z3);
     s.zb = z3[0]; s.zc = z3[1]; s.zf = z3[2];
-    s.zeta = polintd(A.ex, s.zb, s.zc, s.zf, A.ix[1]);
+    s.zeta = tinterp<F>(A, s.zb, s.zc, s.zf);
     return s;
 }
 
@@ -231,15 +241,15 @@ __device__ __forceinline__ void find_currents(const DGrid& g, const StepArgs& A,
     if (surf_out) *surf_out = s;
     if (Z < s.tdepth) Z = s.tdepth + (double)0.001f;
     if (Z > s.zeta) Z = s.zeta - (double)0.001f;
-    const VBracket br = bracket_rho(g, Z, s.zeta, s.tdepth);
-    const VBracket bw = bracket_w(g, Z, s.zeta, s.tdepth);
+    const VBracket br = bracket_rho<sizeof(F) == 4>(g, Z, s.zeta, s.tdepth);
+    const VBracket bw = bracket_w<sizeof(F) == 4>(g, Z, s.zeta, s.tdepth);
     double q[3];
     interp_u<F, NSLOT>(g, X, Y, br, A.slots, q);
-    U = NSLOT == 2 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+    U = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
     interp_v<F, NSLOT>(g, X, Y, br, A.slots, q);
-    V = NSLOT == 2 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+    V = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
     interp_w<F, NSLOT>(g, c, bw, A.slots, q);
-    W = NSLOT == 2 ? polintd(A.ex, q[0], q[0], q[1], A.ix[1]) : polintd(A.ex, q[0], q[1], q[2], A.ix[1]);
+    W = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -649,6 +659,7 @@ __device__ __forceinline__ Band band_of(double p, double h, double serr)
 __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int oq, double& G,
                                          double& Gp)
 {
+#pragma clang fp contract(fast)
     double d = B.a, dd = B.da, e = B.b, de = B.db;
     double dn = B.a, ddn = B.da, en = B.b, den = B.db;
     double w = S(oq), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
@@ -686,6 +697,7 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int
 // full solve at the accepted p: u(2..np-1) into Lo.u (u(1)=u(np)=0)
 __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr& S)
 {
+#pragma clang fp contract(fast)
     const int n = Lo.np - 2;
     double d = B.a, e = B.b, dn = B.a, en = B.b;
     double w = S(Lo.qty), wp = 0.0, l2p = 0.0;
@@ -744,6 +756,7 @@ __device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr& S, i
                                            double k2, double h, double rh, double z1,
                                            PieceCache& pc)
 {
+#pragma clang fp contract(fast)
     const int np = Lo.np;
     // u(i-1..i+2), y(i..i+1), issued together
     const double um1 = (i >= 2) ? S(Lo.u + i - 2) : 0.0;
@@ -786,6 +799,7 @@ __device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, 
                                              const double cff[4], const Surface& sf, double hh,
                                              int k, double& z, double& kh)
 {
+#pragma clang fp contract(fast)
     const int nw = g.N + 1;
     const F* aks = (const F*)g.aks;
     Slots4<F> v[4];
@@ -795,7 +809,7 @@ __device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, 
     const double zb = level_z(g, g.s_w, g.cs_w, k, sf.zb, hh, Sk);
     const double zc = level_z(g, g.s_w, g.cs_w, k, sf.zc, hh, Sk);
     const double zf = level_z(g, g.s_w, g.cs_w, k, sf.zf, hh, Sk);
-    z = polintd(A.ex, zb, zc, zf, A.ix[1]);
+    z = tinterp<F>(A, zb, zc, zf);
     double kk[3];
 #pragma unroll
     for (int t = 0; t < 3; t++) {
@@ -803,7 +817,7 @@ __device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, 
         kk[t] = cff[0] * pick(v[0], s) + cff[1] * pick(v[1], s) + cff[2] * pick(v[2], s) +
                 cff[3] * pick(v[3], s);
     }
-    kh = polintd(A.ex, kk[0], kk[1], kk[2], A.ix[1]);
+    kh = tinterp<F>(A, kk[0], kk[1], kk[2]);
 }
 
 template <typename F>
@@ -811,6 +825,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
                                              const Surface& sf, double Zpar, DevSrc& dv,
                                              const Scr& S, unsigned* capped)
 {
+#pragma clang fp contract(fast)
     const int nw = g.N + 1;
     const int np = A.np;
     const FastLayout Lo(np);
