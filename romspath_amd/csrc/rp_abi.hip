@@ -494,10 +494,10 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     const dim3 bd(block);
     using KFn = void (*)(const DGrid*, StepArgs);
     auto pick = [&](int phase) -> KFn {
-        if (f32 && !nested) return phase == 0 ? step_kernel<float, false, 0> : phase == 1 ? step_kernel<float, false, 1> : step_kernel<float, false, 2>;
-        if (f32) return phase == 0 ? step_kernel<float, true, 0> : phase == 1 ? step_kernel<float, true, 1> : step_kernel<float, true, 2>;
-        if (!nested) return phase == 0 ? step_kernel<double, false, 0> : phase == 1 ? step_kernel<double, false, 1> : step_kernel<double, false, 2>;
-        return phase == 0 ? step_kernel<double, true, 0> : phase == 1 ? step_kernel<double, true, 1> : step_kernel<double, true, 2>;
+        if (f32 && !nested) return phase == 0 ? step_kernel<float, false> : phase == 1 ? advect_kernel<float, false> : vwalk_kernel<float, false>;
+        if (f32) return phase == 0 ? step_kernel<float, true> : phase == 1 ? advect_kernel<float, true> : vwalk_kernel<float, true>;
+        if (!nested) return phase == 0 ? step_kernel<double, false> : phase == 1 ? advect_kernel<double, false> : vwalk_kernel<double, false>;
+        return phase == 0 ? step_kernel<double, true> : phase == 1 ? advect_kernel<double, true> : vwalk_kernel<double, true>;
     };
     if (!vt) {
         hipLaunchKernelGGL(pick(0), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);

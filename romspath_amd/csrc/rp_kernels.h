@@ -46,9 +46,14 @@ struct StepArgs {
 };
 
 // NESTED=false: one grid, all lanes read grid 0 (uniform, scalar loads).
-// PHASE 0: whole step; 1: advection + horizontal walk; 2: the rest.
-template <typename F, bool NESTED, int PHASE>
+// step_kernel: the whole step (no vertical walk); with the vertical walk,
+// advect_kernel (advection + horizontal walk) then vwalk_kernel (the rest).
+template <typename F, bool NESTED>
 __global__ void step_kernel(const DGrid* __restrict__ grids, StepArgs A);
+template <typename F, bool NESTED>
+__global__ void advect_kernel(const DGrid* __restrict__ grids, StepArgs A);
+template <typename F, bool NESTED>
+__global__ void vwalk_kernel(const DGrid* __restrict__ grids, StepArgs A);
 
 template <typename F>
 __global__ void pack_field(const double* __restrict__ src, const double* __restrict__ add, F* dst,

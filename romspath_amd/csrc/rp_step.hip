@@ -834,45 +834,53 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     double cff[4];
     rho_weights(g, c, cff);
     // the bottom and top levels fix the resampling grid; the column is then
-    // walked once, emitting y (linint) and qty (second differences of y)
+    // walked once, kLv levels per batch of loads, emitting y (linint: the
+    // point fz uses the segment of the largest level j<=nw-1 with z(j)<=fz)
+    // and qty (second differences of y, setupq's qty for uniform spacing)
     double z1, k1, zn, kn;
     column_level<F>(g, A, c, cff, sf, hh, 1, z1, k1);
     column_level<F>(g, A, c, cff, sf, hh, nw, zn, kn);
     const double h = (zn - z1) / (double)(np - 1);
     const double rh = 1.0 / h;
     {
-        double zlo = z1, klo = k1, zhi, khi;
-        column_level<F>(g, A, c, cff, sf, hh, 2, zhi, khi);
-        int j = 2;                        // current segment is (j-1, j)
-        double m = (klo - khi) / (zlo - zhi);
-        double b = klo - m * zlo;
+        constexpr int kLv = 4;
+        double zlo = z1, klo = k1;
         double ym1 = 0.0, y0 = k1;
         S(Lo.y + 0) = y0;
-        for (int i = 2; i <= np; i++) {
-            double yi;
-            if (i == np) {
-                yi = kn;
-            } else {
-                const double fz = z1 + (double)(i - 1) * h;
-                while (j < nw && zhi <= fz) {
+        int i = 2;                                   // next point to emit
+        for (int kb = 2; kb <= nw; kb += kLv) {
+            double zk[kLv], kk[kLv];
+#pragma unroll
+            for (int t = 0; t < kLv; t++) {
+                const int k = kb + t;
+                if (k < nw) column_level<F>(g, A, c, cff, sf, hh, k, zk[t], kk[t]);
+                else { zk[t] = zn; kk[t] = kn; }
+            }
+#pragma unroll
+            for (int t = 0; t < kLv; t++) {
+                const int k = kb + t;
+                if (k <= nw) {
+                    const double zhi = zk[t], khi = kk[t];
+                    const double m = (klo - khi) / (zlo - zhi);
+                    const double b = klo - m * zlo;
+                    for (; i <= np; i++) {
+                        double yi;
+                        if (i == np) {
+                            yi = kn;
+                        } else {
+                            const double fz = z1 + (double)(i - 1) * h;
+                            if (k < nw && !(fz < zhi)) break;
+                            yi = m * fz + b;
+                        }
+                        S(Lo.y + i - 1) = yi;
+                        S(Lo.qty + i - 3) = ((yi - y0) - (y0 - ym1)) * rh;   // qty(i-1)
+                        ym1 = y0;
+                        y0 = yi;
+                    }
                     zlo = zhi;
                     klo = khi;
-                    j++;
-                    if (j == nw) {
-                        zhi = zn;
-                        khi = kn;
-                    } else {
-                        column_level<F>(g, A, c, cff, sf, hh, j, zhi, khi);
-                    }
-                    m = (klo - khi) / (zlo - zhi);
-                    b = klo - m * zlo;
                 }
-                yi = m * fz + b;
             }
-            S(Lo.y + i - 1) = yi;
-            if (i >= 3) S(Lo.qty + i - 3) = ((yi - y0) - (y0 - ym1)) * rh;   // qty(i-1)
-            ym1 = y0;
-            y0 = yi;
         }
     }
     // secant search for p (pppack.f90:5236-5298), sfq from G and G'
@@ -1150,7 +1158,7 @@ __device__ __forceinline__ void finish(const DGrid* __restrict__ grids, const St
 }
 
 template <typename F, bool NESTED, int PHASE>
-__global__ void __launch_bounds__(256) step_kernel(const DGrid* __restrict__ grids, StepArgs A)
+__device__ __forceinline__ void step_body(const DGrid* __restrict__ grids, const StepArgs& A)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
@@ -1198,11 +1206,45 @@ __global__ void __launch_bounds__(256) step_kernel(const DGrid* __restrict__ gri
     }
 }
 
-#define RP_INST(F, N, P) template __global__ void step_kernel<F, N, P>(const DGrid*, StepArgs);
-RP_INST(float, false, 0) RP_INST(float, false, 1) RP_INST(float, false, 2)
-RP_INST(float, true, 0) RP_INST(float, true, 1) RP_INST(float, true, 2)
-RP_INST(double, false, 0) RP_INST(double, false, 1) RP_INST(double, false, 2)
-RP_INST(double, true, 0) RP_INST(double, true, 1) RP_INST(double, true, 2)
+// Occupancy targets (waves per SIMD) of the three step kernels; 0 leaves the
+// choice to the compiler.
+#ifndef RP_WAVES_ALL
+#define RP_WAVES_ALL 0
+#endif
+#ifndef RP_WAVES_ADVECT
+#define RP_WAVES_ADVECT 0
+#endif
+#ifndef RP_WAVES_VWALK
+#define RP_WAVES_VWALK 0
+#endif
+#define RP_WAVES(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
+
+template <typename F, bool NESTED>
+__global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ALL)
+    step_kernel(const DGrid* __restrict__ grids, StepArgs A)
+{
+    step_body<F, NESTED, 0>(grids, A);
+}
+
+template <typename F, bool NESTED>
+__global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ADVECT)
+    advect_kernel(const DGrid* __restrict__ grids, StepArgs A)
+{
+    step_body<F, NESTED, 1>(grids, A);
+}
+
+template <typename F, bool NESTED>
+__global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_VWALK)
+    vwalk_kernel(const DGrid* __restrict__ grids, StepArgs A)
+{
+    step_body<F, NESTED, 2>(grids, A);
+}
+
+#define RP_INST(F, N)                                                              \
+    template __global__ void step_kernel<F, N>(const DGrid*, StepArgs);            \
+    template __global__ void advect_kernel<F, N>(const DGrid*, StepArgs);          \
+    template __global__ void vwalk_kernel<F, N>(const DGrid*, StepArgs);
+RP_INST(float, false) RP_INST(float, true) RP_INST(double, false) RP_INST(double, true)
 #undef RP_INST
 
 // ---------------------------------------------------------------------------
