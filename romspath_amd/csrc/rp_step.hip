@@ -873,7 +873,8 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
                             yi = m * fz + b;
                         }
                         S(Lo.y + i - 1) = yi;
-                        S(Lo.qty + i - 3) = ((yi - y0) - (y0 - ym1)) * rh;   // qty(i-1)
+                        // qty(i-1) needs y(i-2): defined for i >= 3 (rows 0..np-3)
+                        if (i >= 3) S(Lo.qty + i - 3) = ((yi - y0) - (y0 - ym1)) * rh;
                         ym1 = y0;
                         y0 = yi;
                     }
