@@ -58,7 +58,9 @@ struct Ctx {
     DGrid tab_host[kMaxGrid];
     // counters
     unsigned long long* active = nullptr;
+    unsigned long long* work = nullptr;
     unsigned* capped = nullptr;
+    int vwalk_blocks_per_cu[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // [form][nested], 0 = not queried
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
     std::string err;
 };
@@ -214,7 +216,8 @@ int rp_create(void** out, const rp_config* cfg)
         delete c;
         return RP_E_HIP;
     }
-    if (dalloc(c, &c->active, 1) || dalloc(c, &c->capped, 1) || dalloc(c, &c->dgrids, kMaxGrid)) {
+    if (dalloc(c, &c->active, 1) || dalloc(c, &c->capped, 1) || dalloc(c, &c->dgrids, kMaxGrid) ||
+        dalloc(c, &c->work, 1)) {
         rp_destroy(c);
         return RP_E_ALLOC;
     }
@@ -433,6 +436,7 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     A.dev = c->use_dev ? c->dev : nullptr;
     A.ndev = c->ndev;
     A.active = c->active;
+    A.work = c->work;
     A.capped = c->capped;
     A.mid = c->mid;
     if (vt && A.np < 3) return fail(c, RP_E_ARG, "smoothing spline needs sub*(N+1) >= 3");
@@ -440,12 +444,26 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     const int block = c->cfg.block > 0 ? c->cfg.block : 256;
     if (block % 64 || block > 256) return fail(c, RP_E_ARG, "block must be a multiple of 64, <= 256");
     const int64_t grid_all = (c->n + block - 1) / block;
+    const bool nested = c->cfg.ngrid > 1;
+    const bool f32 = c->fbytes == 4;
+    using KFn = void (*)(const DGrid*, StepArgs);
+    auto pick = [&](int phase) -> KFn {
+        if (f32 && !nested) return phase == 0 ? step_kernel<float, false> : phase == 1 ? advect_kernel<float, false> : vwalk_kernel<float, false>;
+        if (f32) return phase == 0 ? step_kernel<float, true> : phase == 1 ? advect_kernel<float, true> : vwalk_kernel<float, true>;
+        if (!nested) return phase == 0 ? step_kernel<double, false> : phase == 1 ? advect_kernel<double, false> : vwalk_kernel<double, false>;
+        return phase == 0 ? step_kernel<double, true> : phase == 1 ? advect_kernel<double, true> : vwalk_kernel<double, true>;
+    };
     int64_t grid_vt = grid_all;
     A.scratch = nullptr;
     if (vt) {
-        // persistent grid for the vertical walk: bounded scratch, lanes loop
-        const int64_t cap = (int64_t)c->ncu * 1024 / block;
-        grid_vt = std::min<int64_t>(grid_all, cap);
+        // persistent grid for the vertical walk: exactly the resident blocks
+        int& bpc = c->vwalk_blocks_per_cu[f32 ? 0 : 1][nested ? 1 : 0];
+        if (bpc == 0) {
+            int nb = 0;
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick(2), block, 0));
+            bpc = std::max(nb, 1);
+        }
+        grid_vt = std::min<int64_t>(grid_all, (int64_t)c->ncu * bpc);
         const int64_t T = grid_vt * block;
         const size_t need = (size_t)vturb_rows(A.np, maxN + 1) * T * sizeof(double);
         if (c->scratch_bytes < need) {
@@ -489,21 +507,13 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
         HIPCHK(c, hipEventCreate(&ev.second));
     }
     HIPCHK(c, hipEventRecord(ev.first, c->stream));
-    const bool nested = c->cfg.ngrid > 1;
-    const bool f32 = c->fbytes == 4;
     const dim3 bd(block);
-    using KFn = void (*)(const DGrid*, StepArgs);
-    auto pick = [&](int phase) -> KFn {
-        if (f32 && !nested) return phase == 0 ? step_kernel<float, false> : phase == 1 ? advect_kernel<float, false> : vwalk_kernel<float, false>;
-        if (f32) return phase == 0 ? step_kernel<float, true> : phase == 1 ? advect_kernel<float, true> : vwalk_kernel<float, true>;
-        if (!nested) return phase == 0 ? step_kernel<double, false> : phase == 1 ? advect_kernel<double, false> : vwalk_kernel<double, false>;
-        return phase == 0 ? step_kernel<double, true> : phase == 1 ? advect_kernel<double, true> : vwalk_kernel<double, true>;
-    };
     if (!vt) {
         hipLaunchKernelGGL(pick(0), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
     } else {
         hipLaunchKernelGGL(pick(1), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
         HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemsetAsync(c->work, 0, sizeof(unsigned long long), c->stream));
         hipLaunchKernelGGL(pick(2), dim3((unsigned)grid_vt), bd, 0, c->stream, c->dgrids, A);
     }
     HIPCHK(c, hipGetLastError());
@@ -633,7 +643,7 @@ int rp_destroy(void* ctx)
     dfree(c->wd); dfree(c->zeta); dfree(c->bath);
     dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
     dfree(c->dev); dfree(c->staging); dfree(c->scratch);
-    dfree(c->active); dfree(c->capped); dfree(c->dgrids);
+    dfree(c->active); dfree(c->capped); dfree(c->dgrids); dfree(c->work);
     for (auto& e : c->ev_used) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     for (auto& e : c->ev_free) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -42,6 +42,7 @@ struct StepArgs {
     double* scratch;             // VTurb work arrays, column per lane
     double* mid;                 // [4][n] newX, newY, AdvectZ, clamped z between phases
     unsigned long long* active;  // += particles that took the step
+    unsigned long long* work;    // vwalk_kernel's next unclaimed particle (zeroed per launch)
     unsigned* capped;            // += secant searches stopped at the cap
 };
 

@@ -1158,15 +1158,13 @@ __device__ __forceinline__ void finish(const DGrid* __restrict__ grids, const St
     A.gid[q] = (int8_t)gid;
 }
 
+// One particle of the step.  Returns whether it took the step.
 template <typename F, bool NESTED, int PHASE>
-__device__ __forceinline__ void step_body(const DGrid* __restrict__ grids, const StepArgs& A)
+__device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const StepArgs& A,
+                                         int64_t q, int64_t tid, int64_t T)
 {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t T = (int64_t)gridDim.x * blockDim.x;
-    unsigned long long nact = 0;
-    for (int64_t q = tid; q < A.n; q += T) {
-        if (skips(A, q)) continue;
-        nact++;
+    {
+        if (skips(A, q)) return false;
         const double X0 = A.x[q], Y0 = A.y[q], Z0 = A.z[q];
         const int gid = NESTED ? (int)A.gid[q] : 1;
         const DGrid& g = grids[NESTED ? gid - 1 : 0];
@@ -1181,7 +1179,7 @@ __device__ __forceinline__ void step_body(const DGrid* __restrict__ grids, const
                 A.mid[A.n + q] = newY;
                 A.mid[2 * A.n + q] = AdvectZ;
                 A.mid[3 * A.n + q] = Zpar;
-                continue;
+                return true;
             }
         } else {
             newX = A.mid[q];
@@ -1200,11 +1198,47 @@ __device__ __forceinline__ void step_body(const DGrid* __restrict__ grids, const
         }
         finish<F, NESTED>(grids, A, q, gid, Z0, newX, newY, AdvectZ, TurbV);
     }
-    if (PHASE != 1) {
-        // active-particle count: wave reduction, one atomic per wave
-        for (int off = 32; off > 0; off >>= 1) nact += __shfl_down(nact, off, 64);
-        if ((threadIdx.x & 63) == 0 && nact) atomicAdd(A.active, nact);
+    return true;
+}
+
+__device__ __forceinline__ void count_active(const StepArgs& A, unsigned long long nact)
+{
+    // wave reduction, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) nact += __shfl_down(nact, off, 64);
+    if ((threadIdx.x & 63) == 0 && nact) atomicAdd(A.active, nact);
+}
+
+// one lane per particle (grid covers n)
+template <typename F, bool NESTED, int PHASE>
+__device__ __forceinline__ void step_body(const DGrid* __restrict__ grids, const StepArgs& A)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long nact = 0;
+    if (q < A.n && step_one<F, NESTED, PHASE>(grids, A, q, q, 0)) nact = 1;
+    if (PHASE != 1) count_active(A, nact);
+}
+
+// Persistent form for the vertical walk: a grid sized to the resident
+// waves; each wave claims the next 64 particles from a counter until none
+// are left (every wave reaches the exit once the counter passes n).  Lane t
+// keeps scratch column t.  Dynamic claiming evens out the per-particle cost,
+// which varies with the number of secant iterations.
+template <typename F, bool NESTED>
+__device__ __forceinline__ void vwalk_body(const DGrid* __restrict__ grids, const StepArgs& A)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
+    unsigned long long nact = 0;
+    for (;;) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(A.work, 64ull);
+        b = __shfl(b, 0, 64);
+        if (b >= (unsigned long long)A.n) break;
+        const int64_t q = (int64_t)b + lane;
+        if (q < A.n && step_one<F, NESTED, 2>(grids, A, q, tid, T)) nact++;
     }
+    count_active(A, nact);
 }
 
 // Occupancy targets (waves per SIMD) of the three step kernels; 0 leaves the
@@ -1238,7 +1272,7 @@ template <typename F, bool NESTED>
 __global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_VWALK)
     vwalk_kernel(const DGrid* __restrict__ grids, StepArgs A)
 {
-    step_body<F, NESTED, 2>(grids, A);
+    vwalk_body<F, NESTED>(grids, A);
 }
 
 #define RP_INST(F, N)                                                              \
