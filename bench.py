@@ -1,10 +1,16 @@
 #!/usr/bin/env python3
 """Particle-steps/s of the HIP particle step on one MI355X.
 
-Workload (default): 400x300x30 tanh-shelf grid with a masked island, 16
-drifting Gaussian eddies plus an M2 tide, RK4 advection + vertical random
-displacement (idt 120 s, 10 sub-steps of 12 s) + horizontal random walk
-(Kh 1 m2/s), 1,000,000 particles, FP32 fields with FP64 particle state.
+Workloads:
+  c3 (default): nested grids + Stokes drift.  Parent 600x500x30 (1 km) with a
+     3:1 child 300x300x30 over its centre (grid 1 = child), tanh shelf and a
+     masked island, 16 drifting eddies + M2 tide, 32 child-only eddies, Stokes
+     drift 0.1 exp(2kz) m/s from 30 deg; 4,000,000 particles released half in
+     each grid, RK4 + vertical random displacement (idt 120 s, 10 sub-steps of
+     12 s), hand-off between the grids every step.
+  c2: 400x300x30 single grid of the same family, 1,000,000 particles, with the
+     horizontal random walk (Kh 1 m2/s) on.
+Both: FP32 fields with FP64 particle state (--precision fp64 for the FP64 form).
 
 A "step" is one internal time step of all particles (one rp_step).  The
 timed region covers K steps including the hydro-slot uploads the schedule
@@ -56,7 +62,8 @@ def run_form(cfg, records, steps, warmup, precision):
 
     def do(st):
         for ld in st.loads:
-            ctx.set_hydro_slot(1, ld.slot, records.get(1, ld.record))
+            for g in range(1, cfg.ngrid + 1):
+                ctx.set_hydro_slot(g, ld.slot, records.get(g, ld.record))
         ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index, want_diag=st.output)
 
     for st in sched[:warmup]:
@@ -73,11 +80,12 @@ def run_form(cfg, records, steps, warmup, precision):
     t1 = time.perf_counter()
     a1 = ctx.active_steps()
     avg_ms, launches = ctx.kernel_timing()
+    phase_ms = ctx.phase_timing()
     capped = ctx.secant_capped()
     out = ctx.get_particles()
     run.close()
     return dict(active=a1 - a0, seconds=t1 - t0, avg_kernel_ms=avg_ms, launches=launches,
-                capped=capped, state=out)
+                phase_ms=phase_ms, capped=capped, state=out)
 
 
 def cpu_baseline(cfg, n_sample, steps):
@@ -96,11 +104,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--particles", type=int, default=1_000_000)
+    ap.add_argument("--workload", choices=["c3", "c2"], default="c3")
+    ap.add_argument("--particles", type=int, default=None,
+                    help="default 4,000,000 (c3) or 1,000,000 (c2)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-particles", type=int, default=4000)
+    ap.add_argument("--cpu-particles", type=int, default=12000)
     ap.add_argument("--cpu-steps", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -108,8 +118,22 @@ def main():
               "(every GPU box here has a single MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    log("building workload C2 (400x300x30, 16 eddies + M2, RK4 + VRW + HRW)")
-    cfg = synth.config2(n_particles=args.particles, days=3.0)
+    if args.workload == "c3":
+        npart = args.particles or 4_000_000
+        log("building workload C3 (nested 600x500x30 + 3:1 child 300x300x30, Stokes, RK4 + VRW)")
+        cfg = synth.config3(n_particles=npart, days=5.0)
+        desc = ("C3: nested grids + Stokes drift, parent 600x500x30 + 3:1 child 300x300x30, "
+                "16 drifting eddies + M2 tide + 32 child eddies, RK4 + vertical random walk "
+                f"(10 x 12 s), {npart} particles half in each grid, hand-off every step")
+        dims = {"parent": [600, 500, 30], "child": [300, 300, 30], "refine": 3}
+    else:
+        npart = args.particles or 1_000_000
+        log("building workload C2 (400x300x30, 16 eddies + M2, RK4 + VRW + HRW)")
+        cfg = synth.config2(n_particles=npart, days=3.0)
+        desc = ("C2: 400x300x30 tanh shelf + island, 16 drifting eddies + M2 tide, "
+                "RK4 + vertical random walk (10 x 12 s) + horizontal random walk, "
+                f"{npart} particles")
+        dims = {"grid": [400, 300, 30]}
     prm = cfg.params
     total = args.warmup + args.steps
     need = set()
@@ -118,17 +142,20 @@ def main():
             need.add(ld.record)
     records = RecordSource(cfg)
     for r in sorted(need):
-        records.get(1, r)
-    log(f"  {len(need)} hydro records generated")
+        for g in range(1, cfg.ngrid + 1):
+            records.get(g, r)
+    log(f"  {len(need)} hydro records x {cfg.ngrid} grids generated")
 
     res = run_form(cfg, records, args.steps, args.warmup, args.precision)
     value = res["active"] / res["seconds"]
     log(f"  {args.precision}: {value:.4g} particle-steps/s, kernel {res['avg_kernel_ms']:.3f} ms")
     f = 4 if args.precision == "fp32" else 8
     N = cfg.grids[0].N
-    balg = b_alg(f, N)
+    balg = b_alg(f, N, vturb=prm.VTurbOn, ngrid=cfg.ngrid)
     per_launch = res["active"] / max(res["launches"], 1)
     achieved = balg * per_launch / (res["avg_kernel_ms"] * 1e-3) / 1e9 if res["avg_kernel_ms"] else 0.0
+    adv_ms, walk_ms = res["phase_ms"]
+    log(f"  kernels: advect {adv_ms:.3f} ms, vertical walk {walk_ms:.3f} ms per step")
 
     extra = {}
     if not args.no_fp64_form and args.precision == "fp32":
@@ -136,6 +163,7 @@ def main():
         v64 = r64["active"] / r64["seconds"]
         a, b = res["state"], r64["state"]
         same = a["status"] == b["status"]
+        same &= a["gid"] == b["gid"]
         layer = np.maximum(b["bath"], 1.0) / N
         ex = np.abs(a["x"] - b["x"])[same]
         ey = np.abs(a["y"] - b["y"])[same]
@@ -169,17 +197,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "C2: 400x300x30 tanh shelf + island, 16 drifting eddies + M2 tide, "
-                               "RK4 + vertical random walk (10 x 12 s) + horizontal random walk, "
-                               f"{args.particles} particles, "
-                               f"{'FP32' if f == 4 else 'FP64'} fields / FP64 state",
-                   "particles": args.particles, "grid": [400, 300, 30], "idt_s": prm.idt,
-                   "dt_turb_s": prm.deltat, "field_precision": args.precision,
-                   "parallelism": "single GPU"},
+        "config": dict({"workload": desc + f", {'FP32' if f == 4 else 'FP64'} fields / FP64 state",
+                        "particles": npart, "idt_s": prm.idt, "dt_turb_s": prm.deltat,
+                        "field_precision": args.precision, "parallelism": "single GPU"}, **dims),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "one step = advect_kernel + vwalk_kernel",
                      "b_alg_bytes_per_particle_step": balg,
                      "avg_kernel_ms": res["avg_kernel_ms"],
+                     "advect_kernel_ms": adv_ms, "vwalk_kernel_ms": walk_ms,
                      "active_particles_per_launch": per_launch},
         "cpu_baseline": cpu,
         "secant_capped": res["capped"],

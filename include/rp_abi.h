@@ -137,9 +137,16 @@ int rp_secant_capped(void *ctx, int64_t *count);
 /* Wait for all work queued on the context stream. */
 int rp_synchronize(void *ctx);
 
-/* Average device time per rp_step launch (ms) of the step kernel since the
- * last call, measured with HIP events on the context stream. */
+/* Average device time per rp_step (ms) of its kernels since the last call,
+ * measured with HIP events on the context stream.  No reference counterpart
+ * (the reference has no timers on this path). */
 int rp_kernel_timing(void *ctx, double *avg_ms, int64_t *launches);
+
+/* Split of the averages of the last rp_kernel_timing call: the advection
+ * kernel (RK stages, horizontal walk; the whole step when vertical
+ * turbulence is off) and the vertical-walk kernel (spline fit, sub-steps,
+ * grid hand-off).  No reference counterpart. */
+int rp_phase_timing(void *ctx, double *advect_ms, double *walk_ms);
 
 const char *rp_last_error(void *ctx);
 int rp_destroy(void *ctx);

@@ -27,8 +27,8 @@ PI64 = C.POINTER(C.c_int64)
 
 EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", "rp_step",
            "rp_get_particles", "rp_set_deviates", "rp_sample_deviates", "rp_active_steps",
-           "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_last_error",
-           "rp_destroy"]
+           "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_phase_timing",
+           "rp_last_error", "rp_destroy"]
 
 
 class RpConfig(C.Structure):
@@ -78,6 +78,7 @@ def lib():
         L.rp_secant_capped.argtypes = [vp, PI64]
         L.rp_synchronize.argtypes = [vp]
         L.rp_kernel_timing.argtypes = [vp, PD, PI64]
+        L.rp_phase_timing.argtypes = [vp, PD, PD]
         L.rp_last_error.argtypes = [vp]
         L.rp_last_error.restype = C.c_char_p
         L.rp_destroy.argtypes = [vp]
@@ -238,3 +239,10 @@ class Context:
         k = C.c_int64()
         self._chk("rp_kernel_timing", self._L.rp_kernel_timing(self.h, C.byref(ms), C.byref(k)))
         return ms.value, k.value
+
+    def phase_timing(self):
+        """(advection ms, vertical-walk ms) averages of the last kernel_timing()."""
+        a = C.c_double()
+        b = C.c_double()
+        self._chk("rp_phase_timing", self._L.rp_phase_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value

@@ -61,7 +61,10 @@ struct Ctx {
     unsigned long long* work = nullptr;
     unsigned* capped = nullptr;
     int vwalk_blocks_per_cu[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // [form][nested], 0 = not queried
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+    // per step: events before the step, between its two phases, after it
+    struct StepEvents { hipEvent_t e[3]; };
+    std::vector<StepEvents> ev_used, ev_free;
+    double phase_ms[2] = {0.0, 0.0};     // averages of the last rp_kernel_timing call
     std::string err;
 };
 
@@ -498,26 +501,28 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
             HIPCHK(c, hipStreamSynchronize(c->stream));
         }
     }
-    std::pair<hipEvent_t, hipEvent_t> ev;
+    Ctx::StepEvents ev;
     if (!c->ev_free.empty()) {
         ev = c->ev_free.back();
         c->ev_free.pop_back();
     } else {
-        HIPCHK(c, hipEventCreate(&ev.first));
-        HIPCHK(c, hipEventCreate(&ev.second));
+        for (auto& e : ev.e) HIPCHK(c, hipEventCreate(&e));
     }
-    HIPCHK(c, hipEventRecord(ev.first, c->stream));
+    HIPCHK(c, hipEventRecord(ev.e[0], c->stream));
     const dim3 bd(block);
     if (!vt) {
         hipLaunchKernelGGL(pick(0), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev.e[1], c->stream));
     } else {
         hipLaunchKernelGGL(pick(1), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
         HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(ev.e[1], c->stream));
         HIPCHK(c, hipMemsetAsync(c->work, 0, sizeof(unsigned long long), c->stream));
         hipLaunchKernelGGL(pick(2), dim3((unsigned)grid_vt), bd, 0, c->stream, c->dgrids, A);
     }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(ev.second, c->stream));
+    HIPCHK(c, hipEventRecord(ev.e[2], c->stream));
     c->ev_used.push_back(ev);
     return RP_OK;
 }
@@ -613,16 +618,31 @@ int rp_kernel_timing(void* ctx, double* avg_ms, int64_t* launches)
     Ctx* c = (Ctx*)ctx;
     if (!c || !avg_ms || !launches) return RP_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    double tot = 0.0;
+    double tot = 0.0, ph[2] = {0.0, 0.0};
     for (auto& e : c->ev_used) {
-        float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, e.first, e.second));
-        tot += ms;
+        float a = 0.f, b = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, e.e[0], e.e[1]));
+        HIPCHK(c, hipEventElapsedTime(&b, e.e[1], e.e[2]));
+        ph[0] += a;
+        ph[1] += b;
+        tot += (double)a + (double)b;
         c->ev_free.push_back(e);
     }
+    const double k = c->ev_used.empty() ? 0.0 : 1.0 / (double)c->ev_used.size();
     *launches = (int64_t)c->ev_used.size();
-    *avg_ms = c->ev_used.empty() ? 0.0 : tot / (double)c->ev_used.size();
+    *avg_ms = tot * k;
+    c->phase_ms[0] = ph[0] * k;
+    c->phase_ms[1] = ph[1] * k;
     c->ev_used.clear();
+    return RP_OK;
+}
+
+int rp_phase_timing(void* ctx, double* advect_ms, double* walk_ms)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !advect_ms || !walk_ms) return RP_E_ARG;
+    *advect_ms = c->phase_ms[0];
+    *walk_ms = c->phase_ms[1];
     return RP_OK;
 }
 
@@ -644,8 +664,8 @@ int rp_destroy(void* ctx)
     dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
     dfree(c->dev); dfree(c->staging); dfree(c->scratch);
     dfree(c->active); dfree(c->capped); dfree(c->dgrids); dfree(c->work);
-    for (auto& e : c->ev_used) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-    for (auto& e : c->ev_free) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto& e : c->ev_used) for (auto& x : e.e) (void)hipEventDestroy(x);
+    for (auto& e : c->ev_free) for (auto& x : e.e) (void)hipEventDestroy(x);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return RP_OK;

@@ -39,7 +39,8 @@ module romspath_hip
 
   public :: rp_create, rp_set_grid, rp_set_hydro_slot, rp_set_particles, rp_step, &
             rp_get_particles, rp_set_deviates, rp_sample_deviates, rp_active_steps, &
-            rp_secant_capped, rp_synchronize, rp_kernel_timing, rp_last_error, rp_destroy, &
+            rp_secant_capped, rp_synchronize, rp_kernel_timing, rp_phase_timing, rp_last_error, &
+            rp_destroy, &
             rp_error_text
 
   interface
@@ -133,6 +134,12 @@ module romspath_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: avg_ms
       integer(c_int64_t), intent(out) :: launches
+    end function
+
+    integer(c_int) function rp_phase_timing(ctx, advect_ms, walk_ms) bind(C, name="rp_phase_timing")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: advect_ms, walk_ms
     end function
 
     type(c_ptr) function rp_last_error(ctx) bind(C, name="rp_last_error")

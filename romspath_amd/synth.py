@@ -73,6 +73,10 @@ class Grid:
     cs_r: np.ndarray = field(default=None)
     s_w: np.ndarray = field(default=None)
     cs_w: np.ndarray = field(default=None)
+    # coordinates x = x0 + (i*dx*den)/den: a refined child computes its
+    # points as x0 + (i*1000)/3 so that every third one equals a parent
+    # point bit for bit (the nested maps search for exact matches)
+    den: int = 1
 
     def __post_init__(self):
         if self.s_rho is None:
@@ -99,28 +103,34 @@ class Grid:
         m = self.mask_rho
         return np.ascontiguousarray(m[:-1, :] * m[1:, :], dtype=np.int32)
 
+    def _axis(self, n, d0, d):
+        i = np.arange(n, dtype=np.float64)
+        if self.den == 1:
+            return d0 + i * d
+        return d0 + (i * (d * self.den)) / self.den
+
     @property
     def x_rho(self):
-        i = np.arange(self.L, dtype=np.float64)
-        return np.broadcast_to(self.x0 + (i * self.dx), (self.M, self.L)).copy()
+        return np.broadcast_to(self._axis(self.L, self.x0, self.dx), (self.M, self.L)).copy()
 
     @property
     def y_rho(self):
-        j = np.arange(self.M, dtype=np.float64)
-        return np.broadcast_to((self.y0 + j * self.dy)[:, None], (self.M, self.L)).copy()
+        return np.broadcast_to(self._axis(self.M, self.y0, self.dy)[:, None], (self.M, self.L)).copy()
 
     # coordinates of the staggered points (metres)
     def xy_rho(self):
         return self.x_rho, self.y_rho
 
     def xy_u(self):
-        x = self.x0 + (np.arange(self.L - 1) + 0.5) * self.dx
-        y = self.y0 + np.arange(self.M) * self.dy
+        xr = self._axis(self.L, self.x0, self.dx)
+        x = 0.5 * (xr[:-1] + xr[1:])
+        y = self._axis(self.M, self.y0, self.dy)
         return np.meshgrid(x, y)
 
     def xy_v(self):
-        x = self.x0 + np.arange(self.L) * self.dx
-        y = self.y0 + (np.arange(self.M - 1) + 0.5) * self.dy
+        x = self._axis(self.L, self.x0, self.dx)
+        yr = self._axis(self.M, self.y0, self.dy)
+        y = 0.5 * (yr[:-1] + yr[1:])
         return np.meshgrid(x, y)
 
     def z_w(self, zeta):
@@ -482,3 +492,68 @@ def config_mini(n_particles=600, seed=7, days=0.25, vturb=True, hturb=True,
     prm = RunParams(dt=3600, idt=idt, days=days, VTurbOn=vturb, HTurbOn=hturb,
                     deltat=deltat, ConstantHTurb=(1.0,) * 5, seed=seed)
     return Config("mini", [g], [hyd], prm, p)
+
+
+def _nested(L, M, N, Lc, Mc, i0, j0, refine, n_particles, seed, days, n_eddies, n_child_eddies,
+            rad_scale, zeta_amp, hturb, stokes, name, xmid=200e3, width=25e3, island_r=15e3):
+    """Parent + centred refined child (grid 1 = child, 2 = parent, as the
+    reference orders them).  Child point (1,1) sits on parent point (i0,j0)."""
+    dx = 1000.0
+    hp = _tanh_shelf(L, M, dx, xmid=xmid, width=width)
+    xc, yc = 0.5 * (L - 1) * dx, 0.5 * (M - 1) * dx
+    parent = Grid(L, M, N, dx, dx, 0.0, 0.0, h=hp,
+                  mask_rho=_island_mask(L, M, dx, dx, xc, yc, island_r))
+    x0, y0 = (i0 - 1) * dx, (j0 - 1) * dx
+    dxc = dx / refine
+    child = Grid(Lc, Mc, N, dxc, dxc, x0, y0, h=np.zeros((Mc, Lc)),
+                 mask_rho=np.ones((Mc, Lc), np.int32), den=int(refine))
+    xr, yr = child.xy_rho()
+    child.h = np.ascontiguousarray(20.0 + 90.0 * (1.0 + np.tanh((xr - xmid) / width)))
+    child.mask_rho = np.ascontiguousarray(((xr - xc) ** 2 + (yr - yc) ** 2 >= island_r ** 2).astype(np.int32))
+    rng = np.random.default_rng(seed)
+    pxr, pyr = parent.xy_rho()
+    flow = EddyField(rng, n_eddies, (0.0, (L - 1) * dx), (0.0, (M - 1) * dx),
+                     5e3 * rad_scale, 20e3 * rad_scale, vmax=0.5, drift=0.05, tide=0.2,
+                     scale_grid=(pxr, pyr))
+    small = EddyField(rng, n_child_eddies, (x0, x0 + (Lc - 1) * dxc), (y0, y0 + (Mc - 1) * dxc),
+                      1e3 * rad_scale, 3e3 * rad_scale, vmax=0.2, drift=0.05, tide=0.0,
+                      scale_grid=(xr, yr))
+    hx = _shelf_hx(xmid=xmid, width=width)
+    hyd_p = HydroModel(parent, flow, hx=hx, zeta_amp=zeta_amp, aks_peak=1e-2, stokes=stokes)
+    hyd_c = HydroModel(child, SumField(flow, small), hx=hx, zeta_amp=zeta_amp, aks_peak=1e-2,
+                       stokes=stokes)
+    nh = n_particles // 2
+    Xc, Yc, Zc = _release_uniform(rng, child, nh)
+    # parent releases outside the child's footprint
+    xs, ys = [], []
+    need = n_particles - nh
+    x1c, y1c = x0 + (Lc - 1) * dxc, y0 + (Mc - 1) * dxc
+    while need > 0:
+        X, Y, _ = _release_uniform(rng, parent, 2 * need + 16)
+        xm, ym = (X - 1.0) * dx, (Y - 1.0) * dx
+        out = ~((xm >= x0 - dx) & (xm <= x1c + dx) & (ym >= y0 - dx) & (ym <= y1c + dx))
+        xs.append(X[out][:need])
+        ys.append(Y[out][:need])
+        need -= xs[-1].size
+    Xp, Yp = np.concatenate(xs), np.concatenate(ys)
+    s_ = rng.uniform(-0.98, -0.02, Xp.size)
+    Zp = s_ * bilinear_depth(parent, Xp, Yp)
+    gid = np.r_[np.ones(nh, np.int32), np.full(Xp.size, 2, np.int32)]
+    p = Particles(np.r_[Xc, Xp], np.r_[Yc, Yp], np.r_[Zc, Zp], np.zeros(n_particles), gid=gid)
+    prm = RunParams(dt=3600, idt=120, days=days, VTurbOn=True, HTurbOn=hturb, deltat=12.0,
+                    ConstantHTurb=(1.0,) * 5, seed=seed)
+    return Config(name, [child, parent], [hyd_c, hyd_p], prm, p, refine=(float(refine),))
+
+
+def config3(n_particles=4_000_000, seed=3, days=5.0):
+    """Nested grids + Stokes drift: parent 600x500x30, 3:1 child 300x300x30 at the
+    centre, 32 child-only eddies, u_s = 0.1 exp(2kz) from 30 deg, k = 0.1."""
+    return _nested(600, 500, 30, 300, 300, 251, 201, 3, n_particles, seed, days, 16, 32, 1.0,
+                   0.0, False, (0.1, 0.1, np.deg2rad(30.0)), "C3")
+
+
+def config_mini_nested(n_particles=600, seed=8, days=2 / 24.0, zeta_amp=0.2, hturb=True):
+    """Small C3-family case: parent 48x40x8 and a 3:1 child 30x30x8."""
+    return _nested(48, 40, 8, 30, 30, 20, 15, 3, n_particles, seed, days, 5, 6, 0.25, zeta_amp,
+                   hturb, (0.1, 0.1, np.deg2rad(30.0)), "mini-nested", xmid=24e3, width=6e3,
+                   island_r=3.3e3)

@@ -110,3 +110,44 @@ def test_error_codes_not_crashes():
         run.ctx.step((0.0, 1.0, 2.0), (0.0, 1.0, 2.0), 1, 0, 2, 3, 0)
     assert e.value.code == -1
     run.close()
+
+
+def test_nested_fp64_handoff_matches_oracle():
+    cfg = synth.config_mini_nested(n_particles=600)
+    g, o, act, capped = _run_both(cfg, 60, "fp64", inject=True)
+    _check_exact_discrete(g, o)
+    ex, ey, ez = _err(g, o, cfg)
+    print("nested fp64 max err", ex.max(), ey.max(), ez.max(), "gid counts", np.bincount(o.gid),
+          flush=True)
+    assert ex.max() <= 1e-9 and ey.max() <= 1e-9 and ez.max() <= 1e-9
+    assert act == o.active and capped == 0
+    # the case really hands particles over in both directions
+    assert np.any(o.gid != cfg.particles.gid)
+
+
+def test_nested_fp32_fields_error():
+    cfg = synth.config_mini_nested(n_particles=600)
+    g, o, _, _ = _run_both(cfg, 60, "fp32", inject=True)
+    same = (g["status"] == o.status) & (g["gid"] == o.gid)
+    ex, ey, ez = _err(g, o, cfg)
+    print("nested fp32 err p95", np.percentile(ex[same], 95), np.percentile(ez[same], 95),
+          "agree", same.mean(), flush=True)
+    assert same.mean() > 0.99
+    assert np.percentile(ex[same], 95) <= 1e-4 and np.percentile(ez[same], 95) <= 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_thirty_levels_spline_path(precision):
+    # N=30, sub=4: 124 spline knots per column, the level count of the large grids
+    cfg = synth.config_mini(n_particles=256, N=30, days=1 / 24.0, dob_spread=600.0)
+    g, o, act, capped = _run_both(cfg, 30, precision, inject=True)
+    ex, ey, ez = _err(g, o, cfg)
+    print("N30", precision, "max err", ex.max(), ez.max(), "p95 z", np.percentile(ez, 95), flush=True)
+    if precision == "fp64":
+        _check_exact_discrete(g, o)
+        assert ex.max() <= 1e-9 and ey.max() <= 1e-9 and ez.max() <= 1e-9
+        assert act == o.active and capped == 0
+    else:
+        same = g["status"] == o.status
+        assert same.mean() > 0.99
+        assert np.percentile(ez[same], 95) <= 1e-4
