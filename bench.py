@@ -54,10 +54,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(cfg, records, steps, warmup, precision):
+def run_form(cfg, records, steps, warmup, precision, sort_interval=16):
     prm = cfg.params
     sched = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, warmup + steps))
-    run = GpuRun(cfg, precision=precision, records=records)
+    run = GpuRun(cfg, precision=precision, records=records, sort_interval=sort_interval)
     ctx = run.ctx
 
     def do(st):
@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--particles", type=int, default=None,
                     help="default 4,000,000 (c3) or 1,000,000 (c2)")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--sort-interval", type=int, default=16,
+                    help="re-sort particles by cell every k steps (0 = never)")
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=12000)
@@ -146,7 +148,7 @@ def main():
             records.get(g, r)
     log(f"  {len(need)} hydro records x {cfg.ngrid} grids generated")
 
-    res = run_form(cfg, records, args.steps, args.warmup, args.precision)
+    res = run_form(cfg, records, args.steps, args.warmup, args.precision, args.sort_interval)
     value = res["active"] / res["seconds"]
     log(f"  {args.precision}: {value:.4g} particle-steps/s, kernel {res['avg_kernel_ms']:.3f} ms")
     f = 4 if args.precision == "fp32" else 8
@@ -159,7 +161,7 @@ def main():
 
     extra = {}
     if not args.no_fp64_form and args.precision == "fp32":
-        r64 = run_form(cfg, records, args.steps, args.warmup, "fp64")
+        r64 = run_form(cfg, records, args.steps, args.warmup, "fp64", args.sort_interval)
         v64 = r64["active"] / r64["seconds"]
         a, b = res["state"], r64["state"]
         same = a["status"] == b["status"]
@@ -199,7 +201,8 @@ def main():
         "data": "synthetic",
         "config": dict({"workload": desc + f", {'FP32' if f == 4 else 'FP64'} fields / FP64 state",
                         "particles": npart, "idt_s": prm.idt, "dt_turb_s": prm.deltat,
-                        "field_precision": args.precision, "parallelism": "single GPU"}, **dims),
+                        "field_precision": args.precision, "sort_interval": args.sort_interval,
+                        "parallelism": "single GPU"}, **dims),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "kernel": "one step = advect_kernel + vwalk_kernel",

@@ -14,7 +14,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "lib" / "librp.so"
-SOURCES = ["rp_step.hip", "rp_abi.hip"]
+SOURCES = ["rp_step.hip", "rp_sort.hip", "rp_abi.hip"]
 HEADERS = ["rp_device.h", "rp_kernels.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -64,6 +64,15 @@ struct Ctx {
     // per step: events before the step, between its two phases, after it
     struct StepEvents { hipEvent_t e[3]; };
     std::vector<StepEvents> ev_used, ev_free;
+    // re-sort by cell (cfg.sort_interval > 0)
+    int64_t steps_done = 0;
+    uint32_t* sort_cnt = nullptr;
+    int64_t sort_nb = 0;
+    uint32_t* perm = nullptr;
+    double* alt64[PermArgs::kF64] = {};
+    uint8_t* alt8[PermArgs::kU8] = {};
+    uint32_t* alt_id = nullptr;
+    int64_t alt_n = -1;
     double phase_ms[2] = {0.0, 0.0};     // averages of the last rp_kernel_timing call
     std::string err;
 };
@@ -347,6 +356,7 @@ int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, con
     rc |= dalloc(c, &c->mid, 4 * (size_t)n);
     if (rc) return RP_E_ALLOC;
     c->n = n;
+    c->steps_done = 0;
     if (n == 0) return RP_OK;
     if ((rc = put(c, c->x, x, n * 8)) || (rc = put(c, c->y, y, n * 8)) || (rc = put(c, c->z, z, n * 8)) ||
         (rc = put(c, c->age, age, n * 8)) || (rc = put(c, c->dob, dob, n * 8)) ||
@@ -377,6 +387,73 @@ int rp_set_deviates(void* ctx, const double* normals, int64_t n, int32_t ndev)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->ndev = ndev;
     c->use_dev = true;
+    return RP_OK;
+}
+
+// Counting sort of the particle columns by (grid, rho cell), on the
+// context stream; the columns are gathered into the spare set and the two
+// sets swap.
+int sort_particles(Ctx* c)
+{
+    const int64_t n = c->n;
+    SortArgs K;
+    uint32_t nb = 0;
+    for (int g = 0; g < c->cfg.ngrid; g++) {
+        K.base[g] = nb;
+        K.L[g] = c->grids[g].L;
+        K.M[g] = c->grids[g].M;
+        nb += (uint32_t)c->grids[g].L * (uint32_t)c->grids[g].M;
+    }
+    K.base[c->cfg.ngrid] = nb;
+    const int64_t bins = (int64_t)nb + 1;
+    if (c->sort_nb != bins) {
+        if (dalloc(c, &c->sort_cnt, (size_t)bins)) return RP_E_ALLOC;
+        c->sort_nb = bins;
+    }
+    if (c->alt_n != n) {
+        int rc = dalloc(c, &c->perm, (size_t)n);
+        for (auto& a : c->alt64) rc |= dalloc(c, &a, (size_t)n);
+        for (auto& a : c->alt8) rc |= dalloc(c, &a, (size_t)n);
+        rc |= dalloc(c, &c->alt_id, (size_t)n);
+        if (rc) return RP_E_ALLOC;
+        c->alt_n = n;
+    }
+    K.x = c->x;
+    K.y = c->y;
+    K.status = c->status;
+    K.gid = c->gid;
+    K.isin = c->isin;
+    K.n = n;
+    K.open_boundary = c->cfg.open_boundary != 0;
+    K.ngrid = c->cfg.ngrid;
+    K.cnt = c->sort_cnt;
+    K.perm = c->perm;
+    const unsigned gb = (unsigned)std::min<int64_t>((bins + 255) / 256, 4096);
+    const unsigned gp = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(sort_fill, dim3(gb), dim3(256), 0, c->stream, c->sort_cnt, bins, 0u);
+    hipLaunchKernelGGL(sort_count, dim3(gp), dim3(256), 0, c->stream, K);
+    hipLaunchKernelGGL(sort_scan, dim3(1), dim3(1024), 0, c->stream, c->sort_cnt, bins);
+    hipLaunchKernelGGL(sort_scatter, dim3(gp), dim3(256), 0, c->stream, K);
+    PermArgs P;
+    P.perm = c->perm;
+    P.n = n;
+    double** cur64[PermArgs::kF64] = {&c->x, &c->y, &c->z, &c->age, &c->dob, &c->wd, &c->zeta, &c->bath};
+    uint8_t** cur8[PermArgs::kU8] = {(uint8_t**)&c->status, (uint8_t**)&c->gid, &c->isin};
+    for (int k = 0; k < PermArgs::kF64; k++) {
+        P.s64[k] = *cur64[k];
+        P.d64[k] = c->alt64[k];
+    }
+    for (int k = 0; k < PermArgs::kU8; k++) {
+        P.s8[k] = *cur8[k];
+        P.d8[k] = c->alt8[k];
+    }
+    P.sid = c->id;
+    P.did = c->alt_id;
+    hipLaunchKernelGGL(sort_permute, dim3(gp), dim3(256), 0, c->stream, P);
+    HIPCHK(c, hipGetLastError());
+    for (int k = 0; k < PermArgs::kF64; k++) std::swap(*cur64[k], c->alt64[k]);
+    for (int k = 0; k < PermArgs::kU8; k++) std::swap(*cur8[k], c->alt8[k]);
+    std::swap(c->id, c->alt_id);
     return RP_OK;
 }
 
@@ -508,7 +585,17 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     } else {
         for (auto& e : ev.e) HIPCHK(c, hipEventCreate(&e));
     }
+    c->ev_free.push_back(ev);                 // back to the pool if the step fails below
     HIPCHK(c, hipEventRecord(ev.e[0], c->stream));
+    if (c->cfg.sort_interval > 0 && c->steps_done % c->cfg.sort_interval == 0) {
+        const int rc = sort_particles(c);
+        if (rc) return rc;
+        A.x = c->x; A.y = c->y; A.z = c->z; A.age = c->age; A.dob = c->dob;
+        A.status = c->status; A.gid = c->gid; A.isin = c->isin;
+        A.wd = c->wd; A.zeta = c->zeta; A.bath = c->bath;
+        A.id = c->id;
+    }
+    c->steps_done++;
     const dim3 bd(block);
     if (!vt) {
         hipLaunchKernelGGL(pick(0), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
@@ -523,6 +610,7 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(ev.e[2], c->stream));
+    c->ev_free.pop_back();
     c->ev_used.push_back(ev);
     return RP_OK;
 }
@@ -662,6 +750,9 @@ int rp_destroy(void* ctx)
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->age); dfree(c->dob);
     dfree(c->wd); dfree(c->zeta); dfree(c->bath);
     dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
+    dfree(c->sort_cnt); dfree(c->perm); dfree(c->alt_id);
+    for (auto& a : c->alt64) dfree(a);
+    for (auto& a : c->alt8) dfree(a);
     dfree(c->dev); dfree(c->staging); dfree(c->scratch);
     dfree(c->active); dfree(c->capped); dfree(c->dgrids); dfree(c->work);
     for (auto& e : c->ev_used) for (auto& x : e.e) (void)hipEventDestroy(x);

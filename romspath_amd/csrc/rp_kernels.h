@@ -62,6 +62,38 @@ __global__ void pack_field(const double* __restrict__ src, const double* __restr
 
 __global__ void sample_deviates(uint64_t seed, int64_t step, int64_t n, int ndev, double* out);
 
+// re-sort of the particle columns by cell (rp_sort.hip)
+struct SortArgs {
+    const double *x, *y;
+    const int8_t *status, *gid;
+    const uint8_t* isin;
+    int64_t n;
+    int open_boundary, ngrid;
+    int L[kMaxGrid], M[kMaxGrid];
+    uint32_t base[kMaxGrid + 1];   // first key of each grid; base[ngrid] = parked particles
+    uint32_t* cnt;                 // base[ngrid] + 1 bins
+    uint32_t* perm;                // n: stored index of the particle at each sorted position
+};
+
+struct PermArgs {
+    static constexpr int kF64 = 8;   // x, y, z, age, dob, wd, zeta, bath
+    static constexpr int kU8 = 3;    // status, gid, isin
+    const uint32_t* perm;
+    int64_t n;
+    const double* s64[kF64];
+    double* d64[kF64];
+    const uint8_t* s8[kU8];
+    uint8_t* d8[kU8];
+    const uint32_t* sid;
+    uint32_t* did;
+};
+
+__global__ void sort_fill(uint32_t* a, int64_t n, uint32_t v);
+__global__ void sort_count(SortArgs K);
+__global__ void sort_scan(uint32_t* cnt, int64_t nb);
+__global__ void sort_scatter(SortArgs K);
+__global__ void sort_permute(PermArgs P);
+
 // rows of VTurb scratch per lane for np resampled points and nw w-levels
 inline int64_t vturb_rows(int np, int nw) { return 9LL * np + 2LL * nw; }
 

@@ -612,16 +612,18 @@ __device__ __forceinline__ double vturb(const DGrid& g, const StepArgs& A, const
 //     quantity the secant search needs, with no back substitution;
 //   * only the accepted p gets a full solve, whose u builds the spline
 //     pieces the random-displacement sub-steps visit.
-// Work arrays are read in chunks of kCh rows so their loads overlap.
+// Work arrays are read in chunks of kCh rows so their loads overlap.  The
+// right-hand side qty (second differences of y, setupq's qty for uniform
+// spacing) is formed from y inside each sweep instead of being stored.
 // ---------------------------------------------------------------------------
 constexpr int kCh = 8;
 
 struct FastLayout {
     int np;
-    int qty, y, rd, l1, l2, w, u;
+    int y, rd, l1, w, u;
     __device__ __forceinline__ FastLayout(int np_) : np(np_)
     {
-        qty = 0; y = np; rd = 2 * np; l1 = 3 * np; l2 = 4 * np; w = 5 * np; u = 6 * np;
+        y = 0; rd = np; l1 = 2 * np; w = 3 * np; u = 4 * np;
     }
 };
 
@@ -655,23 +657,45 @@ __device__ __forceinline__ Band band_of(double p, double h, double serr)
     return B;
 }
 
+// qty row r = ((y[r+2]-y[r+1]) - (y[r+1]-y[r])) / h, streamed: the sweep at
+// row m needs row m+1, i.e. y[m+3] with y[m+1], y[m+2] carried.
+struct QtyStream {
+    double ya, yb;
+    __device__ __forceinline__ double first(const Scr& S, int oy, double rh)
+    {
+        const double y0 = S(oy), y1 = S(oy + 1), y2 = S(oy + 2);
+        ya = y1;
+        yb = y2;
+        return ((y2 - y1) - (y1 - y0)) * rh;
+    }
+    __device__ __forceinline__ double next(double yc, double rh)
+    {
+        const double q = ((yc - yb) - (yb - ya)) * rh;
+        ya = yb;
+        yb = yc;
+        return q;
+    }
+};
+
 // G(p) and dG/dp by one LDL' sweep over the n interior unknowns
-__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int oq, double& G,
-                                         double& Gp)
+__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int oy, double rh,
+                                         double& G, double& Gp)
 {
 #pragma clang fp contract(fast)
     double d = B.a, dd = B.da, e = B.b, de = B.db;
     double dn = B.a, ddn = B.da, en = B.b, den = B.db;
-    double w = S(oq), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
+    QtyStream qs;
+    double w = qs.first(S, oy, rh), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
     double g = 0.0, gp = 0.0;
     for (int m0 = 0; m0 < n; m0 += kCh) {
-        double qn[kCh];
+        double yc[kCh];
 #pragma unroll
-        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(oq + m0 + k + 1) : 0.0;
+        for (int k = 0; k < kCh; k++) yc[k] = (m0 + k + 1 < n) ? S(oy + m0 + k + 3) : 0.0;
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
             const int m = m0 + k;
             if (m < n) {
+                const double qn = (m + 1 < n) ? qs.next(yc[k], rh) : 0.0;
                 const double rd = rcp64(d);
                 const double wrd = w * rd;
                 g += w * wrd;
@@ -681,7 +705,7 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int
                 const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
                 const double e1 = en - rat * B.c, de1 = den - drat * B.c - rat * B.dc;
                 const double d2 = B.a - rat2 * B.c, dd2 = B.da - drat2 * B.c - rat2 * B.dc;
-                const double wn = qn[k] - rat * w - l2p * wp;
+                const double wn = qn - rat * w - l2p * wp;
                 const double dwn = -drat * w - rat * dw - dl2p * wp - l2p * dwp;
                 wp = w; dwp = dw; w = wn; dw = dwn;
                 l2p = rat2; dl2p = drat2;
@@ -694,31 +718,35 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int
     Gp = gp;
 }
 
-// full solve at the accepted p: u(2..np-1) into Lo.u (u(1)=u(np)=0)
-__device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr& S)
+// full solve at the accepted p: u(2..np-1) into Lo.u (u(1)=u(np)=0).  The
+// forward sweep keeps 1/d, l1 and w per row; l2 = c/d is formed again on
+// the way back.
+__device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr& S,
+                                        double rh)
 {
 #pragma clang fp contract(fast)
     const int n = Lo.np - 2;
     double d = B.a, e = B.b, dn = B.a, en = B.b;
-    double w = S(Lo.qty), wp = 0.0, l2p = 0.0;
+    QtyStream qs;
+    double w = qs.first(S, Lo.y, rh), wp = 0.0, l2p = 0.0;
     for (int m0 = 0; m0 < n; m0 += kCh) {
-        double qn[kCh];
+        double yc[kCh];
 #pragma unroll
-        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(Lo.qty + m0 + k + 1) : 0.0;
+        for (int k = 0; k < kCh; k++) yc[k] = (m0 + k + 1 < n) ? S(Lo.y + m0 + k + 3) : 0.0;
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
             const int m = m0 + k;
             if (m < n) {
+                const double qn = (m + 1 < n) ? qs.next(yc[k], rh) : 0.0;
                 const double rd = rcp64(d);
                 const double rat = e * rd, rat2 = B.c * rd;
                 S(Lo.rd + m) = rd;
                 S(Lo.w + m) = w;
                 S(Lo.l1 + m) = rat;
-                S(Lo.l2 + m) = rat2;
                 const double d1 = dn - rat * e;
                 const double e1 = en - rat * B.c;
                 const double d2 = B.a - rat2 * B.c;
-                const double wn = qn[k] - rat * w - l2p * wp;
+                const double wn = qn - rat * w - l2p * wp;
                 wp = w; w = wn; l2p = rat2;
                 d = d1; e = e1; dn = d2; en = B.b;
             }
@@ -726,7 +754,7 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
     }
     double up1 = 0.0, up2 = 0.0;
     for (int m1 = n - 1; m1 >= 0; m1 -= kCh) {
-        double wv[kCh], rv[kCh], a1[kCh], a2[kCh];
+        double wv[kCh], rv[kCh], a1[kCh];
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
             const int m = m1 - k;
@@ -734,14 +762,13 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
             wv[k] = ok ? S(Lo.w + m) : 0.0;
             rv[k] = ok ? S(Lo.rd + m) : 0.0;
             a1[k] = ok ? S(Lo.l1 + m) : 0.0;
-            a2[k] = ok ? S(Lo.l2 + m) : 0.0;
         }
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
             const int m = m1 - k;
             if (m >= 0) {
                 double um = wv[k] * rv[k];
-                if (m < n - 1) um = um - up1 * a1[k] - up2 * a2[k];
+                if (m < n - 1) um = um - up1 * a1[k] - up2 * (B.c * rv[k]);
                 S(Lo.u + m + 1) = um;
                 up2 = up1;
                 up1 = um;
@@ -836,7 +863,6 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     // the bottom and top levels fix the resampling grid; the column is then
     // walked once, kLv levels per batch of loads, emitting y (linint: the
     // point fz uses the segment of the largest level j<=nw-1 with z(j)<=fz)
-    // and qty (second differences of y, setupq's qty for uniform spacing)
     double z1, k1, zn, kn;
     column_level<F>(g, A, c, cff, sf, hh, 1, z1, k1);
     column_level<F>(g, A, c, cff, sf, hh, nw, zn, kn);
@@ -845,8 +871,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     {
         constexpr int kLv = 4;
         double zlo = z1, klo = k1;
-        double ym1 = 0.0, y0 = k1;
-        S(Lo.y + 0) = y0;
+        S(Lo.y + 0) = k1;
         int i = 2;                                   // next point to emit
         for (int kb = 2; kb <= nw; kb += kLv) {
             double zk[kLv], kk[kLv];
@@ -873,10 +898,6 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
                             yi = m * fz + b;
                         }
                         S(Lo.y + i - 1) = yi;
-                        // qty(i-1) needs y(i-2): defined for i >= 3 (rows 0..np-3)
-                        if (i >= 3) S(Lo.qty + i - 3) = ((yi - y0) - (y0 - ym1)) * rh;
-                        ym1 = y0;
-                        y0 = yi;
                     }
                     zlo = zhi;
                     klo = khi;
@@ -892,7 +913,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
         p = 1.0;
     } else {
         double G, Gp;
-        stream_G(band_of(0.0, h, serr), n, S, Lo.qty, G, Gp);
+        stream_G(band_of(0.0, h, serr), n, S, Lo.y, rh, G, Gp);
         sfq = 6.0 * G;
         if (s < sfq) {
             const double utru = 0.5 * (G - Gp);
@@ -903,7 +924,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
             int it = 0;
             for (;;) {
                 const double pq = q / (1.0 + q);
-                stream_G(band_of(pq, h, serr), n, S, Lo.qty, G, Gp);
+                stream_G(band_of(pq, h, serr), n, S, Lo.y, rh, G, Gp);
                 sfq = 6.0 * (G + pq * Gp) / ((1.0 + q) * (1.0 + q));
                 if (fabs(sfq - s) <= 0.01 * s) break;
                 if (++it >= kSecantCap) {
@@ -919,7 +940,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
             p = q / (1.0 + q);
         }
     }
-    solve_u(band_of(p, h, serr), Lo, S);
+    solve_u(band_of(p, h, serr), Lo, S, rh);
     const double k2 = 6.0 * (1.0 - p) * (serr * serr);
     PieceCache pc;
     pc.i = -1;
@@ -1268,8 +1289,16 @@ __global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ADVECT)
     step_body<F, NESTED, 1>(grids, A);
 }
 
+// The FP32-field vertical walk is held to 3 waves per SIMD (168 VGPRs):
+// its sweeps wait on work-array loads, and the third wave hides them.
+template <typename F>
+constexpr int vwalk_waves()
+{
+    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (sizeof(F) == 4 ? 3 : 1);
+}
+
 template <typename F, bool NESTED>
-__global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_VWALK)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(vwalk_waves<F>())))
     vwalk_kernel(const DGrid* __restrict__ grids, StepArgs A)
 {
     vwalk_body<F, NESTED>(grids, A);

@@ -37,11 +37,11 @@ def injected_deviates(seed, step_index, n, ndev):
 
 
 class GpuRun:
-    def __init__(self, cfg, precision="fp32", block=0, records=None):
+    def __init__(self, cfg, precision="fp32", block=0, records=None, sort_interval=0):
         self.cfg = cfg
         ng = cfg.ngrid
         self.ctx = abi.Context(abi.make_config(cfg.params, ngrid=ng, precision=precision,
-                                               block=block))
+                                               block=block, sort_interval=sort_interval))
         scl, off = scale_offsets(cfg.grids, cfg.refine) if ng > 1 else (None, None)
         for i, g in enumerate(cfg.grids):
             self.ctx.set_grid(i + 1, g, None if scl is None else scl[i], None if off is None else off[i])
