@@ -88,6 +88,106 @@ def run_form(cfg, records, steps, warmup, precision, sort_interval=16):
                 phase_ms=phase_ms, capped=capped, state=out)
 
 
+def batched_ensembles(runup, window, tune_steps, log_every=120):
+    """Release ensembles (C4): the same window of steps once as one HIP-graph
+    batch per external step over all scenarios, and once as one plain launch
+    sequence per scenario, from the same state.  Returns a dict for the line."""
+    from romspath_amd.autotune import autotune
+    cfg, counts = synth.config4()
+    prm = cfg.params
+    n = cfg.particles.n
+    log(f"C4: {len(counts)} scenarios, {n} particles, run-up {runup} steps, window {window}")
+    steps = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, runup + window))
+    records = RecordSource(cfg)
+    for st in steps:
+        for ld in st.loads:
+            records.get(1, ld.record)
+    run = GpuRun(cfg, precision="fp32", records=records, sort_interval=16)
+    ctx = run.ctx
+
+    def groups(lo, hi):
+        i = lo
+        while i < hi:
+            j = i + 1
+            while j < hi and not steps[j].loads:
+                j += 1
+            yield i, j
+            i = j
+
+    # run-up in graph batches, with the autotuner on its last steps
+    tune_from = max(0, runup - tune_steps)
+    for i, j in groups(0, tune_from):
+        run.load(steps[i])
+        ctx.step_batch(steps[i:j], use_graph=True)
+        if i // log_every != j // log_every:
+            log(f"  run-up {j}/{runup}")
+    it = iter(range(tune_from, runup))
+
+    def next_step():
+        k = next(it, None)
+        if k is None:
+            return False
+        st = steps[k]
+        run.load(st)
+        ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index, want_diag=st.output)
+        return True
+
+    tuned = autotune(ctx, next_step, trial_steps=8, log=log)
+    for k in it:                       # run-up steps the tuner left over
+        st = steps[k]
+        run.load(st)
+        ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index, want_diag=st.output)
+    snap = ctx.get_particles()
+    state = synth.Particles(snap["x"], snap["y"], snap["z"], cfg.particles.dob.copy(),
+                            gid=snap["gid"], status=snap["status"], age=snap["age"],
+                            isin=snap["isin"])
+    b = tuned.best
+
+    def timed(form):
+        ctx.set_particles(state)
+        ctx.set_tuning(b.block, b.sort_interval, b.tile)
+        if form == "plain":
+            ctx.set_scenarios(counts)
+        ctx.synchronize()
+        a0 = ctx.active_steps()
+        t0 = time.perf_counter()
+        for i, j in groups(runup, runup + window):
+            run.load(steps[i])
+            if form == "batched":
+                ctx.step_batch(steps[i:j], use_graph=True)
+            else:
+                for st in steps[i:j]:
+                    for s_ in range(len(counts)):
+                        ctx.step_scenario(s_, st.ex, st.ix, st.ets, st.tb, st.tc, st.tf,
+                                          st.step_index, want_diag=st.output)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        act = ctx.active_steps() - a0
+        out = ctx.get_particles()
+        log(f"  {form}: {(t1 - t0) / window * 1e3:.2f} ms/step, {act / (t1 - t0):.4g} particle-steps/s")
+        return t1 - t0, act, out
+
+    tb_, act_b, out_b = timed("batched")
+    tp_, act_p, out_p = timed("plain")
+    run.close()
+    same = all(np.array_equal(out_b[k], out_p[k]) for k in ("x", "y", "z", "status", "gid"))
+    return {
+        "workload": "C4: C2 grid and fields, 256 release scenarios of round(exp(U[ln 1e3, ln 2e5])) "
+                    "particles in Gaussian clouds (sigma 2 km) around random wet cells, random depths, "
+                    "start times U[0, 24 h); FP32 fields",
+        "scenarios": int(len(counts)), "particles": int(n),
+        "window_steps": window, "runup_steps": runup,
+        "active_particle_steps": int(act_b),
+        "batched": {"ms_per_step": tb_ / window * 1e3, "value": act_b / tb_,
+                    "form": "one HIP graph per external step (30 internal steps), all scenarios"},
+        "plain": {"ms_per_step": tp_ / window * 1e3, "value": act_p / tp_,
+                  "form": "rp_step_scenario per scenario per step (3 launches each)"},
+        "speedup": tp_ / tb_,
+        "identical_results": bool(same),
+        "tuned": tuned.as_dict(),
+    }
+
+
 def cpu_baseline(cfg, n_sample, steps):
     from oracle import oracle as O
     sub = synth.Particles(cfg.particles.x[:n_sample].copy(), cfg.particles.y[:n_sample].copy(),
@@ -114,6 +214,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=12000)
     ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--no-batched", action="store_true", help="skip the C4 ensemble comparison")
+    ap.add_argument("--c4-runup", type=int, default=720, help="steps before the C4 window (24 h)")
+    ap.add_argument("--c4-window", type=int, default=30)
+    ap.add_argument("--c4-tune-steps", type=int, default=96)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -177,6 +281,10 @@ def main():
             "vertical_max": float(ez.max()), "vertical_p95": float(np.percentile(ez, 95)),
             "status_agreement": float(same.mean())}
         log(f"  fp64: {v64:.4g} particle-steps/s")
+
+    if not args.no_batched:
+        extra["batched_ensembles"] = batched_ensembles(args.c4_runup, args.c4_window,
+                                                       args.c4_tune_steps)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -125,6 +125,38 @@ int rp_set_deviates(void *ctx, const double *normals, int64_t n, int32_t ndev);
  * step_index, in the layout of rp_set_deviates. */
 int rp_sample_deviates(void *ctx, int64_t step_index, int64_t n, int32_t ndev, double *out);
 
+/* Tuning of the step, no reference counterpart: threads per block (0 =
+ * 256, else a multiple of 64 up to 256), re-sort of the stored particles by
+ * cell every sort_interval steps (0 = never) and the sort tile in cells
+ * (tile_x x tile_y; 1 x 1 sorts by cell).  Results do not depend on any of
+ * them.  The next step re-sorts. */
+int rp_set_tuning(void *ctx, int32_t block, int32_t sort_interval, int32_t tile_x,
+                  int32_t tile_y);
+
+/* Release ensembles, no reference counterpart (the reference runs one
+ * release per job): the particles of rp_set_particles, in their order, are
+ * nscen consecutive scenarios of counts[s] particles (sum = particle count).
+ * Every scenario is still stepped by rp_step / rp_step_batch together; this
+ * only enables rp_step_scenario. */
+int rp_set_scenarios(void *ctx, int32_t nscen, const int64_t *counts);
+
+/* One internal step of scenario scen (0-based) only: the per-ensemble launch
+ * form of rp_step, with the same arguments.  Each scenario keeps its own
+ * re-sort count and is sorted among its own particles. */
+int rp_step_scenario(void *ctx, int32_t scen, const double ex[3], const double ix[3], int32_t ets,
+                     int32_t t_b, int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag);
+
+/* nsteps consecutive internal steps of all particles (the rp_step calls of
+ * one external step, ROMSPath.f90:515-546, with no hydro load between them):
+ * ex, ix [nsteps][3], ets [nsteps], slots [nsteps][3] = (t_b, t_c, t_f),
+ * step_index [nsteps], want_diag [nsteps] or NULL.  use_graph = 1 captures
+ * the launches of all steps (re-sorts included) into one HIP graph and
+ * launches it once; 0 launches them as rp_step does.  No reference
+ * counterpart. */
+int rp_step_batch(void *ctx, int32_t nsteps, const double *ex, const double *ix,
+                  const int32_t *ets, const int32_t *slots, const int64_t *step_index,
+                  const int32_t *want_diag, int32_t use_graph);
+
 /* Sum over all steps so far of the particles that took the step (those that
  * pass the skip tests of ROMSPath.f90:672-690).  Synchronises the stream. */
 int rp_active_steps(void *ctx, int64_t *count);

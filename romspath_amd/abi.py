@@ -28,6 +28,7 @@ PI64 = C.POINTER(C.c_int64)
 EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", "rp_step",
            "rp_get_particles", "rp_set_deviates", "rp_sample_deviates", "rp_active_steps",
            "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_phase_timing",
+           "rp_set_tuning", "rp_set_scenarios", "rp_step_scenario", "rp_step_batch",
            "rp_last_error", "rp_destroy"]
 
 
@@ -79,6 +80,11 @@ def lib():
         L.rp_synchronize.argtypes = [vp]
         L.rp_kernel_timing.argtypes = [vp, PD, PI64]
         L.rp_phase_timing.argtypes = [vp, PD, PD]
+        L.rp_set_tuning.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+        L.rp_set_scenarios.argtypes = [vp, C.c_int32, PI64]
+        L.rp_step_scenario.argtypes = [vp, C.c_int32, PD, PD, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_int64, C.c_int32]
+        L.rp_step_batch.argtypes = [vp, C.c_int32, PD, PD, PI32, PI32, PI64, PI32, C.c_int32]
         L.rp_last_error.argtypes = [vp]
         L.rp_last_error.restype = C.c_char_p
         L.rp_destroy.argtypes = [vp]
@@ -194,6 +200,35 @@ class Context:
         ixa = (C.c_double * 3)(*ix)
         self._chk("rp_step", self._L.rp_step(self.h, exa, ixa, ets, tb, tc, tf, step_index,
                                              int(want_diag)))
+
+    def set_tuning(self, block=0, sort_interval=0, tile=(1, 1)):
+        self._chk("rp_set_tuning", self._L.rp_set_tuning(self.h, int(block), int(sort_interval),
+                                                         int(tile[0]), int(tile[1])))
+
+    def set_scenarios(self, counts):
+        c = np.ascontiguousarray(counts, dtype=np.int64)
+        self._chk("rp_set_scenarios", self._L.rp_set_scenarios(
+            self.h, int(c.size), c.ctypes.data_as(PI64)))
+
+    def step_scenario(self, scen, ex, ix, ets, tb, tc, tf, step_index, want_diag=False):
+        exa = (C.c_double * 3)(*ex)
+        ixa = (C.c_double * 3)(*ix)
+        self._chk("rp_step_scenario", self._L.rp_step_scenario(
+            self.h, int(scen), exa, ixa, ets, tb, tc, tf, step_index, int(want_diag)))
+
+    def step_batch(self, steps, use_graph=True):
+        """steps: sequence of objects with ex, ix, ets, tb, tc, tf, step_index, output."""
+        k = len(steps)
+        ex = np.ascontiguousarray([st.ex for st in steps], dtype=np.float64).reshape(k, 3)
+        ix = np.ascontiguousarray([st.ix for st in steps], dtype=np.float64).reshape(k, 3)
+        ets = np.ascontiguousarray([st.ets for st in steps], dtype=np.int32)
+        slots = np.ascontiguousarray([(st.tb, st.tc, st.tf) for st in steps], dtype=np.int32)
+        sidx = np.ascontiguousarray([st.step_index for st in steps], dtype=np.int64)
+        diag = np.ascontiguousarray([int(getattr(st, "output", False)) for st in steps],
+                                    dtype=np.int32)
+        self._chk("rp_step_batch", self._L.rp_step_batch(
+            self.h, k, _d(ex), _d(ix), ets.ctypes.data_as(PI32), slots.ctypes.data_as(PI32),
+            sidx.ctypes.data_as(PI64), diag.ctypes.data_as(PI32), int(bool(use_graph))))
 
     def get_particles(self):
         n = self.n

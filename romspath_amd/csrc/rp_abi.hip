@@ -61,8 +61,12 @@ struct Ctx {
     unsigned long long* work = nullptr;
     unsigned* capped = nullptr;
     int vwalk_blocks_per_cu[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // [form][nested], 0 = not queried
-    // per step: events before the step, between its two phases, after it
-    struct StepEvents { hipEvent_t e[3]; };
+    // per step: events before the step, between its two phases, after it;
+    // a captured batch of steps has one entry covering all of its steps
+    struct StepEvents {
+        hipEvent_t e[3];
+        int steps = 1;
+    };
     std::vector<StepEvents> ev_used, ev_free;
     // re-sort by cell (cfg.sort_interval > 0)
     int64_t steps_done = 0;
@@ -73,6 +77,11 @@ struct Ctx {
     uint8_t* alt8[PermArgs::kU8] = {};
     uint32_t* alt_id = nullptr;
     int64_t alt_n = -1;
+    int sort_tile[2] = {1, 1};           // cells per sort tile (x, y)
+    bool ranges_intact = true;           // each scenario's particles within its range
+    // ensembles: scenario s holds stored positions first[s] .. first[s]+count[s]-1
+    std::vector<int64_t> scen_first, scen_count, scen_steps;
+    hipGraphExec_t gexec = nullptr;      // last captured batch of steps
     double phase_ms[2] = {0.0, 0.0};     // averages of the last rp_kernel_timing call
     std::string err;
 };
@@ -196,6 +205,315 @@ void free_grid(GridDev& g)
     dfree(g.mrho); dfree(g.mu); dfree(g.mv);
     dfree(g.zeta); dfree(g.u); dfree(g.v); dfree(g.w); dfree(g.aks);
     g.set = false;
+}
+
+
+// ---------------------------------------------------------------------------
+// re-sort of the stored particle columns
+// ---------------------------------------------------------------------------
+
+// key layout of the sort: tiles per row and first key of each grid; returns
+// the number of bins (the last bin holds the particles that no longer move)
+int64_t sort_keys(const Ctx* c, SortArgs& K)
+{
+    uint32_t nb = 0;
+    K.tile_x = c->sort_tile[0];
+    K.tile_y = c->sort_tile[1];
+    for (int g = 0; g < c->cfg.ngrid; g++) {
+        const int L = c->grids[g].L, M = c->grids[g].M;
+        K.base[g] = nb;
+        K.L[g] = L;
+        K.M[g] = M;
+        K.ntx[g] = (L - 2) / K.tile_x + 1;
+        nb += (uint32_t)K.ntx[g] * (uint32_t)((M - 2) / K.tile_y + 1);
+    }
+    K.base[c->cfg.ngrid] = nb;
+    return (int64_t)nb + 1;
+}
+
+int sort_buffers(Ctx* c, int64_t bins)
+{
+    if (c->sort_nb < bins) {
+        if (dalloc(c, &c->sort_cnt, (size_t)bins)) return RP_E_ALLOC;
+        c->sort_nb = bins;
+    }
+    if (c->alt_n != c->n) {
+        int rc = dalloc(c, &c->perm, (size_t)c->n);
+        for (auto& a : c->alt64) rc |= dalloc(c, &a, (size_t)c->n);
+        for (auto& a : c->alt8) rc |= dalloc(c, &a, (size_t)c->n);
+        rc |= dalloc(c, &c->alt_id, (size_t)c->n);
+        if (rc) return RP_E_ALLOC;
+        c->alt_n = c->n;
+    }
+    return RP_OK;
+}
+
+PermArgs perm_args(Ctx* c, int64_t q0, int64_t cnt, double** cur64[], uint8_t** cur8[])
+{
+    PermArgs P;
+    P.perm = c->perm;
+    P.n = cnt;
+    cur64[0] = &c->x; cur64[1] = &c->y; cur64[2] = &c->z; cur64[3] = &c->age;
+    cur64[4] = &c->dob; cur64[5] = &c->wd; cur64[6] = &c->zeta; cur64[7] = &c->bath;
+    cur8[0] = (uint8_t**)&c->status; cur8[1] = (uint8_t**)&c->gid; cur8[2] = &c->isin;
+    for (int k = 0; k < PermArgs::kF64; k++) {
+        P.s64[k] = *cur64[k] + q0;
+        P.d64[k] = c->alt64[k] + q0;
+    }
+    for (int k = 0; k < PermArgs::kU8; k++) {
+        P.s8[k] = *cur8[k] + q0;
+        P.d8[k] = c->alt8[k] + q0;
+    }
+    P.sid = c->id + q0;
+    P.did = c->alt_id + q0;
+    return P;
+}
+
+void swap_columns(Ctx* c, double** cur64[], uint8_t** cur8[])
+{
+    for (int k = 0; k < PermArgs::kF64; k++) std::swap(*cur64[k], c->alt64[k]);
+    for (int k = 0; k < PermArgs::kU8; k++) std::swap(*cur8[k], c->alt8[k]);
+    std::swap(c->id, c->alt_id);
+}
+
+// Counting sort of stored particles q0 .. q0+cnt-1 by (grid, tile) on the
+// context stream, launches only (buffers from sort_buffers).  The whole set
+// swaps with the spare set; a sub-range is copied back into place.
+int sort_range(Ctx* c, int64_t q0, int64_t cnt)
+{
+    SortArgs K;
+    const int64_t bins = sort_keys(c, K);
+    if (c->sort_nb < bins || c->alt_n != c->n) return fail(c, RP_E_STATE, "sort buffers missing");
+    K.x = c->x + q0;
+    K.y = c->y + q0;
+    K.status = c->status + q0;
+    K.gid = c->gid + q0;
+    K.isin = c->isin + q0;
+    K.n = cnt;
+    K.open_boundary = c->cfg.open_boundary != 0;
+    K.ngrid = c->cfg.ngrid;
+    K.cnt = c->sort_cnt;
+    K.perm = c->perm;
+    const unsigned gb = (unsigned)std::min<int64_t>((bins + 255) / 256, 4096);
+    const unsigned gp = (unsigned)std::min<int64_t>((cnt + 255) / 256, 8192);
+    hipLaunchKernelGGL(sort_fill, dim3(gb), dim3(256), 0, c->stream, c->sort_cnt, bins, 0u);
+    hipLaunchKernelGGL(sort_count, dim3(gp), dim3(256), 0, c->stream, K);
+    hipLaunchKernelGGL(sort_scan, dim3(1), dim3(1024), 0, c->stream, c->sort_cnt, bins);
+    hipLaunchKernelGGL(sort_scatter, dim3(gp), dim3(256), 0, c->stream, K);
+    double** cur64[PermArgs::kF64];
+    uint8_t** cur8[PermArgs::kU8];
+    const PermArgs P = perm_args(c, q0, cnt, cur64, cur8);
+    hipLaunchKernelGGL(sort_permute, dim3(gp), dim3(256), 0, c->stream, P);
+    if (q0 == 0 && cnt == c->n) {
+        HIPCHK(c, hipGetLastError());
+        swap_columns(c, cur64, cur8);
+    } else {
+        hipLaunchKernelGGL(sort_copy_back, dim3(gp), dim3(256), 0, c->stream, P);
+        HIPCHK(c, hipGetLastError());
+    }
+    return RP_OK;
+}
+
+// stored order back to release order (so scenario ranges hold again)
+int restore_order(Ctx* c)
+{
+    double** cur64[PermArgs::kF64];
+    uint8_t** cur8[PermArgs::kU8];
+    const PermArgs P = perm_args(c, 0, c->n, cur64, cur8);
+    const unsigned gp = (unsigned)std::min<int64_t>((c->n + 255) / 256, 8192);
+    hipLaunchKernelGGL(sort_restore, dim3(gp), dim3(256), 0, c->stream, P);
+    HIPCHK(c, hipGetLastError());
+    swap_columns(c, cur64, cur8);
+    c->ranges_intact = true;
+    return RP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one step: host preparation (checks, buffers, grid table) apart from the
+// launches, so that a sequence of steps can be captured into a HIP graph
+// ---------------------------------------------------------------------------
+using KFn = void (*)(const DGrid*, StepArgs);
+
+struct StepPlan {
+    StepArgs A;
+    int block = 256;
+    int64_t grid_all = 0, grid_vt = 0;
+    bool vt = false;
+    KFn k[3] = {nullptr, nullptr, nullptr};   // step / advect / vertical walk
+};
+
+int check_step(Ctx* c, int t_b, int t_c, int t_f, int64_t step_index)
+{
+    if (t_b < 1 || t_b > 3 || t_c < 1 || t_c > 3 || t_f < 1 || t_f > 3)
+        return fail(c, RP_E_ARG, "slot numbers must be 1..3");
+    if (step_index < 0) return fail(c, RP_E_ARG, "negative step index");
+    for (int i = 0; i < c->cfg.ngrid; i++) {
+        const GridDev& g = c->grids[i];
+        if (!g.set) return fail(c, RP_E_STATE, "rp_step: grid " + std::to_string(i + 1) + " not set");
+        if (!g.slot_set[t_b - 1] || !g.slot_set[t_c - 1] || !g.slot_set[t_f - 1])
+            return fail(c, RP_E_STATE, "rp_step: hydro slot not loaded");
+    }
+    return RP_OK;
+}
+
+int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, int t_c, int t_f,
+              int64_t step_index, int want_diag, int64_t q0, int64_t cnt, StepPlan& P)
+{
+    int rc = check_step(c, t_b, t_c, t_f, step_index);
+    if (rc) return rc;
+    int maxN = 0;
+    for (int i = 0; i < c->cfg.ngrid; i++) maxN = std::max(maxN, c->grids[i].N);
+    const bool vt = c->cfg.vturb_on != 0;
+    StepArgs& A = P.A;
+    for (int k = 0; k < 3; k++) {
+        A.ex[k] = ex[k];
+        A.ix[k] = ix[k];
+    }
+    A.ets = ets;
+    {
+        // Lagrange weights of the quadratic through (ex, .) at ix(2)
+        const double x = ix[1], x0 = ex[0], x1 = ex[1], x2 = ex[2];
+        A.pw[0] = (x - x1) * (x - x2) / ((x0 - x1) * (x0 - x2));
+        A.pw[1] = (x - x0) * (x - x2) / ((x1 - x0) * (x1 - x2));
+        A.pw[2] = (x - x0) * (x - x1) / ((x2 - x0) * (x2 - x1));
+    }
+    A.slots.s[0] = t_b - 1;
+    A.slots.s[1] = t_c - 1;
+    A.slots.s[2] = t_f - 1;
+    A.idt = c->cfg.idt;
+    A.deltat = c->cfg.deltat;
+    A.serr = c->cfg.serr;
+    A.smth = c->cfg.smth;
+    A.aksback = c->cfg.aksback;
+    A.vturb_on = vt;
+    A.hturb_on = c->cfg.hturb_on != 0;
+    A.scheme = c->cfg.scheme;
+    A.nsb = c->cfg.nsb;
+    A.vertdist = c->cfg.vertdist;
+    A.open_boundary = c->cfg.open_boundary != 0;
+    A.ngrid = c->cfg.ngrid;
+    A.np = np_of(c, maxN);
+    A.loop = vt ? c->cfg.idt / (int)c->cfg.deltat : 0;
+    A.q0 = q0;
+    A.n = cnt;
+    A.nmid = c->n;
+    A.step_index = step_index;
+    A.seed = c->cfg.seed;
+    A.want_diag = want_diag != 0;
+    A.dev = c->use_dev ? c->dev : nullptr;
+    A.ndev = c->ndev;
+    A.active = c->active;
+    A.work = c->work;
+    A.capped = c->capped;
+    A.mid = c->mid;
+    if (vt && A.np < 3) return fail(c, RP_E_ARG, "smoothing spline needs sub*(N+1) >= 3");
+
+    const int block = c->cfg.block > 0 ? c->cfg.block : 256;
+    if (block % 64 || block > 256) return fail(c, RP_E_ARG, "block must be a multiple of 64, <= 256");
+    P.block = block;
+    P.vt = vt;
+    P.grid_all = (cnt + block - 1) / block;
+    const bool nested = c->cfg.ngrid > 1;
+    const bool f32 = c->fbytes == 4;
+    if (f32 && !nested) {
+        P.k[0] = step_kernel<float, false>; P.k[1] = advect_kernel<float, false>; P.k[2] = vwalk_kernel<float, false>;
+    } else if (f32) {
+        P.k[0] = step_kernel<float, true>; P.k[1] = advect_kernel<float, true>; P.k[2] = vwalk_kernel<float, true>;
+    } else if (!nested) {
+        P.k[0] = step_kernel<double, false>; P.k[1] = advect_kernel<double, false>; P.k[2] = vwalk_kernel<double, false>;
+    } else {
+        P.k[0] = step_kernel<double, true>; P.k[1] = advect_kernel<double, true>; P.k[2] = vwalk_kernel<double, true>;
+    }
+    P.grid_vt = P.grid_all;
+    A.scratch = nullptr;
+    if (vt) {
+        // persistent grid for the vertical walk: exactly the resident blocks
+        int& bpc = c->vwalk_blocks_per_cu[f32 ? 0 : 1][nested ? 1 : 0];
+        if (bpc == 0) {
+            int nb = 0;
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, P.k[2], block, 0));
+            bpc = std::max(nb, 1);
+        }
+        P.grid_vt = std::min<int64_t>(P.grid_all, (int64_t)c->ncu * bpc);
+        const int64_t T = P.grid_vt * block;
+        const size_t need = (size_t)vturb_rows(A.np, maxN + 1) * T * sizeof(double);
+        if (c->scratch_bytes < need) {
+            if ((rc = dalloc(c, (char**)&c->scratch, need))) return rc;
+            c->scratch_bytes = need;
+        }
+        A.scratch = c->scratch;
+    }
+    if (c->cfg.sort_interval > 0) {
+        SortArgs K;
+        if ((rc = sort_buffers(c, sort_keys(c, K)))) return rc;
+    }
+    {
+        DGrid tab[kMaxGrid];
+        std::memset(tab, 0, sizeof(tab));
+        for (int i = 0; i < c->cfg.ngrid; i++) {
+            const GridDev& g = c->grids[i];
+            DGrid& d = tab[i];
+            d.L = g.L; d.M = g.M; d.N = g.N; d.vtransform = g.vtransform; d.hc = g.hc;
+            d.s_rho = g.s_rho; d.cs_r = g.cs_r; d.s_w = g.s_w; d.cs_w = g.cs_w;
+            d.h = g.h; d.pm = g.pm; d.pn = g.pn;
+            d.mrho = g.mrho; d.mu = g.mu; d.mv = g.mv;
+            d.zeta = g.zeta; d.u = g.u; d.v = g.v; d.w = g.w; d.aks = g.aks;
+            d.kh = c->cfg.kh[i];
+            for (int a = 0; a < kMaxGrid; a++)
+                for (int k = 0; k < 2; k++) {
+                    d.scl[a][k] = g.scl[a][k];
+                    d.off[a][k] = g.off[a][k];
+                }
+        }
+        if (std::memcmp(tab, c->tab_host, sizeof(tab)) != 0) {
+            std::memcpy(c->tab_host, tab, sizeof(tab));
+            HIPCHK(c, hipMemcpyAsync(c->dgrids, c->tab_host, sizeof(tab), hipMemcpyHostToDevice,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+    }
+    return RP_OK;
+}
+
+// events for one timed entry, taken from the pool; the entry goes back to
+// the pool (ev_free) until the caller moves it to ev_used
+int take_events(Ctx* c, Ctx::StepEvents& ev)
+{
+    if (!c->ev_free.empty()) {
+        ev = c->ev_free.back();
+        c->ev_free.pop_back();
+    } else {
+        for (auto& e : ev.e) HIPCHK(c, hipEventCreate(&e));
+    }
+    ev.steps = 1;
+    c->ev_free.push_back(ev);
+    return RP_OK;
+}
+
+// the step's launches on the context stream (particle columns as they stand
+// now, after any re-sort); ev records the phase boundaries when given
+int enqueue_step(Ctx* c, StepPlan& P, const Ctx::StepEvents* ev)
+{
+    StepArgs& A = P.A;
+    A.x = c->x; A.y = c->y; A.z = c->z; A.age = c->age; A.dob = c->dob;
+    A.status = c->status; A.gid = c->gid; A.isin = c->isin;
+    A.wd = c->wd; A.zeta = c->zeta; A.bath = c->bath;
+    A.id = c->id;
+    const dim3 bd(P.block);
+    if (!P.vt) {
+        hipLaunchKernelGGL(P.k[0], dim3((unsigned)P.grid_all), bd, 0, c->stream, c->dgrids, A);
+        HIPCHK(c, hipGetLastError());
+        if (ev) HIPCHK(c, hipEventRecord(ev->e[1], c->stream));
+    } else {
+        hipLaunchKernelGGL(P.k[1], dim3((unsigned)P.grid_all), bd, 0, c->stream, c->dgrids, A);
+        HIPCHK(c, hipGetLastError());
+        if (ev) HIPCHK(c, hipEventRecord(ev->e[1], c->stream));
+        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, c->stream, c->work);
+        hipLaunchKernelGGL(P.k[2], dim3((unsigned)P.grid_vt), bd, 0, c->stream, c->dgrids, A);
+    }
+    HIPCHK(c, hipGetLastError());
+    if (ev) HIPCHK(c, hipEventRecord(ev->e[2], c->stream));
+    return RP_OK;
 }
 
 }  // namespace
@@ -357,6 +675,10 @@ int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, con
     if (rc) return RP_E_ALLOC;
     c->n = n;
     c->steps_done = 0;
+    c->ranges_intact = true;
+    c->scen_first.clear();
+    c->scen_count.clear();
+    c->scen_steps.clear();
     if (n == 0) return RP_OK;
     if ((rc = put(c, c->x, x, n * 8)) || (rc = put(c, c->y, y, n * 8)) || (rc = put(c, c->z, z, n * 8)) ||
         (rc = put(c, c->age, age, n * 8)) || (rc = put(c, c->dob, dob, n * 8)) ||
@@ -390,225 +712,166 @@ int rp_set_deviates(void* ctx, const double* normals, int64_t n, int32_t ndev)
     return RP_OK;
 }
 
-// Counting sort of the particle columns by (grid, rho cell), on the
-// context stream; the columns are gathered into the spare set and the two
-// sets swap.
-int sort_particles(Ctx* c)
-{
-    const int64_t n = c->n;
-    SortArgs K;
-    uint32_t nb = 0;
-    for (int g = 0; g < c->cfg.ngrid; g++) {
-        K.base[g] = nb;
-        K.L[g] = c->grids[g].L;
-        K.M[g] = c->grids[g].M;
-        nb += (uint32_t)c->grids[g].L * (uint32_t)c->grids[g].M;
-    }
-    K.base[c->cfg.ngrid] = nb;
-    const int64_t bins = (int64_t)nb + 1;
-    if (c->sort_nb != bins) {
-        if (dalloc(c, &c->sort_cnt, (size_t)bins)) return RP_E_ALLOC;
-        c->sort_nb = bins;
-    }
-    if (c->alt_n != n) {
-        int rc = dalloc(c, &c->perm, (size_t)n);
-        for (auto& a : c->alt64) rc |= dalloc(c, &a, (size_t)n);
-        for (auto& a : c->alt8) rc |= dalloc(c, &a, (size_t)n);
-        rc |= dalloc(c, &c->alt_id, (size_t)n);
-        if (rc) return RP_E_ALLOC;
-        c->alt_n = n;
-    }
-    K.x = c->x;
-    K.y = c->y;
-    K.status = c->status;
-    K.gid = c->gid;
-    K.isin = c->isin;
-    K.n = n;
-    K.open_boundary = c->cfg.open_boundary != 0;
-    K.ngrid = c->cfg.ngrid;
-    K.cnt = c->sort_cnt;
-    K.perm = c->perm;
-    const unsigned gb = (unsigned)std::min<int64_t>((bins + 255) / 256, 4096);
-    const unsigned gp = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(sort_fill, dim3(gb), dim3(256), 0, c->stream, c->sort_cnt, bins, 0u);
-    hipLaunchKernelGGL(sort_count, dim3(gp), dim3(256), 0, c->stream, K);
-    hipLaunchKernelGGL(sort_scan, dim3(1), dim3(1024), 0, c->stream, c->sort_cnt, bins);
-    hipLaunchKernelGGL(sort_scatter, dim3(gp), dim3(256), 0, c->stream, K);
-    PermArgs P;
-    P.perm = c->perm;
-    P.n = n;
-    double** cur64[PermArgs::kF64] = {&c->x, &c->y, &c->z, &c->age, &c->dob, &c->wd, &c->zeta, &c->bath};
-    uint8_t** cur8[PermArgs::kU8] = {(uint8_t**)&c->status, (uint8_t**)&c->gid, &c->isin};
-    for (int k = 0; k < PermArgs::kF64; k++) {
-        P.s64[k] = *cur64[k];
-        P.d64[k] = c->alt64[k];
-    }
-    for (int k = 0; k < PermArgs::kU8; k++) {
-        P.s8[k] = *cur8[k];
-        P.d8[k] = c->alt8[k];
-    }
-    P.sid = c->id;
-    P.did = c->alt_id;
-    hipLaunchKernelGGL(sort_permute, dim3(gp), dim3(256), 0, c->stream, P);
-    HIPCHK(c, hipGetLastError());
-    for (int k = 0; k < PermArgs::kF64; k++) std::swap(*cur64[k], c->alt64[k]);
-    for (int k = 0; k < PermArgs::kU8; k++) std::swap(*cur8[k], c->alt8[k]);
-    std::swap(c->id, c->alt_id);
-    return RP_OK;
-}
-
 int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int32_t t_b,
             int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag)
 {
     Ctx* c = (Ctx*)ctx;
     if (!c || !ex || !ix) return RP_E_ARG;
-    if (t_b < 1 || t_b > 3 || t_c < 1 || t_c > 3 || t_f < 1 || t_f > 3)
-        return fail(c, RP_E_ARG, "slot numbers must be 1..3");
-    if (step_index < 0) return fail(c, RP_E_ARG, "negative step index");
-    int maxN = 0;
-    for (int i = 0; i < c->cfg.ngrid; i++) {
-        const GridDev& g = c->grids[i];
-        if (!g.set) return fail(c, RP_E_STATE, "rp_step: grid " + std::to_string(i + 1) + " not set");
-        if (!g.slot_set[t_b - 1] || !g.slot_set[t_c - 1] || !g.slot_set[t_f - 1])
-            return fail(c, RP_E_STATE, "rp_step: hydro slot not loaded");
-        maxN = std::max(maxN, g.N);
-    }
-    if (c->n == 0) return RP_OK;
-    const bool vt = c->cfg.vturb_on != 0;
-    StepArgs A;
-    for (int k = 0; k < 3; k++) {
-        A.ex[k] = ex[k];
-        A.ix[k] = ix[k];
-    }
-    A.ets = ets;
-    {
-        // Lagrange weights of the quadratic through (ex, .) at ix(2)
-        const double x = ix[1], x0 = ex[0], x1 = ex[1], x2 = ex[2];
-        A.pw[0] = (x - x1) * (x - x2) / ((x0 - x1) * (x0 - x2));
-        A.pw[1] = (x - x0) * (x - x2) / ((x1 - x0) * (x1 - x2));
-        A.pw[2] = (x - x0) * (x - x1) / ((x2 - x0) * (x2 - x1));
-    }
-    A.slots.s[0] = t_b - 1;
-    A.slots.s[1] = t_c - 1;
-    A.slots.s[2] = t_f - 1;
-    A.idt = c->cfg.idt;
-    A.deltat = c->cfg.deltat;
-    A.serr = c->cfg.serr;
-    A.smth = c->cfg.smth;
-    A.aksback = c->cfg.aksback;
-    A.vturb_on = vt;
-    A.hturb_on = c->cfg.hturb_on != 0;
-    A.scheme = c->cfg.scheme;
-    A.nsb = c->cfg.nsb;
-    A.vertdist = c->cfg.vertdist;
-    A.open_boundary = c->cfg.open_boundary != 0;
-    A.ngrid = c->cfg.ngrid;
-    A.np = np_of(c, maxN);
-    A.loop = vt ? c->cfg.idt / (int)c->cfg.deltat : 0;
-    A.n = c->n;
-    A.step_index = step_index;
-    A.seed = c->cfg.seed;
-    A.x = c->x; A.y = c->y; A.z = c->z; A.age = c->age; A.dob = c->dob;
-    A.status = c->status; A.gid = c->gid; A.isin = c->isin;
-    A.wd = c->wd; A.zeta = c->zeta; A.bath = c->bath;
-    A.want_diag = want_diag != 0;
-    A.id = c->id;
-    A.dev = c->use_dev ? c->dev : nullptr;
-    A.ndev = c->ndev;
-    A.active = c->active;
-    A.work = c->work;
-    A.capped = c->capped;
-    A.mid = c->mid;
-    if (vt && A.np < 3) return fail(c, RP_E_ARG, "smoothing spline needs sub*(N+1) >= 3");
-
-    const int block = c->cfg.block > 0 ? c->cfg.block : 256;
-    if (block % 64 || block > 256) return fail(c, RP_E_ARG, "block must be a multiple of 64, <= 256");
-    const int64_t grid_all = (c->n + block - 1) / block;
-    const bool nested = c->cfg.ngrid > 1;
-    const bool f32 = c->fbytes == 4;
-    using KFn = void (*)(const DGrid*, StepArgs);
-    auto pick = [&](int phase) -> KFn {
-        if (f32 && !nested) return phase == 0 ? step_kernel<float, false> : phase == 1 ? advect_kernel<float, false> : vwalk_kernel<float, false>;
-        if (f32) return phase == 0 ? step_kernel<float, true> : phase == 1 ? advect_kernel<float, true> : vwalk_kernel<float, true>;
-        if (!nested) return phase == 0 ? step_kernel<double, false> : phase == 1 ? advect_kernel<double, false> : vwalk_kernel<double, false>;
-        return phase == 0 ? step_kernel<double, true> : phase == 1 ? advect_kernel<double, true> : vwalk_kernel<double, true>;
-    };
-    int64_t grid_vt = grid_all;
-    A.scratch = nullptr;
-    if (vt) {
-        // persistent grid for the vertical walk: exactly the resident blocks
-        int& bpc = c->vwalk_blocks_per_cu[f32 ? 0 : 1][nested ? 1 : 0];
-        if (bpc == 0) {
-            int nb = 0;
-            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick(2), block, 0));
-            bpc = std::max(nb, 1);
-        }
-        grid_vt = std::min<int64_t>(grid_all, (int64_t)c->ncu * bpc);
-        const int64_t T = grid_vt * block;
-        const size_t need = (size_t)vturb_rows(A.np, maxN + 1) * T * sizeof(double);
-        if (c->scratch_bytes < need) {
-            int rc = dalloc(c, (char**)&c->scratch, need);
-            if (rc) return rc;
-            c->scratch_bytes = need;
-        }
-        A.scratch = c->scratch;
-    }
-    {
-        DGrid tab[kMaxGrid];
-        std::memset(tab, 0, sizeof(tab));
-        for (int i = 0; i < c->cfg.ngrid; i++) {
-            const GridDev& g = c->grids[i];
-            DGrid& d = tab[i];
-            d.L = g.L; d.M = g.M; d.N = g.N; d.vtransform = g.vtransform; d.hc = g.hc;
-            d.s_rho = g.s_rho; d.cs_r = g.cs_r; d.s_w = g.s_w; d.cs_w = g.cs_w;
-            d.h = g.h; d.pm = g.pm; d.pn = g.pn;
-            d.mrho = g.mrho; d.mu = g.mu; d.mv = g.mv;
-            d.zeta = g.zeta; d.u = g.u; d.v = g.v; d.w = g.w; d.aks = g.aks;
-            d.kh = c->cfg.kh[i];
-            for (int a = 0; a < kMaxGrid; a++)
-                for (int k = 0; k < 2; k++) {
-                    d.scl[a][k] = g.scl[a][k];
-                    d.off[a][k] = g.off[a][k];
-                }
-        }
-        if (std::memcmp(tab, c->tab_host, sizeof(tab)) != 0) {
-            std::memcpy(c->tab_host, tab, sizeof(tab));
-            HIPCHK(c, hipMemcpyAsync(c->dgrids, c->tab_host, sizeof(tab), hipMemcpyHostToDevice,
-                                     c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-        }
-    }
+    if (c->n == 0) return check_step(c, t_b, t_c, t_f, step_index);
+    StepPlan P;
+    int rc = plan_step(c, ex, ix, ets, t_b, t_c, t_f, step_index, want_diag, 0, c->n, P);
+    if (rc) return rc;
     Ctx::StepEvents ev;
-    if (!c->ev_free.empty()) {
-        ev = c->ev_free.back();
-        c->ev_free.pop_back();
-    } else {
-        for (auto& e : ev.e) HIPCHK(c, hipEventCreate(&e));
-    }
-    c->ev_free.push_back(ev);                 // back to the pool if the step fails below
+    if ((rc = take_events(c, ev))) return rc;
     HIPCHK(c, hipEventRecord(ev.e[0], c->stream));
     if (c->cfg.sort_interval > 0 && c->steps_done % c->cfg.sort_interval == 0) {
-        const int rc = sort_particles(c);
-        if (rc) return rc;
-        A.x = c->x; A.y = c->y; A.z = c->z; A.age = c->age; A.dob = c->dob;
-        A.status = c->status; A.gid = c->gid; A.isin = c->isin;
-        A.wd = c->wd; A.zeta = c->zeta; A.bath = c->bath;
-        A.id = c->id;
+        if ((rc = sort_range(c, 0, c->n))) return rc;
+        c->ranges_intact = c->scen_first.size() <= 1;
     }
     c->steps_done++;
-    const dim3 bd(block);
-    if (!vt) {
-        hipLaunchKernelGGL(pick(0), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(ev.e[1], c->stream));
-    } else {
-        hipLaunchKernelGGL(pick(1), dim3((unsigned)grid_all), bd, 0, c->stream, c->dgrids, A);
-        HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventRecord(ev.e[1], c->stream));
-        HIPCHK(c, hipMemsetAsync(c->work, 0, sizeof(unsigned long long), c->stream));
-        hipLaunchKernelGGL(pick(2), dim3((unsigned)grid_vt), bd, 0, c->stream, c->dgrids, A);
+    if ((rc = enqueue_step(c, P, &ev))) return rc;
+    c->ev_free.pop_back();
+    c->ev_used.push_back(ev);
+    return RP_OK;
+}
+
+int rp_set_tuning(void* ctx, int32_t block, int32_t sort_interval, int32_t tile_x, int32_t tile_y)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (block != 0 && (block < 64 || block > 256 || block % 64))
+        return fail(c, RP_E_ARG, "block must be 0 or a multiple of 64 in 64..256");
+    if (sort_interval < 0) return fail(c, RP_E_ARG, "negative sort interval");
+    if (tile_x < 1 || tile_y < 1 || tile_x > 4096 || tile_y > 4096)
+        return fail(c, RP_E_ARG, "sort tile must be 1..4096 cells");
+    c->cfg.block = block;
+    c->cfg.sort_interval = sort_interval;
+    c->sort_tile[0] = tile_x;
+    c->sort_tile[1] = tile_y;
+    for (auto& f : c->vwalk_blocks_per_cu) f[0] = f[1] = 0;
+    // the next step sorts
+    c->steps_done = 0;
+    for (auto& k : c->scen_steps) k = 0;
+    return RP_OK;
+}
+
+int rp_set_scenarios(void* ctx, int32_t nscen, const int64_t* counts)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || nscen < 1 || !counts) return RP_E_ARG;
+    int64_t tot = 0;
+    for (int32_t s = 0; s < nscen; s++) {
+        if (counts[s] < 0) return fail(c, RP_E_ARG, "negative scenario count");
+        tot += counts[s];
     }
-    HIPCHK(c, hipGetLastError());
+    if (tot != c->n) return fail(c, RP_E_ARG, "scenario counts must add up to the particle count");
+    c->scen_first.assign(nscen, 0);
+    c->scen_count.assign(counts, counts + nscen);
+    c->scen_steps.assign(nscen, 0);
+    for (int32_t s = 1; s < nscen; s++) c->scen_first[s] = c->scen_first[s - 1] + counts[s - 1];
+    return RP_OK;
+}
+
+int rp_step_scenario(void* ctx, int32_t scen, const double ex[3], const double ix[3], int32_t ets,
+                     int32_t t_b, int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !ex || !ix) return RP_E_ARG;
+    if (scen < 0 || scen >= (int32_t)c->scen_first.size())
+        return fail(c, RP_E_ARG, "scenario index out of range (rp_set_scenarios)");
+    const int64_t q0 = c->scen_first[scen], cnt = c->scen_count[scen];
+    if (cnt == 0) return check_step(c, t_b, t_c, t_f, step_index);
+    StepPlan P;
+    int rc = plan_step(c, ex, ix, ets, t_b, t_c, t_f, step_index, want_diag, q0, cnt, P);
+    if (rc) return rc;
+    Ctx::StepEvents ev;
+    if ((rc = take_events(c, ev))) return rc;
+    HIPCHK(c, hipEventRecord(ev.e[0], c->stream));
+    if (!c->ranges_intact) {
+        if ((rc = restore_order(c))) return rc;
+    }
+    if (c->cfg.sort_interval > 0 && c->scen_steps[scen] % c->cfg.sort_interval == 0) {
+        if ((rc = sort_range(c, q0, cnt))) return rc;
+    }
+    c->scen_steps[scen]++;
+    if ((rc = enqueue_step(c, P, &ev))) return rc;
+    c->ev_free.pop_back();
+    c->ev_used.push_back(ev);
+    return RP_OK;
+}
+
+int rp_step_batch(void* ctx, int32_t nsteps, const double* ex, const double* ix, const int32_t* ets,
+                  const int32_t* slots, const int64_t* step_index, const int32_t* want_diag,
+                  int32_t use_graph)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || nsteps < 0 || (nsteps > 0 && (!ex || !ix || !ets || !slots || !step_index)))
+        return RP_E_ARG;
+    if (nsteps == 0) return RP_OK;
+    if (c->n == 0) {
+        for (int32_t i = 0; i < nsteps; i++) {
+            const int rc = check_step(c, slots[3 * i], slots[3 * i + 1], slots[3 * i + 2], step_index[i]);
+            if (rc) return rc;
+        }
+        return RP_OK;
+    }
+    std::vector<StepPlan> plans(nsteps);
+    for (int32_t i = 0; i < nsteps; i++) {
+        const int rc = plan_step(c, ex + 3 * i, ix + 3 * i, ets[i], slots[3 * i], slots[3 * i + 1],
+                                 slots[3 * i + 2], step_index[i], want_diag ? want_diag[i] : 0, 0,
+                                 c->n, plans[i]);
+        if (rc) return rc;
+    }
+    if (c->n > 0 && c->cfg.sort_interval > 0) {
+        SortArgs K;
+        const int rc = sort_buffers(c, sort_keys(c, K));
+        if (rc) return rc;
+    }
+    Ctx::StepEvents ev;
+    int rc = take_events(c, ev);
+    if (rc) return rc;
+    ev.steps = nsteps;
+    HIPCHK(c, hipEventRecord(ev.e[0], c->stream));
+    auto enqueue_all = [&]() -> int {
+        for (int32_t i = 0; i < nsteps; i++) {
+            if (c->cfg.sort_interval > 0 && c->steps_done % c->cfg.sort_interval == 0) {
+                const int r = sort_range(c, 0, c->n);
+                if (r) return r;
+                c->ranges_intact = c->scen_first.size() <= 1;
+            }
+            c->steps_done++;
+            const int r = enqueue_step(c, plans[i], nullptr);
+            if (r) return r;
+        }
+        return RP_OK;
+    };
+    if (use_graph) {
+        if (c->gexec) {
+            // the previous batch may still run from it
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            (void)hipGraphExecDestroy(c->gexec);
+            c->gexec = nullptr;
+        }
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_all();
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, RP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            c->gexec = nullptr;
+            return fail(c, RP_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+        HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    } else {
+        if ((rc = enqueue_all())) return rc;
+    }
+    HIPCHK(c, hipEventRecord(ev.e[1], c->stream));
     HIPCHK(c, hipEventRecord(ev.e[2], c->stream));
     c->ev_free.pop_back();
     c->ev_used.push_back(ev);
@@ -707,6 +970,7 @@ int rp_kernel_timing(void* ctx, double* avg_ms, int64_t* launches)
     if (!c || !avg_ms || !launches) return RP_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double tot = 0.0, ph[2] = {0.0, 0.0};
+    int64_t steps = 0;
     for (auto& e : c->ev_used) {
         float a = 0.f, b = 0.f;
         HIPCHK(c, hipEventElapsedTime(&a, e.e[0], e.e[1]));
@@ -714,10 +978,11 @@ int rp_kernel_timing(void* ctx, double* avg_ms, int64_t* launches)
         ph[0] += a;
         ph[1] += b;
         tot += (double)a + (double)b;
+        steps += e.steps;
         c->ev_free.push_back(e);
     }
-    const double k = c->ev_used.empty() ? 0.0 : 1.0 / (double)c->ev_used.size();
-    *launches = (int64_t)c->ev_used.size();
+    const double k = steps == 0 ? 0.0 : 1.0 / (double)steps;
+    *launches = steps;
     *avg_ms = tot * k;
     c->phase_ms[0] = ph[0] * k;
     c->phase_ms[1] = ph[1] * k;
@@ -750,6 +1015,7 @@ int rp_destroy(void* ctx)
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->age); dfree(c->dob);
     dfree(c->wd); dfree(c->zeta); dfree(c->bath);
     dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     dfree(c->sort_cnt); dfree(c->perm); dfree(c->alt_id);
     for (auto& a : c->alt64) dfree(a);
     for (auto& a : c->alt8) dfree(a);

@@ -26,7 +26,8 @@ struct StepArgs {
     double vertdist;
     int open_boundary, ngrid;
     int np, loop;
-    int64_t n;
+    int64_t q0, n;               // stored particles q0 .. q0+n-1 take this launch
+    int64_t nmid;                // row stride of mid (all stored particles)
     int64_t step_index;
     uint64_t seed;
     double *x, *y, *z, *age;
@@ -70,6 +71,8 @@ struct SortArgs {
     int64_t n;
     int open_boundary, ngrid;
     int L[kMaxGrid], M[kMaxGrid];
+    int tile_x, tile_y;            // cells per sort tile
+    int ntx[kMaxGrid];             // tiles per row of each grid
     uint32_t base[kMaxGrid + 1];   // first key of each grid; base[ngrid] = parked particles
     uint32_t* cnt;                 // base[ngrid] + 1 bins
     uint32_t* perm;                // n: stored index of the particle at each sorted position
@@ -93,6 +96,9 @@ __global__ void sort_count(SortArgs K);
 __global__ void sort_scan(uint32_t* cnt, int64_t nb);
 __global__ void sort_scatter(SortArgs K);
 __global__ void sort_permute(PermArgs P);
+__global__ void sort_copy_back(PermArgs P);
+__global__ void sort_restore(PermArgs P);
+__global__ void zero_u64(unsigned long long* p);
 
 // rows of VTurb scratch per lane for np resampled points and nw w-levels
 inline int64_t vturb_rows(int np, int nw) { return 9LL * np + 2LL * nw; }

@@ -1,7 +1,8 @@
 // rp_sort.hip -- periodic re-sort of the stored particle columns by grid cell.
 //
-// Counting sort on the key (grid, J, I) of each particle's rho cell: count,
-// exclusive scan, scatter to a permutation, gather every column through it.
+// Counting sort on the key (grid, tile row, tile column) of each particle's
+// rho cell, tiles of tile_x x tile_y cells: count, exclusive scan, scatter
+// to a permutation, gather every column through it.
 // Particles that no longer move (status 9, or out through an open boundary)
 // share the last key.  The order inside a cell is whatever the scatter's
 // atomics give; no result depends on it: every particle's step reads only
@@ -21,7 +22,7 @@ __device__ __forceinline__ uint32_t cell_key(const SortArgs& K, int64_t q)
     const double X = fmin(fmax(K.x[q], 1.0), (double)(L - 1));
     const double Y = fmin(fmax(K.y[q], 1.0), (double)(M - 1));
     const int I = (int)X, J = (int)Y;
-    return K.base[g] + (uint32_t)((J - 1) * L + (I - 1));
+    return K.base[g] + (uint32_t)((J - 1) / K.tile_y * K.ntx[g] + (I - 1) / K.tile_x);
 }
 
 __global__ void __launch_bounds__(256) sort_fill(uint32_t* a, int64_t n, uint32_t v)
@@ -94,6 +95,39 @@ __global__ void __launch_bounds__(256) sort_permute(PermArgs P)
         for (int k = 0; k < PermArgs::kU8; k++) P.d8[k][i] = P.s8[k][s];
         P.did[i] = P.sid[s];
     }
+}
+
+// write the gathered columns back over the source (sort of a sub-range)
+__global__ void __launch_bounds__(256) sort_copy_back(PermArgs P)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < PermArgs::kF64; k++) const_cast<double*>(P.s64[k])[i] = P.d64[k][i];
+#pragma unroll
+        for (int k = 0; k < PermArgs::kU8; k++) const_cast<uint8_t*>(P.s8[k])[i] = P.d8[k][i];
+        const_cast<uint32_t*>(P.sid)[i] = P.did[i];
+    }
+}
+
+// back to release order: the particle with id r goes to position r
+__global__ void __launch_bounds__(256) sort_restore(PermArgs P)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = P.sid[i];
+#pragma unroll
+        for (int k = 0; k < PermArgs::kF64; k++) P.d64[k][r] = P.s64[k][i];
+#pragma unroll
+        for (int k = 0; k < PermArgs::kU8; k++) P.d8[k][r] = P.s8[k][i];
+        P.did[r] = r;
+    }
+}
+
+// counter reset as a kernel, so that a captured launch sequence holds no memset
+__global__ void zero_u64(unsigned long long* p)
+{
+    if (threadIdx.x == 0) p[0] = 0ull;
 }
 
 }  // namespace rp

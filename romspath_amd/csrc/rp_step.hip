@@ -1197,16 +1197,16 @@ __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const 
             else advect<F, 3>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
             if (PHASE == 1) {
                 A.mid[q] = newX;
-                A.mid[A.n + q] = newY;
-                A.mid[2 * A.n + q] = AdvectZ;
-                A.mid[3 * A.n + q] = Zpar;
+                A.mid[A.nmid + q] = newY;
+                A.mid[2 * A.nmid + q] = AdvectZ;
+                A.mid[3 * A.nmid + q] = Zpar;
                 return true;
             }
         } else {
             newX = A.mid[q];
-            newY = A.mid[A.n + q];
-            AdvectZ = A.mid[2 * A.n + q];
-            Zpar = A.mid[3 * A.n + q];
+            newY = A.mid[A.nmid + q];
+            AdvectZ = A.mid[2 * A.nmid + q];
+            Zpar = A.mid[3 * A.nmid + q];
         }
         if (A.vturb_on) {
             const RhoCell c0 = rho_cell(g, X0, Y0);
@@ -1233,9 +1233,9 @@ __device__ __forceinline__ void count_active(const StepArgs& A, unsigned long lo
 template <typename F, bool NESTED, int PHASE>
 __device__ __forceinline__ void step_body(const DGrid* __restrict__ grids, const StepArgs& A)
 {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long nact = 0;
-    if (q < A.n && step_one<F, NESTED, PHASE>(grids, A, q, q, 0)) nact = 1;
+    if (r < A.n && step_one<F, NESTED, PHASE>(grids, A, A.q0 + r, r, 0)) nact = 1;
     if (PHASE != 1) count_active(A, nact);
 }
 
@@ -1256,8 +1256,8 @@ __device__ __forceinline__ void vwalk_body(const DGrid* __restrict__ grids, cons
         if (lane == 0) b = atomicAdd(A.work, 64ull);
         b = __shfl(b, 0, 64);
         if (b >= (unsigned long long)A.n) break;
-        const int64_t q = (int64_t)b + lane;
-        if (q < A.n && step_one<F, NESTED, 2>(grids, A, q, tid, T)) nact++;
+        const int64_t r = (int64_t)b + lane;
+        if (r < A.n && step_one<F, NESTED, 2>(grids, A, A.q0 + r, tid, T)) nact++;
     }
     count_active(A, nact);
 }

@@ -40,7 +40,7 @@ module romspath_hip
   public :: rp_create, rp_set_grid, rp_set_hydro_slot, rp_set_particles, rp_step, &
             rp_get_particles, rp_set_deviates, rp_sample_deviates, rp_active_steps, &
             rp_secant_capped, rp_synchronize, rp_kernel_timing, rp_phase_timing, rp_last_error, &
-            rp_destroy, &
+            rp_destroy, rp_set_tuning, rp_set_scenarios, rp_step_scenario, rp_step_batch, &
             rp_error_text
 
   interface
@@ -140,6 +140,39 @@ module romspath_hip
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: advect_ms, walk_ms
+    end function
+
+    integer(c_int) function rp_set_tuning(ctx, block, sort_interval, tile_x, tile_y) &
+        bind(C, name="rp_set_tuning")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: block, sort_interval, tile_x, tile_y
+    end function
+
+    integer(c_int) function rp_set_scenarios(ctx, nscen, counts) bind(C, name="rp_set_scenarios")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nscen
+      integer(c_int64_t), intent(in) :: counts(*)
+    end function
+
+    integer(c_int) function rp_step_scenario(ctx, scen, ex, ix, ets, t_b, t_c, t_f, step_index, &
+        want_diag) bind(C, name="rp_step_scenario")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: scen, ets, t_b, t_c, t_f, want_diag
+      real(c_double), intent(in) :: ex(3), ix(3)
+      integer(c_int64_t), value :: step_index
+    end function
+
+    integer(c_int) function rp_step_batch(ctx, nsteps, ex, ix, ets, slots, step_index, want_diag, &
+        use_graph) bind(C, name="rp_step_batch")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsteps, use_graph
+      real(c_double), intent(in) :: ex(3, *), ix(3, *)
+      integer(c_int32_t), intent(in) :: ets(*), slots(3, *), want_diag(*)
+      integer(c_int64_t), intent(in) :: step_index(*)
     end function
 
     type(c_ptr) function rp_last_error(ctx) bind(C, name="rp_last_error")

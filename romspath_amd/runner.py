@@ -48,18 +48,51 @@ class GpuRun:
         self.ctx.set_particles(cfg.particles)
         self.records = records or RecordSource(cfg)
 
-    def run(self, max_steps=None, inject=False, on_step=None, diag_every_step=False):
+    def load(self, st):
+        for ld in st.loads:
+            for g in range(1, self.cfg.ngrid + 1):
+                self.ctx.set_hydro_slot(g, ld.slot, self.records.get(g, ld.record))
+
+    def run(self, max_steps=None, inject=False, on_step=None, diag_every_step=False,
+            mode="step", counts=None, use_graph=True):
+        """mode "step": one rp_step per internal step; "batch": the steps of each
+        external step in one rp_step_batch (one HIP graph when use_graph);
+        "scenario": rp_step_scenario for each of the scenarios given by counts."""
         prm = self.cfg.params
         n = self.cfg.particles.n
         ndev = ndev_of(prm)
-        for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps):
-            for ld in st.loads:
-                for g in range(1, self.cfg.ngrid + 1):
-                    self.ctx.set_hydro_slot(g, ld.slot, self.records.get(g, ld.record))
+        steps = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps))
+        if diag_every_step:
+            for st in steps:
+                st.output = True
+        if mode == "scenario":
+            self.ctx.set_scenarios(counts)
+        if mode == "batch":
+            if inject:
+                raise ValueError("injected deviates are per step: use mode='step' or 'scenario'")
+            i = 0
+            while i < len(steps):
+                j = i + 1
+                while j < len(steps) and not steps[j].loads:
+                    j += 1
+                self.load(steps[i])
+                self.ctx.step_batch(steps[i:j], use_graph=use_graph)
+                if on_step is not None:
+                    for st in steps[i:j]:
+                        on_step(st, self)
+                i = j
+            return self.ctx.get_particles()
+        for st in steps:
+            self.load(st)
             if inject:
                 self.ctx.set_deviates(injected_deviates(prm.seed, st.step_index, n, ndev))
-            self.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index,
-                          want_diag=diag_every_step or st.output)
+            if mode == "scenario":
+                for s_ in range(len(counts)):
+                    self.ctx.step_scenario(s_, st.ex, st.ix, st.ets, st.tb, st.tc, st.tf,
+                                           st.step_index, want_diag=st.output)
+            else:
+                self.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index,
+                              want_diag=st.output)
             if on_step is not None:
                 on_step(st, self)
         return self.ctx.get_particles()

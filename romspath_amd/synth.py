@@ -557,3 +557,50 @@ def config_mini_nested(n_particles=600, seed=8, days=2 / 24.0, zeta_amp=0.2, htu
     return _nested(48, 40, 8, 30, 30, 20, 15, 3, n_particles, seed, days, 5, 6, 0.25, zeta_amp,
                    hturb, (0.1, 0.1, np.deg2rad(30.0)), "mini-nested", xmid=24e3, width=6e3,
                    island_r=3.3e3)
+
+
+def config4(n_scen=256, seed=4, days=3.0, nmin=1_000, nmax=200_000, sigma_km=2.0,
+            start_spread=86400.0, field_seed=2):
+    """Batched release ensembles on the C2 grid and fields: n_scen scenarios of
+    round(exp(U[ln nmin, ln nmax])) particles, each a Gaussian cloud (sigma 2 km)
+    around a random fully wet cell at random depths, released together at a
+    scenario start time U[0, 24 h) (whole seconds).  Particles are stored
+    scenario after scenario; returns (Config, counts)."""
+    base = config2(n_particles=1, seed=field_seed, days=days)
+    g = base.grids[0]
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_scen))).astype(np.int64)
+    m = g.mask_rho
+    sig = sigma_km * 1000.0 / g.dx
+    xs, ys, zs, dobs = [], [], [], []
+    for s_ in range(n_scen):
+        # centre: a random cell whose four corners are wet, away from the edges
+        while True:
+            cx = rng.uniform(4.0, g.L - 4.0)
+            cy = rng.uniform(4.0, g.M - 4.0)
+            I, J = int(cx), int(cy)
+            if m[J - 1, I - 1] + m[J - 1, I] + m[J, I] + m[J, I - 1] == 4:
+                break
+        n = int(counts[s_])
+        X = np.empty(0)
+        Y = np.empty(0)
+        while X.size < n:
+            k = 2 * (n - X.size) + 16
+            x = rng.normal(cx, sig, k)
+            y = rng.normal(cy, sig, k)
+            ok = (x >= 2.0) & (x < g.L - 2.0) & (y >= 2.0) & (y < g.M - 2.0)
+            x, y = x[ok], y[ok]
+            I = x.astype(np.int64)
+            J = y.astype(np.int64)
+            wet = (m[J - 1, I - 1] + m[J - 1, I] + m[J, I] + m[J, I - 1]) == 4
+            X = np.r_[X, x[wet]]
+            Y = np.r_[Y, y[wet]]
+        X, Y = X[:n], Y[:n]
+        Z = rng.uniform(-0.98, -0.02, n) * bilinear_depth(g, X, Y)
+        xs.append(X)
+        ys.append(Y)
+        zs.append(Z)
+        dobs.append(np.full(n, float(np.floor(rng.uniform(0.0, start_spread)))))
+    p = Particles(np.concatenate(xs), np.concatenate(ys), np.concatenate(zs), np.concatenate(dobs))
+    cfg = Config("C4", base.grids, base.hydro, base.params, p)
+    return cfg, counts
