@@ -218,7 +218,20 @@ def main():
     ap.add_argument("--c4-runup", type=int, default=720, help="steps before the C4 window (24 h)")
     ap.add_argument("--c4-window", type=int, default=30)
     ap.add_argument("--c4-tune-steps", type=int, default=96)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device (reference reading for the HBM "
+                         "byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 1 GiB")
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(every GPU box here has a single MI355X); run with --gpus 1", file=sys.stderr)

@@ -286,6 +286,13 @@ struct Scr {
     __device__ __forceinline__ double& operator()(int row) const { return p[(int64_t)row * T]; }
 };
 
+// the same lane column viewed as FP32 rows (the FP32-field form's work arrays)
+struct Scr32 {
+    float* p;
+    int64_t T;
+    __device__ __forceinline__ float& operator()(int row) const { return p[(int64_t)row * T]; }
+};
+
 struct VtLayout {
     int np, nw;
     int fitz, y, h, r, qty, d, l1, l2, u, z, pkh;
@@ -612,18 +619,19 @@ __device__ __forceinline__ double vturb(const DGrid& g, const StepArgs& A, const
 //     quantity the secant search needs, with no back substitution;
 //   * only the accepted p gets a full solve, whose u builds the spline
 //     pieces the random-displacement sub-steps visit.
-// Work arrays are read in chunks of kCh rows so their loads overlap.  The
-// right-hand side qty (second differences of y, setupq's qty for uniform
-// spacing) is formed from y inside each sweep instead of being stored.
+// Work arrays are FP32 rows (the arithmetic stays FP64): qty, formed in FP64
+// from the FP64 resampled values and rounded once, so each sweep sees qty to
+// a relative 6e-8 per entry; y for the pieces; the factor rows and u.  They
+// are read in chunks of kCh rows so their loads overlap.
 // ---------------------------------------------------------------------------
 constexpr int kCh = 8;
 
 struct FastLayout {
     int np;
-    int y, rd, l1, w, u;
+    int q, y, rd, l1, w, u;
     __device__ __forceinline__ FastLayout(int np_) : np(np_)
     {
-        y = 0; rd = np; l1 = 2 * np; w = 3 * np; u = 4 * np;
+        q = 0; y = np; rd = 2 * np; l1 = 3 * np; w = 4 * np; u = 5 * np;
     }
 };
 
@@ -657,45 +665,23 @@ __device__ __forceinline__ Band band_of(double p, double h, double serr)
     return B;
 }
 
-// qty row r = ((y[r+2]-y[r+1]) - (y[r+1]-y[r])) / h, streamed: the sweep at
-// row m needs row m+1, i.e. y[m+3] with y[m+1], y[m+2] carried.
-struct QtyStream {
-    double ya, yb;
-    __device__ __forceinline__ double first(const Scr& S, int oy, double rh)
-    {
-        const double y0 = S(oy), y1 = S(oy + 1), y2 = S(oy + 2);
-        ya = y1;
-        yb = y2;
-        return ((y2 - y1) - (y1 - y0)) * rh;
-    }
-    __device__ __forceinline__ double next(double yc, double rh)
-    {
-        const double q = ((yc - yb) - (yb - ya)) * rh;
-        ya = yb;
-        yb = yc;
-        return q;
-    }
-};
-
 // G(p) and dG/dp by one LDL' sweep over the n interior unknowns
-__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int oy, double rh,
-                                         double& G, double& Gp)
+__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, int oq, double& G,
+                                         double& Gp)
 {
 #pragma clang fp contract(fast)
     double d = B.a, dd = B.da, e = B.b, de = B.db;
     double dn = B.a, ddn = B.da, en = B.b, den = B.db;
-    QtyStream qs;
-    double w = qs.first(S, oy, rh), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
+    double w = (double)S(oq), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
     double g = 0.0, gp = 0.0;
     for (int m0 = 0; m0 < n; m0 += kCh) {
-        double yc[kCh];
+        float qn[kCh];
 #pragma unroll
-        for (int k = 0; k < kCh; k++) yc[k] = (m0 + k + 1 < n) ? S(oy + m0 + k + 3) : 0.0;
+        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(oq + m0 + k + 1) : 0.0f;
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
             const int m = m0 + k;
             if (m < n) {
-                const double qn = (m + 1 < n) ? qs.next(yc[k], rh) : 0.0;
                 const double rd = rcp64(d);
                 const double wrd = w * rd;
                 g += w * wrd;
@@ -705,7 +691,7 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int
                 const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
                 const double e1 = en - rat * B.c, de1 = den - drat * B.c - rat * B.dc;
                 const double d2 = B.a - rat2 * B.c, dd2 = B.da - drat2 * B.c - rat2 * B.dc;
-                const double wn = qn - rat * w - l2p * wp;
+                const double wn = (double)qn[k] - rat * w - l2p * wp;
                 const double dwn = -drat * w - rat * dw - dl2p * wp - l2p * dwp;
                 wp = w; dwp = dw; w = wn; dw = dwn;
                 l2p = rat2; dl2p = drat2;
@@ -721,32 +707,29 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr& S, int
 // full solve at the accepted p: u(2..np-1) into Lo.u (u(1)=u(np)=0).  The
 // forward sweep keeps 1/d, l1 and w per row; l2 = c/d is formed again on
 // the way back.
-__device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr& S,
-                                        double rh)
+__device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr32& S)
 {
 #pragma clang fp contract(fast)
     const int n = Lo.np - 2;
     double d = B.a, e = B.b, dn = B.a, en = B.b;
-    QtyStream qs;
-    double w = qs.first(S, Lo.y, rh), wp = 0.0, l2p = 0.0;
+    double w = (double)S(Lo.q), wp = 0.0, l2p = 0.0;
     for (int m0 = 0; m0 < n; m0 += kCh) {
-        double yc[kCh];
+        float qn[kCh];
 #pragma unroll
-        for (int k = 0; k < kCh; k++) yc[k] = (m0 + k + 1 < n) ? S(Lo.y + m0 + k + 3) : 0.0;
+        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(Lo.q + m0 + k + 1) : 0.0f;
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
             const int m = m0 + k;
             if (m < n) {
-                const double qn = (m + 1 < n) ? qs.next(yc[k], rh) : 0.0;
                 const double rd = rcp64(d);
                 const double rat = e * rd, rat2 = B.c * rd;
-                S(Lo.rd + m) = rd;
-                S(Lo.w + m) = w;
-                S(Lo.l1 + m) = rat;
+                S(Lo.rd + m) = (float)rd;
+                S(Lo.w + m) = (float)w;
+                S(Lo.l1 + m) = (float)rat;
                 const double d1 = dn - rat * e;
                 const double e1 = en - rat * B.c;
                 const double d2 = B.a - rat2 * B.c;
-                const double wn = qn - rat * w - l2p * wp;
+                const double wn = (double)qn[k] - rat * w - l2p * wp;
                 wp = w; w = wn; l2p = rat2;
                 d = d1; e = e1; dn = d2; en = B.b;
             }
@@ -754,41 +737,42 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
     }
     double up1 = 0.0, up2 = 0.0;
     for (int m1 = n - 1; m1 >= 0; m1 -= kCh) {
-        double wv[kCh], rv[kCh], a1[kCh];
+        float wv[kCh], rv[kCh], a1[kCh];
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
             const int m = m1 - k;
             const bool ok = m >= 0;
-            wv[k] = ok ? S(Lo.w + m) : 0.0;
-            rv[k] = ok ? S(Lo.rd + m) : 0.0;
-            a1[k] = ok ? S(Lo.l1 + m) : 0.0;
+            wv[k] = ok ? S(Lo.w + m) : 0.0f;
+            rv[k] = ok ? S(Lo.rd + m) : 0.0f;
+            a1[k] = ok ? S(Lo.l1 + m) : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
             const int m = m1 - k;
             if (m >= 0) {
-                double um = wv[k] * rv[k];
-                if (m < n - 1) um = um - up1 * a1[k] - up2 * (B.c * rv[k]);
-                S(Lo.u + m + 1) = um;
+                const double r = (double)rv[k];
+                double um = (double)wv[k] * r;
+                if (m < n - 1) um = um - up1 * (double)a1[k] - up2 * (B.c * r);
+                S(Lo.u + m + 1) = (float)um;
                 up2 = up1;
                 up1 = um;
             }
         }
     }
-    S(Lo.u + 0) = 0.0;
-    S(Lo.u + Lo.np - 1) = 0.0;
+    S(Lo.u + 0) = 0.0f;
+    S(Lo.u + Lo.np - 1) = 0.0f;
 }
 
-__device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr& S, int i, double p,
+__device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr32& S, int i, double p,
                                            double k2, double h, double rh, double z1,
                                            PieceCache& pc)
 {
 #pragma clang fp contract(fast)
     const int np = Lo.np;
     // u(i-1..i+2), y(i..i+1), issued together
-    const double um1 = (i >= 2) ? S(Lo.u + i - 2) : 0.0;
+    const double um1 = (i >= 2) ? (double)S(Lo.u + i - 2) : 0.0;
     const double u0 = S(Lo.u + i - 1), u1 = S(Lo.u + i);
-    const double u2 = (i + 2 <= np) ? S(Lo.u + i + 1) : 0.0;
+    const double u2 = (i + 2 <= np) ? (double)S(Lo.u + i + 1) : 0.0;
     const double y0 = S(Lo.y + i - 1), y1 = S(Lo.y + i);
     const double r_i = (i >= 2) ? (u0 - um1) * rh : 0.0;
     const double r_i1 = (u1 - u0) * rh;
@@ -805,7 +789,7 @@ __device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr& S, i
     pc.brk = z1 + (double)(i - 1) * h;
 }
 
-__device__ __forceinline__ double ppvalu_fast(const FastLayout& Lo, const Scr& S, PieceCache& pc,
+__device__ __forceinline__ double ppvalu_fast(const FastLayout& Lo, const Scr32& S, PieceCache& pc,
                                               double x, int jderiv, double p, double k2, double h,
                                               double rh, double z1)
 {
@@ -850,7 +834,7 @@ __device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, 
 template <typename F>
 __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, const RhoCell& c,
                                              const Surface& sf, double Zpar, DevSrc& dv,
-                                             const Scr& S, unsigned* capped)
+                                             const Scr32& S, unsigned* capped)
 {
 #pragma clang fp contract(fast)
     const int nw = g.N + 1;
@@ -871,7 +855,8 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     {
         constexpr int kLv = 4;
         double zlo = z1, klo = k1;
-        S(Lo.y + 0) = k1;
+        double ym1 = 0.0, y0 = k1;
+        S(Lo.y + 0) = (float)k1;
         int i = 2;                                   // next point to emit
         for (int kb = 2; kb <= nw; kb += kLv) {
             double zk[kLv], kk[kLv];
@@ -897,7 +882,11 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
                             if (k < nw && !(fz < zhi)) break;
                             yi = m * fz + b;
                         }
-                        S(Lo.y + i - 1) = yi;
+                        S(Lo.y + i - 1) = (float)yi;
+                        // qty row i-3 needs y(i-2): defined for i >= 3 (rows 0..np-3)
+                        if (i >= 3) S(Lo.q + i - 3) = (float)(((yi - y0) - (y0 - ym1)) * rh);
+                        ym1 = y0;
+                        y0 = yi;
                     }
                     zlo = zhi;
                     klo = khi;
@@ -913,7 +902,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
         p = 1.0;
     } else {
         double G, Gp;
-        stream_G(band_of(0.0, h, serr), n, S, Lo.y, rh, G, Gp);
+        stream_G(band_of(0.0, h, serr), n, S, Lo.q, G, Gp);
         sfq = 6.0 * G;
         if (s < sfq) {
             const double utru = 0.5 * (G - Gp);
@@ -924,7 +913,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
             int it = 0;
             for (;;) {
                 const double pq = q / (1.0 + q);
-                stream_G(band_of(pq, h, serr), n, S, Lo.y, rh, G, Gp);
+                stream_G(band_of(pq, h, serr), n, S, Lo.q, G, Gp);
                 sfq = 6.0 * (G + pq * Gp) / ((1.0 + q) * (1.0 + q));
                 if (fabs(sfq - s) <= 0.01 * s) break;
                 if (++it >= kSecantCap) {
@@ -940,7 +929,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
             p = q / (1.0 + q);
         }
     }
-    solve_u(band_of(p, h, serr), Lo, S, rh);
+    solve_u(band_of(p, h, serr), Lo, S);
     const double k2 = 6.0 * (1.0 - p) * (serr * serr);
     PieceCache pc;
     pc.i = -1;
@@ -1211,11 +1200,13 @@ __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const 
         if (A.vturb_on) {
             const RhoCell c0 = rho_cell(g, X0, Y0);
             const Surface s0 = surface_at<F>(g, A, c0);
-            const Scr S{A.scratch + tid, T};
-            if (sizeof(F) == 4)
+            if (sizeof(F) == 4) {
+                const Scr32 S{(float*)A.scratch + tid, T};
                 TurbV = vturb_fast<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
-            else
+            } else {
+                const Scr S{A.scratch + tid, T};
                 TurbV = vturb<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
+            }
         }
         finish<F, NESTED>(grids, A, q, gid, Z0, newX, newY, AdvectZ, TurbV);
     }
@@ -1289,12 +1280,13 @@ __global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ADVECT)
     step_body<F, NESTED, 1>(grids, A);
 }
 
-// The FP32-field vertical walk is held to 3 waves per SIMD (168 VGPRs):
-// its sweeps wait on work-array loads, and the third wave hides them.
+// The FP32-field vertical walk is held to 4 waves per SIMD (128 VGPRs, a
+// small spill): its sweeps and gathers wait on memory, and more resident
+// waves hide more of that than the spill costs.
 template <typename F>
 constexpr int vwalk_waves()
 {
-    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (sizeof(F) == 4 ? 3 : 1);
+    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (sizeof(F) == 4 ? 4 : 1);
 }
 
 template <typename F, bool NESTED>
