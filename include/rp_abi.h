@@ -157,6 +157,33 @@ int rp_step_batch(void *ctx, int32_t nsteps, const double *ex, const double *ix,
                   const int32_t *ets, const int32_t *slots, const int64_t *step_index,
                   const int32_t *want_diag, int32_t use_graph);
 
+/* Velocity perturbation, no reference counterpart: du, dv (m/s) added to the
+ * interpolated u, v of every RK stage, from uniform cubic B-splines on an
+ * nx x ny control net (4..64 each way) spread over each grid's rho-index
+ * domain [1, L] x [1, M].  coef [2][ny][nx] (du then dv, x fastest); NULL
+ * removes it.  Zero coefficients leave every result unchanged. */
+int rp_set_perturbation(void *ctx, int32_t nx, int32_t ny, const double *coef);
+
+/* Adjoint recording, no reference counterpart.  ckpt_every > 0 starts a
+ * recorded run at the current particle state: every following rp_step runs
+ * the whole step in one kernel with the vertical walk's spline on its
+ * reference-order FP64 path (for both field precisions), stores the state
+ * every ckpt_every steps, a copy of the hydro slots whenever they changed,
+ * and the injected deviates of each step if any; no re-sort happens while
+ * recording.  0 ends recording and frees what it kept. */
+int rp_adjoint_record(void *ctx, int32_t ckpt_every);
+
+/* Reverse sweep over the recorded steps, no reference counterpart.  Given
+ * lam = dL/d(x, y, z) of the particles' current state (release order; x, y
+ * in grid-index units of each particle's grid, z in metres), returns
+ * g = dL/d(x, y, z) of the state when recording started and, if g_pert is
+ * not NULL, dL/dcoef of the perturbation ([2][ny][nx]).  Segments between
+ * checkpoints are recomputed forward, then each step is differentiated
+ * exactly (forward-mode dual numbers seeded at the step's start) and
+ * lambda <- J^T lambda.  The particle state is left as it was. */
+int rp_adjoint(void *ctx, int64_t n, const double *lam_x, const double *lam_y,
+               const double *lam_z, double *g_x, double *g_y, double *g_z, double *g_pert);
+
 /* Sum over all steps so far of the particles that took the step (those that
  * pass the skip tests of ROMSPath.f90:672-690).  Synchronises the stream. */
 int rp_active_steps(void *ctx, int64_t *count);

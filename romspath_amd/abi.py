@@ -29,6 +29,7 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_get_particles", "rp_set_deviates", "rp_sample_deviates", "rp_active_steps",
            "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_phase_timing",
            "rp_set_tuning", "rp_set_scenarios", "rp_step_scenario", "rp_step_batch",
+           "rp_set_perturbation", "rp_adjoint_record", "rp_adjoint",
            "rp_last_error", "rp_destroy"]
 
 
@@ -85,6 +86,9 @@ def lib():
         L.rp_step_scenario.argtypes = [vp, C.c_int32, PD, PD, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32, C.c_int64, C.c_int32]
         L.rp_step_batch.argtypes = [vp, C.c_int32, PD, PD, PI32, PI32, PI64, PI32, C.c_int32]
+        L.rp_set_perturbation.argtypes = [vp, C.c_int32, C.c_int32, PD]
+        L.rp_adjoint_record.argtypes = [vp, C.c_int32]
+        L.rp_adjoint.argtypes = [vp, C.c_int64, PD, PD, PD, PD, PD, PD, PD]
         L.rp_last_error.argtypes = [vp]
         L.rp_last_error.restype = C.c_char_p
         L.rp_destroy.argtypes = [vp]
@@ -229,6 +233,29 @@ class Context:
         self._chk("rp_step_batch", self._L.rp_step_batch(
             self.h, k, _d(ex), _d(ix), ets.ctypes.data_as(PI32), slots.ctypes.data_as(PI32),
             sidx.ctypes.data_as(PI64), diag.ctypes.data_as(PI32), int(bool(use_graph))))
+
+    def set_perturbation(self, coef):
+        """coef: array [2][ny][nx] (du, dv in m/s) or None."""
+        if coef is None:
+            self._chk("rp_set_perturbation", self._L.rp_set_perturbation(self.h, 0, 0, None))
+            return
+        c = _f64(coef)
+        self._chk("rp_set_perturbation", self._L.rp_set_perturbation(
+            self.h, int(c.shape[2]), int(c.shape[1]), _d(c)))
+
+    def adjoint_record(self, ckpt_every):
+        self._chk("rp_adjoint_record", self._L.rp_adjoint_record(self.h, int(ckpt_every)))
+
+    def adjoint(self, lam, want_pert=False, pert_shape=None):
+        """lam: dict or tuple of dL/dx, dL/dy, dL/dz (release order).  Returns
+        (gx, gy, gz) and, with want_pert, dL/dcoef shaped pert_shape."""
+        lx, ly, lz = (_f64(a) for a in lam)
+        n = self.n
+        gx, gy, gz = np.empty(n), np.empty(n), np.empty(n)
+        gp = np.empty(pert_shape) if want_pert else None
+        self._chk("rp_adjoint", self._L.rp_adjoint(self.h, n, _d(lx), _d(ly), _d(lz), _d(gx), _d(gy),
+                                                   _d(gz), _d(gp)))
+        return (gx, gy, gz), gp
 
     def get_particles(self):
         n = self.n

@@ -14,8 +14,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "lib" / "librp.so"
-SOURCES = ["rp_step.hip", "rp_sort.hip", "rp_abi.hip"]
-HEADERS = ["rp_device.h", "rp_kernels.h"]
+SOURCES = ["rp_step.hip", "rp_adjoint.hip", "rp_sort.hip", "rp_abi.hip"]
+HEADERS = ["rp_device.h", "rp_kernels.h", "rp_dual.h", "rp_step_dev.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -54,22 +54,42 @@ def build_fortran(verbose: bool = True) -> Path:
     return DRIVER
 
 
-def build(verbose: bool = True) -> Path:
-    LIB.parent.mkdir(parents=True, exist_ok=True)
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *FLAGS, "-I", str(ROOT / "include"), "-o", str(tmp),
-           *[str(CSRC / s) for s in SOURCES]]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
-    if verbose and r.stderr.strip():
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+
+
+def _check(proc, what, verbose):
+    out, err = proc.communicate()
+    if proc.returncode != 0:
+        sys.stderr.write(out + err)
+        raise RuntimeError(f"{what} failed ({proc.returncode})")
+    if verbose and err.strip():
         # hipcc's host pass does not know the packed-fp32 feature: harmless
-        lines = [l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l]
+        lines = [l for l in err.splitlines() if "packed-fp32-ops" not in l]
         if lines:
             sys.stderr.write("\n".join(lines) + "\n")
+
+
+def build(verbose: bool = True) -> Path:
+    """Compile every source to an object in parallel, then link librp.so."""
+    LIB.parent.mkdir(parents=True, exist_ok=True)
+    objdir = LIB.parent / "obj"
+    objdir.mkdir(exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    procs = []
+    objs = []
+    for src in SOURCES:
+        obj = objdir / (Path(src).stem + ".o")
+        objs.append(str(obj))
+        procs.append((src, _run([HIPCC, *compile_flags, "-I", str(ROOT / "include"), "-c",
+                                 str(CSRC / src), "-o", str(obj)], verbose)))
+    for src, proc in procs:
+        _check(proc, f"hipcc {src}", verbose)
+    tmp = LIB.with_suffix(".so.tmp")
+    _check(_run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs],
+                verbose), "link", verbose)
     os.replace(tmp, LIB)
     build_fortran(verbose)
     return LIB

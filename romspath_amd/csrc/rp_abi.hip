@@ -82,6 +82,24 @@ struct Ctx {
     // ensembles: scenario s holds stored positions first[s] .. first[s]+count[s]-1
     std::vector<int64_t> scen_first, scen_count, scen_steps;
     hipGraphExec_t gexec = nullptr;      // last captured batch of steps
+    // velocity perturbation (rp_set_perturbation), device copy or null
+    double* pert = nullptr;
+    int pert_nx = 0, pert_ny = 0;
+    // adjoint recording (rp_adjoint_record)
+    struct AdjSnap {
+        void* f[kMaxGrid][5];         // copies of zeta, u, v, w, aks of every grid
+        DGrid* dtab;                  // device grid table pointing at them
+    };
+    struct AdjRec {
+        int K = 0;                    // checkpoint interval, 0 = not recording
+        bool dirty = true;            // hydro slots changed since the last snapshot
+        std::vector<StepArgs> args;   // per recorded step (particle pointers set at launch)
+        std::vector<int> snap;        // snapshot of each step
+        std::vector<AdjSnap> snaps;
+        std::vector<char*> ckpt;      // state at steps 0, K, 2K, ...
+        std::vector<double*> devs;    // injected deviates of each step, or null
+        unsigned long long* active = nullptr;   // counter for recomputed steps
+    } adj;
     double phase_ms[2] = {0.0, 0.0};     // averages of the last rp_kernel_timing call
     std::string err;
 };
@@ -406,6 +424,9 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     A.work = c->work;
     A.capped = c->capped;
     A.mid = c->mid;
+    A.pert = c->pert;
+    A.pert_nx = c->pert_nx;
+    A.pert_ny = c->pert_ny;
     if (vt && A.np < 3) return fail(c, RP_E_ARG, "smoothing spline needs sub*(N+1) >= 3");
 
     const int block = c->cfg.block > 0 ? c->cfg.block : 256;
@@ -514,6 +535,179 @@ int enqueue_step(Ctx* c, StepPlan& P, const Ctx::StepEvents* ev)
     HIPCHK(c, hipGetLastError());
     if (ev) HIPCHK(c, hipEventRecord(ev->e[2], c->stream));
     return RP_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// adjoint recording and backward sweep
+// ---------------------------------------------------------------------------
+size_t field_bytes(const Ctx* c, const GridDev& g, int f)
+{
+    const size_t fb = (size_t)c->fbytes * 4;   // four interleaved slots
+    switch (f) {
+    case 0: return (size_t)g.L * g.M * fb;
+    case 1: return (size_t)(g.L - 1) * g.M * g.N * fb;
+    case 2: return (size_t)g.L * (g.M - 1) * g.N * fb;
+    default: return (size_t)g.L * g.M * (g.N + 1) * fb;
+    }
+}
+
+// bytes of one saved particle state: x y z age (FP64), status gid isin (bytes)
+size_t state_bytes(int64_t n) { return (size_t)n * (4 * sizeof(double) + 3); }
+
+void copy_state(Ctx* c, char* buf, bool save)
+{
+    const int64_t n = c->n;
+    double* d[4] = {c->x, c->y, c->z, c->age};
+    void* b[3] = {c->status, c->gid, c->isin};
+    for (int k = 0; k < 4; k++) {
+        char* p = buf + (size_t)k * n * sizeof(double);
+        if (save) (void)hipMemcpyAsync(p, d[k], n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+        else (void)hipMemcpyAsync(d[k], p, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+    }
+    for (int k = 0; k < 3; k++) {
+        char* p = buf + (size_t)4 * n * sizeof(double) + (size_t)k * n;
+        if (save) (void)hipMemcpyAsync(p, b[k], n, hipMemcpyDeviceToDevice, c->stream);
+        else (void)hipMemcpyAsync(b[k], p, n, hipMemcpyDeviceToDevice, c->stream);
+    }
+}
+
+void free_recording(Ctx* c)
+{
+    Ctx::AdjRec& R = c->adj;
+    for (auto& sn : R.snaps) {
+        for (int g = 0; g < kMaxGrid; g++)
+            for (int f = 0; f < 5; f++) dfree(sn.f[g][f]);
+        dfree(sn.dtab);
+    }
+    for (auto& p : R.ckpt) dfree(p);
+    for (auto& p : R.devs) dfree(p);
+    R.snaps.clear();
+    R.ckpt.clear();
+    R.devs.clear();
+    R.args.clear();
+    R.snap.clear();
+    R.dirty = true;
+}
+
+// copy of the current hydro slots of every grid, with its device grid table
+int take_snapshot(Ctx* c)
+{
+    Ctx::AdjSnap sn;
+    std::memset(&sn, 0, sizeof(sn));
+    DGrid tab[kMaxGrid];
+    std::memcpy(tab, c->tab_host, sizeof(tab));
+    for (int g = 0; g < c->cfg.ngrid; g++) {
+        GridDev& G = c->grids[g];
+        void* src[5] = {G.zeta, G.u, G.v, G.w, G.aks};
+        for (int f = 0; f < 5; f++) {
+            const size_t bytes = field_bytes(c, G, f);
+            if (dalloc(c, (char**)&sn.f[g][f], bytes)) return RP_E_ALLOC;
+            HIPCHK(c, hipMemcpyAsync(sn.f[g][f], src[f], bytes, hipMemcpyDeviceToDevice, c->stream));
+        }
+        tab[g].zeta = sn.f[g][0]; tab[g].u = sn.f[g][1]; tab[g].v = sn.f[g][2];
+        tab[g].w = sn.f[g][3]; tab[g].aks = sn.f[g][4];
+    }
+    if (dalloc(c, &sn.dtab, kMaxGrid)) return RP_E_ALLOC;
+    HIPCHK(c, hipMemcpyAsync(sn.dtab, tab, sizeof(tab), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->adj.snaps.push_back(sn);
+    c->adj.dirty = false;
+    return RP_OK;
+}
+
+using AdjFwdFn = void (*)(const DGrid*, StepArgs);
+using AdjStepFn = void (*)(const DGrid*, StepArgs, AdjArgs);
+
+AdjFwdFn adj_forward_fn(const Ctx* c)
+{
+    const bool f32 = c->fbytes == 4, nested = c->cfg.ngrid > 1;
+    if (f32) return nested ? adj_forward_kernel<float, true> : adj_forward_kernel<float, false>;
+    return nested ? adj_forward_kernel<double, true> : adj_forward_kernel<double, false>;
+}
+
+AdjStepFn adj_step_fn(const Ctx* c)
+{
+    const bool f32 = c->fbytes == 4, nested = c->cfg.ngrid > 1;
+    if (f32) return nested ? adj_step_kernel<float, true> : adj_step_kernel<float, false>;
+    return nested ? adj_step_kernel<double, true> : adj_step_kernel<double, false>;
+}
+
+// persistent grid of a queue kernel and the scratch it needs (rows x elem bytes per lane)
+int queue_grid(Ctx* c, const void* fn, int block, size_t dyn_lds, int64_t n, int64_t& grid)
+{
+    int nb = 0;
+    HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, dyn_lds));
+    grid = std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, (int64_t)c->ncu * std::max(nb, 1)));
+    return RP_OK;
+}
+
+int ensure_scratch(Ctx* c, size_t need)
+{
+    if (c->scratch_bytes < need) {
+        const int rc = dalloc(c, (char**)&c->scratch, need);
+        if (rc) return rc;
+        c->scratch_bytes = need;
+    }
+    return RP_OK;
+}
+
+int max_levels(const Ctx* c)
+{
+    int maxN = 0;
+    for (int i = 0; i < c->cfg.ngrid; i++) maxN = std::max(maxN, c->grids[i].N);
+    return maxN;
+}
+
+// one forward step of a recorded run (spline on the reference-order path)
+int adj_forward_launch(Ctx* c, StepArgs A, const DGrid* tab)
+{
+    const int block = 256;
+    const AdjFwdFn fn = adj_forward_fn(c);
+    int64_t grid = 0;
+    int rc = queue_grid(c, (const void*)fn, block, 0, A.n, grid);
+    if (rc) return rc;
+    if (A.vturb_on) {
+        const int64_t T = grid * block;
+        if ((rc = ensure_scratch(c, (size_t)vturb_rows(A.np, max_levels(c) + 1) * T * sizeof(double))))
+            return rc;
+        A.scratch = c->scratch;
+    }
+    A.x = c->x; A.y = c->y; A.z = c->z; A.age = c->age; A.dob = c->dob;
+    A.status = c->status; A.gid = c->gid; A.isin = c->isin;
+    A.wd = c->wd; A.zeta = c->zeta; A.bath = c->bath;
+    A.id = c->id;
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, c->stream, c->work);
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(block), 0, c->stream, tab, A);
+    HIPCHK(c, hipGetLastError());
+    return RP_OK;
+}
+
+// recorded rp_step: checkpoint, field snapshot and deviates as needed, then the step
+int record_step(Ctx* c, StepArgs A)
+{
+    Ctx::AdjRec& R = c->adj;
+    const int64_t k = (int64_t)R.args.size();
+    if (k % R.K == 0) {
+        char* buf = nullptr;
+        if (dalloc(c, &buf, state_bytes(c->n))) return RP_E_ALLOC;
+        copy_state(c, buf, true);
+        R.ckpt.push_back(buf);
+    }
+    if (R.dirty) {
+        const int rc = take_snapshot(c);
+        if (rc) return rc;
+    }
+    double* dv = nullptr;
+    if (A.dev) {
+        if (dalloc(c, &dv, (size_t)c->n * c->ndev)) return RP_E_ALLOC;
+        HIPCHK(c, hipMemcpyAsync(dv, A.dev, (size_t)c->n * c->ndev * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+    }
+    R.devs.push_back(dv);
+    R.snap.push_back((int)R.snaps.size() - 1);
+    R.args.push_back(A);
+    return adj_forward_launch(c, A, c->dgrids);
 }
 
 }  // namespace
@@ -638,6 +832,7 @@ int rp_set_hydro_slot(void* ctx, int32_t ng, int32_t slot, const double* zeta, c
     GridDev& g = c->grids[ng - 1];
     const int s = slot - 1;
     int rc;
+    c->adj.dirty = true;
     if ((rc = upload_field(c, g.zeta, zeta, nullptr, g.L, g.M, 1, s)) ||
         (rc = upload_field(c, g.u, u, su, g.L - 1, g.M, g.N, s)) ||
         (rc = upload_field(c, g.v, v, sv, g.L, g.M - 1, g.N, s)) ||
@@ -676,6 +871,8 @@ int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, con
     c->n = n;
     c->steps_done = 0;
     c->ranges_intact = true;
+    free_recording(c);
+    c->adj.K = 0;
     c->scen_first.clear();
     c->scen_count.clear();
     c->scen_steps.clear();
@@ -723,6 +920,17 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     if (rc) return rc;
     Ctx::StepEvents ev;
     if ((rc = take_events(c, ev))) return rc;
+    if (c->adj.K > 0) {
+        // recorded run: no re-sort (the adjoint indexes the stored order)
+        HIPCHK(c, hipEventRecord(ev.e[0], c->stream));
+        if ((rc = record_step(c, P.A))) return rc;
+        HIPCHK(c, hipEventRecord(ev.e[1], c->stream));
+        HIPCHK(c, hipEventRecord(ev.e[2], c->stream));
+        c->steps_done++;
+        c->ev_free.pop_back();
+        c->ev_used.push_back(ev);
+        return RP_OK;
+    }
     HIPCHK(c, hipEventRecord(ev.e[0], c->stream));
     if (c->cfg.sort_interval > 0 && c->steps_done % c->cfg.sort_interval == 0) {
         if ((rc = sort_range(c, 0, c->n))) return rc;
@@ -779,6 +987,7 @@ int rp_step_scenario(void* ctx, int32_t scen, const double ex[3], const double i
     if (!c || !ex || !ix) return RP_E_ARG;
     if (scen < 0 || scen >= (int32_t)c->scen_first.size())
         return fail(c, RP_E_ARG, "scenario index out of range (rp_set_scenarios)");
+    if (c->adj.K > 0) return fail(c, RP_E_STATE, "rp_step_scenario: a recorded run steps with rp_step");
     const int64_t q0 = c->scen_first[scen], cnt = c->scen_count[scen];
     if (cnt == 0) return check_step(c, t_b, t_c, t_f, step_index);
     StepPlan P;
@@ -808,6 +1017,7 @@ int rp_step_batch(void* ctx, int32_t nsteps, const double* ex, const double* ix,
     if (!c || nsteps < 0 || (nsteps > 0 && (!ex || !ix || !ets || !slots || !step_index)))
         return RP_E_ARG;
     if (nsteps == 0) return RP_OK;
+    if (c->adj.K > 0) return fail(c, RP_E_STATE, "rp_step_batch: a recorded run steps with rp_step");
     if (c->n == 0) {
         for (int32_t i = 0; i < nsteps; i++) {
             const int rc = check_step(c, slots[3 * i], slots[3 * i + 1], slots[3 * i + 2], step_index[i]);
@@ -875,6 +1085,161 @@ int rp_step_batch(void* ctx, int32_t nsteps, const double* ex, const double* ix,
     HIPCHK(c, hipEventRecord(ev.e[2], c->stream));
     c->ev_free.pop_back();
     c->ev_used.push_back(ev);
+    return RP_OK;
+}
+
+int rp_set_perturbation(void* ctx, int32_t nx, int32_t ny, const double* coef)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (!coef) {
+        dfree(c->pert);
+        c->pert_nx = c->pert_ny = 0;
+        return RP_OK;
+    }
+    if (nx < 4 || ny < 4 || nx > 64 || ny > 64)
+        return fail(c, RP_E_ARG, "perturbation control net must be 4..64 points each way");
+    if (c->adj.K > 0 && (nx != c->pert_nx || ny != c->pert_ny || !c->pert))
+        return fail(c, RP_E_STATE, "perturbation shape fixed while recording");
+    const size_t cnt = 2 * (size_t)nx * ny;
+    if (!c->pert || nx != c->pert_nx || ny != c->pert_ny) {
+        if (dalloc(c, &c->pert, cnt)) return RP_E_ALLOC;
+    }
+    c->pert_nx = nx;
+    c->pert_ny = ny;
+    HIPCHK(c, hipMemcpyAsync(c->pert, coef, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RP_OK;
+}
+
+int rp_adjoint_record(void* ctx, int32_t ckpt_every)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || ckpt_every < 0) return RP_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    free_recording(c);
+    c->adj.K = ckpt_every;
+    if (ckpt_every > 0) {
+        if (c->n == 0) return fail(c, RP_E_STATE, "rp_adjoint_record: no particles");
+        if (!c->adj.active && dalloc(c, &c->adj.active, 1)) return RP_E_ALLOC;
+    }
+    return RP_OK;
+}
+
+int rp_adjoint(void* ctx, int64_t n, const double* lam_x, const double* lam_y, const double* lam_z,
+               double* g_x, double* g_y, double* g_z, double* g_pert)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !lam_x || !lam_y || !lam_z || !g_x || !g_y || !g_z) return RP_E_ARG;
+    Ctx::AdjRec& R = c->adj;
+    if (R.K <= 0 || R.args.empty()) return fail(c, RP_E_STATE, "rp_adjoint: no recorded steps");
+    if (n != c->n) return fail(c, RP_E_ARG, "particle count differs from rp_set_particles");
+    if (g_pert && !c->pert) return fail(c, RP_E_STATE, "rp_adjoint: no perturbation set");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t nsteps = (int64_t)R.args.size();
+    const int K = R.K;
+    // lambda in stored order
+    std::vector<uint32_t> ids(n);
+    HIPCHK(c, hipMemcpy(ids.data(), c->id, n * 4, hipMemcpyDeviceToHost));
+    std::vector<double> hl(3 * (size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        hl[i] = lam_x[ids[i]];
+        hl[n + i] = lam_y[ids[i]];
+        hl[2 * n + i] = lam_z[ids[i]];
+    }
+    double *lam = nullptr, *gp = nullptr;
+    char *fin = nullptr, *seg = nullptr;
+    const size_t npert = c->pert ? 2 * (size_t)c->pert_nx * c->pert_ny : 0;
+    // per saved step: x y z (FP64) + status gid isin
+    const size_t seg_step = (size_t)n * (3 * sizeof(double) + 3);
+    auto cleanup = [&]() { dfree(lam); dfree(gp); dfree(fin); dfree(seg); };
+    if (dalloc(c, &lam, 3 * (size_t)n) || (g_pert && dalloc(c, &gp, npert)) ||
+        dalloc(c, &fin, state_bytes(n)) || dalloc(c, &seg, seg_step * K)) {
+        cleanup();
+        return RP_E_ALLOC;
+    }
+    int rc = RP_OK;
+    auto run = [&]() -> int {
+        HIPCHK(c, hipMemcpyAsync(lam, hl.data(), hl.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        if (gp) hipLaunchKernelGGL(adj_fill, dim3(8), dim3(256), 0, c->stream, gp, (int64_t)npert, 0.0);
+        copy_state(c, fin, true);
+        const int block = 256;
+        const AdjStepFn sfn = adj_step_fn(c);
+        const size_t lds = gp ? npert * sizeof(double) : 0;
+        int64_t grid_b = 0;
+        int r = queue_grid(c, (const void*)sfn, block, lds, n, grid_b);
+        if (r) return r;
+        const int64_t Tb = grid_b * block;
+        const int np = R.args[0].np;
+        auto state_at = [&](int64_t j) {
+            AdjState s;
+            char* b = seg + seg_step * j;
+            s.x = (double*)b;
+            s.y = s.x + n;
+            s.z = s.y + n;
+            s.status = (int8_t*)(s.z + n);
+            s.gid = s.status + n;
+            s.isin = (uint8_t*)(s.gid + n);
+            return s;
+        };
+        const int64_t nseg = (nsteps + K - 1) / K;
+        const unsigned gsave = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+        for (int64_t sgi = nseg - 1; sgi >= 0; sgi--) {
+            const int64_t k0 = sgi * K, k1 = std::min<int64_t>(k0 + K, nsteps);
+            copy_state(c, R.ckpt[sgi], false);
+            for (int64_t k = k0; k < k1; k++) {
+                const AdjState st = state_at(k - k0);
+                hipLaunchKernelGGL(adj_save_kernel, dim3(gsave), dim3(256), 0, c->stream, st, c->x,
+                                   c->y, c->z, c->status, c->gid, c->isin, n);
+                if (k + 1 < k1) {
+                    StepArgs A = R.args[k];
+                    A.dev = R.devs[k];
+                    A.active = R.active;
+                    if ((r = adj_forward_launch(c, A, R.snaps[R.snap[k]].dtab))) return r;
+                }
+            }
+            for (int64_t k = k1 - 1; k >= k0; k--) {
+                StepArgs A = R.args[k];
+                const AdjState st = state_at(k - k0);
+                A.x = st.x; A.y = st.y; A.z = st.z;
+                A.status = st.status; A.gid = st.gid; A.isin = st.isin;
+                A.age = c->age; A.dob = c->dob; A.id = c->id;
+                A.wd = c->wd; A.zeta = c->zeta; A.bath = c->bath;
+                A.want_diag = 0;
+                A.dev = R.devs[k];
+                A.active = R.active;
+                if (A.vturb_on) {
+                    const size_t need = (size_t)vturb_rows(np, max_levels(c) + 1) * Tb * sizeof(Dual<3>);
+                    if ((r = ensure_scratch(c, need))) return r;
+                }
+                A.scratch = c->scratch;
+                AdjArgs J;
+                J.lx = lam;
+                J.ly = lam + n;
+                J.lz = lam + 2 * n;
+                J.gpert = gp;
+                hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, c->stream, c->work);
+                hipLaunchKernelGGL(sfn, dim3((unsigned)grid_b), dim3(block), lds, c->stream,
+                                   R.snaps[R.snap[k]].dtab, A, J);
+                HIPCHK(c, hipGetLastError());
+            }
+        }
+        copy_state(c, fin, false);
+        HIPCHK(c, hipMemcpyAsync(hl.data(), lam, hl.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        if (gp) HIPCHK(c, hipMemcpyAsync(g_pert, gp, npert * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return RP_OK;
+    };
+    rc = run();
+    cleanup();
+    if (rc) return rc;
+    for (int64_t i = 0; i < n; i++) {
+        g_x[ids[i]] = hl[i];
+        g_y[ids[i]] = hl[n + i];
+        g_z[ids[i]] = hl[2 * n + i];
+    }
     return RP_OK;
 }
 
@@ -1016,6 +1381,9 @@ int rp_destroy(void* ctx)
     dfree(c->wd); dfree(c->zeta); dfree(c->bath);
     dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    free_recording(c);
+    dfree(c->adj.active);
+    dfree(c->pert);
     dfree(c->sort_cnt); dfree(c->perm); dfree(c->alt_id);
     for (auto& a : c->alt64) dfree(a);
     for (auto& a : c->alt8) dfree(a);

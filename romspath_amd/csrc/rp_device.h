@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "rp_dual.h"
+
 namespace rp {
 
 constexpr int kMaxGrid = 5;
@@ -75,8 +77,8 @@ __device__ __forceinline__ double pick(const Slots4<F>& o, int s)
 // interpolation_module.f90:72-109  (exact 3-point quadratic, Newton form
 // based at the node nearest x)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double polintd(const double xa[3], double y0, double y1, double y2,
-                                          double x)
+template <class R>
+__device__ __forceinline__ R polintd(const double xa[3], R y0, R y1, R y2, double x)
 {
     int ns = 1;
     double dif = fabs(x - xa[0]);
@@ -84,10 +86,11 @@ __device__ __forceinline__ double polintd(const double xa[3], double y0, double 
     if (dift < dif) { ns = 2; dif = dift; }
     dift = fabs(x - xa[2]);
     if (dift < dif) { ns = 3; dif = dift; }
-    double c = (xa[1] - x) * ((y2 - y1) / (xa[1] - xa[2]));
+    R c = (xa[1] - x) * ((y2 - y1) / (xa[1] - xa[2]));
     c = c - (xa[1] - x) * ((y1 - y0) / (xa[0] - xa[1]));
     c = c / (xa[0] - xa[2]);
-    double a, b;
+    R a;
+    double b;
     if (ns == 3) {
         a = (y2 - y1) / (xa[1] - xa[2]);
         b = xa[2] - x;
@@ -95,20 +98,21 @@ __device__ __forceinline__ double polintd(const double xa[3], double y0, double 
         a = (y1 - y0) / (xa[0] - xa[1]);
         b = xa[0] - x;
     }
-    const double yn = ns == 1 ? y0 : (ns == 2 ? y1 : y2);
+    const R yn = ns == 1 ? y0 : (ns == 2 ? y1 : y2);
     const double xn = ns == 1 ? xa[0] : (ns == 2 ? xa[1] : xa[2]);
     return yn + (xn - x) * a + b * c;
 }
 
 // interpolation_module.f90:1230-1273 (masked inverse-distance weights)
-__device__ __forceinline__ void getcoeff(double X, double Y, const double m[4], double cff[4])
+template <class R>
+__device__ __forceinline__ void getcoeff(R X, R Y, const double m[4], R cff[4])
 {
-    const double d0 = sqrt(X * X + Y * Y);
-    const double d1 = sqrt((1 - X) * (1 - X) + Y * Y);
-    const double d2 = sqrt((1 - X) * (1 - X) + (1 - Y) * (1 - Y));
-    const double d3 = sqrt(X * X + (1 - Y) * (1 - Y));
-    const double w0 = m[0] / d0, w1 = m[1] / d1, w2 = m[2] / d2, w3 = m[3] / d3;
-    const double wt = w0 + w1 + w2 + w3;
+    const R d0 = sqrt(X * X + Y * Y);
+    const R d1 = sqrt((1.0 - X) * (1.0 - X) + Y * Y);
+    const R d2 = sqrt((1.0 - X) * (1.0 - X) + (1.0 - Y) * (1.0 - Y));
+    const R d3 = sqrt(X * X + (1.0 - Y) * (1.0 - Y));
+    const R w0 = m[0] / d0, w1 = m[1] / d1, w2 = m[2] / d2, w3 = m[3] / d3;
+    const R wt = w0 + w1 + w2 + w3;
     if (wt == 0.0) {
         cff[0] = cff[1] = cff[2] = cff[3] = 0.0;
     } else {
@@ -116,50 +120,53 @@ __device__ __forceinline__ void getcoeff(double X, double Y, const double m[4], 
     }
 }
 
-__device__ __forceinline__ void bilin(double x, double y, double c[4])
+template <class R>
+__device__ __forceinline__ void bilin(R x, R y, R c[4])
 {
-    c[0] = (1 - x) * (1 - y);
-    c[1] = x * (1 - y);
+    c[0] = (1.0 - x) * (1.0 - y);
+    c[1] = x * (1.0 - y);
     c[2] = x * y;
-    c[3] = (1 - x) * y;
+    c[3] = (1.0 - x) * y;
 }
 
 // grid_module.f90:513-598.  S(k) does not depend on zeta, so callers that
 // need several slots at one level compute it once (same value bit for bit).
-__device__ __forceinline__ double level_S(const DGrid& g, const double* s, const double* cs,
-                                          int k, double h)
+template <class R>
+__device__ __forceinline__ R level_S(const DGrid& g, const double* s, const double* cs, int k, R h)
 {
     if (g.vtransform == 2) return (g.hc * s[k - 1] + h * cs[k - 1]) / (g.hc + h);
     if (g.vtransform == 1) return g.hc * s[k - 1] + (h - g.hc) * cs[k - 1];
-    return 0.0;
+    return R(0.0);
 }
-__device__ __forceinline__ double level_z(const DGrid& g, const double* s, const double* cs,
-                                          int k, double zeta, double h, double S)
+template <class R>
+__device__ __forceinline__ R level_z(const DGrid& g, const double* s, const double* cs, int k,
+                                     R zeta, R h, R S)
 {
     if (g.vtransform == 2) return zeta + (zeta + h) * S;
     if (g.vtransform == 1) return S + zeta * (1.0 + S / h);
     return zeta * (1.0 + s[k - 1]) + g.hc * s[k - 1] + (h - g.hc) * cs[k - 1];
 }
-__device__ __forceinline__ double level(const DGrid& g, const double* s, const double* cs, int k,
-                                        double zeta, double h)
+template <class R>
+__device__ __forceinline__ R level(const DGrid& g, const double* s, const double* cs, int k, R zeta,
+                                   R h)
 {
     return level_z(g, s, cs, k, zeta, h, level_S(g, s, cs, k, h));
 }
 
 // Level z(k) for one column: the FP32-field form multiplies by 1/(hc+h)
 // computed once instead of dividing at every level.
-template <bool FAST>
+template <bool FAST, class R = double>
 struct Levels {
     const DGrid& g;
     const double *s, *cs;
-    double zeta, h, inv;
-    __device__ __forceinline__ Levels(const DGrid& g_, const double* s_, const double* cs_,
-                                      double zeta_, double h_)
+    R zeta, h, inv;
+    __device__ __forceinline__ Levels(const DGrid& g_, const double* s_, const double* cs_, R zeta_,
+                                      R h_)
         : g(g_), s(s_), cs(cs_), zeta(zeta_), h(h_)
     {
-        inv = (FAST && g.vtransform == 2) ? 1.0 / (g.hc + h) : 0.0;
+        inv = (FAST && g.vtransform == 2) ? R(1.0 / (g.hc + h)) : R(0.0);
     }
-    __device__ __forceinline__ double operator()(int k) const
+    __device__ __forceinline__ R operator()(int k) const
     {
         if (FAST && g.vtransform == 2) return zeta + (zeta + h) * ((g.hc * s[k - 1] + h * cs[k - 1]) * inv);
         return level(g, s, cs, k, zeta, h);
@@ -169,8 +176,8 @@ struct Levels {
 // First k in [1, nz] with level(k) > Z (nz+1 if none).  The reference scans
 // linearly (interpolation_module.f90:533-577); the levels are non-decreasing
 // in k, so the predicate is monotone and bisection returns the same index.
-template <class Lev>
-__device__ __forceinline__ int first_above(const Lev& lev, int nz, double Z)
+template <class Lev, class R>
+__device__ __forceinline__ int first_above(const Lev& lev, int nz, R Z)
 {
     int lo = 1, hi = nz + 1;
     while (lo < hi) {
@@ -182,49 +189,51 @@ __device__ __forceinline__ int first_above(const Lev& lev, int nz, double Z)
 }
 
 // Vertical bracket of getInterp3d (interpolation_module.f90:528-589).
-struct VBracket {
+template <class R = double>
+struct VBracketT {
     int ibot, itop;
-    double c0, c1;
+    R c0, c1;
 };
+using VBracket = VBracketT<double>;
 
-template <bool FAST>
-__device__ __forceinline__ VBracket bracket_rho(const DGrid& g, double Z, double zeta, double depth)
+template <bool FAST, class R>
+__device__ __forceinline__ VBracketT<R> bracket_rho(const DGrid& g, R Z, R zeta, R depth)
 {
-    const double h = -1.0 * depth;
+    const R h = -1.0 * depth;
     const int nz = g.N;
-    const Levels<FAST> lev(g, g.s_rho, g.cs_r, zeta, h);
+    const Levels<FAST, R> lev(g, g.s_rho, g.cs_r, zeta, h);
     const int i = first_above(lev, nz, Z);
-    VBracket b;
+    VBracketT<R> b;
     if (i > nz) {
         b.ibot = b.itop = nz; b.c0 = 0.5; b.c1 = 0.5;
     } else if (i == 1) {
         b.ibot = b.itop = 1; b.c0 = 0.5; b.c1 = 0.5;
     } else {
-        const double z1 = lev(i - 1);
-        const double z2 = lev(i);
-        const double Zr = (Z - z1) / (z2 - z1);
+        const R z1 = lev(i - 1);
+        const R z2 = lev(i);
+        const R Zr = (Z - z1) / (z2 - z1);
         b.ibot = i - 1; b.itop = i; b.c0 = 1.0 - Zr; b.c1 = Zr;
     }
     return b;
 }
 
-template <bool FAST>
-__device__ __forceinline__ VBracket bracket_w(const DGrid& g, double Z, double zeta, double depth)
+template <bool FAST, class R>
+__device__ __forceinline__ VBracketT<R> bracket_w(const DGrid& g, R Z, R zeta, R depth)
 {
-    const double h = -1.0 * depth;
+    const R h = -1.0 * depth;
     const int nz = g.N + 1;
-    const Levels<FAST> lev(g, g.s_w, g.cs_w, zeta, h);
+    const Levels<FAST, R> lev(g, g.s_w, g.cs_w, zeta, h);
     const int i = first_above(lev, nz, Z);
-    VBracket b;
+    VBracketT<R> b;
     if (i > nz) {
         // z at or above the top w level: two copies of level nz (the
         // reference's weight is undefined there; the value is not).
         b.ibot = b.itop = nz; b.c0 = 0.5; b.c1 = 0.5;
     } else {
         const int ib = i > 1 ? i - 1 : 1;
-        const double z1 = lev(ib);
-        const double z2 = lev(i);
-        const double Zr = (Z - z1) / (z2 - z1);
+        const R z1 = lev(ib);
+        const R z2 = lev(i);
+        const R Zr = (Z - z1) / (z2 - z1);
         b.ibot = ib; b.itop = i; b.c0 = 1.0 - Zr; b.c1 = Zr;
     }
     return b;

@@ -45,6 +45,8 @@ struct StepArgs {
     unsigned long long* active;  // += particles that took the step
     unsigned long long* work;    // vwalk_kernel's next unclaimed particle (zeroed per launch)
     unsigned* capped;            // += secant searches stopped at the cap
+    const double* pert;          // velocity perturbation coefficients [2][ny][nx] or null
+    int pert_nx, pert_ny;
 };
 
 // NESTED=false: one grid, all lanes read grid 0 (uniform, scalar loads).
@@ -99,6 +101,29 @@ __global__ void sort_permute(PermArgs P);
 __global__ void sort_copy_back(PermArgs P);
 __global__ void sort_restore(PermArgs P);
 __global__ void zero_u64(unsigned long long* p);
+
+// ---------------------------------------------------------------------------
+// adjoint (rp_adjoint.hip)
+// ---------------------------------------------------------------------------
+struct AdjState {                // particle state at the start of one step
+    double *x, *y, *z;
+    int8_t *status, *gid;
+    uint8_t* isin;
+};
+
+struct AdjArgs {
+    double *lx, *ly, *lz;        // lambda = dL/d(x, y, z), stored order, updated in place
+    double* gpert;               // dL/dc [2][ny][nx] accumulated, or null
+};
+
+template <typename F, bool NESTED>
+__global__ void adj_forward_kernel(const DGrid* __restrict__ grids, StepArgs A);
+template <typename F, bool NESTED>
+__global__ void adj_step_kernel(const DGrid* __restrict__ grids, StepArgs A, AdjArgs J);
+__global__ void adj_save_kernel(AdjState dst, const double* x, const double* y, const double* z,
+                                const int8_t* status, const int8_t* gid, const uint8_t* isin,
+                                int64_t n);
+__global__ void adj_fill(double* a, int64_t n, double v);
 
 // rows of VTurb scratch per lane for np resampled points and nw w-levels
 inline int64_t vturb_rows(int np, int nw) { return 9LL * np + 2LL * nw; }

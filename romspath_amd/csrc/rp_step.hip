@@ -1,1217 +1,7 @@
-// rp_step.hip -- the particle-step kernel (one lane per particle) and the
-// field-upload kernels.  Replaces update_particles (ROMSPath.f90:662-1003) and
-// everything it calls for Behavior=0.
-#include "rp_kernels.h"
+// rp_step.hip -- the particle-step kernels and the field-upload kernels.
+#include "rp_step_dev.h"
 
 namespace rp {
-
-// polintd(ex, y, ix(2)) (interpolation_module.f90:72-109).  The FP64 form
-// evaluates the reference's Newton formula; the FP32-field form uses the
-// Lagrange weights of the same quadratic, computed once per launch.
-template <typename F>
-__device__ __forceinline__ double tinterp(const StepArgs& A, double y0, double y1, double y2)
-{
-    if (sizeof(F) == 8) return polintd(A.ex, y0, y1, y2, A.ix[1]);
-    return A.pw[0] * y0 + A.pw[1] * y1 + A.pw[2] * y2;
-}
-
-// ---------------------------------------------------------------------------
-// horizontal stencil of a rho-point field: floor cell, bilinear weights, the
-// four corner offsets (I,J) (I+1,J) (I+1,J+1) (I,J+1) in an (L, M) array.
-// ---------------------------------------------------------------------------
-struct RhoCell {
-    int I, J;
-    double x, y;
-    int64_t o[4];
-};
-
-// Corner indices are clamped into the grid.  For positions inside the grid
-// this changes nothing; it keeps every read in bounds where the reference
-// indexes outside its arrays (after a failed nested hand-off,
-// ROMSPath.f90:837-843, or for a non-finite position).
-__device__ __forceinline__ RhoCell rho_cell(const DGrid& g, double X, double Y)
-{
-    RhoCell c;
-    c.I = (int)floor(X);
-    c.J = (int)floor(Y);
-    c.x = X - c.I;
-    c.y = Y - c.J;
-    const int i0 = min(max(c.I, 1), g.L), i1 = min(max(c.I + 1, 1), g.L);
-    const int j0 = min(max(c.J, 1), g.M), j1 = min(max(c.J + 1, 1), g.M);
-    c.o[0] = (int64_t)(j0 - 1) * g.L + (i0 - 1);
-    c.o[1] = (int64_t)(j0 - 1) * g.L + (i1 - 1);
-    c.o[2] = (int64_t)(j1 - 1) * g.L + (i1 - 1);
-    c.o[3] = (int64_t)(j1 - 1) * g.L + (i0 - 1);
-    return c;
-}
-
-// getInterp2D("depth"|"pm"|"pn") -- plain bilinear (interpolation_module.f90:421-507)
-__device__ __forceinline__ double bilinear2d(const double* f, const RhoCell& c)
-{
-    double w[4];
-    bilin(c.x, c.y, w);
-    return w[0] * f[c.o[0]] + w[1] * f[c.o[1]] + w[2] * f[c.o[2]] + w[3] * f[c.o[3]];
-}
-
-// getInterp2D("zeta") at slots b, c, f: always inverse-distance weighted
-// (getcoeff is called unconditionally, interpolation_module.f90:486).
-template <typename F>
-__device__ __forceinline__ void zeta3(const DGrid& g, const RhoCell& c, const SlotSel& S,
-                                      double out[3])
-{
-    double m[4], w[4];
-    for (int k = 0; k < 4; k++) m[k] = (double)g.mrho[c.o[k]];
-    getcoeff(c.x, c.y, m, w);
-    const F* z = (const F*)g.zeta;
-    Slots4<F> v[4];
-    for (int k = 0; k < 4; k++) ld4(z + c.o[k] * 4, v[k]);
-    for (int t = 0; t < 3; t++) {
-        const int s = S.s[t];
-        out[t] = w[0] * pick(v[0], s) + w[1] * pick(v[1], s) + w[2] * pick(v[2], s) +
-                 w[3] * pick(v[3], s);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// getInterp3d for u, v, w at the slots b, c, f (interpolation_module.f90:509-962)
-// The vertical bracket is shared by every slot: the reference recomputes it
-// per call from the same (z, zeta, depth), so the values are identical.
-// ---------------------------------------------------------------------------
-template <typename F, int NSLOT>
-__device__ __forceinline__ void blend3(const F* f, const int64_t col[4], int nk,
-                                       const double cff[4], const bool zero[4],
-                                       const VBracket& vb, const SlotSel& S, double out[3])
-{
-    double zb[3] = {0, 0, 0}, zt[3] = {0, 0, 0};
-    for (int k = 0; k < 4; k++) {
-        Slots4<F> lb, lt;
-        ld4(f + (col[k] * nk + (vb.ibot - 1)) * 4, lb);
-        ld4(f + (col[k] * nk + (vb.itop - 1)) * 4, lt);
-#pragma unroll
-        for (int t = 0; t < NSLOT; t++) {
-            const int s = S.s[t];
-            const double b = zero[k] ? 0.0 : pick(lb, s);
-            const double a = zero[k] ? 0.0 : pick(lt, s);
-            if (k == 0) {
-                zb[t] = cff[0] * b;
-                zt[t] = cff[0] * a;
-            } else {
-                zb[t] = zb[t] + cff[k] * b;
-                zt[t] = zt[t] + cff[k] * a;
-            }
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < NSLOT; t++) out[t] = vb.c0 * zb[t] + vb.c1 * zt[t];
-}
-
-template <typename F, int NSLOT>
-__device__ __forceinline__ void interp_u(const DGrid& g, double X, double Y, const VBracket& vb,
-                                        const SlotSel& S, double out[3])
-{
-    const int Jf = (int)floor(Y);
-    const double y = Y - Jf;
-    const int Jnode = min(max(Jf, 1), g.M - 1);   // in-range positions: unchanged
-    const double uvI = X - 0.5;
-    const int Inode = (int)floor(uvI);
-    const double x = uvI - Inode;
-    double cff[4];
-    bilin(x, y, cff);
-    const int xiu = g.L - 1;
-    int ia, ib;
-    double m[4];
-    auto MU = [&](int i, int j) { return (double)g.mu[(int64_t)(j - 1) * xiu + (i - 1)]; };
-    if (Inode >= xiu) {
-        ia = ib = xiu;
-        m[0] = MU(xiu, Jnode); m[1] = 0; m[2] = 0; m[3] = MU(xiu, Jnode + 1);
-    } else if (Inode < 1) {
-        ia = ib = 1;
-        m[0] = 0; m[1] = MU(1, Jnode); m[2] = MU(1, Jnode + 1); m[3] = 0;
-    } else {
-        ia = Inode; ib = Inode + 1;
-        m[0] = MU(ia, Jnode); m[1] = MU(ib, Jnode); m[2] = MU(ib, Jnode + 1); m[3] = MU(ia, Jnode + 1);
-    }
-    const double nwater = m[0] + m[1] + m[2] + m[3];
-    bool zero[4];
-    if (nwater < 4.0 && nwater > 0.0) {
-        for (int k = 0; k < 4; k++) zero[k] = (m[k] == 0.0);
-    } else if (nwater == 0.0) {
-        for (int k = 0; k < 4; k++) { zero[k] = true; cff[k] = 0.0; }
-    } else {
-        for (int k = 0; k < 4; k++) zero[k] = false;
-    }
-    int64_t col[4];
-    col[0] = (int64_t)(Jnode - 1) * xiu + (ia - 1);
-    col[1] = (int64_t)(Jnode - 1) * xiu + (ib - 1);
-    col[2] = (int64_t)(Jnode) * xiu + (ib - 1);
-    col[3] = (int64_t)(Jnode) * xiu + (ia - 1);
-    blend3<F, NSLOT>((const F*)g.u, col, g.N, cff, zero, vb, S, out);
-}
-
-template <typename F, int NSLOT>
-__device__ __forceinline__ void interp_v(const DGrid& g, double X, double Y, const VBracket& vb,
-                                        const SlotSel& S, double out[3])
-{
-    int Inode = (int)floor(X);
-    const double x = X - Inode;
-    const double uvJ = Y - 0.5;
-    int Jnode = (int)floor(uvJ);
-    if (Jnode < 1) Jnode = 1;
-    const int etav = g.M - 1;
-    if (Jnode > etav) Inode = etav;   // interpolation_module.f90:696 assigns Inode
-    Inode = min(max(Inode, 1), g.L - 1);   // in-range positions: unchanged
-    const double y = uvJ - Jnode;
-    double cff[4];
-    bilin(x, y, cff);
-    int ja, jb;
-    double m[4];
-    auto MV = [&](int i, int j) { return (double)g.mv[(int64_t)(j - 1) * g.L + (i - 1)]; };
-    if (Jnode >= etav) {
-        ja = jb = etav;
-        m[0] = MV(Inode, etav); m[1] = MV(Inode + 1, etav); m[2] = 0; m[3] = 0;
-    } else {
-        ja = Jnode; jb = Jnode + 1;
-        m[0] = MV(Inode, ja); m[1] = MV(Inode + 1, ja); m[2] = MV(Inode + 1, jb); m[3] = MV(Inode, jb);
-    }
-    const double nwater = m[0] + m[1] + m[2] + m[3];
-    bool zero[4];
-    if (nwater < 4.0 && nwater > 0.0) {
-        for (int k = 0; k < 4; k++) zero[k] = (m[k] == 0.0);
-    } else if (nwater == 0.0) {
-        for (int k = 0; k < 4; k++) { zero[k] = true; cff[k] = 0.0; }
-    } else {
-        for (int k = 0; k < 4; k++) zero[k] = false;
-    }
-    int64_t col[4];
-    col[0] = (int64_t)(ja - 1) * g.L + (Inode - 1);
-    col[1] = (int64_t)(ja - 1) * g.L + (Inode);
-    col[2] = (int64_t)(jb - 1) * g.L + (Inode);
-    col[3] = (int64_t)(jb - 1) * g.L + (Inode - 1);
-    blend3<F, NSLOT>((const F*)g.v, col, g.N, cff, zero, vb, S, out);
-}
-
-// masked rho-point weights for w and AKs: bilinear, or IDW when a corner is dry
-__device__ __forceinline__ void rho_weights(const DGrid& g, const RhoCell& c, double cff[4])
-{
-    double m[4];
-    for (int k = 0; k < 4; k++) m[k] = (double)g.mrho[c.o[k]];
-    bilin(c.x, c.y, cff);
-    if (m[0] + m[1] + m[2] + m[3] < 4.0) getcoeff(c.x, c.y, m, cff);
-}
-
-template <typename F, int NSLOT>
-__device__ __forceinline__ void interp_w(const DGrid& g, const RhoCell& c, const VBracket& vb,
-                                        const SlotSel& S, double out[3])
-{
-    double cff[4];
-    rho_weights(g, c, cff);
-    const bool zero[4] = {false, false, false, false};
-    blend3<F, NSLOT>((const F*)g.w, c.o, g.N + 1, cff, zero, vb, S, out);
-}
-
-// ---------------------------------------------------------------------------
-// find_currents (advection_module.f90:129-350), version=2: all stages use
-// time ix(2).  Z is clamped in place, as the reference's INTENT(INOUT) Zpar.
-// ---------------------------------------------------------------------------
-struct Surface {
-    double tdepth, zb, zc, zf, zeta;
-};
-
-template <typename F>
-__device__ __forceinline__ Surface surface_at(const DGrid& g, const StepArgs& A, const RhoCell& c)
-{
-    Surface s;
-    s.tdepth = -1.0 * bilinear2d(g.h, c);
-    double z3[3];
-    zeta3<F>(g, c, A.slots, z3);
-    s.zb = z3[0]; s.zc = z3[1]; s.zf = z3[2];
-    s.zeta = tinterp<F>(A, s.zb, s.zc, s.zf);
-    return s;
-}
-
-// NSLOT=2 on the first external step, whose u, v, w use slots (b,b,c)
-// (advection_module.f90:277-334); 3 otherwise.
-template <typename F, int NSLOT>
-__device__ __forceinline__ void find_currents(const DGrid& g, const StepArgs& A, double X,
-                                              double Y, double& Z, double& U, double& V,
-                                              double& W, Surface* surf_out)
-{
-    const RhoCell c = rho_cell(g, X, Y);
-    const Surface s = surface_at<F>(g, A, c);
-    if (surf_out) *surf_out = s;
-    if (Z < s.tdepth) Z = s.tdepth + (double)0.001f;
-    if (Z > s.zeta) Z = s.zeta - (double)0.001f;
-    const VBracket br = bracket_rho<sizeof(F) == 4>(g, Z, s.zeta, s.tdepth);
-    const VBracket bw = bracket_w<sizeof(F) == 4>(g, Z, s.zeta, s.tdepth);
-    double q[3];
-    interp_u<F, NSLOT>(g, X, Y, br, A.slots, q);
-    U = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
-    interp_v<F, NSLOT>(g, X, Y, br, A.slots, q);
-    V = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
-    interp_w<F, NSLOT>(g, c, bw, A.slots, q);
-    W = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
-}
-
-// ---------------------------------------------------------------------------
-// deviates: injected (test hook) or Philox, in the fixed per-particle layout
-// [HTurb x, HTurb y, VTurb sub-step 1..loop]
-// ---------------------------------------------------------------------------
-struct DevSrc {
-    const double* inj;
-    uint64_t seed, id, step;
-    int blk;
-    double buf[4];
-    __device__ __forceinline__ double get(int j)
-    {
-        if (inj) return inj[j];
-        const int b = j >> 2;
-        if (b != blk) {
-            normals4(seed, id, step, (uint32_t)b, buf);
-            blk = b;
-        }
-        const int r = j & 3;
-        return r == 0 ? buf[0] : (r == 1 ? buf[1] : (r == 2 ? buf[2] : buf[3]));
-    }
-};
-
-// ---------------------------------------------------------------------------
-// ver_turb_module.f90:31-329 with pppack.f90 smooth/setupq/chol1d/ppvalu.
-// Per-lane work arrays live in a global scratch area, one column per lane
-// (row r of lane t at r*T + t), so every access of a wavefront is one
-// contiguous 512-byte segment.
-// ---------------------------------------------------------------------------
-struct Scr {
-    double* p;
-    int64_t T;
-    __device__ __forceinline__ double& operator()(int row) const { return p[(int64_t)row * T]; }
-};
-
-// the same lane column viewed as FP32 rows (the FP32-field form's work arrays)
-struct Scr32 {
-    float* p;
-    int64_t T;
-    __device__ __forceinline__ float& operator()(int row) const { return p[(int64_t)row * T]; }
-};
-
-struct VtLayout {
-    int np, nw;
-    int fitz, y, h, r, qty, d, l1, l2, u, z, pkh;
-    __device__ __forceinline__ VtLayout(int np_, int nw_) : np(np_), nw(nw_)
-    {
-        fitz = 0; y = np; h = 2 * np; r = 3 * np; qty = 4 * np; d = 5 * np; l1 = 6 * np;
-        l2 = 7 * np; u = 8 * np; z = 9 * np; pkh = 9 * np + nw;
-    }
-};
-
-// chol1d (pppack.f90:1428-1566) for dy(:)=serr: the setupq products are
-// rebuilt row by row from r(i)=serr/h(i) (bit-identical to setupq's
-// v(:,1..3,5..7)), elimination and forward substitution share one sweep.
-// Returns sum_i qu(i)^2 dy(i)^2 (the dot_product of smooth) and, when asked,
-// utru of the p=0 solve.
-__device__ __forceinline__ double chol1d_dev(double p, const VtLayout& Lo, const Scr& S, double dy2,
-                             bool want_utru, double* utru)
-{
-    const int np = Lo.np;
-    const double six1mp = 6.0 * (1.0 - p);
-    const double twop = 2.0 * p;
-    auto R = [&](int i) { return S(Lo.r + i - 1); };
-    auto H = [&](int i) { return S(Lo.h + i - 1); };
-    auto row = [&](int i, double& d0, double& e0, double& f0) {
-        // setupq (pppack.f90:4793-4813) then chol1d's matrix (pppack.f90:1509-1512)
-        const double ri = R(i), rim1 = R(i - 1);
-        const double v1 = rim1, v3 = ri;
-        const double v2 = (-ri) - rim1;
-        const double v5 = v1 * v1 + v2 * v2 + v3 * v3;
-        double v6 = 0.0, v7 = 0.0;
-        if (i <= np - 2) {
-            const double rip1 = R(i + 1);
-            const double v1n = ri, v2n = (-rip1) - ri;
-            v6 = v2 * v1n + v3 * v2n;
-            if (i <= np - 3) v7 = v3 * rip1;
-        }
-        d0 = six1mp * v5 + twop * (H(i - 1) + H(i));
-        e0 = six1mp * v6 + p * H(i);
-        f0 = six1mp * v7;
-    };
-    if (np < 4) {
-        double d0, e0, f0;
-        row(2, d0, e0, f0);
-        S(Lo.u + 0) = 0.0;
-        S(Lo.u + 1) = S(Lo.qty + 1) / d0;
-        S(Lo.u + 2) = 0.0;
-    } else {
-        double di, ei, fi, dn, en, fn, dnn, enn, fnn;
-        row(2, di, ei, fi);
-        row(3, dn, en, fn);
-        double u_im1 = 0.0, u_i = S(Lo.qty + 1), l2_im1 = 0.0;
-        S(Lo.u + 0) = 0.0;
-        S(Lo.u + 1) = u_i;
-        for (int i = 2; i <= np - 2; i++) {
-            double rat = ei / di;
-            dn = dn - rat * ei;
-            en = en - rat * fi;
-            const double l1 = rat;
-            rat = fi / di;
-            if (i + 2 <= np - 1) {
-                row(i + 2, dnn, enn, fnn);
-            } else {
-                dnn = 0.0; enn = 0.0; fnn = 0.0;
-            }
-            dnn = dnn - rat * fi;
-            const double l2 = rat;
-            S(Lo.d + i - 1) = di;
-            S(Lo.l1 + i - 1) = l1;
-            S(Lo.l2 + i - 1) = l2;
-            const double u_ip1 = S(Lo.qty + i) - l1 * u_i - l2_im1 * u_im1;
-            S(Lo.u + i) = u_ip1;
-            di = dn; ei = en; fi = fn;
-            dn = dnn; en = enn; fn = fnn;
-            u_im1 = u_i; u_i = u_ip1; l2_im1 = l2;
-        }
-        // back substitution (pppack.f90:1545-1550); di is v(np-1,1)
-        S(Lo.u + np - 1) = 0.0;
-        double u_p2 = 0.0;
-        double u_p1 = u_i / di;
-        S(Lo.u + np - 2) = u_p1;
-        for (int i = np - 2; i >= 2; i--) {
-            const double ui = S(Lo.u + i - 1) / S(Lo.d + i - 1) - u_p1 * S(Lo.l1 + i - 1) -
-                              u_p2 * S(Lo.l2 + i - 1);
-            S(Lo.u + i - 1) = ui;
-            u_p2 = u_p1;
-            u_p1 = ui;
-        }
-    }
-    // qu = Q*u (pppack.f90:1556-1563) fused with smooth's dot_product and utru
-    double sum = 0.0, ut = 0.0, prev = 0.0;
-    double um1 = S(Lo.u + 0);
-    for (int i = 2; i <= np; i++) {
-        const double ui = S(Lo.u + i - 1);
-        const double hi = H(i - 1);
-        const double raw = (ui - um1) / hi;
-        const double qu = raw - prev;
-        sum = sum + (qu * qu) * dy2;
-        if (want_utru) ut = ut + hi * (um1 * (um1 + ui) + ui * ui);
-        prev = raw;
-        um1 = ui;
-    }
-    const double qn = -prev;
-    sum = sum + (qn * qn) * dy2;
-    if (want_utru) *utru = ut;
-    return sum;
-}
-
-// interv (pppack.f90:3363-3576) on the nearly uniform breakpoints fitz: a
-// guess from the mean spacing, corrected against the stored breakpoints, so
-// the result is the reference's: the largest i<=np-1 with fitz(i)<=x, else 1.
-__device__ __forceinline__ int interv_dev(const VtLayout& Lo, const Scr& S, double x, double f1,
-                                          double dz)
-{
-    const int np = Lo.np;
-    const double gss = (x - f1) / dz;
-    int i;
-    if (!(gss >= 0.0)) i = 1;
-    else if (gss >= (double)(np - 1)) i = np - 1;
-    else i = (int)gss + 1;
-    while (i > 1 && S(Lo.fitz + i - 1) > x) i--;
-    while (i < np - 1 && S(Lo.fitz + i) <= x) i++;
-    return i;
-}
-
-struct PieceCache {
-    int i;
-    double c[4];
-    double brk;
-};
-
-// Coefficients of piece i of the smoothing spline (smooth's final loop,
-// pppack.f90:5305-5314), formed on demand from u, h and y.
-__device__ __forceinline__ void spline_piece(const VtLayout& Lo, const Scr& S, int i, double p, double k2,
-                             PieceCache& pc)
-{
-    const int np = Lo.np;
-    auto U = [&](int j) { return S(Lo.u + j - 1); };
-    auto H = [&](int j) { return S(Lo.h + j - 1); };
-    auto raw = [&](int j) { return (U(j) - U(j - 1)) / H(j - 1); };   // j >= 2
-    const double r_i = (i >= 2) ? raw(i) : 0.0;
-    const double r_i1 = raw(i + 1);
-    const double qu_i = r_i1 - r_i;
-    double qu_i1;
-    if (i + 1 == np) qu_i1 = -r_i1;
-    else qu_i1 = raw(i + 2) - r_i1;
-    const double a1 = S(Lo.y + i - 1) - k2 * qu_i;
-    const double a1n = S(Lo.y + i) - k2 * qu_i1;
-    const double a3 = U(i) * 6.0 * p;
-    const double a3n = U(i + 1) * 6.0 * p;
-    const double hi = H(i);
-    const double a4 = (a3n - a3) / hi;
-    const double a2 = (a1n - a1) / hi - (a3 + a4 / 3.0 * hi) / 2.0 * hi;
-    pc.i = i;
-    pc.c[0] = a1; pc.c[1] = a2; pc.c[2] = a3; pc.c[3] = a4;
-    pc.brk = S(Lo.fitz + i - 1);
-}
-
-// The Horner loop of ppvalu (pppack.f90:4349-4357) for k=4, written out:
-// value=(value/fmmjdr)*h+coef(m,i) for fmmjdr=4-jderiv..1; its first step
-// yields coef(4,i) exactly and the last divides by 1.
-__device__ __forceinline__ double horner(const PieceCache& pc, double h, int jderiv)
-{
-    if (jderiv == 0) return ((pc.c[3] / 3.0 * h + pc.c[2]) / 2.0 * h + pc.c[1]) * h + pc.c[0];
-    return (pc.c[3] / 2.0 * h + pc.c[2]) * h + pc.c[1];
-}
-
-// ppvalu (pppack.f90:4246-4362), k=4
-__device__ __forceinline__ double ppvalu_dev(const VtLayout& Lo, const Scr& S, PieceCache& pc,
-                                             double x, int jderiv, double p, double k2,
-                                             double f1, double dz)
-{
-    const int i = interv_dev(Lo, S, x, f1, dz);
-    if (i != pc.i) spline_piece(Lo, S, i, p, k2, pc);
-    const double h = x - pc.brk;
-    return horner(pc, h, jderiv);
-}
-
-template <typename F>
-__device__ __forceinline__ double vturb(const DGrid& g, const StepArgs& A, const RhoCell& c, const Surface& sf,
-                        double Zpar, DevSrc& dv, const Scr& S, unsigned* capped)
-{
-    const int nw = g.N + 1;
-    const int np = A.np;
-    const VtLayout Lo(np, nw);
-    const double tdepth = sf.tdepth, zeta = sf.zeta;
-    const double h = -1.0 * tdepth;
-    // AKs column at the particle (getInterpAKs, interpolation_module.f90:1175-1229)
-    double cff[4];
-    rho_weights(g, c, cff);
-    const F* aks = (const F*)g.aks;
-    for (int k = 1; k <= nw; k++) {
-        const double Sk = level_S(g, g.s_w, g.cs_w, k, h);
-        const double zb = level_z(g, g.s_w, g.cs_w, k, sf.zb, h, Sk);
-        const double zc = level_z(g, g.s_w, g.cs_w, k, sf.zc, h, Sk);
-        const double zf = level_z(g, g.s_w, g.cs_w, k, sf.zf, h, Sk);
-        S(Lo.z + k - 1) = polintd(A.ex, zb, zc, zf, A.ix[1]);
-        Slots4<F> v[4];
-        for (int q = 0; q < 4; q++) ld4(aks + (c.o[q] * nw + (k - 1)) * 4, v[q]);
-        double kh[3];
-        for (int t = 0; t < 3; t++) {
-            const int s = A.slots.s[t];
-            kh[t] = cff[0] * pick(v[0], s) + cff[1] * pick(v[1], s) + cff[2] * pick(v[2], s) +
-                    cff[3] * pick(v[3], s);
-        }
-        S(Lo.pkh + k - 1) = polintd(A.ex, kh[0], kh[1], kh[2], A.ix[1]);
-    }
-    // resample to np uniform points with linint (ver_turb_module.f90:126-136;
-    // interpolation_module.f90:27-61).  fitz increases, so the bracket is
-    // found by a forward merge; it equals linint's bisection result.
-    const double z1 = S(Lo.z + 0), zn = S(Lo.z + nw - 1);
-    const double dz = (zn - z1) / (double)(np - 1);
-    S(Lo.fitz + 0) = z1;
-    S(Lo.fitz + np - 1) = zn;
-    S(Lo.y + 0) = S(Lo.pkh + 0);
-    S(Lo.y + np - 1) = S(Lo.pkh + nw - 1);
-    {
-        double fz = z1;
-        int jlo = 1;
-        double zlo = z1, zhi = S(Lo.z + 1);
-        for (int i = 2; i <= np - 1; i++) {
-            fz = fz + dz;
-            S(Lo.fitz + i - 1) = fz;
-            while (jlo < nw - 1 && zhi <= fz) {
-                jlo++;
-                zlo = zhi;
-                zhi = S(Lo.z + jlo);
-            }
-            const double ylo = S(Lo.pkh + jlo - 1), yhi = S(Lo.pkh + jlo);
-            const double m = (ylo - yhi) / (zlo - zhi);
-            const double b = ylo - m * zlo;
-            S(Lo.y + i - 1) = m * fz + b;
-        }
-    }
-    // setupq (pppack.f90:4726-4825) with dy(:)=serr
-    const double serr = A.serr;
-    {
-        double prev = 0.0;
-        for (int i = 1; i <= np - 1; i++) {
-            const double hi = S(Lo.fitz + i) - S(Lo.fitz + i - 1);
-            S(Lo.h + i - 1) = hi;
-            S(Lo.r + i - 1) = serr / hi;
-            const double diff = (S(Lo.y + i) - S(Lo.y + i - 1)) / hi;
-            if (i >= 2) S(Lo.qty + i - 1) = diff - prev;
-            prev = diff;
-        }
-    }
-    // smooth (pppack.f90:5236-5298)
-    const double dy2 = serr * serr;
-    const double s = A.smth;
-    double p, q, sfq;
-    if (s <= 0.0) {
-        p = 1.0;
-        chol1d_dev(p, Lo, S, dy2, false, nullptr);
-        sfq = 0.0;
-    } else {
-        p = 0.0;
-        double utru = 0.0;
-        sfq = 36.0 * chol1d_dev(p, Lo, S, dy2, true, &utru);
-        if (s < sfq) {
-            const double ooss = 1.0 / sqrt(s);
-            double oosf = 1.0 / sqrt(sfq);
-            q = -(oosf - ooss) * sfq / (6.0 * utru * oosf);
-            double prevq = 0.0, prevsf = oosf;
-            int it = 0;
-            for (;;) {
-                sfq = 36.0 * chol1d_dev(q / (1.0 + q), Lo, S, dy2, false, nullptr) /
-                      ((1.0 + q) * (1.0 + q));
-                if (fabs(sfq - s) <= 0.01 * s) break;
-                if (++it >= kSecantCap) {   // the reference has no cap
-                    atomicAdd(capped, 1u);
-                    break;
-                }
-                oosf = 1.0 / sqrt(sfq);
-                const double change = (q - prevq) / (oosf - prevsf) * (oosf - ooss);
-                prevq = q;
-                q = q - change;
-                prevsf = oosf;
-            }
-            p = q / (1.0 + q);
-        }
-    }
-    const double k2 = 6.0 * (1.0 - p) * dy2;
-    // RDM sub-steps (ver_turb_module.f90:231-293)
-    PieceCache pc;
-    pc.i = -1;
-    const double deltat = A.deltat;
-    double ParZc = Zpar;
-    for (int i = 1; i <= A.loop; i++) {
-        double Kprimec;
-        if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
-        else Kprimec = ppvalu_dev(Lo, S, pc, ParZc, 1, p, k2, z1, dz);
-        const double KprimeZc = Kprimec * deltat;
-        const double Z3rdc = ParZc + 0.5 * KprimeZc;
-        double KH3rdc;
-        if (Z3rdc < tdepth || Z3rdc > zeta) {
-            KH3rdc = A.aksback;
-        } else {
-            KH3rdc = ppvalu_dev(Lo, S, pc, Z3rdc, 0, p, k2, z1, dz);
-            if (KH3rdc < A.aksback) KH3rdc = A.aksback;
-        }
-        const double DEV = dv.get(1 + i);
-        const double r = 1.0;
-        const double dTz = KprimeZc + DEV * sqrt(2.0 / r * KH3rdc * deltat);
-        ParZc = ParZc + dTz;
-        if (ParZc <= tdepth) ParZc = tdepth + fabs(ParZc - tdepth);
-        if (ParZc >= zeta) ParZc = zeta - fabs(ParZc - zeta);
-    }
-    return ParZc - Zpar;
-}
-
-// ---------------------------------------------------------------------------
-// VTurb, FP32-field form.  Same model and the same secant search for the
-// smoothing parameter as above, evaluated without per-iteration work arrays:
-//
-//   * the resampled points are taken as exactly uniform (spacing dz; the
-//     reference accumulates fitz(i)=fitz(i-1)+dz, which differs in the last
-//     bits), so chol1d's matrix T(p) = 6(1-p) V + p W is pentadiagonal
-//     Toeplitz, V = r^2 (Q'Q), W = h tridiag(1,4,1), r = serr/dz;
-//   * smooth's sum_i (a(i,1) dy(i))^2 = u'V u for the solution of T u = qty.
-//     With G(p) = qty' T(p)^-1 qty (= sum w_i^2/d_i of an LDL' sweep) and
-//     u'T u = G, dG/dp = -u'(dT/dp)u = 6 u'Vu - u'Wu, it follows that
-//     u'Vu = (G + p G')/6 and, at p=0, utru = u'Wu/2 = (G - G')/2.  One
-//     forward LDL' sweep carrying derivatives in p therefore gives every
-//     quantity the secant search needs, with no back substitution;
-//   * only the accepted p gets a full solve, whose u builds the spline
-//     pieces the random-displacement sub-steps visit.
-// Work arrays are FP32 rows (the arithmetic stays FP64): qty, formed in FP64
-// from the FP64 resampled values and rounded once, so each sweep sees qty to
-// a relative 6e-8 per entry; y for the pieces; the factor rows and u.  They
-// are read in chunks of kCh rows so their loads overlap.
-// ---------------------------------------------------------------------------
-constexpr int kCh = 8;
-
-struct FastLayout {
-    int np;
-    int q, y, rd, l1, w, u;
-    __device__ __forceinline__ FastLayout(int np_) : np(np_)
-    {
-        q = 0; y = np; rd = 2 * np; l1 = 3 * np; w = 4 * np; u = 5 * np;
-    }
-};
-
-// 1/d to within an ulp: hardware estimate and two Newton steps
-__device__ __forceinline__ double rcp64(double d)
-{
-    double r = __builtin_amdgcn_rcp(d);
-    double e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-d, r, 1.0);
-    return fma(r, e, r);
-}
-
-// Toeplitz entries of chol1d's matrix (pppack.f90:1509-1512) for uniform h
-struct Band {
-    double a, b, c, da, db, dc;
-};
-
-__device__ __forceinline__ Band band_of(double p, double h, double serr)
-{
-    const double r = serr / h;
-    const double v5 = 6.0 * r * r, v6 = -4.0 * r * r, v7 = r * r;
-    const double six1mp = 6.0 * (1.0 - p), twop = 2.0 * p;
-    Band B;
-    B.a = six1mp * v5 + twop * (h + h);
-    B.b = six1mp * v6 + p * h;
-    B.c = six1mp * v7;
-    B.da = -6.0 * v5 + 4.0 * h;
-    B.db = -6.0 * v6 + h;
-    B.dc = -6.0 * v7;
-    return B;
-}
-
-// G(p) and dG/dp by one LDL' sweep over the n interior unknowns
-__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, int oq, double& G,
-                                         double& Gp)
-{
-#pragma clang fp contract(fast)
-    double d = B.a, dd = B.da, e = B.b, de = B.db;
-    double dn = B.a, ddn = B.da, en = B.b, den = B.db;
-    double w = (double)S(oq), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
-    double g = 0.0, gp = 0.0;
-    for (int m0 = 0; m0 < n; m0 += kCh) {
-        float qn[kCh];
-#pragma unroll
-        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(oq + m0 + k + 1) : 0.0f;
-#pragma unroll
-        for (int k = 0; k < kCh; k++) {
-            const int m = m0 + k;
-            if (m < n) {
-                const double rd = rcp64(d);
-                const double wrd = w * rd;
-                g += w * wrd;
-                gp += (2.0 * w * dw - wrd * w * dd) * rd;
-                const double rat = e * rd, drat = (de - rat * dd) * rd;
-                const double rat2 = B.c * rd, drat2 = (B.dc - rat2 * dd) * rd;
-                const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
-                const double e1 = en - rat * B.c, de1 = den - drat * B.c - rat * B.dc;
-                const double d2 = B.a - rat2 * B.c, dd2 = B.da - drat2 * B.c - rat2 * B.dc;
-                const double wn = (double)qn[k] - rat * w - l2p * wp;
-                const double dwn = -drat * w - rat * dw - dl2p * wp - l2p * dwp;
-                wp = w; dwp = dw; w = wn; dw = dwn;
-                l2p = rat2; dl2p = drat2;
-                d = d1; dd = dd1; e = e1; de = de1;
-                dn = d2; ddn = dd2; en = B.b; den = B.db;
-            }
-        }
-    }
-    G = g;
-    Gp = gp;
-}
-
-// full solve at the accepted p: u(2..np-1) into Lo.u (u(1)=u(np)=0).  The
-// forward sweep keeps 1/d, l1 and w per row; l2 = c/d is formed again on
-// the way back.
-__device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr32& S)
-{
-#pragma clang fp contract(fast)
-    const int n = Lo.np - 2;
-    double d = B.a, e = B.b, dn = B.a, en = B.b;
-    double w = (double)S(Lo.q), wp = 0.0, l2p = 0.0;
-    for (int m0 = 0; m0 < n; m0 += kCh) {
-        float qn[kCh];
-#pragma unroll
-        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(Lo.q + m0 + k + 1) : 0.0f;
-#pragma unroll
-        for (int k = 0; k < kCh; k++) {
-            const int m = m0 + k;
-            if (m < n) {
-                const double rd = rcp64(d);
-                const double rat = e * rd, rat2 = B.c * rd;
-                S(Lo.rd + m) = (float)rd;
-                S(Lo.w + m) = (float)w;
-                S(Lo.l1 + m) = (float)rat;
-                const double d1 = dn - rat * e;
-                const double e1 = en - rat * B.c;
-                const double d2 = B.a - rat2 * B.c;
-                const double wn = (double)qn[k] - rat * w - l2p * wp;
-                wp = w; w = wn; l2p = rat2;
-                d = d1; e = e1; dn = d2; en = B.b;
-            }
-        }
-    }
-    double up1 = 0.0, up2 = 0.0;
-    for (int m1 = n - 1; m1 >= 0; m1 -= kCh) {
-        float wv[kCh], rv[kCh], a1[kCh];
-#pragma unroll
-        for (int k = 0; k < kCh; k++) {
-            const int m = m1 - k;
-            const bool ok = m >= 0;
-            wv[k] = ok ? S(Lo.w + m) : 0.0f;
-            rv[k] = ok ? S(Lo.rd + m) : 0.0f;
-            a1[k] = ok ? S(Lo.l1 + m) : 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < kCh; k++) {
-            const int m = m1 - k;
-            if (m >= 0) {
-                const double r = (double)rv[k];
-                double um = (double)wv[k] * r;
-                if (m < n - 1) um = um - up1 * (double)a1[k] - up2 * (B.c * r);
-                S(Lo.u + m + 1) = (float)um;
-                up2 = up1;
-                up1 = um;
-            }
-        }
-    }
-    S(Lo.u + 0) = 0.0f;
-    S(Lo.u + Lo.np - 1) = 0.0f;
-}
-
-__device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr32& S, int i, double p,
-                                           double k2, double h, double rh, double z1,
-                                           PieceCache& pc)
-{
-#pragma clang fp contract(fast)
-    const int np = Lo.np;
-    // u(i-1..i+2), y(i..i+1), issued together
-    const double um1 = (i >= 2) ? (double)S(Lo.u + i - 2) : 0.0;
-    const double u0 = S(Lo.u + i - 1), u1 = S(Lo.u + i);
-    const double u2 = (i + 2 <= np) ? (double)S(Lo.u + i + 1) : 0.0;
-    const double y0 = S(Lo.y + i - 1), y1 = S(Lo.y + i);
-    const double r_i = (i >= 2) ? (u0 - um1) * rh : 0.0;
-    const double r_i1 = (u1 - u0) * rh;
-    const double qu_i = r_i1 - r_i;
-    const double qu_i1 = (i + 1 == np) ? -r_i1 : (u2 - u1) * rh - r_i1;
-    const double a1 = y0 - k2 * qu_i;
-    const double a1n = y1 - k2 * qu_i1;
-    const double a3 = u0 * 6.0 * p;
-    const double a3n = u1 * 6.0 * p;
-    const double a4 = (a3n - a3) * rh;
-    const double a2 = (a1n - a1) * rh - (a3 + a4 / 3.0 * h) / 2.0 * h;
-    pc.i = i;
-    pc.c[0] = a1; pc.c[1] = a2; pc.c[2] = a3; pc.c[3] = a4;
-    pc.brk = z1 + (double)(i - 1) * h;
-}
-
-__device__ __forceinline__ double ppvalu_fast(const FastLayout& Lo, const Scr32& S, PieceCache& pc,
-                                              double x, int jderiv, double p, double k2, double h,
-                                              double rh, double z1)
-{
-    const int np = Lo.np;
-    const double gss = (x - z1) * rh;
-    int i;
-    if (!(gss >= 0.0)) i = 1;
-    else if (gss >= (double)(np - 1)) i = np - 1;
-    else i = (int)gss + 1;
-    if (i != pc.i) piece_fast(Lo, S, i, p, k2, h, rh, z1, pc);
-    return horner(pc, x - pc.brk, jderiv);
-}
-
-// AKs at w level k of the particle's column, time-interpolated, and the
-// level's depth (getInterpAKs + getWlevel + polintd, ver_turb_module.f90:100-124)
-template <typename F>
-__device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, const RhoCell& c,
-                                             const double cff[4], const Surface& sf, double hh,
-                                             int k, double& z, double& kh)
-{
-#pragma clang fp contract(fast)
-    const int nw = g.N + 1;
-    const F* aks = (const F*)g.aks;
-    Slots4<F> v[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) ld4(aks + (c.o[q] * nw + (k - 1)) * 4, v[q]);
-    const double Sk = level_S(g, g.s_w, g.cs_w, k, hh);
-    const double zb = level_z(g, g.s_w, g.cs_w, k, sf.zb, hh, Sk);
-    const double zc = level_z(g, g.s_w, g.cs_w, k, sf.zc, hh, Sk);
-    const double zf = level_z(g, g.s_w, g.cs_w, k, sf.zf, hh, Sk);
-    z = tinterp<F>(A, zb, zc, zf);
-    double kk[3];
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-        const int s = A.slots.s[t];
-        kk[t] = cff[0] * pick(v[0], s) + cff[1] * pick(v[1], s) + cff[2] * pick(v[2], s) +
-                cff[3] * pick(v[3], s);
-    }
-    kh = tinterp<F>(A, kk[0], kk[1], kk[2]);
-}
-
-template <typename F>
-__device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, const RhoCell& c,
-                                             const Surface& sf, double Zpar, DevSrc& dv,
-                                             const Scr32& S, unsigned* capped)
-{
-#pragma clang fp contract(fast)
-    const int nw = g.N + 1;
-    const int np = A.np;
-    const FastLayout Lo(np);
-    const double tdepth = sf.tdepth, zeta = sf.zeta;
-    const double hh = -1.0 * tdepth;
-    double cff[4];
-    rho_weights(g, c, cff);
-    // the bottom and top levels fix the resampling grid; the column is then
-    // walked once, kLv levels per batch of loads, emitting y (linint: the
-    // point fz uses the segment of the largest level j<=nw-1 with z(j)<=fz)
-    double z1, k1, zn, kn;
-    column_level<F>(g, A, c, cff, sf, hh, 1, z1, k1);
-    column_level<F>(g, A, c, cff, sf, hh, nw, zn, kn);
-    const double h = (zn - z1) / (double)(np - 1);
-    const double rh = 1.0 / h;
-    {
-        constexpr int kLv = 4;
-        double zlo = z1, klo = k1;
-        double ym1 = 0.0, y0 = k1;
-        S(Lo.y + 0) = (float)k1;
-        int i = 2;                                   // next point to emit
-        for (int kb = 2; kb <= nw; kb += kLv) {
-            double zk[kLv], kk[kLv];
-#pragma unroll
-            for (int t = 0; t < kLv; t++) {
-                const int k = kb + t;
-                if (k < nw) column_level<F>(g, A, c, cff, sf, hh, k, zk[t], kk[t]);
-                else { zk[t] = zn; kk[t] = kn; }
-            }
-#pragma unroll
-            for (int t = 0; t < kLv; t++) {
-                const int k = kb + t;
-                if (k <= nw) {
-                    const double zhi = zk[t], khi = kk[t];
-                    const double m = (klo - khi) / (zlo - zhi);
-                    const double b = klo - m * zlo;
-                    for (; i <= np; i++) {
-                        double yi;
-                        if (i == np) {
-                            yi = kn;
-                        } else {
-                            const double fz = z1 + (double)(i - 1) * h;
-                            if (k < nw && !(fz < zhi)) break;
-                            yi = m * fz + b;
-                        }
-                        S(Lo.y + i - 1) = (float)yi;
-                        // qty row i-3 needs y(i-2): defined for i >= 3 (rows 0..np-3)
-                        if (i >= 3) S(Lo.q + i - 3) = (float)(((yi - y0) - (y0 - ym1)) * rh);
-                        ym1 = y0;
-                        y0 = yi;
-                    }
-                    zlo = zhi;
-                    klo = khi;
-                }
-            }
-        }
-    }
-    // secant search for p (pppack.f90:5236-5298), sfq from G and G'
-    const double serr = A.serr, s = A.smth;
-    const int n = np - 2;
-    double p = 0.0, q = 0.0, sfq;
-    if (s <= 0.0) {
-        p = 1.0;
-    } else {
-        double G, Gp;
-        stream_G(band_of(0.0, h, serr), n, S, Lo.q, G, Gp);
-        sfq = 6.0 * G;
-        if (s < sfq) {
-            const double utru = 0.5 * (G - Gp);
-            const double ooss = 1.0 / sqrt(s);
-            double oosf = 1.0 / sqrt(sfq);
-            q = -(oosf - ooss) * sfq / (6.0 * utru * oosf);
-            double prevq = 0.0, prevsf = oosf;
-            int it = 0;
-            for (;;) {
-                const double pq = q / (1.0 + q);
-                stream_G(band_of(pq, h, serr), n, S, Lo.q, G, Gp);
-                sfq = 6.0 * (G + pq * Gp) / ((1.0 + q) * (1.0 + q));
-                if (fabs(sfq - s) <= 0.01 * s) break;
-                if (++it >= kSecantCap) {
-                    atomicAdd(capped, 1u);
-                    break;
-                }
-                oosf = 1.0 / sqrt(sfq);
-                const double change = (q - prevq) / (oosf - prevsf) * (oosf - ooss);
-                prevq = q;
-                q = q - change;
-                prevsf = oosf;
-            }
-            p = q / (1.0 + q);
-        }
-    }
-    solve_u(band_of(p, h, serr), Lo, S);
-    const double k2 = 6.0 * (1.0 - p) * (serr * serr);
-    PieceCache pc;
-    pc.i = -1;
-    const double deltat = A.deltat;
-    double ParZc = Zpar;
-    for (int i = 1; i <= A.loop; i++) {
-        double Kprimec;
-        if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
-        else Kprimec = ppvalu_fast(Lo, S, pc, ParZc, 1, p, k2, h, rh, z1);
-        const double KprimeZc = Kprimec * deltat;
-        const double Z3rdc = ParZc + 0.5 * KprimeZc;
-        double KH3rdc;
-        if (Z3rdc < tdepth || Z3rdc > zeta) {
-            KH3rdc = A.aksback;
-        } else {
-            KH3rdc = ppvalu_fast(Lo, S, pc, Z3rdc, 0, p, k2, h, rh, z1);
-            if (KH3rdc < A.aksback) KH3rdc = A.aksback;
-        }
-        const double DEV = dv.get(1 + i);
-        const double dTz = KprimeZc + DEV * sqrt(2.0 * KH3rdc * deltat);
-        ParZc = ParZc + dTz;
-        if (ParZc <= tdepth) ParZc = tdepth + fabs(ParZc - tdepth);
-        if (ParZc >= zeta) ParZc = zeta - fabs(ParZc - zeta);
-    }
-    return ParZc - Zpar;
-}
-
-// boundary_module.f90:43-143 with the 3-point inside() of
-// interpolation_module.f90:269-420 specialised to the unit cell.
-__device__ __forceinline__ bool inside3(const double xb[3], const double yb[3], double xo, double yo)
-{
-    double X[4] = {xb[0], xb[1], xb[2], xb[0]};
-    double Y[4] = {yb[0], yb[1], yb[2], yb[0]};
-    int crossings = 0;
-    bool sel[3];
-    for (int k = 0; k < 3; k++)
-        sel[k] = ((X[k + 1] - xo) * (xo - X[k]) >= 0.0) && (X[k] != X[k + 1]);
-    for (int k = 0; k < 3; k++) {
-        if (!sel[k]) continue;
-        const double dx1 = xo - X[k];
-        const double dx2 = X[k + 1] - xo;
-        const double dxy = dx2 * (yo - Y[k]) - dx1 * (Y[k + 1] - yo);
-        int inc = 0;
-        if ((X[k] == xo) && (Y[k] == yo)) {
-            return true;
-        } else if (((dx1 == 0.0) && (yo >= Y[k])) || ((dx2 == 0.0) && (yo >= Y[k + 1]))) {
-            inc = 1;
-        } else if ((dx1 * dx2 > 0.0) && ((X[k + 1] - X[k]) * dxy >= 0.0)) {
-            inc = 2;
-        }
-        if (X[k + 1] > X[k]) crossings += inc;
-        else crossings -= inc;
-    }
-    return crossings != 0;
-}
-
-__device__ __forceinline__ void bounds(const DGrid& g, double Xp, double Yp, bool& ingrid, bool& obound)
-{
-    const double ecut = (double)0.1f, fcut = 1.0 - ecut;
-    ingrid = false;
-    obound = false;
-    const int I = (int)floor(Xp), J = (int)floor(Yp);
-    const double X = Xp - (double)I, Y = Yp - (double)J;
-    if ((I > 1) && (J > 1) && (I < g.L - 1) && (J < g.M - 1)) {
-        const int64_t b = (int64_t)(J - 1) * g.L + (I - 1);
-        const double m11 = g.mrho[b], m21 = g.mrho[b + 1];
-        const double m22 = g.mrho[b + g.L + 1], m12 = g.mrho[b + g.L];
-        const int nm = (int)(m11 + m21 + m22 + m12);
-        if (nm == 4) ingrid = true;
-        if (nm == 3) {
-            double tX[3] = {0, 0, 0}, tY[3] = {0, 0, 0};
-            const double mm[4] = {m11, m12, m21, m22};   // (it,jt) = (1,1) (1,2) (2,1) (2,2)
-            int n = 0;
-            for (int q = 0; q < 4; q++) {
-                if (mm[q] == 1.0 && n < 3) {
-                    tX[n] = (double)(q >> 1);
-                    tY[n] = (double)(q & 1);
-                    n++;
-                }
-            }
-            if (inside3(tX, tY, X, Y)) ingrid = true;
-        }
-        if (nm == 2) {
-            if (m11 == 1.0 && (X < ecut) && (Y < ecut)) ingrid = true;
-            if (m21 == 1.0 && (X > fcut) && (Y < ecut)) ingrid = true;
-            if (m22 == 1.0 && (X > fcut) && (Y > fcut)) ingrid = true;
-            if (m12 == 1.0 && (X < ecut) && (Y > fcut)) ingrid = true;
-        }
-        if ((I <= 1) || (J <= 1) || (I >= g.L - 1) || (J >= g.M - 1)) obound = true;
-    } else {
-        obound = true;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// the step: update_particles (ROMSPath.f90:662-1003), one lane per particle,
-// grid-stride over the particles.  PHASE 0 does everything in one pass (used
-// without vertical turbulence); with it, PHASE 1 (advection + horizontal walk)
-// hands newX, newY, AdvectZ and the clamped z to PHASE 2 (vertical walk,
-// grid hand-off, reflection, status), so each kernel keeps a smaller
-// register footprint.  Lane t of PHASE 2 uses scratch column t.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool skips(const StepArgs& A, int64_t q)
-{
-    if (A.ix[2] <= A.dob[q]) return true;                    // ROMSPath.f90:672
-    if (A.status[q] == ST_UNSET) return true;                // ROMSPath.f90:676
-    if (A.open_boundary && !A.isin[q]) return true;          // ROMSPath.f90:688-690
-    return false;
-}
-
-__device__ __forceinline__ DevSrc dev_of(const StepArgs& A, int64_t q)
-{
-    DevSrc dv;
-    dv.inj = A.dev ? A.dev + A.id[q] * (int64_t)A.ndev : nullptr;
-    dv.seed = A.seed;
-    dv.id = A.id[q];
-    dv.step = A.step_index;
-    dv.blk = -1;
-    return dv;
-}
-
-// RKAdvect (advection_module.f90:20-125) + HTurb (hor_turb_module.f90:30-51,
-// ROMSPath.f90:766-768).  Returns newX/newY (ROMSPath.f90:821-822).
-template <typename F, int NSLOT>
-__device__ __forceinline__ void advect(const DGrid& g, const StepArgs& A, DevSrc& dv, double X0,
-                                       double Y0, double& Zpar, double& newX, double& newY,
-                                       double& AdvectZ)
-{
-    const RhoCell c0 = rho_cell(g, X0, Y0);
-    const double pm = bilinear2d(g.pm, c0);
-    const double pn = bilinear2d(g.pn, c0);
-    double AdvectX = 0.0, AdvectY = 0.0, TurbHx = 0.0, TurbHy = 0.0;
-    AdvectZ = 0.0;
-    if (A.scheme == 1) {
-        // every stage starts from the step's start point with half a step (idt/2)
-        const double idt = (double)A.idt;
-        double Uad, Vad, Wad;
-        find_currents<F, NSLOT>(g, A, X0, Y0, Zpar, Uad, Vad, Wad, nullptr);
-        const double k1u = Uad * pm, k1v = Vad * pn, k1w = Wad;
-        double x1 = X0 + Uad * pm * idt / 2.0;
-        double y1 = Y0 + Vad * pn * idt / 2.0;
-        double z1 = Zpar + Wad * idt / 2.0;
-        find_currents<F, NSLOT>(g, A, x1, y1, z1, Uad, Vad, Wad, nullptr);
-        const double k2u = Uad * pm, k2v = Vad * pn, k2w = Wad;
-        double x2 = X0 + Uad * pm * idt / 2.0;
-        double y2 = Y0 + Vad * pn * idt / 2.0;
-        double z2 = Zpar + Wad * idt / 2.0;
-        find_currents<F, NSLOT>(g, A, x2, y2, z2, Uad, Vad, Wad, nullptr);
-        const double k3u = Uad * pm, k3v = Vad * pn, k3w = Wad;
-        double x3 = X0 + Uad * pm * idt / 2.0;
-        double y3 = Y0 + Vad * pn * idt / 2.0;
-        double z3 = Zpar + Wad * idt / 2.0;
-        find_currents<F, NSLOT>(g, A, x3, y3, z3, Uad, Vad, Wad, nullptr);
-        const double k4u = Uad * pm, k4v = Vad * pn, k4w = Wad;
-        const double PU = (k1u + 2.0 * k2u + 2.0 * k3u + k4u) / 6.0;
-        const double PV = (k1v + 2.0 * k2v + 2.0 * k3v + k4v) / 6.0;
-        const double PW = (k1w + 2.0 * k2w + 2.0 * k3w + k4w) / 6.0;
-        AdvectX = idt * PU;
-        AdvectY = idt * PV;
-        AdvectZ = idt * PW;
-    }
-    if (A.hturb_on) {
-        const double KM = g.kh, r = 1.0;
-        TurbHx = dv.get(0) * sqrt(2.0 / r * KM * (double)A.idt);
-        TurbHy = dv.get(1) * sqrt(2.0 / r * KM * (double)A.idt);
-    }
-    TurbHx = TurbHx * pm;
-    TurbHy = TurbHy * pn;
-    newX = X0 + AdvectX + TurbHx;
-    newY = Y0 + AdvectY + TurbHy;
-}
-
-// grid hand-off, new-position depth and zeta, vertical reflection and status
-// (ROMSPath.f90:828-892)
-template <typename F, bool NESTED>
-__device__ __forceinline__ void finish(const DGrid* __restrict__ grids, const StepArgs& A,
-                                       int64_t q, int gid, double Z0, double newX, double newY,
-                                       double AdvectZ, double TurbV)
-{
-    const double Zbehav = 0.0;
-    double tempX = newX, tempY = newY;
-    bool ingrid = false, obound = false;
-    const int ngrid = NESTED ? A.ngrid : 1;
-    for (int i = 1; i <= ngrid; i++) {
-        const DGrid& gc = grids[NESTED ? gid - 1 : 0];
-        tempX = gc.scl[i - 1][0] * newX + gc.off[i - 1][0];
-        tempY = gc.scl[i - 1][1] * newY + gc.off[i - 1][1];
-        bounds(grids[NESTED ? i - 1 : 0], tempX, tempY, ingrid, obound);
-        if (ingrid) {
-            gid = i;
-            break;
-        }
-    }
-    const DGrid& gn = grids[NESTED ? gid - 1 : 0];
-    const RhoCell cn = rho_cell(gn, tempX, tempY);
-    const Surface sn = surface_at<F>(gn, A, cn);
-    const double tdepth = sn.tdepth, Pzeta = sn.zeta;
-    if (A.want_diag) {
-        A.wd[q] = (-1.0 * tdepth) + Pzeta;
-        A.zeta[q] = Pzeta;
-        A.bath[q] = -1.0 * tdepth;
-    }
-    bool hbot = false, htop = false;
-    double newZ;
-    if (A.nsb == 1) {
-        newZ = Pzeta - A.vertdist;
-    } else if (A.nsb == 2) {
-        newZ = tdepth + A.vertdist;
-    } else {
-        newZ = Z0 + AdvectZ + TurbV + Zbehav;
-        if (newZ < tdepth) {
-            newZ = tdepth + fabs(newZ - tdepth);
-            hbot = true;
-        }
-        if (newZ > Pzeta) {
-            newZ = Pzeta - fabs(newZ - Pzeta);
-            htop = true;
-        }
-    }
-    int8_t st;
-    if (ingrid) {
-        A.isin[q] = 1;
-        A.x[q] = tempX;
-        A.y[q] = tempY;
-        A.z[q] = newZ;
-        st = hbot ? ST_BOT : (htop ? ST_TOP : ST_MOVED);
-    } else if (obound) {
-        A.isin[q] = 0;
-        gid = A.ngrid;
-        st = ST_EXIT;
-    } else {
-        A.isin[q] = 1;
-        st = ST_LAND;
-    }
-    A.status[q] = st;
-    A.gid[q] = (int8_t)gid;
-}
-
-// One particle of the step.  Returns whether it took the step.
-template <typename F, bool NESTED, int PHASE>
-__device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const StepArgs& A,
-                                         int64_t q, int64_t tid, int64_t T)
-{
-    {
-        if (skips(A, q)) return false;
-        const double X0 = A.x[q], Y0 = A.y[q], Z0 = A.z[q];
-        const int gid = NESTED ? (int)A.gid[q] : 1;
-        const DGrid& g = grids[NESTED ? gid - 1 : 0];
-        DevSrc dv = dev_of(A, q);
-        double Zpar = Z0, newX, newY, AdvectZ, TurbV = 0.0;
-        if (PHASE != 2) {
-            A.age[q] = A.age[q] + (double)(float)A.idt;
-            if (A.ets == 1) advect<F, 2>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
-            else advect<F, 3>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
-            if (PHASE == 1) {
-                A.mid[q] = newX;
-                A.mid[A.nmid + q] = newY;
-                A.mid[2 * A.nmid + q] = AdvectZ;
-                A.mid[3 * A.nmid + q] = Zpar;
-                return true;
-            }
-        } else {
-            newX = A.mid[q];
-            newY = A.mid[A.nmid + q];
-            AdvectZ = A.mid[2 * A.nmid + q];
-            Zpar = A.mid[3 * A.nmid + q];
-        }
-        if (A.vturb_on) {
-            const RhoCell c0 = rho_cell(g, X0, Y0);
-            const Surface s0 = surface_at<F>(g, A, c0);
-            if (sizeof(F) == 4) {
-                const Scr32 S{(float*)A.scratch + tid, T};
-                TurbV = vturb_fast<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
-            } else {
-                const Scr S{A.scratch + tid, T};
-                TurbV = vturb<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
-            }
-        }
-        finish<F, NESTED>(grids, A, q, gid, Z0, newX, newY, AdvectZ, TurbV);
-    }
-    return true;
-}
 
 __device__ __forceinline__ void count_active(const StepArgs& A, unsigned long long nact)
 {

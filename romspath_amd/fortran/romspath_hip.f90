@@ -41,6 +41,7 @@ module romspath_hip
             rp_get_particles, rp_set_deviates, rp_sample_deviates, rp_active_steps, &
             rp_secant_capped, rp_synchronize, rp_kernel_timing, rp_phase_timing, rp_last_error, &
             rp_destroy, rp_set_tuning, rp_set_scenarios, rp_step_scenario, rp_step_batch, &
+            rp_set_perturbation, rp_adjoint_record, rp_adjoint, &
             rp_error_text
 
   interface
@@ -163,6 +164,29 @@ module romspath_hip
       integer(c_int32_t), value :: scen, ets, t_b, t_c, t_f, want_diag
       real(c_double), intent(in) :: ex(3), ix(3)
       integer(c_int64_t), value :: step_index
+    end function
+
+    integer(c_int) function rp_set_perturbation(ctx, nx, ny, coef) bind(C, name="rp_set_perturbation")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nx, ny
+      real(c_double), intent(in) :: coef(*)
+    end function
+
+    integer(c_int) function rp_adjoint_record(ctx, ckpt_every) bind(C, name="rp_adjoint_record")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ckpt_every
+    end function
+
+    integer(c_int) function rp_adjoint(ctx, n, lam_x, lam_y, lam_z, g_x, g_y, g_z, g_pert) &
+        bind(C, name="rp_adjoint")
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: lam_x(*), lam_y(*), lam_z(*)
+      real(c_double), intent(out) :: g_x(*), g_y(*), g_z(*)
+      type(c_ptr), value :: g_pert
     end function
 
     integer(c_int) function rp_step_batch(ctx, nsteps, ex, ix, ets, slots, step_index, want_diag, &
