@@ -188,6 +188,115 @@ def batched_ensembles(runup, window, tune_steps, log_every=120):
     }
 
 
+def adjoint_cost(n, steps, ckpt, fd_check=True):
+    """Adjoint (C5): C2 grid and fields, n particles, vertical random walk on,
+    16x16 velocity perturbation (zero), checkpoints every ckpt steps, over a
+    window of `steps` internal steps.  Loss = sum of squared horizontal
+    distances (km) from final positions to seeded targets minus a soft count
+    (box of 20 x 20 km smoothed by a Gaussian of 2 km) of particles inside a
+    seeded target box.  Times: the recorded forward (the step the adjoint
+    differentiates), the reverse sweep, and the plain FP32-field forward."""
+    from scipy.special import ndtr
+    base = synth.config2(n_particles=n, seed=5, days=2.0)
+    cfg = base
+    prm = cfg.params
+    g = cfg.grids[0]
+    records = RecordSource(cfg)
+    for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, steps):
+        for ld in st.loads:
+            records.get(1, ld.record)
+    rng = np.random.default_rng(55)
+    tx = rng.uniform(5.0, g.L - 5.0, n)
+    ty = rng.uniform(5.0, g.M - 5.0, n)
+    bx0, by0 = rng.uniform(20.0, g.L - 40.0), rng.uniform(20.0, g.M - 40.0)
+    sig, beta = 2.0, 10.0
+
+    def soft(u, lo):
+        a = (u - lo) / sig
+        b = (u - lo - 20.0) / sig
+        phi = ndtr(a) - ndtr(b)
+        dphi = (np.exp(-0.5 * a * a) - np.exp(-0.5 * b * b)) / (sig * np.sqrt(2 * np.pi))
+        return phi, dphi
+
+    def loss_terms(o):
+        dx, dy = o["x"] - tx, o["y"] - ty
+        px, dpx = soft(o["x"], bx0)
+        py, dpy = soft(o["y"], by0)
+        L = dx * dx + dy * dy - beta * px * py
+        lam = (2 * dx - beta * dpx * py, 2 * dy - beta * px * dpy, np.zeros_like(dx))
+        return L, lam
+
+    def forward(record, parts=None):
+        c2 = cfg if parts is None else synth.Config(cfg.name, cfg.grids, cfg.hydro, cfg.params,
+                                                   parts, cfg.refine)
+        run = GpuRun(c2, precision="fp32", records=records, sort_interval=0)
+        run.ctx.set_perturbation(np.zeros((2, 16, 16)))
+        if record:
+            run.ctx.adjoint_record(ckpt)
+        run.ctx.synchronize()
+        t0 = time.perf_counter()
+        out = run.run(max_steps=steps)
+        return run, out, time.perf_counter() - t0
+
+    log(f"C5: adjoint, {n} particles, {steps} steps, checkpoints every {ckpt}")
+    run, _, t_plain = forward(False)
+    run.close()
+    run, out, t_fwd = forward(True)
+    L, lam = loss_terms(out)
+    t0 = time.perf_counter()
+    (gx, gy, gz), gp = run.ctx.adjoint(lam, want_pert=True, pert_shape=(2, 16, 16))
+    t_adj = time.perf_counter() - t0
+    run.close()
+    # release (x, y, s): z0 = s h(x0, y0) with zeta 0 at release
+    P = cfg.particles
+    I = np.floor(P.x).astype(int)
+    J = np.floor(P.y).astype(int)
+    fx, fy = P.x - I, P.y - J
+    h = g.h
+    h00, h10, h11, h01 = h[J - 1, I - 1], h[J - 1, I], h[J, I], h[J, I - 1]
+    hh = (1 - fx) * (1 - fy) * h00 + fx * (1 - fy) * h10 + fx * fy * h11 + (1 - fx) * fy * h01
+    dhdx = (1 - fy) * (h10 - h00) + fy * (h11 - h01)
+    dhdy = (1 - fx) * (h01 - h00) + fx * (h11 - h10)
+    s_rel = P.z / hh
+    g_s = gz * hh
+    g_x = gx + gz * s_rel * dhdx
+    g_y = gy + gz * s_rel * dhdy
+    log(f"  plain {t_plain:.2f} s, recorded forward {t_fwd:.2f} s, adjoint {t_adj:.2f} s "
+        f"({t_adj / t_fwd:.2f} x forward)")
+    res = {
+        "workload": f"C5: C2 grid and fields, {n} particles, RK4 + vertical random walk, "
+                    f"{steps}-step window of the 2-day run, checkpoint every {ckpt} steps, "
+                    "FP32 fields, FP64 checkpoints",
+        "particles": n, "steps": steps, "checkpoint_every": ckpt,
+        "t_forward_recorded_s": t_fwd, "t_adjoint_s": t_adj, "t_forward_fp32_form_s": t_plain,
+        "adjoint_over_forward": t_adj / t_fwd,
+        "adjoint_over_fp32_form_forward": t_adj / t_plain,
+        "loss": float(L.sum()),
+        "grad_norms": {"release_x": float(np.linalg.norm(g_x)), "release_y": float(np.linalg.norm(g_y)),
+                       "release_s": float(np.linalg.norm(g_s)), "perturbation": float(np.linalg.norm(gp))},
+    }
+    if fd_check:
+        # central differences of the per-particle loss in release x, all particles at once
+        fds = []
+        ok = np.ones(n, dtype=bool)
+        for eps in (1e-4, 1e-5):
+            vals = []
+            for sgn in (1.0, -1.0):
+                parts = synth.Particles(P.x + sgn * eps, P.y.copy(), P.z.copy(), P.dob.copy())
+                r, o, _ = forward(False, parts)
+                r.close()
+                ok &= o["status"] == out["status"]
+                vals.append(loss_terms(o)[0])
+            fds.append((vals[0] - vals[1]) / (2 * eps))
+        floor = 1e-3 * np.median(np.abs(fds[0][ok]))
+        trusted = ok & (np.abs(fds[0] - fds[1]) <= 1e-5 * np.maximum(np.abs(fds[0]), floor))
+        nrm = float(np.linalg.norm((gx - fds[1])[trusted]) / np.linalg.norm(fds[1][trusted]))
+        res["fd_check_release_x"] = {"eps": [1e-4, 1e-5], "trusted_particles": int(trusted.sum()),
+                                     "particles": n, "normwise_relative_error": nrm}
+        log(f"  FD check d/dx0: {trusted.sum()}/{n} trusted, norm-wise rel {nrm:.2e}")
+    return res
+
+
 def cpu_baseline(cfg, n_sample, steps):
     from oracle import oracle as O
     sub = synth.Particles(cfg.particles.x[:n_sample].copy(), cfg.particles.y[:n_sample].copy(),
@@ -218,6 +327,10 @@ def main():
     ap.add_argument("--c4-runup", type=int, default=720, help="steps before the C4 window (24 h)")
     ap.add_argument("--c4-window", type=int, default=30)
     ap.add_argument("--c4-tune-steps", type=int, default=96)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the C5 adjoint measurement")
+    ap.add_argument("--c5-particles", type=int, default=262_144)
+    ap.add_argument("--c5-steps", type=int, default=128)
+    ap.add_argument("--c5-ckpt", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (reference reading for the HBM "
                          "byte counters) and exit")
@@ -298,6 +411,9 @@ def main():
     if not args.no_batched:
         extra["batched_ensembles"] = batched_ensembles(args.c4_runup, args.c4_window,
                                                        args.c4_tune_steps)
+
+    if not args.no_adjoint:
+        extra["adjoint"] = adjoint_cost(args.c5_particles, args.c5_steps, args.c5_ckpt)
 
     cpu = None
     if not args.no_cpu_baseline:
