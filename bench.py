@@ -387,6 +387,11 @@ def main():
     per_launch = res["active"] / max(res["launches"], 1)
     achieved = balg * per_launch / (res["avg_kernel_ms"] * 1e-3) / 1e9 if res["avg_kernel_ms"] else 0.0
     adv_ms, walk_ms = res["phase_ms"]
+    traffic = None
+    tr_file = ROOT / "profiles" / "hbm_traffic_c3_fp32.json"
+    if args.workload == "c3" and args.precision == "fp32" and tr_file.exists():
+        tr = json.loads(tr_file.read_text())
+        traffic = (tr["read_bytes_per_particle_step"] + tr["write_bytes_per_particle_step"]) * per_launch
     log(f"  kernels: advect {adv_ms:.3f} ms, vertical walk {walk_ms:.3f} ms per step")
 
     extra = {}
@@ -441,7 +446,9 @@ def main():
                         "field_precision": args.precision, "sort_interval": args.sort_interval,
                         "parallelism": "single GPU"}, **dims),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per step, "
+                                       "profiles/hbm_traffic_c3_fp32.md" if traffic else None,
                      "kernel": "one step = advect_kernel + vwalk_kernel",
                      "b_alg_bytes_per_particle_step": balg,
                      "avg_kernel_ms": res["avg_kernel_ms"],
