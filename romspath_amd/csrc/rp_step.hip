@@ -63,20 +63,29 @@ __global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ALL)
     step_body<F, NESTED, 0>(grids, A);
 }
 
+// The FP32-field advection kernel is held to 3 waves per SIMD (168 VGPRs and
+// a small spill): its RK stages wait on field gathers, and the extra waves
+// hide them better than the spill costs.
+template <typename F>
+constexpr int advect_waves()
+{
+    return RP_WAVES_ADVECT > 0 ? RP_WAVES_ADVECT : (sizeof(F) == 4 ? 3 : 1);
+}
+
 template <typename F, bool NESTED>
-__global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ADVECT)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(advect_waves<F>())))
     advect_kernel(const DGrid* __restrict__ grids, StepArgs A)
 {
     step_body<F, NESTED, 1>(grids, A);
 }
 
-// The FP32-field vertical walk is held to 4 waves per SIMD (128 VGPRs, a
-// small spill): its sweeps and gathers wait on memory, and more resident
-// waves hide more of that than the spill costs.
+// The FP32-field vertical walk is held to 5 waves per SIMD (96 VGPRs, a
+// spill of about 250 B): its sweeps and gathers wait on memory, and more
+// resident waves hide more of that than the spill costs.
 template <typename F>
 constexpr int vwalk_waves()
 {
-    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (sizeof(F) == 4 ? 4 : 1);
+    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (sizeof(F) == 4 ? 5 : 1);
 }
 
 template <typename F, bool NESTED>
