@@ -23,6 +23,7 @@ struct GridDev {
     double *h = nullptr, *pm = nullptr, *pn = nullptr;
     int8_t *mrho = nullptr, *mu = nullptr, *mv = nullptr;
     void *zeta = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *aks = nullptr;
+    float* cur[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // FP32 form: fields at ix(2)
     double scl[kMaxGrid][2], off[kMaxGrid][2];
     bool set = false;
     bool slot_set[3] = {false, false, false};
@@ -82,6 +83,12 @@ struct Ctx {
     // ensembles: scenario s holds stored positions first[s] .. first[s]+count[s]-1
     std::vector<int64_t> scen_first, scen_count, scen_steps;
     hipGraphExec_t gexec = nullptr;      // last captured batch of steps
+    // current fields (FP32 form): what they were last built for
+    uint64_t field_gen = 0;              // bumped by every hydro upload
+    uint64_t cur_gen = ~0ull;
+    double cur_pw[3] = {0.0, 0.0, 0.0};
+    int cur_slots[3] = {-1, -1, -1};
+    int cur_first = -1;
     // velocity perturbation (rp_set_perturbation), device copy or null
     double* pert = nullptr;
     int pert_nx = 0, pert_ny = 0;
@@ -222,6 +229,7 @@ void free_grid(GridDev& g)
     dfree(g.h); dfree(g.pm); dfree(g.pn);
     dfree(g.mrho); dfree(g.mu); dfree(g.mv);
     dfree(g.zeta); dfree(g.u); dfree(g.v); dfree(g.w); dfree(g.aks);
+    for (auto& p : g.cur) dfree(p);
     g.set = false;
 }
 
@@ -358,7 +366,20 @@ struct StepPlan {
     int64_t grid_all = 0, grid_vt = 0;
     bool vt = false;
     KFn k[3] = {nullptr, nullptr, nullptr};   // step / advect / vertical walk
+    bool cur = false;                          // FP32 form: build the current fields first
+    CurJobs jobs;
 };
+
+// points of field f (zeta, u, v, w, aks) of a grid
+int64_t field_points(const GridDev& g, int f)
+{
+    switch (f) {
+    case 0: return (int64_t)g.L * g.M;
+    case 1: return (int64_t)(g.L - 1) * g.M * g.N;
+    case 2: return (int64_t)g.L * (g.M - 1) * g.N;
+    default: return (int64_t)g.L * g.M * (g.N + 1);
+    }
+}
 
 int check_step(Ctx* c, int t_b, int t_c, int t_f, int64_t step_index)
 {
@@ -437,9 +458,9 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     const bool nested = c->cfg.ngrid > 1;
     const bool f32 = c->fbytes == 4;
     if (f32 && !nested) {
-        P.k[0] = step_kernel<float, false>; P.k[1] = advect_kernel<float, false>; P.k[2] = vwalk_kernel<float, false>;
+        P.k[0] = step_kernel<Cur, false>; P.k[1] = advect_kernel<Cur, false>; P.k[2] = vwalk_kernel<Cur, false>;
     } else if (f32) {
-        P.k[0] = step_kernel<float, true>; P.k[1] = advect_kernel<float, true>; P.k[2] = vwalk_kernel<float, true>;
+        P.k[0] = step_kernel<Cur, true>; P.k[1] = advect_kernel<Cur, true>; P.k[2] = vwalk_kernel<Cur, true>;
     } else if (!nested) {
         P.k[0] = step_kernel<double, false>; P.k[1] = advect_kernel<double, false>; P.k[2] = vwalk_kernel<double, false>;
     } else {
@@ -468,6 +489,30 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
         SortArgs K;
         if ((rc = sort_buffers(c, sort_keys(c, K)))) return rc;
     }
+    P.cur = f32;
+    if (f32) {
+        // jobs of the current-field kernel: zeta and AKs at (b, c, f), u, v, w
+        // at (b, b, c) on the first external step, (b, c, f) after it
+        CurJobs& J = P.jobs;
+        J.njobs = 0;
+        for (int k = 0; k < 3; k++) J.pw[k] = A.pw[k];
+        for (int gi = 0; gi < c->cfg.ngrid; gi++) {
+            GridDev& g = c->grids[gi];
+            void* src[5] = {g.zeta, g.u, g.v, g.w, g.aks};
+            for (int f = 0; f < 5; f++) {
+                const int64_t npt = field_points(g, f);
+                if (!g.cur[f] && (rc = dalloc(c, &g.cur[f], (size_t)npt))) return rc;
+                const int j = J.njobs++;
+                J.src[j] = (const float*)src[f];
+                J.dst[j] = g.cur[f];
+                J.n[j] = npt;
+                const bool first = ets == 1 && f >= 1 && f <= 3;
+                J.slot[j][0] = A.slots.s[0];
+                J.slot[j][1] = first ? A.slots.s[0] : A.slots.s[1];
+                J.slot[j][2] = first ? A.slots.s[1] : A.slots.s[2];
+            }
+        }
+    }
     {
         DGrid tab[kMaxGrid];
         std::memset(tab, 0, sizeof(tab));
@@ -479,6 +524,7 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
             d.h = g.h; d.pm = g.pm; d.pn = g.pn;
             d.mrho = g.mrho; d.mu = g.mu; d.mv = g.mv;
             d.zeta = g.zeta; d.u = g.u; d.v = g.v; d.w = g.w; d.aks = g.aks;
+            d.czeta = g.cur[0]; d.cu = g.cur[1]; d.cv = g.cur[2]; d.cw = g.cur[3]; d.caks = g.cur[4];
             d.kh = c->cfg.kh[i];
             for (int a = 0; a < kMaxGrid; a++)
                 for (int k = 0; k < 2; k++) {
@@ -521,6 +567,22 @@ int enqueue_step(Ctx* c, StepPlan& P, const Ctx::StepEvents* ev)
     A.wd = c->wd; A.zeta = c->zeta; A.bath = c->bath;
     A.id = c->id;
     const dim3 bd(P.block);
+    if (P.cur && P.jobs.njobs > 0) {
+        // built once per step time and slot set (the scenarios of a step share them)
+        const int first = A.ets == 1 ? 1 : 0;
+        if (c->cur_gen != c->field_gen || c->cur_pw[0] != A.pw[0] || c->cur_pw[1] != A.pw[1] ||
+            c->cur_pw[2] != A.pw[2] || c->cur_first != first ||
+            c->cur_slots[0] != A.slots.s[0] || c->cur_slots[1] != A.slots.s[1] ||
+            c->cur_slots[2] != A.slots.s[2]) {
+            hipLaunchKernelGGL(current_fields, dim3(512, (unsigned)P.jobs.njobs), dim3(256), 0,
+                               c->stream, P.jobs);
+            HIPCHK(c, hipGetLastError());
+            c->cur_gen = c->field_gen;
+            for (int k = 0; k < 3; k++) c->cur_pw[k] = A.pw[k];
+            c->cur_first = first;
+            for (int k = 0; k < 3; k++) c->cur_slots[k] = A.slots.s[k];
+        }
+    }
     if (!P.vt) {
         hipLaunchKernelGGL(P.k[0], dim3((unsigned)P.grid_all), bd, 0, c->stream, c->dgrids, A);
         HIPCHK(c, hipGetLastError());
@@ -833,6 +895,7 @@ int rp_set_hydro_slot(void* ctx, int32_t ng, int32_t slot, const double* zeta, c
     const int s = slot - 1;
     int rc;
     c->adj.dirty = true;
+    c->field_gen++;
     if ((rc = upload_field(c, g.zeta, zeta, nullptr, g.L, g.M, 1, s)) ||
         (rc = upload_field(c, g.u, u, su, g.L - 1, g.M, g.N, s)) ||
         (rc = upload_field(c, g.v, v, sv, g.L, g.M - 1, g.N, s)) ||

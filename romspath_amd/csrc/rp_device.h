@@ -40,9 +40,32 @@ struct DGrid {
     const double *h, *pm, *pn;           // (xi_rho, eta_rho) column-major
     const int8_t *mrho, *mu, *mv;        // masks
     const void *zeta, *u, *v, *w, *aks;  // FP32 or FP64, slot-interleaved
+    // FP32 "current" fields of the step: the three slots already combined at
+    // the step's time ix(2), one value per point (same point order, no slot)
+    const float *czeta, *cu, *cv, *cw, *caks;
     double kh;                           // ConstantHTurb(ng)
     double scl[kMaxGrid][2], off[kMaxGrid][2];
 };
+
+// Field kinds of the step code: double and float read the slot-interleaved
+// fields (FP64 parity form, and the FP32 form's reference-order spline path
+// used by the adjoint); Cur reads the FP32 current fields.  Every stage and
+// the spline column interpolate in time at the same ix(2), and the temporal
+// interpolation is linear in the field values, so combining the slots once
+// per point and step is the same interpolation in another order.
+struct Cur {};
+template <typename F>
+constexpr bool is_cur()
+{
+    return __is_same(F, Cur);
+}
+// the FP32-field form's arithmetic economies (Lagrange time weights, one
+// reciprocal per column, the streaming spline) apply to float and Cur
+template <typename F>
+constexpr bool is_fast()
+{
+    return is_cur<F>() || sizeof(F) == 4;
+}
 
 // ---------------------------------------------------------------------------
 // loads

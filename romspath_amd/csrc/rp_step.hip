@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ALL)
 template <typename F>
 constexpr int advect_waves()
 {
-    return RP_WAVES_ADVECT > 0 ? RP_WAVES_ADVECT : (sizeof(F) == 4 ? 3 : 1);
+    return RP_WAVES_ADVECT > 0 ? RP_WAVES_ADVECT : (is_fast<F>() ? 3 : 1);
 }
 
 template <typename F, bool NESTED>
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(advect
 template <typename F>
 constexpr int vwalk_waves()
 {
-    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (sizeof(F) == 4 ? 5 : 1);
+    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (is_fast<F>() ? 5 : 1);
 }
 
 template <typename F, bool NESTED>
@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(vwalk_
     template __global__ void step_kernel<F, N>(const DGrid*, StepArgs);            \
     template __global__ void advect_kernel<F, N>(const DGrid*, StepArgs);          \
     template __global__ void vwalk_kernel<F, N>(const DGrid*, StepArgs);
-RP_INST(float, false) RP_INST(float, true) RP_INST(double, false) RP_INST(double, true)
+RP_INST(Cur, false) RP_INST(Cur, true) RP_INST(double, false) RP_INST(double, true)
 #undef RP_INST
 
 // ---------------------------------------------------------------------------
@@ -138,6 +138,22 @@ __global__ void sample_deviates(uint64_t seed, int64_t step, int64_t n, int ndev
     dv.step = (uint64_t)step;
     dv.blk = -1;
     for (int j = 0; j < ndev; j++) out[q * ndev + j] = dv.get(j);
+}
+
+__global__ void __launch_bounds__(256) current_fields(CurJobs J)
+{
+    const int j = blockIdx.y;
+    if (j >= J.njobs) return;
+    const float* src = J.src[j];
+    float* dst = J.dst[j];
+    const int s0 = J.slot[j][0], s1 = J.slot[j][1], s2 = J.slot[j][2];
+    const double p0 = J.pw[0], p1 = J.pw[1], p2 = J.pw[2];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < J.n[j];
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = reinterpret_cast<const float4*>(src)[e];
+        auto pk = [&](int t) { return (double)(t == 0 ? v.x : (t == 1 ? v.y : (t == 2 ? v.z : v.w))); };
+        dst[e] = (float)(p0 * pk(s0) + p1 * pk(s1) + p2 * pk(s2));
+    }
 }
 
 }  // namespace rp

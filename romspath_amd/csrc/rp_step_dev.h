@@ -16,6 +16,7 @@ namespace rp {
 template <typename F, class R>
 __device__ __forceinline__ R tinterp(const StepArgs& A, R y0, R y1, R y2)
 {
+    if constexpr (is_cur<F>()) return y0;   // current fields: already at ix(2)
     if (sizeof(F) == 8) return polintd(A.ex, y0, y1, y2, A.ix[1]);
     return A.pw[0] * y0 + A.pw[1] * y1 + A.pw[2] * y2;
 }
@@ -72,6 +73,13 @@ __device__ __forceinline__ void zeta3(const DGrid& g, const RhoCellT<R>& c, cons
     R w[4];
     for (int k = 0; k < 4; k++) m[k] = (double)g.mrho[c.o[k]];
     getcoeff(c.x, c.y, m, w);
+    if constexpr (is_cur<F>()) {
+        const float* z = g.czeta;
+        out[0] = w[0] * (double)z[c.o[0]] + w[1] * (double)z[c.o[1]] + w[2] * (double)z[c.o[2]] +
+                 w[3] * (double)z[c.o[3]];
+        out[1] = out[2] = out[0];
+        return;
+    } else {
     const F* z = (const F*)g.zeta;
     Slots4<F> v[4];
     for (int k = 0; k < 4; k++) ld4(z + c.o[k] * 4, v[k]);
@@ -79,6 +87,7 @@ __device__ __forceinline__ void zeta3(const DGrid& g, const RhoCellT<R>& c, cons
         const int s = S.s[t];
         out[t] = w[0] * pick(v[0], s) + w[1] * pick(v[1], s) + w[2] * pick(v[2], s) +
                  w[3] * pick(v[3], s);
+    }
     }
 }
 
@@ -88,10 +97,23 @@ __device__ __forceinline__ void zeta3(const DGrid& g, const RhoCellT<R>& c, cons
 // per call from the same (z, zeta, depth), so the values are identical.
 // ---------------------------------------------------------------------------
 template <typename F, int NSLOT, class R>
-__device__ __forceinline__ void blend3(const F* f, const int64_t col[4], int nk, const R cff[4],
-                                       const bool zero[4], const VBracketT<R>& vb,
+__device__ __forceinline__ void blend3(const void* fv, const float* fc, const int64_t col[4], int nk,
+                                       const R cff[4], const bool zero[4], const VBracketT<R>& vb,
                                        const SlotSel& S, R out[3])
 {
+    if constexpr (is_cur<F>()) {
+        // one value per point: the current field fc
+        R zb = 0.0, zt = 0.0;
+        for (int k = 0; k < 4; k++) {
+            const double b = zero[k] ? 0.0 : (double)fc[col[k] * nk + (vb.ibot - 1)];
+            const double a = zero[k] ? 0.0 : (double)fc[col[k] * nk + (vb.itop - 1)];
+            zb = zb + cff[k] * b;
+            zt = zt + cff[k] * a;
+        }
+        out[0] = out[1] = out[2] = vb.c0 * zb + vb.c1 * zt;
+        return;
+    } else {
+    const F* f = (const F*)fv;
     R zb[3] = {0.0, 0.0, 0.0}, zt[3] = {0.0, 0.0, 0.0};
     for (int k = 0; k < 4; k++) {
         Slots4<F> lb, lt;
@@ -113,6 +135,7 @@ __device__ __forceinline__ void blend3(const F* f, const int64_t col[4], int nk,
     }
 #pragma unroll
     for (int t = 0; t < NSLOT; t++) out[t] = vb.c0 * zb[t] + vb.c1 * zt[t];
+    }
 }
 
 template <typename F, int NSLOT, class R>
@@ -155,7 +178,7 @@ __device__ __forceinline__ void interp_u(const DGrid& g, R X, R Y, const VBracke
     col[1] = (int64_t)(Jnode - 1) * xiu + (ib - 1);
     col[2] = (int64_t)(Jnode) * xiu + (ib - 1);
     col[3] = (int64_t)(Jnode) * xiu + (ia - 1);
-    blend3<F, NSLOT>((const F*)g.u, col, g.N, cff, zero, vb, S, out);
+    blend3<F, NSLOT>(g.u, g.cu, col, g.N, cff, zero, vb, S, out);
 }
 
 template <typename F, int NSLOT, class R>
@@ -197,7 +220,7 @@ __device__ __forceinline__ void interp_v(const DGrid& g, R X, R Y, const VBracke
     col[1] = (int64_t)(ja - 1) * g.L + (Inode);
     col[2] = (int64_t)(jb - 1) * g.L + (Inode);
     col[3] = (int64_t)(jb - 1) * g.L + (Inode - 1);
-    blend3<F, NSLOT>((const F*)g.v, col, g.N, cff, zero, vb, S, out);
+    blend3<F, NSLOT>(g.v, g.cv, col, g.N, cff, zero, vb, S, out);
 }
 
 // masked rho-point weights for w and AKs: bilinear, or IDW when a corner is dry
@@ -217,7 +240,7 @@ __device__ __forceinline__ void interp_w(const DGrid& g, const RhoCellT<R>& c,
     R cff[4];
     rho_weights(g, c, cff);
     const bool zero[4] = {false, false, false, false};
-    blend3<F, NSLOT>((const F*)g.w, c.o, g.N + 1, cff, zero, vb, S, out);
+    blend3<F, NSLOT>(g.w, g.cw, c.o, g.N + 1, cff, zero, vb, S, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,8 +327,8 @@ __device__ __forceinline__ void find_currents(const DGrid& g, const StepArgs& A,
     if (surf_out) *surf_out = s;
     if (Z < s.tdepth) Z = s.tdepth + (double)0.001f;
     if (Z > s.zeta) Z = s.zeta - (double)0.001f;
-    const VBracketT<R> br = bracket_rho<sizeof(F) == 4>(g, Z, s.zeta, s.tdepth);
-    const VBracketT<R> bw = bracket_w<sizeof(F) == 4>(g, Z, s.zeta, s.tdepth);
+    const VBracketT<R> br = bracket_rho<is_fast<F>()>(g, Z, s.zeta, s.tdepth);
+    const VBracketT<R> bw = bracket_w<is_fast<F>()>(g, Z, s.zeta, s.tdepth);
     R q[3];
     interp_u<F, NSLOT>(g, X, Y, br, A.slots, q);
     U = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
@@ -906,6 +929,16 @@ __device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, 
 {
 #pragma clang fp contract(fast)
     const int nw = g.N + 1;
+    if constexpr (is_cur<F>()) {
+        // zeta and AKs already at ix(2): one level depth, one value per corner
+        const float* ak = g.caks;
+        const double a0 = ak[c.o[0] * nw + (k - 1)], a1 = ak[c.o[1] * nw + (k - 1)];
+        const double a2 = ak[c.o[2] * nw + (k - 1)], a3 = ak[c.o[3] * nw + (k - 1)];
+        const double Sk = level_S(g, g.s_w, g.cs_w, k, hh);
+        z = level_z(g, g.s_w, g.cs_w, k, sf.zeta, hh, Sk);
+        kh = cff[0] * a0 + cff[1] * a1 + cff[2] * a2 + cff[3] * a3;
+        return;
+    } else {
     const F* aks = (const F*)g.aks;
     Slots4<F> v[4];
 #pragma unroll
@@ -923,6 +956,7 @@ __device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, 
                 cff[3] * pick(v[3], s);
     }
     kh = tinterp<F>(A, kk[0], kk[1], kk[2]);
+    }
 }
 
 template <typename F>
@@ -1342,7 +1376,7 @@ __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const 
         if (A.vturb_on) {
             const RhoCell c0 = rho_cell(g, X0, Y0);
             const Surface s0 = surface_at<F>(g, A, c0);
-            if (sizeof(F) == 4 && !FAITHFUL) {
+            if constexpr (is_fast<F>() && !FAITHFUL) {
                 const Scr32 S{(float*)A.scratch + tid, T};
                 TurbV = vturb_fast<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
             } else {
