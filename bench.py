@@ -39,6 +39,9 @@ from romspath_amd.runner import GpuRun, RecordSource  # noqa: E402
 from romspath_amd.schedule import schedule  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_VALU_TFLOPS = 78.6          # MI355X FP64 vector, FMA counted as 2
+# VALU lane-instructions per second: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+PEAK_VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # minimal stencil bytes per active particle-step, no reuse between particles:
 # per RK stage 88f+12 (h 4f, zeta 12f, u/v/w 72f, masks 12 B) x4, pm/pn 8f,
 # post-step depth+zeta 16f + 4 B per grid tested, VTurb (16+12(N+1))f + 4,
@@ -393,6 +396,25 @@ def main():
         tr = json.loads(tr_file.read_text())
         traffic = (tr["read_bytes_per_particle_step"] + tr["write_bytes_per_particle_step"]) * per_launch
     log(f"  kernels: advect {adv_ms:.3f} ms, vertical walk {walk_ms:.3f} ms per step")
+    # FP64 vector roofline of the same kernels: FLOP and VALU instructions per
+    # particle-step from rocprofv3 SQ counters of this workload
+    compute = None
+    va_file = ROOT / "profiles" / "valu_c3_fp32.json"
+    if args.workload == "c3" and args.precision == "fp32" and va_file.exists() and res["avg_kernel_ms"]:
+        va = json.loads(va_file.read_text())
+        ks = res["avg_kernel_ms"] * 1e-3
+        flop = sum(va["fp64_flop_per_particle_step"].values())
+        instr = sum(va["valu_lane_instr_per_particle_step"].values())
+        ach = flop * per_launch / ks / 1e12
+        compute = {"bound": "fp64 valu", "achieved": ach, "peak": PEAK_FP64_VALU_TFLOPS,
+                   "unit": "TFLOP/s", "frac": ach / PEAK_FP64_VALU_TFLOPS,
+                   "fp64_flop_per_particle_step": flop,
+                   "valu_lane_instr_per_particle_step": instr,
+                   "valu_issue_frac": instr * per_launch / ks / PEAK_VALU_LANE_OPS,
+                   "source": "rocprofv3 SQ_INSTS_VALU(_FMA/_MUL/_ADD_F64) per dispatch, "
+                             "profiles/valu_c3_fp32.md"}
+        log(f"  FP64 VALU: {ach:.1f} TFLOP/s ({ach / PEAK_FP64_VALU_TFLOPS:.1%} of peak), "
+            f"VALU issue {compute['valu_issue_frac']:.1%}")
 
     extra = {}
     if not args.no_fp64_form and args.precision == "fp32":
@@ -454,6 +476,7 @@ def main():
                      "avg_kernel_ms": res["avg_kernel_ms"],
                      "advect_kernel_ms": adv_ms, "vwalk_kernel_ms": walk_ms,
                      "active_particles_per_launch": per_launch},
+        "compute_roofline": compute,
         "cpu_baseline": cpu,
         "secant_capped": res["capped"],
     }

@@ -381,6 +381,9 @@ int64_t field_points(const GridDev& g, int f)
     }
 }
 
+// levels per column of the current AKs field
+int cur_column(const GridDev& g) { return (g.N + 1 + 3) & ~3; }
+
 int check_step(Ctx* c, int t_b, int t_c, int t_f, int64_t step_index)
 {
     if (t_b < 1 || t_b > 3 || t_c < 1 || t_c > 3 || t_f < 1 || t_f > 3)
@@ -500,12 +503,17 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
             GridDev& g = c->grids[gi];
             void* src[5] = {g.zeta, g.u, g.v, g.w, g.aks};
             for (int f = 0; f < 5; f++) {
-                const int64_t npt = field_points(g, f);
+                // AKs columns padded to a multiple of four levels
+                const int kin = f == 4 ? g.N + 1 : 1;
+                const int kout = f == 4 ? cur_column(g) : 1;
+                const int64_t npt = field_points(g, f) / kin * kout;
                 if (!g.cur[f] && (rc = dalloc(c, &g.cur[f], (size_t)npt))) return rc;
                 const int j = J.njobs++;
                 J.src[j] = (const float*)src[f];
                 J.dst[j] = g.cur[f];
                 J.n[j] = npt;
+                J.kin[j] = kin;
+                J.kout[j] = kout;
                 const bool first = ets == 1 && f >= 1 && f <= 3;
                 J.slot[j][0] = A.slots.s[0];
                 J.slot[j][1] = first ? A.slots.s[0] : A.slots.s[1];
@@ -525,6 +533,7 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
             d.mrho = g.mrho; d.mu = g.mu; d.mv = g.mv;
             d.zeta = g.zeta; d.u = g.u; d.v = g.v; d.w = g.w; d.aks = g.aks;
             d.czeta = g.cur[0]; d.cu = g.cur[1]; d.cv = g.cur[2]; d.cw = g.cur[3]; d.caks = g.cur[4];
+            d.nwc = cur_column(g);
             d.kh = c->cfg.kh[i];
             for (int a = 0; a < kMaxGrid; a++)
                 for (int k = 0; k < 2; k++) {

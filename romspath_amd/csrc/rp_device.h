@@ -41,8 +41,11 @@ struct DGrid {
     const int8_t *mrho, *mu, *mv;        // masks
     const void *zeta, *u, *v, *w, *aks;  // FP32 or FP64, slot-interleaved
     // FP32 "current" fields of the step: the three slots already combined at
-    // the step's time ix(2), one value per point (same point order, no slot)
+    // the step's time ix(2), one value per point (same point order, no slot);
+    // an AKs column holds nwc >= N+1 values (padded to a multiple of four so
+    // that four levels of a corner arrive in one aligned 16-byte load)
     const float *czeta, *cu, *cv, *cw, *caks;
+    int nwc;
     double kh;                           // ConstantHTurb(ng)
     double scl[kMaxGrid][2], off[kMaxGrid][2];
 };

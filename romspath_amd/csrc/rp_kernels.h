@@ -65,13 +65,17 @@ __global__ void pack_field(const double* __restrict__ src, const double* __restr
 
 __global__ void sample_deviates(uint64_t seed, int64_t step, int64_t n, int ndev, double* out);
 
-// current fields of a step: dst[e] = sum_t pw[t] src[4 e + slot[t]] in FP64,
-// stored FP32; one job per (grid, field), blockIdx.y selects the job
+// current fields of a step: dst[e] = sum_t pw[t] src[4 e' + slot[t]] in FP64,
+// stored FP32; one job per (grid, field), blockIdx.y selects the job.  A job
+// copies columns of kin levels into columns of kout >= kin levels: output e
+// = col kout + k takes source point e' = col kin + k for k < kin and is 0
+// in the padding.  n counts output points.
 constexpr int kMaxCurJobs = 5 * kMaxGrid;
 struct CurJobs {
     const float* src[kMaxCurJobs];
     float* dst[kMaxCurJobs];
     int64_t n[kMaxCurJobs];
+    int kin[kMaxCurJobs], kout[kMaxCurJobs];
     int slot[kMaxCurJobs][3];
     double pw[3];
     int njobs;

@@ -148,9 +148,20 @@ __global__ void __launch_bounds__(256) current_fields(CurJobs J)
     float* dst = J.dst[j];
     const int s0 = J.slot[j][0], s1 = J.slot[j][1], s2 = J.slot[j][2];
     const double p0 = J.pw[0], p1 = J.pw[1], p2 = J.pw[2];
+    const int kin = J.kin[j], kout = J.kout[j];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < J.n[j];
          e += (int64_t)gridDim.x * blockDim.x) {
-        const float4 v = reinterpret_cast<const float4*>(src)[e];
+        int64_t si = e;
+        if (kin != kout) {
+            const int64_t col = e / kout;
+            const int k = (int)(e - col * kout);
+            if (k >= kin) {
+                dst[e] = 0.0f;
+                continue;
+            }
+            si = col * kin + k;
+        }
+        const float4 v = reinterpret_cast<const float4*>(src)[si];
         auto pk = [&](int t) { return (double)(t == 0 ? v.x : (t == 1 ? v.y : (t == 2 ? v.z : v.w))); };
         dst[e] = (float)(p0 * pk(s0) + p1 * pk(s1) + p2 * pk(s2));
     }

@@ -932,8 +932,9 @@ __device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, 
     if constexpr (is_cur<F>()) {
         // zeta and AKs already at ix(2): one level depth, one value per corner
         const float* ak = g.caks;
-        const double a0 = ak[c.o[0] * nw + (k - 1)], a1 = ak[c.o[1] * nw + (k - 1)];
-        const double a2 = ak[c.o[2] * nw + (k - 1)], a3 = ak[c.o[3] * nw + (k - 1)];
+        const int nwc = g.nwc;
+        const double a0 = ak[c.o[0] * nwc + (k - 1)], a1 = ak[c.o[1] * nwc + (k - 1)];
+        const double a2 = ak[c.o[2] * nwc + (k - 1)], a3 = ak[c.o[3] * nwc + (k - 1)];
         const double Sk = level_S(g, g.s_w, g.cs_w, k, hh);
         z = level_z(g, g.s_w, g.cs_w, k, sf.zeta, hh, Sk);
         kh = cff[0] * a0 + cff[1] * a1 + cff[2] * a2 + cff[3] * a3;
@@ -976,28 +977,52 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     // walked once, kLv levels per batch of loads, emitting y (linint: the
     // point fz uses the segment of the largest level j<=nw-1 with z(j)<=fz)
     double z1, k1, zn, kn;
+    constexpr int kLv = 4;
+    // levels kb..kb+kLv-1 of the column (those in 2..nw are used)
+    auto batch = [&](int kb, double zk[kLv], double kk[kLv]) {
+        if constexpr (is_cur<F>()) {
+            // kb-1 is a multiple of four: one aligned 16-byte load per corner
+            const int nwc = g.nwc;
+            const float4 a0 = *reinterpret_cast<const float4*>(g.caks + c.o[0] * nwc + (kb - 1));
+            const float4 a1 = *reinterpret_cast<const float4*>(g.caks + c.o[1] * nwc + (kb - 1));
+            const float4 a2 = *reinterpret_cast<const float4*>(g.caks + c.o[2] * nwc + (kb - 1));
+            const float4 a3 = *reinterpret_cast<const float4*>(g.caks + c.o[3] * nwc + (kb - 1));
+            const float v0[kLv] = {a0.x, a0.y, a0.z, a0.w}, v1[kLv] = {a1.x, a1.y, a1.z, a1.w};
+            const float v2[kLv] = {a2.x, a2.y, a2.z, a2.w}, v3[kLv] = {a3.x, a3.y, a3.z, a3.w};
+#pragma unroll
+            for (int t = 0; t < kLv; t++) {
+                const int k = kb + t;
+                if (k >= 2 && k <= nw) {
+                    const double Sk = level_S(g, g.s_w, g.cs_w, k, hh);
+                    zk[t] = level_z(g, g.s_w, g.cs_w, k, zeta, hh, Sk);
+                    kk[t] = cff[0] * (double)v0[t] + cff[1] * (double)v1[t] + cff[2] * (double)v2[t] +
+                            cff[3] * (double)v3[t];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < kLv; t++) {
+                const int k = kb + t;
+                if (k >= 2 && k <= nw) column_level<F>(g, A, c, cff, sf, hh, k, zk[t], kk[t]);
+            }
+        }
+    };
     column_level<F>(g, A, c, cff, sf, hh, 1, z1, k1);
     column_level<F>(g, A, c, cff, sf, hh, nw, zn, kn);
     const double h = (zn - z1) / (double)(np - 1);
     const double rh = 1.0 / h;
     {
-        constexpr int kLv = 4;
         double zlo = z1, klo = k1;
         double ym1 = 0.0, y0 = k1;
         S(Lo.y + 0) = (float)k1;
         int i = 2;                                   // next point to emit
-        for (int kb = 2; kb <= nw; kb += kLv) {
+        for (int kb = 1; kb <= nw; kb += kLv) {
             double zk[kLv], kk[kLv];
+            batch(kb, zk, kk);
 #pragma unroll
             for (int t = 0; t < kLv; t++) {
                 const int k = kb + t;
-                if (k < nw) column_level<F>(g, A, c, cff, sf, hh, k, zk[t], kk[t]);
-                else { zk[t] = zn; kk[t] = kn; }
-            }
-#pragma unroll
-            for (int t = 0; t < kLv; t++) {
-                const int k = kb + t;
-                if (k <= nw) {
+                if (k >= 2 && k <= nw) {
                     const double zhi = zk[t], khi = kk[t];
                     const double m = (klo - khi) / (zlo - zhi);
                     const double b = klo - m * zlo;
