@@ -89,7 +89,10 @@ int rp_set_grid(void *ctx, int32_t ng, const rp_grid *g);
  * v (xi_v,eta_v,s_rho); w and aks (xi_rho,eta_rho,s_w).  su/sv (Stokes drift,
  * may be NULL) are added to u/v in FP64 before storage, as under -DSTOKES
  * (hydrodynamic_module.f90:608-657).  Asynchronous on the context stream; the
- * host arrays may be reused when the call returns. */
+ * host arrays may be reused when the call returns.  The arrays are copied to
+ * pinned host memory before the call returns; the host-to-device copies run on
+ * a second stream, overlapping the steps already queued, and the FP64 -> field
+ * precision conversion runs on the context stream after those steps. */
 int rp_set_hydro_slot(void *ctx, int32_t ng, int32_t slot, const double *zeta, const double *u,
                       const double *v, const double *w, const double *aks, const double *su,
                       const double *sv);

@@ -48,9 +48,29 @@ struct Ctx {
     double* dev = nullptr;
     int ndev = 0;
     bool use_dev = false;
-    // uploads
-    double* staging = nullptr;
-    size_t staging_elems = 0;
+    // hydro uploads (upload_field): the caller's FP64 arrays are copied into
+    // pinned host buffers before the call returns, the host-to-device copies
+    // run on their own stream while queued steps execute, and each field's
+    // pack kernel runs in order on the step stream.  Device staging is kept
+    // per (grid, field), so one record's copies never wait for the steps
+    // still reading the previous one.
+    hipStream_t up_stream = nullptr;
+    struct Pinned {
+        double* p = nullptr;
+        size_t elems = 0;
+        hipEvent_t done = nullptr;      // its last host-to-device copy finished
+        bool used = false;
+    };
+    Pinned pinned[2];
+    int pinned_next = 0;
+    struct FieldStage {
+        double* dev = nullptr;
+        size_t elems = 0;
+        hipEvent_t copied = nullptr;    // on up_stream: the record is on the device
+        hipEvent_t packed = nullptr;    // on stream: the pack kernel has read it
+        bool used = false;
+    };
+    FieldStage fstage[kMaxGrid][5];
     // VTurb scratch
     double* scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -181,43 +201,69 @@ double status_value(int8_t c)
     }
 }
 
-int ensure_staging(Ctx* c, size_t elems)
+int make_event(Ctx* c, hipEvent_t& e)
 {
-    if (c->staging_elems >= elems) return RP_OK;
-    int rc = dalloc(c, &c->staging, 2 * elems);
-    if (rc) return rc;
-    c->staging_elems = elems;
+    if (!e) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return RP_OK;
 }
 
 // Upload one FP64 Fortran (nx, ny, nk) field into physical slot `slot` of
 // the interleaved device array, adding `add` (may be null) in FP64 first.
-int upload_field(Ctx* c, void* dst, const double* src, const double* add, int nx, int ny, int nk,
-                 int slot)
+// Field f of grid gi has its own device staging buffer.  The call returns
+// once the caller's arrays have been read; the conversion is ordered on the
+// step stream, after the steps already queued there and before later ones.
+int upload_field(Ctx* c, int gi, int f, void* dst, const double* src, const double* add, int nx,
+                 int ny, int nk, int slot)
 {
     const size_t elems = (size_t)nx * ny * nk;
-    int rc = ensure_staging(c, elems);
-    if (rc) return rc;
-    rc = put(c, c->staging, src, elems * sizeof(double));
-    if (rc) return rc;
-    const double* dadd = nullptr;
-    if (add) {
-        rc = put(c, c->staging + elems, add, elems * sizeof(double));
-        if (rc) return rc;
-        dadd = c->staging + elems;
+    const size_t need = add ? 2 * elems : elems;
+    int rc;
+    if (!c->up_stream) HIPCHK(c, hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking));
+    Ctx::FieldStage& fs = c->fstage[gi][f];
+    if ((rc = make_event(c, fs.copied)) || (rc = make_event(c, fs.packed))) return rc;
+    if (fs.elems < need) {
+        if (fs.used) HIPCHK(c, hipEventSynchronize(fs.packed));
+        fs.elems = 0;
+        if ((rc = dalloc(c, &fs.dev, need))) return rc;
+        fs.elems = need;
+    } else if (fs.used) {
+        // the copy must not overwrite what the previous pack still reads
+        HIPCHK(c, hipStreamWaitEvent(c->up_stream, fs.packed, 0));
     }
+    const double* part[2] = {src, add};
+    for (int k = 0; k < (add ? 2 : 1); k++) {
+        Ctx::Pinned& h = c->pinned[c->pinned_next];
+        c->pinned_next ^= 1;
+        if ((rc = make_event(c, h.done))) return rc;
+        if (h.used) HIPCHK(c, hipEventSynchronize(h.done));
+        if (h.elems < elems) {
+            if (h.p) (void)hipHostFree(h.p);
+            h.p = nullptr;
+            h.elems = 0;
+            HIPCHK(c, hipHostMalloc((void**)&h.p, elems * sizeof(double), hipHostMallocDefault));
+            h.elems = elems;
+        }
+        std::memcpy(h.p, part[k], elems * sizeof(double));
+        HIPCHK(c, hipMemcpyAsync(fs.dev + k * elems, h.p, elems * sizeof(double),
+                                 hipMemcpyHostToDevice, c->up_stream));
+        HIPCHK(c, hipEventRecord(h.done, c->up_stream));
+        h.used = true;
+    }
+    HIPCHK(c, hipEventRecord(fs.copied, c->up_stream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, fs.copied, 0));
+    const double* dadd = add ? fs.dev + elems : nullptr;
     const int block = 256;
     const int64_t want = ((int64_t)elems + block - 1) / block;
     const int grid = (int)std::min<int64_t>(want, (int64_t)c->ncu * 16);
     if (c->fbytes == 4)
-        hipLaunchKernelGGL(pack_field<float>, dim3(grid), dim3(block), 0, c->stream, c->staging,
-                           dadd, (float*)dst, nx, ny, nk, slot);
+        hipLaunchKernelGGL(pack_field<float>, dim3(grid), dim3(block), 0, c->stream, fs.dev, dadd,
+                           (float*)dst, nx, ny, nk, slot);
     else
-        hipLaunchKernelGGL(pack_field<double>, dim3(grid), dim3(block), 0, c->stream, c->staging,
-                           dadd, (double*)dst, nx, ny, nk, slot);
+        hipLaunchKernelGGL(pack_field<double>, dim3(grid), dim3(block), 0, c->stream, fs.dev, dadd,
+                           (double*)dst, nx, ny, nk, slot);
     HIPCHK(c, hipGetLastError());
-    // the staging buffer is reused by the next upload: finish this one first
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipEventRecord(fs.packed, c->stream));
+    fs.used = true;
     return RP_OK;
 }
 
@@ -905,11 +951,12 @@ int rp_set_hydro_slot(void* ctx, int32_t ng, int32_t slot, const double* zeta, c
     int rc;
     c->adj.dirty = true;
     c->field_gen++;
-    if ((rc = upload_field(c, g.zeta, zeta, nullptr, g.L, g.M, 1, s)) ||
-        (rc = upload_field(c, g.u, u, su, g.L - 1, g.M, g.N, s)) ||
-        (rc = upload_field(c, g.v, v, sv, g.L, g.M - 1, g.N, s)) ||
-        (rc = upload_field(c, g.w, w, nullptr, g.L, g.M, g.N + 1, s)) ||
-        (rc = upload_field(c, g.aks, aks, nullptr, g.L, g.M, g.N + 1, s)))
+    const int gi = ng - 1;
+    if ((rc = upload_field(c, gi, 0, g.zeta, zeta, nullptr, g.L, g.M, 1, s)) ||
+        (rc = upload_field(c, gi, 1, g.u, u, su, g.L - 1, g.M, g.N, s)) ||
+        (rc = upload_field(c, gi, 2, g.v, v, sv, g.L, g.M - 1, g.N, s)) ||
+        (rc = upload_field(c, gi, 3, g.w, w, nullptr, g.L, g.M, g.N + 1, s)) ||
+        (rc = upload_field(c, gi, 4, g.aks, aks, nullptr, g.L, g.M, g.N + 1, s)))
         return rc;
     g.slot_set[s] = true;
     return RP_OK;
@@ -1459,7 +1506,19 @@ int rp_destroy(void* ctx)
     dfree(c->sort_cnt); dfree(c->perm); dfree(c->alt_id);
     for (auto& a : c->alt64) dfree(a);
     for (auto& a : c->alt8) dfree(a);
-    dfree(c->dev); dfree(c->staging); dfree(c->scratch);
+    if (c->up_stream) (void)hipStreamSynchronize(c->up_stream);
+    for (auto& h : c->pinned) {
+        if (h.p) (void)hipHostFree(h.p);
+        if (h.done) (void)hipEventDestroy(h.done);
+    }
+    for (auto& row : c->fstage)
+        for (auto& fs : row) {
+            dfree(fs.dev);
+            if (fs.copied) (void)hipEventDestroy(fs.copied);
+            if (fs.packed) (void)hipEventDestroy(fs.packed);
+        }
+    if (c->up_stream) (void)hipStreamDestroy(c->up_stream);
+    dfree(c->dev); dfree(c->scratch);
     dfree(c->active); dfree(c->capped); dfree(c->dgrids); dfree(c->work);
     for (auto& e : c->ev_used) for (auto& x : e.e) (void)hipEventDestroy(x);
     for (auto& e : c->ev_free) for (auto& x : e.e) (void)hipEventDestroy(x);
