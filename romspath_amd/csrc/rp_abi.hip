@@ -526,6 +526,12 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
             bpc = std::max(nb, 1);
         }
         P.grid_vt = std::min<int64_t>(P.grid_all, (int64_t)c->ncu * bpc);
+        if (f32) {
+            // the FP32 rows (6 np per lane) are addressed through one buffer
+            // descriptor, whose extent is below 2^31 bytes
+            const int64_t per_block = 6LL * A.np * block * 4;
+            P.grid_vt = std::max<int64_t>(1, std::min<int64_t>(P.grid_vt, 0x7FFFFFFFLL / per_block));
+        }
         const int64_t T = P.grid_vt * block;
         const size_t need = (size_t)vturb_rows(A.np, maxN + 1) * T * sizeof(double);
         if (c->scratch_bytes < need) {
@@ -533,6 +539,7 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
             c->scratch_bytes = need;
         }
         A.scratch = c->scratch;
+        A.scratch_bytes = (uint32_t)std::min<size_t>(c->scratch_bytes, 0x7FFFFFFF);
     }
     if (c->cfg.sort_interval > 0) {
         SortArgs K;

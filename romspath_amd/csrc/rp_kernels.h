@@ -41,6 +41,7 @@ struct StepArgs {
     const double* dev;           // injected deviates [id][ndev] or null
     int ndev;
     double* scratch;             // VTurb work arrays, column per lane
+    uint32_t scratch_bytes;      // extent of scratch addressed as FP32 rows (< 2^31)
     double* mid;                 // [4][n] newX, newY, AdvectZ, clamped z between phases
     unsigned long long* active;  // += particles that took the step
     unsigned long long* work;    // vwalk_kernel's next unclaimed particle (zeroed per launch)

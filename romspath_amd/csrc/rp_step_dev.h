@@ -376,10 +376,36 @@ struct ScrT {
 using Scr = ScrT<double>;
 
 // the same lane column viewed as FP32 rows (the FP32-field form's work arrays)
+// The same scratch viewed as FP32 rows of T lanes, read and written by
+// buffer instructions over one descriptor: the lane's column is its vector
+// offset, and for a row index that is the same across the wave, S(row), the
+// row's byte offset is a scalar offset, so a row access needs no address
+// arithmetic per lane.  S.at(row) takes a per-lane row.  Accesses beyond
+// the descriptor's extent read 0 and are not written.
 struct Scr32 {
-    float* p;
-    int64_t T;
-    __device__ __forceinline__ float& operator()(int row) const { return p[(int64_t)row * T]; }
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t voff;   // lane * 4
+    uint32_t rowb;   // T * 4
+    __device__ __forceinline__ Scr32(float* base, uint32_t bytes, uint32_t lane, uint32_t T)
+        : r(__builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, 0x00020000)), voff(lane * 4u),
+          rowb(T * 4u)
+    {
+    }
+    struct Ref {
+        const Scr32& s;
+        uint32_t vo, so;
+        __device__ __forceinline__ operator float() const
+        {
+            return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(s.r, vo, so, 0));
+        }
+        __device__ __forceinline__ const Ref& operator=(float v) const
+        {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), s.r, vo, so, 0);
+            return *this;
+        }
+    };
+    __device__ __forceinline__ Ref operator()(int row) const { return Ref{*this, voff, (uint32_t)row * rowb}; }
+    __device__ __forceinline__ Ref at(int row) const { return Ref{*this, voff + (uint32_t)row * rowb, 0u}; }
 };
 
 struct VtLayout {
@@ -887,10 +913,10 @@ __device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr32& S,
 #pragma clang fp contract(fast)
     const int np = Lo.np;
     // u(i-1..i+2), y(i..i+1), issued together
-    const double um1 = (i >= 2) ? (double)S(Lo.u + i - 2) : 0.0;
-    const double u0 = S(Lo.u + i - 1), u1 = S(Lo.u + i);
-    const double u2 = (i + 2 <= np) ? (double)S(Lo.u + i + 1) : 0.0;
-    const double y0 = S(Lo.y + i - 1), y1 = S(Lo.y + i);
+    const double um1 = (i >= 2) ? (double)S.at(Lo.u + i - 2) : 0.0;
+    const double u0 = (float)S.at(Lo.u + i - 1), u1 = (float)S.at(Lo.u + i);
+    const double u2 = (i + 2 <= np) ? (double)S.at(Lo.u + i + 1) : 0.0;
+    const double y0 = (float)S.at(Lo.y + i - 1), y1 = (float)S.at(Lo.y + i);
     const double r_i = (i >= 2) ? (u0 - um1) * rh : 0.0;
     const double r_i1 = (u1 - u0) * rh;
     const double qu_i = r_i1 - r_i;
@@ -1035,9 +1061,9 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
                             if (k < nw && !(fz < zhi)) break;
                             yi = m * fz + b;
                         }
-                        S(Lo.y + i - 1) = (float)yi;
+                        S.at(Lo.y + i - 1) = (float)yi;
                         // qty row i-3 needs y(i-2): defined for i >= 3 (rows 0..np-3)
-                        if (i >= 3) S(Lo.q + i - 3) = (float)(((yi - y0) - (y0 - ym1)) * rh);
+                        if (i >= 3) S.at(Lo.q + i - 3) = (float)(((yi - y0) - (y0 - ym1)) * rh);
                         ym1 = y0;
                         y0 = yi;
                     }
@@ -1402,7 +1428,7 @@ __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const 
             const RhoCell c0 = rho_cell(g, X0, Y0);
             const Surface s0 = surface_at<F>(g, A, c0);
             if constexpr (is_fast<F>() && !FAITHFUL) {
-                const Scr32 S{(float*)A.scratch + tid, T};
+                const Scr32 S((float*)A.scratch, A.scratch_bytes, (uint32_t)tid, (uint32_t)T);
                 TurbV = vturb_fast<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
             } else {
                 const Scr S{A.scratch + tid, T};
