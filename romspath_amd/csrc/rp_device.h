@@ -293,6 +293,46 @@ __host__ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t
     return c;
 }
 
+// sin and cos of 2 pi (x + 1/2) 2^-32 for a 32-bit integer x.  The angle is
+// reduced exactly on the integer: the top two bits give the quadrant, and
+// the rest is folded to an angle t in (0, pi/4] (its complement where it is
+// past pi/4), where the Taylor series of sin and cos through t^17 and t^18
+// leave a truncation error below 1e-19.
+__host__ __device__ __forceinline__ void sincos_2pi_u32(uint32_t x, double& s, double& c)
+{
+    const uint32_t q = x >> 30, m = x & 0x3FFFFFFFu;
+    const bool fold = m >= 0x20000000u;
+    const double f = fold ? (double)(0x40000000u - m) - 0.5 : (double)m + 0.5;
+    const double t = f * (6.283185307179586 * 2.3283064365386963e-10);   // 2 pi 2^-32
+    const double t2 = t * t;
+    double ps = 1.0 / 355687428096000.0;                 // 1/17!
+    ps = ps * t2 - 1.0 / 1307674368000.0;                // 1/15!
+    ps = ps * t2 + 1.0 / 6227020800.0;                   // 1/13!
+    ps = ps * t2 - 1.0 / 39916800.0;                     // 1/11!
+    ps = ps * t2 + 1.0 / 362880.0;                       // 1/9!
+    ps = ps * t2 - 1.0 / 5040.0;
+    ps = ps * t2 + 1.0 / 120.0;
+    ps = ps * t2 - 1.0 / 6.0;
+    const double st = t + t * (t2 * ps);
+    double pc = 1.0 / 6402373705728000.0;                // 1/18!
+    pc = pc * t2 - 1.0 / 20922789888000.0;               // 1/16!
+    pc = pc * t2 + 1.0 / 87178291200.0;                  // 1/14!
+    pc = pc * t2 - 1.0 / 479001600.0;                    // 1/12!
+    pc = pc * t2 + 1.0 / 3628800.0;                      // 1/10!
+    pc = pc * t2 - 1.0 / 40320.0;
+    pc = pc * t2 + 1.0 / 720.0;
+    pc = pc * t2 - 1.0 / 24.0;
+    pc = pc * t2 + 0.5;
+    const double ct = 1.0 - t2 * pc;
+    const double s1 = fold ? ct : st, c1 = fold ? st : ct;
+    switch (q) {
+    case 0: s = s1; c = c1; break;
+    case 1: s = c1; c = -s1; break;
+    case 2: s = -s1; c = -c1; break;
+    default: s = -c1; c = s1; break;
+    }
+}
+
 // Four standard normals from one Philox block: two Box-Muller pairs on
 // uniforms (x + 1/2) 2^-32 in (0,1), the resolution of genrand_real3
 // (random_module.f90:240-247).  The reference keeps only the cosine branch
@@ -307,13 +347,11 @@ __device__ __forceinline__ void normals4(uint64_t seed, uint64_t id, uint64_t st
     c.w = (uint32_t)step;
     const U4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32) ^ (uint32_t)(step >> 32));
     const double s = 2.3283064365386963e-10;  // 2^-32
-    const double u0 = ((double)r.x + 0.5) * s, u1 = ((double)r.y + 0.5) * s;
-    const double u2 = ((double)r.z + 0.5) * s, u3 = ((double)r.w + 0.5) * s;
-    const double twopi = 6.283185307179586;
+    const double u0 = ((double)r.x + 0.5) * s, u2 = ((double)r.z + 0.5) * s;
     const double ra = sqrt(-2.0 * log(u0)), rb = sqrt(-2.0 * log(u2));
     double sa, ca, sb, cb;
-    sincos(twopi * u1, &sa, &ca);
-    sincos(twopi * u3, &sb, &cb);
+    sincos_2pi_u32(r.y, sa, ca);     // angle 2 pi u1, u1 = (r.y + 1/2) 2^-32
+    sincos_2pi_u32(r.w, sb, cb);
     out[0] = ra * ca; out[1] = ra * sa; out[2] = rb * cb; out[3] = rb * sb;
 }
 

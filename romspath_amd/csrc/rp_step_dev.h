@@ -795,7 +795,8 @@ struct GSweep {
         const double rd = rcp64(d);
         const double wrd = w * rd;
         g += w * wrd;
-        gp += (2.0 * w * dw - wrd * w * dd) * rd;
+        // d(w^2/d)/dp = (2 w dw - w^2 dd / d) / d = wrd (2 dw - wrd dd)
+        gp += wrd * ((dw + dw) - wrd * dd);
         const double rat = e * rd, drat = (de - rat * dd) * rd;
         const double rat2 = B.c * rd, drat2 = (B.dc - rat2 * dd) * rd;
         const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
