@@ -36,7 +36,8 @@ enum {
     RP_E_HIP = -3,        /* HIP runtime error */
     RP_E_ALLOC = -4,      /* device allocation failed */
     RP_E_RANGE = -5,      /* size beyond a limit */
-    RP_E_NODEV = -6       /* no usable GPU */
+    RP_E_NODEV = -6,      /* no usable GPU */
+    RP_E_IO = -7          /* file could not be written */
 };
 
 enum { RP_PREC_FP32_FIELDS = 0, RP_PREC_FP64 = 1 };
@@ -117,6 +118,16 @@ int rp_step(void *ctx, const double ex[3], const double ix[3], int32_t ets, int3
 int rp_get_particles(void *ctx, int64_t n, double *x, double *y, double *z, double *age,
                      double *status, int32_t *gid, uint8_t *isin, double *wd, double *zeta,
                      double *bath);
+
+/* Trajectory output support (dataOutput, ROMSPath.f90:1012-1025).
+ * rp_set_lonlat: GRIDS(ng)%lon_rho / lat_rho, (xi_rho, eta_rho) FP64, which
+ * InitGrid reads from lon_rho/lat_rho, or x_rho/y_rho on Cartesian grids
+ * (grid_module.f90:266-300).  rp_sample_lonlat: getInterp2D("lon"/"lat")
+ * (interpolation_module.f90:421-507) at every particle's X, Y on its grid,
+ * in release order; synchronises the stream.  The corners are clamped into
+ * the grid as in the step (the reference would index outside its arrays). */
+int rp_set_lonlat(void *ctx, int32_t ng, const double *lon, const double *lat);
+int rp_sample_lonlat(void *ctx, int64_t n, double *lon, double *lat);
 
 /* Test hook: use these normal deviates instead of the RNG for the following
  * steps.  normals is [n][ndev] in release order, ndev = 2 + idt/int(deltat):

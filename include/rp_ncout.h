@@ -1,0 +1,77 @@
+/*
+ * rp_ncout.h -- trajectory output of librp.so: the reference's netCDF file
+ * (createNetCDF / writeNetCDF, ROMSPath.f90:1140-2157), written by the
+ * library itself because no netCDF library is present.
+ *
+ * The file is netCDF classic with 64-bit offsets (CDF-2), readable by every
+ * netCDF-3/-4 reader.  The reference creates NF90_NETCDF4 (HDF5) files with
+ * deflate; the schema is the same: dimensions numpar and time (unlimited),
+ * and in the reference's order the variables
+ *   model_time(time)            double  "Model time", units = time_units
+ *   dob(numpar)                 float   date of birth (s), written at create
+ *   age(time, numpar)           double  par(:,pAge)
+ *   lon, lat(time, numpar)      float   getInterp2D("lon"/"lat") at the particle
+ *   zp(time, numpar)            float   par(:,pZ)
+ *   status(time, numpar)        float   par(:,pStatus)
+ *   GID(time, numpar)           float   par(:,pGID), only when Ngrid > 1
+ *   WaterDepth, zeta, h         float   par(:,pWD/pZeta/pBath), when asked for
+ * with the reference's long_name / units / field attributes and the global
+ * attributes of createNetCDF that apply to the hot path.  Values are rounded
+ * to float exactly as nf90_put_var converts double arrays into NF_FLOAT
+ * variables (round to nearest).
+ *
+ * Host code only: nothing here touches the GPU.  Arrays are in release
+ * order (as rp_get_particles returns them).  Every function returns 0 or a
+ * negative RP_E* code (rp_abi.h); rp_nc_error() gives the message.
+ */
+#ifndef RP_NCOUT_H
+#define RP_NCOUT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4 };
+
+/* Global attributes (createNetCDF, ROMSPath.f90:1650-1760). */
+typedef struct {
+    const char *time_units;      /* model_time units (GRID_MOD time_units) */
+    const char *run_name;        /* RunName, may be NULL */
+    int32_t idt;                 /* adv_time_step */
+    float days;                  /* days_run */
+    int32_t behavior;            /* Behavior */
+    int32_t open_boundary;       /* OpenOceanBoundary -> "TRUE" when set */
+    int32_t vturb_on, hturb_on;  /* Vertical_Turb / Horiz_Turb */
+    int32_t seed;
+    double deltat, serr, smth, sub, aksback;
+} rp_nc_meta;
+
+/* createNetCDF(par(:,pDOB)) (ROMSPath.f90:449, 1140-1868): creates the file
+ * for numpar particles and writes dob.  flags: RP_NC_* optional variables;
+ * GID is defined when ngrid > 1. */
+int rp_nc_create(void **nc, const char *path, int64_t numpar, int32_t ngrid, int32_t flags,
+                 const rp_nc_meta *meta, const double *dob);
+
+/* writeNetCDF (ROMSPath.f90:1870-2157): appends one time record.
+ * model_time = DBLE(time) + reftime.  gid, wd, zeta and bath may be NULL
+ * when the file has no such variable. */
+int rp_nc_write(void *nc, double model_time, const double *age, const double *lon,
+                const double *lat, const double *zp, const double *status, const int32_t *gid,
+                const double *wd, const double *zeta, const double *bath);
+
+/* Records written so far. */
+int64_t rp_nc_records(void *nc);
+
+/* Message of the last failure of this writer. */
+const char *rp_nc_error(void *nc);
+
+/* Closes the file and frees the writer. */
+int rp_nc_close(void *nc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RP_NCOUT_H */

@@ -278,3 +278,27 @@ def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None):
             on_step(st, state)
     state.active = active
     return state
+
+
+def interp2d_rho(field, X, Y):
+    """getInterp2D for the plain-bilinear variables ("lon", "lat", "depth",
+    "pm", "pn"; interpolation_module.f90:421-507) at fractional 1-based rho
+    indices X, Y.  field is (eta_rho, xi_rho), i.e. Fortran (xi, eta).
+    Inode = floor(X), weights (1-x)(1-y), x(1-y), xy, (1-x)y on corners
+    (I,J), (I+1,J), (I+1,J+1), (I,J+1), summed left to right.  Corners are
+    clamped into the grid as the device code does (the reference would read
+    outside its arrays there)."""
+    X = np.asarray(X, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+    M, L = field.shape
+    I = np.floor(X).astype(np.int64)
+    J = np.floor(Y).astype(np.int64)
+    x = X - I
+    y = Y - J
+    i0, i1 = np.clip(I, 1, L) - 1, np.clip(I + 1, 1, L) - 1
+    j0, j1 = np.clip(J, 1, M) - 1, np.clip(J + 1, 1, M) - 1
+    c1 = (1 - x) * (1 - y)
+    c2 = x * (1 - y)
+    c3 = x * y
+    c4 = (1 - x) * y
+    return c1 * field[j0, i0] + c2 * field[j0, i1] + c3 * field[j1, i1] + c4 * field[j1, i0]

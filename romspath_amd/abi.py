@@ -18,7 +18,7 @@ RP_PREC_FP32_FIELDS = 0
 RP_PREC_FP64 = 1
 
 ERRORS = {-1: "RP_E_ARG", -2: "RP_E_STATE", -3: "RP_E_HIP", -4: "RP_E_ALLOC",
-          -5: "RP_E_RANGE", -6: "RP_E_NODEV"}
+          -5: "RP_E_RANGE", -6: "RP_E_NODEV", -7: "RP_E_IO"}
 
 PD = C.POINTER(C.c_double)
 PI32 = C.POINTER(C.c_int32)
@@ -30,7 +30,20 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_phase_timing",
            "rp_set_tuning", "rp_set_scenarios", "rp_step_scenario", "rp_step_batch",
            "rp_set_perturbation", "rp_adjoint_record", "rp_adjoint",
+           "rp_set_lonlat", "rp_sample_lonlat",
+           "rp_nc_create", "rp_nc_write", "rp_nc_records", "rp_nc_error", "rp_nc_close",
            "rp_last_error", "rp_destroy"]
+# entry points whose result is not an int status
+_RESTYPE = {"rp_last_error": C.c_char_p, "rp_nc_error": C.c_char_p, "rp_nc_records": C.c_int64}
+
+
+class RpNcMeta(C.Structure):
+    """rp_nc_meta of include/rp_ncout.h: the file's global attributes."""
+    _fields_ = [("time_units", C.c_char_p), ("run_name", C.c_char_p), ("idt", C.c_int32),
+                ("days", C.c_float), ("behavior", C.c_int32), ("open_boundary", C.c_int32),
+                ("vturb_on", C.c_int32), ("hturb_on", C.c_int32), ("seed", C.c_int32),
+                ("deltat", C.c_double), ("serr", C.c_double), ("smth", C.c_double),
+                ("sub", C.c_double), ("aksback", C.c_double)]
 
 
 class RpConfig(C.Structure):
@@ -89,12 +102,18 @@ def lib():
         L.rp_set_perturbation.argtypes = [vp, C.c_int32, C.c_int32, PD]
         L.rp_adjoint_record.argtypes = [vp, C.c_int32]
         L.rp_adjoint.argtypes = [vp, C.c_int64, PD, PD, PD, PD, PD, PD, PD]
+        L.rp_set_lonlat.argtypes = [vp, C.c_int32, PD, PD]
+        L.rp_sample_lonlat.argtypes = [vp, C.c_int64, PD, PD]
+        L.rp_nc_create.argtypes = [C.POINTER(vp), C.c_char_p, C.c_int64, C.c_int32, C.c_int32,
+                                   C.POINTER(RpNcMeta), PD]
+        L.rp_nc_write.argtypes = [vp, C.c_double, PD, PD, PD, PD, PD, PI32, PD, PD, PD]
+        L.rp_nc_records.argtypes = [vp]
+        L.rp_nc_error.argtypes = [vp]
+        L.rp_nc_close.argtypes = [vp]
         L.rp_last_error.argtypes = [vp]
-        L.rp_last_error.restype = C.c_char_p
         L.rp_destroy.argtypes = [vp]
         for name in EXPORTS:
-            if name != "rp_last_error":
-                getattr(L, name).restype = C.c_int
+            getattr(L, name).restype = _RESTYPE.get(name, C.c_int)
         _lib = L
     return _lib
 
@@ -276,6 +295,15 @@ class Context:
             return
         a = _f64(normals)
         self._chk("rp_set_deviates", self._L.rp_set_deviates(self.h, _d(a), a.shape[0], a.shape[1]))
+
+    def set_lonlat(self, ng, lon, lat):
+        lo, la = _f64(lon), _f64(lat)
+        self._chk("rp_set_lonlat", self._L.rp_set_lonlat(self.h, ng, _d(lo), _d(la)))
+
+    def sample_lonlat(self):
+        lon, lat = np.empty(self.n), np.empty(self.n)
+        self._chk("rp_sample_lonlat", self._L.rp_sample_lonlat(self.h, self.n, _d(lon), _d(lat)))
+        return lon, lat
 
     def sample_deviates(self, step_index, n, ndev):
         out = np.empty((n, ndev))

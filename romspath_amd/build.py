@@ -16,6 +16,9 @@ CSRC = PKG / "csrc"
 LIB = PKG / "lib" / "librp.so"
 SOURCES = ["rp_step.hip", "rp_adjoint.hip", "rp_sort.hip", "rp_abi.hip"]
 HEADERS = ["rp_device.h", "rp_kernels.h", "rp_dual.h", "rp_step_dev.h"]
+# host-only C++ (no device code): the trajectory file writer
+HOST_SOURCES = ["rp_ncout.cpp"]
+CXX = os.environ.get("CXX", "g++")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -85,8 +88,13 @@ def build(verbose: bool = True) -> Path:
         objs.append(str(obj))
         procs.append((src, _run([HIPCC, *compile_flags, "-I", str(ROOT / "include"), "-c",
                                  str(CSRC / src), "-o", str(obj)], verbose)))
+    for src in HOST_SOURCES:
+        obj = objdir / (Path(src).stem + ".o")
+        objs.append(str(obj))
+        procs.append((src, _run([CXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", str(ROOT / "include"),
+                                 "-c", str(CSRC / src), "-o", str(obj)], verbose)))
     for src, proc in procs:
-        _check(proc, f"hipcc {src}", verbose)
+        _check(proc, f"compile {src}", verbose)
     tmp = LIB.with_suffix(".so.tmp")
     _check(_run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs],
                 verbose), "link", verbose)

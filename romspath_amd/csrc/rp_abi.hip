@@ -24,6 +24,7 @@ struct GridDev {
     int8_t *mrho = nullptr, *mu = nullptr, *mv = nullptr;
     void *zeta = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *aks = nullptr;
     float* cur[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // FP32 form: fields at ix(2)
+    double *lon = nullptr, *lat = nullptr;   // lon_rho / lat_rho (x_rho / y_rho), output only
     double scl[kMaxGrid][2], off[kMaxGrid][2];
     bool set = false;
     bool slot_set[3] = {false, false, false};
@@ -276,6 +277,7 @@ void free_grid(GridDev& g)
     dfree(g.mrho); dfree(g.mu); dfree(g.mv);
     dfree(g.zeta); dfree(g.u); dfree(g.v); dfree(g.w); dfree(g.aks);
     for (auto& p : g.cur) dfree(p);
+    dfree(g.lon); dfree(g.lat);
     g.set = false;
 }
 
@@ -1403,6 +1405,60 @@ int rp_get_particles(void* ctx, int64_t n, double* x, double* y, double* z, doub
         std::vector<uint8_t> u8(n);
         HIPCHK(c, hipMemcpy(u8.data(), c->isin, n, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < n; i++) isin[ids[i]] = u8[i];
+    }
+    return RP_OK;
+}
+
+int rp_set_lonlat(void* ctx, int32_t ng, const double* lon, const double* lat)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (ng < 1 || ng > c->cfg.ngrid || !c->grids[ng - 1].set)
+        return fail(c, RP_E_STATE, "rp_set_lonlat: grid not set");
+    if (!lon || !lat) return fail(c, RP_E_ARG, "null lon/lat");
+    GridDev& g = c->grids[ng - 1];
+    const size_t nr = (size_t)g.L * g.M;
+    int rc;
+    if ((rc = dalloc(c, &g.lon, nr)) || (rc = dalloc(c, &g.lat, nr))) return rc;
+    if ((rc = put(c, g.lon, lon, nr * 8)) || (rc = put(c, g.lat, lat, nr * 8))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RP_OK;
+}
+
+int rp_sample_lonlat(void* ctx, int64_t n, double* lon, double* lat)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !lon || !lat) return RP_E_ARG;
+    if (n != c->n) return fail(c, RP_E_ARG, "particle count differs from rp_set_particles");
+    LonLatArgs A;
+    std::memset(&A, 0, sizeof(A));
+    for (int i = 0; i < c->cfg.ngrid; i++) {
+        const GridDev& g = c->grids[i];
+        if (!g.lon || !g.lat) return fail(c, RP_E_STATE, "rp_sample_lonlat: rp_set_lonlat missing");
+        A.lon[i] = g.lon;
+        A.lat[i] = g.lat;
+        A.L[i] = g.L;
+        A.M[i] = g.M;
+    }
+    if (n == 0) return RP_OK;
+    double* d = nullptr;
+    int rc = dalloc(c, &d, 2 * (size_t)n);
+    if (rc) return rc;
+    A.x = c->x; A.y = c->y; A.gid = c->gid; A.n = n;
+    A.out_lon = d; A.out_lat = d + n;
+    const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->ncu * 8);
+    hipLaunchKernelGGL(sample_lonlat, dim3(grid), dim3(256), 0, c->stream, A);
+    hipError_t e = hipGetLastError();
+    std::vector<double> h(2 * (size_t)n);
+    std::vector<uint32_t> ids(n);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, 2 * (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ids.data(), c->id, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(d);
+    if (e != hipSuccess) return fail(c, RP_E_HIP, std::string("rp_sample_lonlat: ") + hipGetErrorString(e));
+    for (int64_t i = 0; i < n; i++) {
+        lon[ids[i]] = h[i];
+        lat[ids[i]] = h[n + i];
     }
     return RP_OK;
 }

@@ -66,6 +66,19 @@ __global__ void pack_field(const double* __restrict__ src, const double* __restr
 
 __global__ void sample_deviates(uint64_t seed, int64_t step, int64_t n, int ndev, double* out);
 
+// getInterp2D("lon"/"lat") at every particle's position on its grid
+// (interpolation_module.f90:421-507, called from dataOutput, ROMSPath.f90:1012-1025)
+struct LonLatArgs {
+    const double* lon[kMaxGrid];
+    const double* lat[kMaxGrid];
+    int L[kMaxGrid], M[kMaxGrid];
+    const double *x, *y;
+    const int8_t* gid;
+    int64_t n;
+    double *out_lon, *out_lat;
+};
+__global__ void sample_lonlat(LonLatArgs A);
+
 // current fields of a step: dst[e] = sum_t pw[t] src[4 e' + slot[t]] in FP64,
 // stored FP32; one job per (grid, field), blockIdx.y selects the job.  A job
 // copies columns of kin levels into columns of kout >= kin levels: output e

@@ -140,6 +140,20 @@ __global__ void sample_deviates(uint64_t seed, int64_t step, int64_t n, int ndev
     for (int j = 0; j < ndev; j++) out[q * ndev + j] = dv.get(j);
 }
 
+__global__ void __launch_bounds__(256) sample_lonlat(LonLatArgs A)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < A.n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int gi = (int)A.gid[q] - 1;
+        DGrid g;
+        g.L = A.L[gi];
+        g.M = A.M[gi];
+        const RhoCell c = rho_cell(g, A.x[q], A.y[q]);
+        A.out_lon[q] = bilinear2d(A.lon[gi], c);
+        A.out_lat[q] = bilinear2d(A.lat[gi], c);
+    }
+}
+
 __global__ void __launch_bounds__(256) current_fields(CurJobs J)
 {
     const int j = blockIdx.y;

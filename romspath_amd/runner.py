@@ -45,6 +45,8 @@ class GpuRun:
         scl, off = scale_offsets(cfg.grids, cfg.refine) if ng > 1 else (None, None)
         for i, g in enumerate(cfg.grids):
             self.ctx.set_grid(i + 1, g, None if scl is None else scl[i], None if off is None else off[i])
+            # Cartesian grids: InitGrid reads x_rho / y_rho into lon_rho / lat_rho
+            self.ctx.set_lonlat(i + 1, g.x_rho, g.y_rho)
         self.ctx.set_particles(cfg.particles)
         self.records = records or RecordSource(cfg)
 
@@ -54,10 +56,12 @@ class GpuRun:
                 self.ctx.set_hydro_slot(g, ld.slot, self.records.get(g, ld.record))
 
     def run(self, max_steps=None, inject=False, on_step=None, diag_every_step=False,
-            mode="step", counts=None, use_graph=True):
+            mode="step", counts=None, use_graph=True, output=None):
         """mode "step": one rp_step per internal step; "batch": the steps of each
         external step in one rp_step_batch (one HIP graph when use_graph);
-        "scenario": rp_step_scenario for each of the scenarios given by counts."""
+        "scenario": rp_step_scenario for each of the scenarios given by counts.
+        output: a TrajectoryFile that gets a record whenever the reference's
+        printdt reaches iprint (ROMSPath.f90:532-543), at model time ix(3)."""
         prm = self.cfg.params
         n = self.cfg.particles.n
         ndev = ndev_of(prm)
@@ -93,6 +97,9 @@ class GpuRun:
             else:
                 self.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index,
                               want_diag=st.output)
+            if output is not None and st.output:
+                from .output import data_output
+                data_output(self.ctx, output, float(int(st.ix[2])))
             if on_step is not None:
                 on_step(st, self)
         return self.ctx.get_particles()
