@@ -26,6 +26,16 @@ module romspath_hip
     integer(c_int32_t) :: reserved(5)
   end type rp_config
 
+  ! rp_nc_meta (include/rp_ncout.h): global attributes of the trajectory file
+  integer(c_int32_t), parameter, public :: RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4
+  type, bind(C), public :: rp_nc_meta
+    type(c_ptr) :: time_units, run_name
+    integer(c_int32_t) :: idt
+    real(c_float) :: days
+    integer(c_int32_t) :: behavior, open_boundary, vturb_on, hturb_on, seed
+    real(c_double) :: deltat, serr, smth, sub, aksback
+  end type rp_nc_meta
+
   ! rp_grid: GRIDDATA after InitGrid.  C's scl[5][2] is Fortran scl(2,5):
   ! scl(:,i) is the reference's GRIDS(ng)%scl(i,:).
   type, bind(C), public :: rp_grid
@@ -42,6 +52,7 @@ module romspath_hip
             rp_secant_capped, rp_synchronize, rp_kernel_timing, rp_phase_timing, rp_last_error, &
             rp_destroy, rp_set_tuning, rp_set_scenarios, rp_step_scenario, rp_step_batch, &
             rp_set_perturbation, rp_adjoint_record, rp_adjoint, &
+            rp_set_lonlat, rp_sample_lonlat, rp_nc_create, rp_nc_write, rp_nc_close, &
             rp_error_text
 
   interface
@@ -197,6 +208,50 @@ module romspath_hip
       real(c_double), intent(in) :: ex(3, *), ix(3, *)
       integer(c_int32_t), intent(in) :: ets(*), slots(3, *), want_diag(*)
       integer(c_int64_t), intent(in) :: step_index(*)
+    end function
+
+    ! GRIDS(ng)%lon_rho / lat_rho (x_rho / y_rho on Cartesian grids)
+    integer(c_int) function rp_set_lonlat(ctx, ng, lon, lat) bind(C, name="rp_set_lonlat")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ng
+      real(c_double), intent(in) :: lon(*), lat(*)
+    end function
+
+    ! getInterp2D("lon"/"lat") at every particle (dataOutput, ROMSPath.f90:1012-1025)
+    integer(c_int) function rp_sample_lonlat(ctx, n, lon, lat) bind(C, name="rp_sample_lonlat")
+      import :: c_int, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(out) :: lon(*), lat(*)
+    end function
+
+    ! createNetCDF(par(:,pDOB)) (ROMSPath.f90:449, 1140-1868); path NUL-terminated
+    integer(c_int) function rp_nc_create(nc, path, numpar, ngrid, flags, meta, dob) &
+        bind(C, name="rp_nc_create")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_char, c_double, rp_nc_meta
+      type(c_ptr), intent(out) :: nc
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: numpar
+      integer(c_int32_t), value :: ngrid, flags
+      type(rp_nc_meta), intent(in) :: meta
+      real(c_double), intent(in) :: dob(*)
+    end function
+
+    ! writeNetCDF (ROMSPath.f90:1870-2157): one record
+    integer(c_int) function rp_nc_write(nc, model_time, age, lon, lat, zp, status, gid, wd, zeta, &
+        bath) bind(C, name="rp_nc_write")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: nc
+      real(c_double), value :: model_time
+      real(c_double), intent(in) :: age(*), lon(*), lat(*), zp(*), status(*)
+      integer(c_int32_t), intent(in) :: gid(*)
+      type(c_ptr), value :: wd, zeta, bath
+    end function
+
+    integer(c_int) function rp_nc_close(nc) bind(C, name="rp_nc_close")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nc
     end function
 
     type(c_ptr) function rp_last_error(ctx) bind(C, name="rp_last_error")

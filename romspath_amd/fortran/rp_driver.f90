@@ -8,7 +8,9 @@
 ! library exists in this environment); particles come from a text file of
 ! "X Y Z DOB" lines (fractional rho indices, metres, seconds).
 !
-! usage: rp_driver <namelist file> <particle file> <output file>
+! usage: rp_driver <namelist file> <particle file> <output file> [<netCDF file>]
+! With a fourth argument the run also writes the reference's trajectory file,
+! one record whenever printdt reaches iprint (ROMSPath.f90:532-543).
 program rp_driver
   use iso_c_binding
   use romspath_hip
@@ -44,7 +46,13 @@ program rp_driver
   type(rp_config) :: cfg
   type(rp_grid) :: g
   type(c_ptr) :: ctx
-  character(len=256) :: nmlfile, parfile, outfile
+  character(len=256) :: nmlfile, parfile, outfile, ncfile
+  type(c_ptr) :: nc
+  type(rp_nc_meta), target :: meta
+  character(kind=c_char, len=32), target :: tunits
+  double precision, allocatable :: plon(:), plat(:), xr(:, :), yr(:, :)
+  integer :: printdt
+  logical :: write_nc
   integer :: rc, ets, its, stepT, stepIT, t_b, t_c, t_f, tstep, ip, i, j, k, iu
   double precision :: ex(3), ix(3), x, y, Lx, Ly, th
   integer(c_int64_t) :: step_index, nact
@@ -54,6 +62,8 @@ program rp_driver
   call get_command_argument(1, nmlfile)
   call get_command_argument(2, parfile)
   call get_command_argument(3, outfile)
+  call get_command_argument(4, ncfile)
+  write_nc = len_trim(ncfile) > 0
   ConstantHTurb = 0.d0
   refine = 1.d0
   open (newunit=iu, file=trim(nmlfile), status='old', action='read')
@@ -157,6 +167,33 @@ program rp_driver
   rc = rp_set_particles(ctx, nl, px, py, pz, page, pdob, pst, pgid, pin)
   if (rc /= 0) call die('rp_set_particles', rc)
 
+  ! --- trajectory file (initNetCDF / createNetCDF, ROMSPath.f90:446-449) ---
+  if (write_nc) then
+    allocate (plon(numpar), plat(numpar), xr(L, M), yr(L, M))
+    do j = 1, M
+      do i = 1, L
+        xr(i, j) = (i - 1) * dx       ! Cartesian: InitGrid reads x_rho/y_rho as lon/lat
+        yr(i, j) = (j - 1) * dx
+      end do
+    end do
+    rc = rp_set_lonlat(ctx, 1_c_int32_t, xr, yr)
+    if (rc /= 0) call die('rp_set_lonlat', rc)
+    tunits = 'seconds since model start' // c_null_char
+    meta%time_units = c_loc(tunits)
+    meta%run_name = c_null_ptr
+    meta%idt = idt
+    meta%days = days
+    meta%behavior = 0
+    meta%open_boundary = 1
+    meta%vturb_on = merge(1, 0, VTurbOn)
+    meta%hturb_on = merge(1, 0, HTurbOn)
+    meta%seed = 1
+    meta%deltat = deltat; meta%serr = serr; meta%smth = smth; meta%sub = sub; meta%aksback = AKSback
+    rc = rp_nc_create(nc, trim(ncfile) // c_null_char, nl, 1_c_int32_t, 0_c_int32_t, meta, pdob)
+    if (rc /= 0) call die('rp_nc_create', rc)
+  end if
+  printdt = 0
+
   ! --- time loop (ROMSPath.f90:168-174, 465-546) ---
   t_b = 1; t_c = 2; t_f = 3; tstep = 1
   stepT = int(days * 86400.0) / dt
@@ -181,6 +218,12 @@ program rp_driver
                    int(t_f, c_int32_t), step_index, 0_c_int32_t)
       if (rc /= 0) call die('rp_step', rc)
       step_index = step_index + 1
+      ! PRINT OUTPUT TO FILE (ROMSPath.f90:532-543)
+      printdt = printdt + idt
+      if (printdt >= iprint) then
+        if (write_nc) call data_output(int(ix(3)))
+        printdt = 0
+      end if
     end do
     tstep = tstep + 1
   end do
@@ -196,9 +239,24 @@ program rp_driver
   close (iu)
   write (*, '(A,I0,A,I0,A,F9.4,A)') 'rp_driver: ', step_index, ' steps, ', nact, &
        ' particle-steps, ', ms, ' ms per step kernel'
+  if (write_nc) then
+    rc = rp_nc_close(nc)
+    if (rc /= 0) call die('rp_nc_close', rc)
+  end if
   rc = rp_destroy(ctx)
 
 contains
+
+  ! dataOutput (ROMSPath.f90:1012-1025): lon/lat at the particles, one record
+  subroutine data_output(time)
+    integer, intent(in) :: time
+    rc = rp_get_particles(ctx, nl, px, py, pz, page, pst, pgid, pin, wd, pzeta, bath)
+    if (rc /= 0) call die('rp_get_particles', rc)
+    rc = rp_sample_lonlat(ctx, nl, plon, plat)
+    if (rc /= 0) call die('rp_sample_lonlat', rc)
+    rc = rp_nc_write(nc, dble(time), page, plon, plat, pz, pst, pgid, c_null_ptr, c_null_ptr, c_null_ptr)
+    if (rc /= 0) call die('rp_nc_write', rc)
+  end subroutine data_output
 
   double precision function stretch(s)
     double precision, intent(in) :: s

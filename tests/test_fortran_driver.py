@@ -5,7 +5,10 @@ import subprocess
 import numpy as np
 import pytest
 
+from scipy.io import netcdf_file
+
 from oracle import oracle as O
+from oracle.oracle import interp2d_rho
 from romspath_amd import synth
 from romspath_amd.build import DRIVER
 
@@ -27,7 +30,8 @@ def test_fortran_driver_matches_oracle(tmp_path):
     np.savetxt(tmp_path / "par.txt", np.c_[p.x, p.y, p.z, p.dob], fmt="%.17e")
     print("case fortran driver C1", n, "particles", flush=True)
     r = subprocess.run([str(DRIVER), str(tmp_path / "run.nml"), str(tmp_path / "par.txt"),
-                        str(tmp_path / "out.txt")], capture_output=True, text=True, timeout=120)
+                        str(tmp_path / "out.txt"), str(tmp_path / "traj.nc")],
+                       capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     got = np.loadtxt(tmp_path / "out.txt")
@@ -36,6 +40,21 @@ def test_fortran_driver_matches_oracle(tmp_path):
     assert np.abs(got[:, 0] - o.x).max() <= 1e-9
     assert np.abs(got[:, 1] - o.y).max() <= 1e-9
     assert np.abs(got[:, 2] - o.z).max() <= 1e-9 * 50.0 / 20
+    # the trajectory file: one record per hour (iprint = 3600 s), the last
+    # one holding the final state; lon/lat are getInterp2D of x_rho/y_rho
+    f = netcdf_file(str(tmp_path / "traj.nc"), "r", mmap=False)
+    try:
+        v = f.variables
+        assert np.array_equal(v["model_time"][:], [3600.0, 7200.0, 10800.0])
+        assert np.array_equal(v["dob"][:], p.dob.astype(np.float32))
+        assert np.array_equal(v["zp"][2], got[:, 2].astype(np.float32))
+        assert np.array_equal(v["status"][2], got[:, 3].astype(np.float32))
+        g = cfg.grids[0]
+        assert np.array_equal(v["lon"][2], interp2d_rho(g.x_rho, got[:, 0], got[:, 1]).astype(np.float32))
+        assert np.array_equal(v["lat"][2], interp2d_rho(g.y_rho, got[:, 0], got[:, 1]).astype(np.float32))
+        assert "GID" not in v
+    finally:
+        f.close()
 
 
 def test_fortran_driver_is_built():
