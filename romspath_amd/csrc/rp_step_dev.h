@@ -406,6 +406,20 @@ struct Scr32 {
     };
     __device__ __forceinline__ Ref operator()(int row) const { return Ref{*this, voff, (uint32_t)row * rowb}; }
     __device__ __forceinline__ Ref at(int row) const { return Ref{*this, voff + (uint32_t)row * rowb, 0u}; }
+    // rows written once and read once later: non-temporal (cache policy NT),
+    // so they do not displace the qty row the secant sweeps read eight times
+    __device__ __forceinline__ void st_nt(int row, float v) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, (uint32_t)row * rowb, 2);
+    }
+    __device__ __forceinline__ void st_nt_at(int row, float v) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff + (uint32_t)row * rowb, 0u, 2);
+    }
+    __device__ __forceinline__ float ld_nt(int row) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, (uint32_t)row * rowb, 2));
+    }
 };
 
 struct VtLayout {
@@ -845,9 +859,9 @@ struct FSweep {
 #pragma clang fp contract(fast)
         const double rd = rcp64(d);
         const double rat = e * rd, rat2 = B.c * rd;
-        S(Lo.rd + m) = (float)rd;
-        S(Lo.w + m) = (float)w;
-        S(Lo.l1 + m) = (float)rat;
+        S.st_nt(Lo.rd + m, (float)rd);
+        S.st_nt(Lo.w + m, (float)w);
+        S.st_nt(Lo.l1 + m, (float)rat);
         const double d1 = dn - rat * e;
         const double e1 = en - rat * B.c;
         const double d2 = B.a - rat2 * B.c;
@@ -867,7 +881,7 @@ struct BSweep {
         const double r = (double)rv;
         double um = (double)wv * r;
         if (m < n - 1) um = um - up1 * (double)a1 - up2 * (c * r);
-        S(Lo.u + m + 1) = (float)um;
+        S.st_nt(Lo.u + m + 1, (float)um);
         up2 = up1;
         up1 = um;
     }
@@ -895,16 +909,16 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
         float wv[kCh], rv[kCh], a1[kCh];
 #pragma unroll
         for (int k = 0; k < kCh; k++) {
-            wv[k] = S(Lo.w + m1 - k);
-            rv[k] = S(Lo.rd + m1 - k);
-            a1[k] = S(Lo.l1 + m1 - k);
+            wv[k] = S.ld_nt(Lo.w + m1 - k);
+            rv[k] = S.ld_nt(Lo.rd + m1 - k);
+            a1[k] = S.ld_nt(Lo.l1 + m1 - k);
         }
 #pragma unroll
         for (int k = 0; k < kCh; k++) bw.row(Lo, S, m1 - k, wv[k], rv[k], a1[k]);
     }
-    for (; m1 >= 0; m1--) bw.row(Lo, S, m1, S(Lo.w + m1), S(Lo.rd + m1), S(Lo.l1 + m1));
-    S(Lo.u + 0) = 0.0f;
-    S(Lo.u + Lo.np - 1) = 0.0f;
+    for (; m1 >= 0; m1--) bw.row(Lo, S, m1, S.ld_nt(Lo.w + m1), S.ld_nt(Lo.rd + m1), S.ld_nt(Lo.l1 + m1));
+    S.st_nt(Lo.u + 0, 0.0f);
+    S.st_nt(Lo.u + Lo.np - 1, 0.0f);
 }
 
 __device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr32& S, int i, double p,
@@ -1041,7 +1055,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     {
         double zlo = z1, klo = k1;
         double ym1 = 0.0, y0 = k1;
-        S(Lo.y + 0) = (float)k1;
+        S.st_nt(Lo.y + 0, (float)k1);
         int i = 2;                                   // next point to emit
         for (int kb = 1; kb <= nw; kb += kLv) {
             double zk[kLv], kk[kLv];
@@ -1062,7 +1076,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
                             if (k < nw && !(fz < zhi)) break;
                             yi = m * fz + b;
                         }
-                        S.at(Lo.y + i - 1) = (float)yi;
+                        S.st_nt_at(Lo.y + i - 1, (float)yi);
                         // qty row i-3 needs y(i-2): defined for i >= 3 (rows 0..np-3)
                         if (i >= 3) S.at(Lo.q + i - 3) = (float)(((yi - y0) - (y0 - ym1)) * rh);
                         ym1 = y0;
