@@ -866,6 +866,10 @@ int rp_create(void** out, const rp_config* cfg)
         delete c;
         return RP_E_HIP;
     }
+    if (upload_p0_tables() != 0) {
+        rp_destroy(c);
+        return RP_E_HIP;
+    }
     if (dalloc(c, &c->active, 1) || dalloc(c, &c->capped, 1) || dalloc(c, &c->dgrids, kMaxGrid) ||
         dalloc(c, &c->work, 1)) {
         rp_destroy(c);

@@ -79,6 +79,9 @@ struct LonLatArgs {
 };
 __global__ void sample_lonlat(LonLatArgs A);
 
+// tables of the FP32-field form's p = 0 spline sweep; once per device, 0 on success
+int upload_p0_tables();
+
 // current fields of a step: dst[e] = sum_t pw[t] src[4 e' + slot[t]] in FP64,
 // stored FP32; one job per (grid, field), blockIdx.y selects the job.  A job
 // copies columns of kin levels into columns of kout >= kin levels: output e

@@ -1,4 +1,6 @@
 // rp_step.hip -- the particle-step kernels and the field-upload kernels.
+#include <vector>
+
 #include "rp_step_dev.h"
 
 namespace rp {
@@ -138,6 +140,36 @@ __global__ void sample_deviates(uint64_t seed, int64_t step, int64_t n, int ndev
     dv.step = (uint64_t)step;
     dv.blk = -1;
     for (int j = 0; j < ndev; j++) out[q * ndev + j] = dv.get(j);
+}
+
+// Tables of the p = 0 spline sweep (stream_G_p0): the forward LDL' sweep of
+// the Toeplitz band (6, -4, 1) with derivatives along (4, 1, 0), per row
+// 1/d, (1/d)^2 dd, l1, l2, dl1, dl2.  Uploaded once per device.
+int upload_p0_tables()
+{
+    static bool done[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1;
+    if (done[dev]) return 0;
+    std::vector<double> T(6 * (size_t)kP0Rows);
+    const double a = 6.0, b = -4.0, c = 1.0, da = 4.0, db = 1.0, dc = 0.0;
+    double d = a, dd = da, e = b, de = db, dn = a, ddn = da, en = b, den = db;
+    for (int m = 0; m < kP0Rows; m++) {
+        const double rd = 1.0 / d;
+        const double rat = e * rd, drat = (de - rat * dd) * rd;
+        const double rat2 = c * rd, drat2 = (dc - rat2 * dd) * rd;
+        double* t = &T[6 * (size_t)m];
+        t[0] = rd; t[1] = rd * rd * dd; t[2] = rat; t[3] = rat2; t[4] = drat; t[5] = drat2;
+        const double d1 = dn - rat * e, dd1 = ddn - drat * e - rat * de;
+        const double e1 = en - rat * c, de1 = den - drat * c - rat * dc;
+        const double d2 = a - rat2 * c, dd2 = da - drat2 * c - rat2 * dc;
+        d = d1; dd = dd1; e = e1; de = de1;
+        dn = d2; ddn = dd2; en = b; den = db;
+    }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kP0Tab), T.data(), T.size() * sizeof(double)) != hipSuccess)
+        return -1;
+    done[dev] = true;
+    return 0;
 }
 
 __global__ void __launch_bounds__(256) sample_lonlat(LonLatArgs A)

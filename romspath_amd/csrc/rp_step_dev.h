@@ -848,6 +848,53 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, i
     Gp = sw.gp;
 }
 
+// The p = 0 sweep from universal tables.  At p = 0 the band is
+// s (6, -4, 1) with s = 6 r^2, and its p-derivative is -s (6, -4, 1) +
+// h (4, 1, 0).  The LDL' pivots of a banded Toeplitz matrix are homogeneous
+// (degree 1 for d, 0 for l1 and l2) and row m of the forward elimination
+// depends only on the leading m x m block, so one table for (6, -4, 1) and
+// its derivative along (4, 1, 0) serves every s, h and n:
+//   rd = T1 / s, l1 = rat, l2 = rat2, d(l1)/dp = k drat, d(l2)/dp = k drat2,
+//   dd/dp = -s d^ + h dd^, with k = h / s.
+// With omega = (dw/dp) / k the sweep accumulates A = sum w^2 T1,
+// B = sum w omega T1, C = sum w^2 T2 (T2 = rd^2 dd^), and
+//   G = A / s,  dG/dp = (A + k (2 B - C)) / s.
+constexpr int kP0Rows = 1024;
+static __constant__ double kP0Tab[kP0Rows * 6];   // T1, T2, rat, rat2, drat, drat2
+
+__device__ __forceinline__ void stream_G_p0(double h, double serr, int n, const Scr32& S, int oq,
+                                            double& G, double& Gp)
+{
+#pragma clang fp contract(fast)
+    const double r = serr / h;
+    const double s = 6.0 * r * r;
+    const double kap = h / s;
+    double w = (double)S(oq), om = 0.0, wp = 0.0, omp = 0.0;
+    double A = 0.0, B = 0.0, C = 0.0, r2p = 0.0, dr2p = 0.0;
+    auto row = [&](int m, double qn) {
+        const double* T = kP0Tab + 6 * m;
+        const double ww = w * w;
+        A += ww * T[0];
+        B += (w * om) * T[0];
+        C += ww * T[1];
+        const double wn = qn - T[2] * w - r2p * wp;
+        const double omn = -(T[4] * w + dr2p * wp) - T[2] * om - r2p * omp;
+        wp = w; w = wn; omp = om; om = omn;
+        r2p = T[3]; dr2p = T[5];
+    };
+    int m0 = 0;
+    for (; m0 + kCh <= n; m0 += kCh) {
+        float qn[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; k++) qn[k] = S(oq + m0 + k + 1);
+#pragma unroll
+        for (int k = 0; k < kCh; k++) row(m0 + k, (double)qn[k]);
+    }
+    for (; m0 < n; m0++) row(m0, (double)S(oq + m0 + 1));
+    G = A / s;
+    Gp = (A + kap * (2.0 * B - C)) / s;
+}
+
 // full solve at the accepted p: u(2..np-1) into Lo.u (u(1)=u(np)=0).  The
 // forward sweep keeps 1/d, l1 and w per row; l2 = c/d is formed again on
 // the way back.
@@ -1097,7 +1144,8 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
         p = 1.0;
     } else {
         double G, Gp;
-        stream_G(band_of(0.0, h, serr), n, S, Lo.q, G, Gp);
+        if (n <= kP0Rows) stream_G_p0(h, serr, n, S, Lo.q, G, Gp);
+        else stream_G(band_of(0.0, h, serr), n, S, Lo.q, G, Gp);
         sfq = 6.0 * G;
         if (s < sfq) {
             const double utru = 0.5 * (G - Gp);
