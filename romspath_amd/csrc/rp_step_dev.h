@@ -761,6 +761,14 @@ __device__ __forceinline__ R vturb(const DGrid& g, const StepArgs& A, const RhoC
 // ---------------------------------------------------------------------------
 constexpr int kCh = 8;
 
+// Secant sweeps (0-based) from which the factor rows are kept.  With the
+// spline parameters of ROMSPath.data (serr 1e-4, smth 0.6, sub 4) the search
+// on resampled AKs profiles took exactly 7 sweeps in every case sampled, so
+// storing from the 7th makes the final solve a back substitution.  A search
+// that ends earlier falls back to the full solve and one that ends later
+// stores again; the result is the same either way.
+constexpr int kStoreFrom = 6;
+
 struct FastLayout {
     int np;
     int q, y, rd, l1, w, u;
@@ -803,10 +811,19 @@ __device__ __forceinline__ Band band_of(double p, double h, double serr)
 struct GSweep {
     double d, dd, e, de, dn, ddn, en, den;
     double w, dw, wp, dwp, l2p, dl2p, g, gp;
-    __device__ __forceinline__ void row(const Band& B, double qn)
+    // STORE: also keep 1/d, w and l1 of the row, exactly as the forward half
+    // of the full solve at this p would (same operations, same order)
+    template <bool STORE = false>
+    __device__ __forceinline__ void row(const Band& B, double qn, const Scr32* S = nullptr,
+                                        const FastLayout* Lo = nullptr, int m = 0)
     {
 #pragma clang fp contract(fast)
         const double rd = rcp64(d);
+        if constexpr (STORE) {
+            S->st_nt(Lo->rd + m, (float)rd);
+            S->st_nt(Lo->w + m, (float)w);
+            S->st_nt(Lo->l1 + m, (float)(e * rd));
+        }
         const double wrd = w * rd;
         g += w * wrd;
         // d(w^2/d)/dp = (2 w dw - w^2 dd / d) / d = wrd (2 dw - wrd dd)
@@ -827,9 +844,11 @@ struct GSweep {
 
 // G(p) and dG/dp by one LDL' sweep over the n interior unknowns.  qty row n
 // holds 0, so every chunk of rows reads the next row's qty without a guard.
-__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, int oq, double& G,
-                                         double& Gp)
+template <bool STORE = false>
+__device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, const FastLayout& Lo,
+                                         double& G, double& Gp)
 {
+    const int oq = Lo.q;
     GSweep sw;
     sw.d = B.a; sw.dd = B.da; sw.e = B.b; sw.de = B.db;
     sw.dn = B.a; sw.ddn = B.da; sw.en = B.b; sw.den = B.db;
@@ -841,9 +860,9 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, i
 #pragma unroll
         for (int k = 0; k < kCh; k++) qn[k] = S(oq + m0 + k + 1);
 #pragma unroll
-        for (int k = 0; k < kCh; k++) sw.row(B, (double)qn[k]);
+        for (int k = 0; k < kCh; k++) sw.template row<STORE>(B, (double)qn[k], &S, &Lo, m0 + k);
     }
-    for (; m0 < n; m0++) sw.row(B, (double)S(oq + m0 + 1));
+    for (; m0 < n; m0++) sw.template row<STORE>(B, (double)S(oq + m0 + 1), &S, &Lo, m0);
     G = sw.g;
     Gp = sw.gp;
 }
@@ -934,6 +953,8 @@ struct BSweep {
     }
 };
 
+__device__ __forceinline__ void back_u(const Band& B, const FastLayout& Lo, const Scr32& S);
+
 __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr32& S)
 {
     const int n = Lo.np - 2;
@@ -949,6 +970,13 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
         for (int k = 0; k < kCh; k++) fw.row(B, Lo, S, m0 + k, (double)qn[k]);
     }
     for (; m0 < n; m0++) fw.row(B, Lo, S, m0, (double)S(Lo.q + m0 + 1));
+    back_u(B, Lo, S);
+}
+
+// back substitution from the stored 1/d, w, l1 rows: u(2..np-1) into Lo.u
+__device__ __forceinline__ void back_u(const Band& B, const FastLayout& Lo, const Scr32& S)
+{
+    const int n = Lo.np - 2;
     BSweep bw;
     bw.up1 = 0.0; bw.up2 = 0.0; bw.c = B.c; bw.n = n;
     int m1 = n - 1;
@@ -1140,12 +1168,13 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     const int n = np - 2;
     S(Lo.q + n) = 0.0f;   // read as the next row's qty by the last row of a sweep
     double p = 0.0, q = 0.0, sfq;
+    bool stored = false;
     if (s <= 0.0) {
         p = 1.0;
     } else {
         double G, Gp;
         if (n <= kP0Rows) stream_G_p0(h, serr, n, S, Lo.q, G, Gp);
-        else stream_G(band_of(0.0, h, serr), n, S, Lo.q, G, Gp);
+        else stream_G(band_of(0.0, h, serr), n, S, Lo, G, Gp);
         sfq = 6.0 * G;
         if (s < sfq) {
             const double utru = 0.5 * (G - Gp);
@@ -1156,7 +1185,12 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
             int it = 0;
             for (;;) {
                 const double pq = q / (1.0 + q);
-                stream_G(band_of(pq, h, serr), n, S, Lo.q, G, Gp);
+                // from the sweep the search usually ends on, keep the sweep's
+                // factor rows, so that the accepted p needs only the back
+                // substitution
+                stored = it >= kStoreFrom;
+                if (stored) stream_G<true>(band_of(pq, h, serr), n, S, Lo, G, Gp);
+                else stream_G(band_of(pq, h, serr), n, S, Lo, G, Gp);
                 sfq = 6.0 * (G + pq * Gp) / ((1.0 + q) * (1.0 + q));
                 if (fabs(sfq - s) <= 0.01 * s) break;
                 if (++it >= kSecantCap) {
@@ -1172,7 +1206,8 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
             p = q / (1.0 + q);
         }
     }
-    solve_u(band_of(p, h, serr), Lo, S);
+    if (stored) back_u(band_of(p, h, serr), Lo, S);
+    else solve_u(band_of(p, h, serr), Lo, S);
     const double k2 = 6.0 * (1.0 - p) * (serr * serr);
     PieceCache pc;
     pc.i = -1;
