@@ -214,22 +214,39 @@ __device__ __forceinline__ int first_above(const Lev& lev, int nz, R Z)
     return lo;
 }
 
+// The same index found by walking from a guess (the index of the previous RK
+// stage, or of the rho bracket for the w bracket): down while the level below
+// is above Z, then up while level k is not.  With a guess off by one or two
+// this takes two or three level evaluations instead of the bisection's five.
+template <class Lev, class R>
+__device__ __forceinline__ int first_above_near(const Lev& lev, int nz, R Z, int guess)
+{
+    int k = min(max(guess, 1), nz + 1);
+    while (k > 1 && lev(k - 1) > Z) k--;
+    while (k <= nz && !(lev(k) > Z)) k++;
+    return k;
+}
+
 // Vertical bracket of getInterp3d (interpolation_module.f90:528-589).
+// k: the first level above Z (the scan's result), kept as the next guess.
 template <class R = double>
 struct VBracketT {
     int ibot, itop;
     R c0, c1;
+    int k;
 };
 using VBracket = VBracketT<double>;
 
 template <bool FAST, class R>
-__device__ __forceinline__ VBracketT<R> bracket_rho(const DGrid& g, R Z, R zeta, R depth)
+__device__ __forceinline__ VBracketT<R> bracket_rho(const DGrid& g, R Z, R zeta, R depth,
+                                                    int guess = 0)
 {
     const R h = -1.0 * depth;
     const int nz = g.N;
     const Levels<FAST, R> lev(g, g.s_rho, g.cs_r, zeta, h);
-    const int i = first_above(lev, nz, Z);
+    const int i = guess > 0 ? first_above_near(lev, nz, Z, guess) : first_above(lev, nz, Z);
     VBracketT<R> b;
+    b.k = i;
     if (i > nz) {
         b.ibot = b.itop = nz; b.c0 = 0.5; b.c1 = 0.5;
     } else if (i == 1) {
@@ -244,13 +261,15 @@ __device__ __forceinline__ VBracketT<R> bracket_rho(const DGrid& g, R Z, R zeta,
 }
 
 template <bool FAST, class R>
-__device__ __forceinline__ VBracketT<R> bracket_w(const DGrid& g, R Z, R zeta, R depth)
+__device__ __forceinline__ VBracketT<R> bracket_w(const DGrid& g, R Z, R zeta, R depth,
+                                                  int guess = 0)
 {
     const R h = -1.0 * depth;
     const int nz = g.N + 1;
     const Levels<FAST, R> lev(g, g.s_w, g.cs_w, zeta, h);
-    const int i = first_above(lev, nz, Z);
+    const int i = guess > 0 ? first_above_near(lev, nz, Z, guess) : first_above(lev, nz, Z);
     VBracketT<R> b;
+    b.k = i;
     if (i > nz) {
         // z at or above the top w level: two copies of level nz (the
         // reference's weight is undefined there; the value is not).

@@ -318,17 +318,21 @@ __device__ __forceinline__ SurfaceT<R> surface_at(const DGrid& g, const StepArgs
 
 // NSLOT=2 on the first external step, whose u, v, w use slots (b,b,c)
 // (advection_module.f90:277-334); 3 otherwise.
+// hint (may be null): the rho bracket's level index of the previous stage,
+// in and out; 0 = none.  The w bracket starts from this stage's rho index.
 template <typename F, int NSLOT, class R>
 __device__ __forceinline__ void find_currents(const DGrid& g, const StepArgs& A, R X, R Y, R& Z,
-                                              R& U, R& V, R& W, SurfaceT<R>* surf_out)
+                                              R& U, R& V, R& W, SurfaceT<R>* surf_out,
+                                              int* hint = nullptr)
 {
     const RhoCellT<R> c = rho_cell(g, X, Y);
     const SurfaceT<R> s = surface_at<F>(g, A, c);
     if (surf_out) *surf_out = s;
     if (Z < s.tdepth) Z = s.tdepth + (double)0.001f;
     if (Z > s.zeta) Z = s.zeta - (double)0.001f;
-    const VBracketT<R> br = bracket_rho<is_fast<F>()>(g, Z, s.zeta, s.tdepth);
-    const VBracketT<R> bw = bracket_w<is_fast<F>()>(g, Z, s.zeta, s.tdepth);
+    const VBracketT<R> br = bracket_rho<is_fast<F>()>(g, Z, s.zeta, s.tdepth, hint ? *hint : 0);
+    const VBracketT<R> bw = bracket_w<is_fast<F>()>(g, Z, s.zeta, s.tdepth, hint ? br.k : 0);
+    if (hint) *hint = br.k;
     R q[3];
     interp_u<F, NSLOT>(g, X, Y, br, A.slots, q);
     U = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
@@ -1357,26 +1361,27 @@ __device__ __forceinline__ void advect(const DGrid& g, const StepArgs& A, DevSrc
         // every stage starts from the step's start point with half a step (idt/2)
         const double idt = (double)A.idt;
         R Uad, Vad, Wad;
+        int hint = 0;     // rho level index of the previous stage (the stages are close)
         if (rec) { rec->x[0] = X0; rec->y[0] = Y0; rec->z[0] = Zpar; }
-        find_currents<F, NSLOT>(g, A, X0, Y0, Zpar, Uad, Vad, Wad, (SurfaceT<R>*)nullptr);
+        find_currents<F, NSLOT>(g, A, X0, Y0, Zpar, Uad, Vad, Wad, (SurfaceT<R>*)nullptr, &hint);
         const R k1u = Uad * pm, k1v = Vad * pn, k1w = Wad;
         R x1 = X0 + Uad * pm * idt / 2.0;
         R y1 = Y0 + Vad * pn * idt / 2.0;
         R z1 = Zpar + Wad * idt / 2.0;
         if (rec) { rec->x[1] = x1; rec->y[1] = y1; rec->z[1] = z1; }
-        find_currents<F, NSLOT>(g, A, x1, y1, z1, Uad, Vad, Wad, (SurfaceT<R>*)nullptr);
+        find_currents<F, NSLOT>(g, A, x1, y1, z1, Uad, Vad, Wad, (SurfaceT<R>*)nullptr, &hint);
         const R k2u = Uad * pm, k2v = Vad * pn, k2w = Wad;
         R x2 = X0 + Uad * pm * idt / 2.0;
         R y2 = Y0 + Vad * pn * idt / 2.0;
         R z2 = Zpar + Wad * idt / 2.0;
         if (rec) { rec->x[2] = x2; rec->y[2] = y2; rec->z[2] = z2; }
-        find_currents<F, NSLOT>(g, A, x2, y2, z2, Uad, Vad, Wad, (SurfaceT<R>*)nullptr);
+        find_currents<F, NSLOT>(g, A, x2, y2, z2, Uad, Vad, Wad, (SurfaceT<R>*)nullptr, &hint);
         const R k3u = Uad * pm, k3v = Vad * pn, k3w = Wad;
         R x3 = X0 + Uad * pm * idt / 2.0;
         R y3 = Y0 + Vad * pn * idt / 2.0;
         R z3 = Zpar + Wad * idt / 2.0;
         if (rec) { rec->x[3] = x3; rec->y[3] = y3; rec->z[3] = z3; }
-        find_currents<F, NSLOT>(g, A, x3, y3, z3, Uad, Vad, Wad, (SurfaceT<R>*)nullptr);
+        find_currents<F, NSLOT>(g, A, x3, y3, z3, Uad, Vad, Wad, (SurfaceT<R>*)nullptr, &hint);
         const R k4u = Uad * pm, k4v = Vad * pn, k4w = Wad;
         const R PU = (k1u + 2.0 * k2u + 2.0 * k3u + k4u) / 6.0;
         const R PV = (k1v + 2.0 * k2v + 2.0 * k3v + k4v) / 6.0;
