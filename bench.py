@@ -300,15 +300,28 @@ def adjoint_cost(n, steps, ckpt, fd_check=True):
     return res
 
 
-def cpu_baseline(cfg, n_sample, steps):
+def cpu_baseline(cfg, n_sample, steps, threads=1, records=None):
+    """The oracle on the first n_sample particles for `steps` steps.  Timed:
+    the particle-step calls only (hydro records, like the GPU run's, are
+    generated outside the timed region; injected deviates are drawn outside)."""
     from oracle import oracle as O
     sub = synth.Particles(cfg.particles.x[:n_sample].copy(), cfg.particles.y[:n_sample].copy(),
                           cfg.particles.z[:n_sample].copy(), cfg.particles.dob[:n_sample].copy())
     scfg = synth.Config(cfg.name, cfg.grids, cfg.hydro, cfg.params, sub, cfg.refine)
-    t0 = time.perf_counter()
-    st = O.run_oracle(scfg, max_steps=steps, inject=True)
-    dt = time.perf_counter() - t0
+    O.set_threads(threads)
+    st = O.run_oracle(scfg, max_steps=steps, inject=True, records=records)
+    O.set_threads(1)
+    dt = st.step_seconds
     return st.active / dt, st.active, dt
+
+
+def host_cores():
+    """CPU threads this process may use (the affinity set, capped by OMP_NUM_THREADS)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
 
 
 def main():
@@ -325,6 +338,8 @@ def main():
     ap.add_argument("--no-fp64-form", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=12000)
+    ap.add_argument("--cpu-mt-particles", type=int, default=160000,
+                    help="sample of the all-cores oracle run")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-batched", action="store_true", help="skip the C4 ensemble comparison")
     ap.add_argument("--c4-runup", type=int, default=720, help="steps before the C4 window (24 h)")
@@ -445,10 +460,19 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log(f"CPU baseline: oracle, {args.cpu_particles} particles x {args.cpu_steps} steps, 1 thread")
-        rate, act, secs = cpu_baseline(cfg, args.cpu_particles, args.cpu_steps)
-        cpu = {"value": rate, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"first {args.cpu_particles} particles of the same workload, "
-                         f"{args.cpu_steps} internal steps, FP64 C restatement, {secs:.1f} s"}
+        rate1, act1, secs1 = cpu_baseline(cfg, args.cpu_particles, args.cpu_steps, 1, records)
+        nt = host_cores()
+        log(f"CPU baseline: oracle, {args.cpu_mt_particles} particles x {args.cpu_steps} steps, "
+            f"{nt} threads")
+        rate, act, secs = cpu_baseline(cfg, args.cpu_mt_particles, args.cpu_steps, nt, records)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": nt, "kind": "port",
+               "sample": f"first {args.cpu_mt_particles} particles of the same workload, "
+                         f"{args.cpu_steps} internal steps, FP64 C restatement, OpenMP over "
+                         f"particles on {nt} threads, {secs:.1f} s",
+               "single_thread": {"value": rate1, "cores": 1,
+                                 "sample": f"first {args.cpu_particles} particles, "
+                                           f"{args.cpu_steps} steps, {secs1:.1f} s"}}
+        log(f"  {rate1:.4g} particle-steps/s on 1 thread, {rate:.4g} on {nt}")
 
     line = {
         "metric": "particle-steps/s",

@@ -90,6 +90,8 @@ def lib():
         L.og_vturb.restype = C.c_double
         L.og_vturb.argtypes = [C.POINTER(OgGrid), C.POINTER(OgParams), C.POINTER(OgTime),
                                C.c_double, C.c_double, C.c_double, PD]
+        L.og_set_threads.argtypes = [C.c_int]
+        L.og_set_threads.restype = None
         L.og_update_particles.restype = C.c_int64
         L.og_update_particles.argtypes = [C.POINTER(OgGrid), C.POINTER(OgParams), C.POINTER(OgTime),
                                           C.POINTER(OgParticles), C.c_int64, PD, C.c_int]
@@ -241,9 +243,12 @@ def ref_pppack():
 
 # ---- whole runs on the oracle (same schedule as the product driver) ----
 
-def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None):
+def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None, records=None):
     """Run cfg on the oracle; deviates are the same seeded per-step blocks the
-    GPU tests inject.  Returns the OracleState."""
+    GPU tests inject.  Returns the OracleState; its step_seconds is the time
+    spent in the particle-step calls alone (records, deviates and slot copies
+    excluded).  records: a RecordSource to share (None = a new one)."""
+    import time
     import sys
     root = str(HERE.parent)
     if root not in sys.path:
@@ -261,9 +266,10 @@ def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None):
         grids = [OracleGrid(cfg.grids[0])]
     P = make_params(prm, ng)
     state = OracleState(particles if particles is not None else cfg.particles)
-    recs = RecordSource(cfg)
+    recs = records if records is not None else RecordSource(cfg)
     ndev = ndev_of(prm)
     active = 0
+    step_seconds = 0.0
     for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps):
         for ld in st.loads:
             for i, g in enumerate(grids):
@@ -273,10 +279,13 @@ def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None):
             dev = injected_deviates(prm.seed, st.step_index, state.n, ndev)
         else:
             dev = np.zeros((state.n, ndev))
+        t0 = time.perf_counter()
         active += update_particles(grids, P, T, state, dev, ndev)
+        step_seconds += time.perf_counter() - t0
         if on_step is not None:
             on_step(st, state)
     state.active = active
+    state.step_seconds = step_seconds
     return state
 
 
@@ -302,3 +311,8 @@ def interp2d_rho(field, X, Y):
     c3 = x * y
     c4 = (1 - x) * y
     return c1 * field[j0, i0] + c2 * field[j0, i1] + c3 * field[j1, i1] + c4 * field[j1, i0]
+
+
+def set_threads(n):
+    """Threads of the oracle's particle loop (the result does not depend on it)."""
+    lib().og_set_threads(int(n))

@@ -19,6 +19,9 @@
  *   - the w-level scan guards its lower index (interpolation_module.f90:572
  *     would read s_w(0) if z sat exactly on the bed).
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "rp_oracle.h"
 
 #include <math.h>
@@ -760,11 +763,26 @@ double og_vturb(const og_grid *g, const og_params *P, const og_time *T, double X
     return ParZc - Zpar;
 }
 
+/* Threads of the particle loop (OpenMP builds only; the reference is
+ * serial, and each particle's step reads and writes only its own columns, so
+ * the result does not depend on the count). */
+void og_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
 /* ROMSPath.f90:662-1003, behaviour off (Behavior=0: behave() returns zeros) */
 int64_t og_update_particles(const og_grid *grids, const og_params *P, const og_time *T,
                             og_particles *p, int64_t n, const double *dev, int ndev)
 {
     int64_t active = 0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : active) schedule(dynamic, 64)
+#endif
     for (int64_t q = 0; q < n; q++) {
         if (T->ix[2] <= p->dob[q]) continue;
         if (p->status[q] == 9.0) continue;

@@ -92,6 +92,7 @@ double og_vturb(const og_grid *g, const og_params *P, const og_time *T, double X
  * dev: injected normal deviates, [n][ndev] with ndev = 2 + idt/int(deltat);
  * columns 0,1 feed HTurb (x then y), columns 2.. feed VTurb's sub-steps.
  * Returns the number of particles that took the step (the active count). */
+void og_set_threads(int n);
 int64_t og_update_particles(const og_grid *grids, const og_params *P, const og_time *T,
                             og_particles *p, int64_t n, const double *dev, int ndev);
 
