@@ -139,3 +139,113 @@ def test_perturbation_gradient_matches_finite_differences():
         print(f"coef[{comp},{j},{i}]: adjoint {gp[comp, j, i]:.10e} fd {fd:.10e} rel {rel:.2e}",
               flush=True)
         assert rel <= 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Tangent-linear checks.  The Jacobian of one step at the current state is
+# the step recorded alone and its adjoint with unit lambdas (row r of J is
+# J^T e_r); that is the tangent-linear model of the step, evaluated in
+# Dual<3> numbers.
+# ---------------------------------------------------------------------------
+from oracle import oracle as O                      # noqa: E402
+from romspath_amd.runner import injected_deviates, ndev_of  # noqa: E402
+from romspath_amd.schedule import schedule          # noqa: E402
+
+
+def _step(run, st):
+    prm = run.cfg.params
+    n = run.cfg.particles.n
+    run.load(st)
+    run.ctx.set_deviates(injected_deviates(prm.seed, st.step_index, n, ndev_of(prm)))
+    run.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index)
+
+
+def _step_jacobian(run, st):
+    n = run.cfg.particles.n
+    run.ctx.adjoint_record(1)
+    _step(run, st)
+    J = np.empty((n, 3, 3))
+    for r in range(3):
+        lam = [np.zeros(n), np.zeros(n), np.zeros(n)]
+        lam[r] = np.ones(n)
+        (gx, gy, gz), _ = run.ctx.adjoint(lam)
+        J[:, r, 0], J[:, r, 1], J[:, r, 2] = gx, gy, gz
+    run.ctx.adjoint_record(0)
+    return J
+
+
+def test_step_jacobian_matches_reference_step_differences():
+    # the tangent-linear of one step against central differences of the
+    # oracle's step (the FP64 restatement of the reference) at the same state
+    K = 20
+    cfg = _case(n=256, seed=12)
+    n = cfg.particles.n
+    prm = cfg.params
+    sched = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, K + 1))
+    print(f"case tangent-linear of step {K} vs oracle differences, {n} particles", flush=True)
+    run = GpuRun(cfg, precision="fp64")
+    for st in sched[:K]:
+        _step(run, st)
+    J = _step_jacobian(run, sched[K])
+    out = run.ctx.get_particles()
+    run.close()
+    base = O.run_oracle(cfg, max_steps=K + 1)
+    assert np.array_equal(out["x"], base.x) and np.array_equal(out["z"], base.z)
+
+    def shifted(dim, eps):
+        def on_step(st, state):
+            if st.step_index == K - 1:
+                (state.x, state.y, state.z)[dim][:] += eps
+        return O.run_oracle(cfg, max_steps=K + 1, on_step=on_step)
+
+    fd = {}
+    ok = np.ones(n, dtype=bool)
+    for eps in (1e-4, 1e-5):
+        F = np.empty((n, 3, 3))
+        for dim in range(3):
+            op, om = shifted(dim, eps), shifted(dim, -eps)
+            ok &= (op.status == base.status) & (om.status == base.status)
+            F[:, 0, dim] = (op.x - om.x) / (2 * eps)
+            F[:, 1, dim] = (op.y - om.y) / (2 * eps)
+            F[:, 2, dim] = (op.z - om.z) / (2 * eps)
+        fd[eps] = F
+    f1, f2 = fd[1e-4].reshape(n, 9), fd[1e-5].reshape(n, 9)
+    scale = np.maximum(np.abs(f1).max(axis=1, keepdims=True), 1e-12)
+    trusted = ok & (np.abs(f1 - f2) <= 1e-5 * scale).all(axis=1)
+    d = (J.reshape(n, 9) - f2)[trusted]
+    nrm = np.linalg.norm(d) / np.linalg.norm(f2[trusted])
+    print(f"step Jacobian: {trusted.sum()}/{ok.sum()} particles trusted, norm-wise rel {nrm:.2e}",
+          flush=True)
+    assert trusted.sum() >= 0.9 * ok.sum()
+    assert nrm <= 1e-5
+
+
+def test_adjoint_is_the_transpose_of_the_tangent_linear():
+    # lambda^T (J_N ... J_1) from the per-step tangent-linear models equals the
+    # checkpointed reverse sweep of the same run (segments of 8, recomputed)
+    N = 24
+    cfg = _case(n=256, seed=13)
+    n = cfg.particles.n
+    prm = cfg.params
+    sched = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, N))
+    print(f"case adjoint vs tangent-linear product over {N} steps", flush=True)
+    run = GpuRun(cfg, precision="fp64")
+    P = np.tile(np.eye(3), (n, 1, 1))
+    for st in sched:
+        P = _step_jacobian(run, st) @ P
+    a = run.ctx.get_particles()
+    run.close()
+    run = GpuRun(cfg, precision="fp64")
+    run.ctx.adjoint_record(8)
+    for st in sched:
+        _step(run, st)
+    b = run.ctx.get_particles()
+    lam = np.random.default_rng(3).standard_normal((n, 3))
+    (gx, gy, gz), _ = run.ctx.adjoint((lam[:, 0], lam[:, 1], lam[:, 2]))
+    run.close()
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["z"], b["z"])
+    g = np.stack([gx, gy, gz], axis=1)
+    want = np.einsum("ni,nij->nj", lam, P)
+    err = np.abs(g - want).max() / np.abs(want).max()
+    print(f"adjoint vs tangent-linear: max rel {err:.2e}", flush=True)
+    assert err <= 1e-10
