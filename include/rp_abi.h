@@ -129,6 +129,23 @@ int rp_get_particles(void *ctx, int64_t n, double *x, double *y, double *z, doub
 int rp_set_lonlat(void *ctx, int32_t ng, const double *lon, const double *lat);
 int rp_sample_lonlat(void *ctx, int64_t n, double *lon, double *lat);
 
+/* ini_ROMSPath's placement of the released particles (ROMSPath.f90:322-403):
+ * for grid ng = 1..Ngrid in order, LL2ij (interpolation_module.f90:110-225:
+ * bisection of the index box with inbox/inside, then the Cartesian linear
+ * map or the spherical local-triangle map) and bounds; the first grid that
+ * holds a particle takes it (X, Y, GID = ng, status 0, isIn), and zbounds
+ * (boundary_module.f90:145-179, zeta of slot t_b) may lift z to zeta - 0.01
+ * or set status 9 again.  Others keep X = Y = 1, GID = Ngrid, status 9,
+ * isIn 0.  lon, lat, z are the release file's columns; z is updated in
+ * place.  Needs rp_set_lonlat for every grid and slot t_b loaded.  The
+ * results go to rp_set_particles; synchronises the stream.
+ * rp_set_grid_angle: GRIDS(ng)%angle (may be NULL: 0) and spherical
+ * (GRIDS(1)%spherical decides for all grids, as in LL2ij). */
+int rp_set_grid_angle(void *ctx, int32_t ng, int32_t spherical, const double *angle);
+int rp_locate_particles(void *ctx, int64_t n, const double *lon, const double *lat, double *z,
+                        int32_t t_b, double *x, double *y, double *status, int32_t *gid,
+                        uint8_t *isin);
+
 /* Test hook: use these normal deviates instead of the RNG for the following
  * steps.  normals is [n][ndev] in release order, ndev = 2 + idt/int(deltat):
  * columns 0,1 feed HTurb (x, y), the rest VTurb's sub-steps in order.

@@ -90,6 +90,11 @@ def lib():
         L.og_vturb.restype = C.c_double
         L.og_vturb.argtypes = [C.POINTER(OgGrid), C.POINTER(OgParams), C.POINTER(OgTime),
                                C.c_double, C.c_double, C.c_double, PD]
+        L.og_ll2ij.argtypes = [PD, PD, PD, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, PD, PD]
+        L.og_ll2ij.restype = None
+        L.og_locate.argtypes = [C.POINTER(OgGrid), C.c_int, C.POINTER(PD), C.POINTER(PD), C.POINTER(PD),
+                                C.c_int, C.c_int, C.c_int64, PD, PD, PD, PD, PD, PD, PI32, PU8]
+        L.og_locate.restype = None
         L.og_set_threads.argtypes = [C.c_int]
         L.og_set_threads.restype = None
         L.og_update_particles.restype = C.c_int64
@@ -316,3 +321,64 @@ def interp2d_rho(field, X, Y):
 def set_threads(n):
     """Threads of the oracle's particle loop (the result does not depend on it)."""
     lib().og_set_threads(int(n))
+
+
+def ll2ij(lon, lat, inlon, inlat, spherical=0, angle=None):
+    """LL2ij (interpolation_module.f90:110-225) for each point; lon, lat (and
+    angle) are (eta_rho, xi_rho) arrays, i.e. Fortran (xi, eta)."""
+    lon = np.ascontiguousarray(lon, dtype=np.float64)
+    lat = np.ascontiguousarray(lat, dtype=np.float64)
+    ang = None if angle is None else np.ascontiguousarray(angle, dtype=np.float64)
+    M, L = lon.shape
+    I = np.empty(len(inlon))
+    J = np.empty(len(inlon))
+    i_, j_ = C.c_double(), C.c_double()
+    for k, (x, y) in enumerate(zip(inlon, inlat)):
+        lib().og_ll2ij(dptr(lon), dptr(lat), None if ang is None else dptr(ang), L, M, spherical,
+                       float(x), float(y), C.byref(i_), C.byref(j_))
+        I[k], J[k] = i_.value, j_.value
+    return I, J
+
+
+def locate(cfg, inlon, inlat, z, lonlat=None, spherical=0, angle=None, tb=1, records=None):
+    """ini_ROMSPath's placement of released particles (ROMSPath.f90:322-403) on
+    cfg's grids with hydro records 1-3 in slots 1-3.  lonlat: per grid (lon,
+    lat) arrays, default (x_rho, y_rho) as InitGrid reads on Cartesian grids.
+    Returns dict x, y, z, status, gid, isin."""
+    import sys
+    root = str(HERE.parent)
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    from romspath_amd.nesting import scale_offsets
+    from romspath_amd.runner import RecordSource
+
+    ng = cfg.ngrid
+    if ng > 1:
+        scl, off = scale_offsets(cfg.grids, cfg.refine)
+        grids = [OracleGrid(g, scl[i], off[i]) for i, g in enumerate(cfg.grids)]
+    else:
+        grids = [OracleGrid(cfg.grids[0])]
+    recs = records if records is not None else RecordSource(cfg)
+    for slot in (1, 2, 3):
+        for i, g in enumerate(grids):
+            g.set_slot(slot, recs.get(i + 1, slot))
+    if lonlat is None:
+        lonlat = [(g.x_rho, g.y_rho) for g in cfg.grids]
+    keep = [(np.ascontiguousarray(a, dtype=np.float64), np.ascontiguousarray(b, dtype=np.float64))
+            for a, b in lonlat]
+    lons = (PD * ng)(*[dptr(a) for a, _ in keep])
+    lats = (PD * ng)(*[dptr(b) for _, b in keep])
+    angs = None
+    if angle is not None:
+        akeep = [np.ascontiguousarray(a, dtype=np.float64) for a in angle]
+        angs = (PD * ng)(*[dptr(a) for a in akeep])
+    n = len(inlon)
+    lo = np.ascontiguousarray(inlon, dtype=np.float64)
+    la = np.ascontiguousarray(inlat, dtype=np.float64)
+    out = {"x": np.empty(n), "y": np.empty(n), "z": np.array(z, dtype=np.float64),
+           "status": np.empty(n), "gid": np.empty(n, dtype=np.int32), "isin": np.empty(n, dtype=np.uint8)}
+    arr = (OgGrid * ng)(*[g.s for g in grids])
+    lib().og_locate(arr, ng, lons, lats, angs, spherical, tb, n, dptr(lo), dptr(la), dptr(out["z"]),
+                    dptr(out["x"]), dptr(out["y"]), dptr(out["status"]), iptr(out["gid"]),
+                    out["isin"].ctypes.data_as(PU8))
+    return out

@@ -384,6 +384,178 @@ int og_inside(int nb, const double *xb_in, const double *yb_in, double xo, doubl
     return crossings != 0;
 }
 
+/* inside() (interpolation_module.f90:261-420) for a boundary of any length:
+ * xb, yb hold nb vertices and one more slot for the closing vertex. */
+static int og_inside_n(int nb, double *xb, double *yb, double xo, double yo)
+{
+    int crossings = 0;
+    xb[nb] = xb[0];
+    yb[nb] = yb[0];
+    for (int k = 0; k < nb; k++) {
+        if (!(((xb[k + 1] - xo) * (xo - xb[k]) >= 0.0) && (xb[k] != xb[k + 1]))) continue;
+        double dx1 = xo - xb[k];
+        double dx2 = xb[k + 1] - xo;
+        double dxy = dx2 * (yo - yb[k]) - dx1 * (yb[k + 1] - yo);
+        int inc = 0;
+        if ((xb[k] == xo) && (yb[k] == yo)) {
+            crossings = 1;
+            break;
+        } else if (((dx1 == 0.0) && (yo >= yb[k])) || ((dx2 == 0.0) && (yo >= yb[k + 1]))) {
+            inc = 1;
+        } else if ((dx1 * dx2 > 0.0) && ((xb[k + 1] - xb[k]) * dxy >= 0.0)) {
+            inc = 2;
+        }
+        if (xb[k + 1] > xb[k]) crossings += inc;
+        else crossings -= inc;
+    }
+    return crossings != 0;
+}
+
+/* inbox (interpolation_module.f90:227-259): the boundary of the index box
+ * [Li,Ui] x [Lj,Uj] of the (L, M) column-major lon/lat arrays, in the
+ * reference's vertex order, then inside(). */
+static int og_inbox(int Li, int Ui, int Lj, int Uj, const double *lon, const double *lat, int L,
+                    double xo, double yo)
+{
+    const int nb = 2 * (Uj - Lj + Ui - Li);
+    double *xb = (double *)malloc(sizeof(double) * (size_t)(nb + 1));
+    double *yb = (double *)malloc(sizeof(double) * (size_t)(nb + 1));
+#define LL(a, i, j) (a)[(size_t)((j) - 1) * L + ((i) - 1)]
+    int shft = 1 - Li;
+    for (int i = Li; i <= Ui - 1; i++) { xb[i + shft - 1] = LL(lon, i, Lj); yb[i + shft - 1] = LL(lat, i, Lj); }
+    shft = 1 - Lj + Ui - Li;
+    for (int j = Lj; j <= Uj - 1; j++) { xb[j + shft - 1] = LL(lon, Ui, j); yb[j + shft - 1] = LL(lat, Ui, j); }
+    shft = 1 + Uj - Lj + 2 * Ui - Li;
+    for (int i = Ui; i >= Li + 1; i--) { xb[shft - i - 1] = LL(lon, i, Uj); yb[shft - i - 1] = LL(lat, i, Uj); }
+    shft = 1 + 2 * Uj - Lj + 2 * (Ui - Li);
+    for (int j = Uj; j >= Lj + 1; j--) { xb[shft - j - 1] = LL(lon, Li, j); yb[shft - j - 1] = LL(lat, Li, j); }
+#undef LL
+    const int in = og_inside_n(nb, xb, yb, xo, yo);
+    free(xb);
+    free(yb);
+    return in;
+}
+
+/* LL2ij (interpolation_module.f90:110-225) for one point: bisection of the
+ * index box with inbox, then the Cartesian linear map or the spherical
+ * local-triangle map (spherical = GRIDS(1)%spherical). */
+void og_ll2ij(const double *lon, const double *lat, const double *angle, int L, int M,
+              int spherical, double inlon, double inlat, double *Ipar, double *Jpar)
+{
+    const double pi = 3.14159265358979323846, deg2rad = pi / 180.0, Eradius = 6371315.0;
+#define LL(a, i, j) (a)[(size_t)((j) - 1) * L + ((i) - 1)]
+    int Imin = 1, Imax = L, Jmin = 1, Jmax = M;
+    int found = og_inbox(Imin, Imax, Jmin, Jmax, lon, lat, L, inlon, inlat);
+    if (found) {
+        while (((Imax - Imin) > 1) || ((Jmax - Jmin) > 1)) {
+            if ((Imax - Imin) > 1) {
+                int i0 = (Imin + Imax) / 2;
+                if (og_inbox(Imin, i0, Jmin, Jmax, lon, lat, L, inlon, inlat)) Imax = i0;
+                else Imin = i0;
+            }
+            if ((Jmax - Jmin) > 1) {
+                int j0 = (Jmin + Jmax) / 2;
+                if (og_inbox(Imin, Imax, Jmin, j0, lon, lat, L, inlon, inlat)) Jmax = j0;
+                else Jmin = j0;
+            }
+        }
+    }
+    found = og_inbox(Imin, Imax, Jmin, Jmax, lon, lat, L, inlon, inlat);
+    if (spherical == 0) {
+        double xmin = LL(lon, Imin, Jmin), xmax = LL(lon, Imax, Jmax);
+        double ymin = LL(lat, Imin, Jmin), ymax = LL(lat, Imax, Jmax);
+        double xfac = ((double)Imax - (double)Imin) / (xmax - xmin);
+        double yfac = ((double)Jmax - (double)Jmin) / (ymax - ymin);
+        double dx = (inlon - xmin) * xfac;
+        double dy = (inlat - ymin) * yfac;
+        *Ipar = (double)Imin + dx;
+        *Jpar = (double)Jmin + dy;
+    } else if (found) {
+        double yfac = Eradius * deg2rad;
+        double xfac = yfac * cos(inlat * deg2rad);
+        double xpp = (inlon - LL(lon, Imin, Jmin)) * xfac;
+        double ypp = (inlat - LL(lat, Imin, Jmin)) * yfac;
+        double t1 = (LL(lon, Imin + 1, Jmin) - LL(lon, Imin, Jmin + 1)) * xfac;
+        double t2 = (LL(lat, Imin + 1, Jmin) - LL(lat, Imin, Jmin + 1)) * yfac;
+        double diag2 = t1 * t1 + t2 * t2;
+        t1 = (LL(lon, Imin, Jmin) - LL(lon, Imin + 1, Jmin)) * xfac;
+        t2 = (LL(lat, Imin, Jmin) - LL(lat, Imin + 1, Jmin)) * yfac;
+        double aa2 = t1 * t1 + t2 * t2;
+        t1 = (LL(lon, Imin, Jmin) - LL(lon, Imin, Jmin + 1)) * xfac;
+        t2 = (LL(lat, Imin, Jmin) - LL(lat, Imin, Jmin + 1)) * yfac;
+        double bb2 = t1 * t1 + t2 * t2;
+        double phi = asin((diag2 - aa2 - bb2) / (2.0 * sqrt(aa2 * bb2)));
+        double ang = angle ? LL(angle, Imin, Jmin) : 0.0;
+        double dx = xpp * cos(ang) + ypp * sin(ang);
+        double dy = ypp * cos(ang) - xpp * sin(ang);
+        dx = dx + dy * tan(phi);
+        dy = dy / cos(phi);
+        dx = fmin(fmax(0.0, dx / sqrt(aa2)), 1.0);
+        dy = fmin(fmax(0.0, dy / sqrt(bb2)), 1.0);
+        *Ipar = (double)Imin + dx;
+        *Jpar = (double)Jmin + dy;
+    } else {
+        *Ipar = -1.0;
+        *Jpar = -1.0;
+    }
+#undef LL
+}
+
+/* zbounds (boundary_module.f90:145-179): depth bilinear, zeta (IDW) at slot t */
+void og_zbounds(const og_grid *g, double Ipar, double Jpar, double *Zpar, int *inz, int t)
+{
+    *inz = 0;
+    int I = (int)floor(Ipar);
+    if (I > 0) {
+        double tdepth = -1.0 * og_interp2d(g, 0, 1, Ipar, Jpar);
+        double tzeta = og_interp2d(g, 3, t, Ipar, Jpar);
+        if ((*Zpar < tzeta) && (*Zpar > tdepth)) *inz = 1;
+        if (*Zpar >= tzeta) {
+            *Zpar = tzeta - 0.01;
+            *inz = 1;
+        }
+    }
+}
+
+/* ini_ROMSPath's placement of the released particles (ROMSPath.f90:322-403):
+ * status 9, GID Ngrid, X = Y = 1 and isIn false to start; then for each grid
+ * in order, LL2ij and bounds, and the first grid that holds a particle takes
+ * it (status 0, zbounds at slot t_b, status 9 again if z is outside). */
+void og_locate(const og_grid *grids, int ngrid, const double *const *lon, const double *const *lat,
+               const double *const *angle, int spherical, int tb, int64_t n, const double *inlon,
+               const double *inlat, double *z, double *x, double *y, double *status, int32_t *gid,
+               uint8_t *isin)
+{
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t q = 0; q < n; q++) {
+        x[q] = 1.0;
+        y[q] = 1.0;
+        status[q] = 9.0;
+        gid[q] = ngrid;
+        isin[q] = 0;
+        for (int ng = 1; ng <= ngrid; ng++) {
+            const og_grid *g = &grids[ng - 1];
+            double I, J;
+            og_ll2ij(lon[ng - 1], lat[ng - 1], angle ? angle[ng - 1] : NULL, g->L, g->M, spherical,
+                     inlon[q], inlat[q], &I, &J);
+            int nmask, ingrid, obound;
+            og_bounds(g, I, J, &nmask, &ingrid, &obound);
+            if (!isin[q] && ingrid) {
+                x[q] = I;
+                y[q] = J;
+                gid[q] = ng;
+                status[q] = 0.0;
+                isin[q] = 1;
+                int inz;
+                og_zbounds(g, I, J, &z[q], &inz, tb);
+                if (!inz) status[q] = 9.0;
+            }
+        }
+    }
+}
+
 /* boundary_module.f90:43-143 */
 void og_bounds(const og_grid *g, double X, double Y, int *nmask, int *ingrid, int *obound)
 {

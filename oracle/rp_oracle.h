@@ -93,6 +93,13 @@ double og_vturb(const og_grid *g, const og_params *P, const og_time *T, double X
  * columns 0,1 feed HTurb (x then y), columns 2.. feed VTurb's sub-steps.
  * Returns the number of particles that took the step (the active count). */
 void og_set_threads(int n);
+void og_ll2ij(const double *lon, const double *lat, const double *angle, int L, int M,
+              int spherical, double inlon, double inlat, double *Ipar, double *Jpar);
+void og_zbounds(const og_grid *g, double Ipar, double Jpar, double *Zpar, int *inz, int t);
+void og_locate(const og_grid *grids, int ngrid, const double *const *lon, const double *const *lat,
+               const double *const *angle, int spherical, int tb, int64_t n, const double *inlon,
+               const double *inlat, double *z, double *x, double *y, double *status, int32_t *gid,
+               uint8_t *isin);
 int64_t og_update_particles(const og_grid *grids, const og_params *P, const og_time *T,
                             og_particles *p, int64_t n, const double *dev, int ndev);
 

@@ -30,7 +30,7 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_phase_timing",
            "rp_set_tuning", "rp_set_scenarios", "rp_step_scenario", "rp_step_batch",
            "rp_set_perturbation", "rp_adjoint_record", "rp_adjoint",
-           "rp_set_lonlat", "rp_sample_lonlat",
+           "rp_set_lonlat", "rp_sample_lonlat", "rp_set_grid_angle", "rp_locate_particles",
            "rp_nc_create", "rp_nc_write", "rp_nc_records", "rp_nc_error", "rp_nc_close",
            "rp_last_error", "rp_destroy"]
 # entry points whose result is not an int status
@@ -103,6 +103,8 @@ def lib():
         L.rp_adjoint_record.argtypes = [vp, C.c_int32]
         L.rp_adjoint.argtypes = [vp, C.c_int64, PD, PD, PD, PD, PD, PD, PD]
         L.rp_set_lonlat.argtypes = [vp, C.c_int32, PD, PD]
+        L.rp_set_grid_angle.argtypes = [vp, C.c_int32, C.c_int32, PD]
+        L.rp_locate_particles.argtypes = [vp, C.c_int64, PD, PD, PD, C.c_int32, PD, PD, PD, PI32, PU8]
         L.rp_sample_lonlat.argtypes = [vp, C.c_int64, PD, PD]
         L.rp_nc_create.argtypes = [C.POINTER(vp), C.c_char_p, C.c_int64, C.c_int32, C.c_int32,
                                    C.POINTER(RpNcMeta), PD]
@@ -299,6 +301,22 @@ class Context:
     def set_lonlat(self, ng, lon, lat):
         lo, la = _f64(lon), _f64(lat)
         self._chk("rp_set_lonlat", self._L.rp_set_lonlat(self.h, ng, _d(lo), _d(la)))
+
+    def set_grid_angle(self, ng, spherical, angle=None):
+        a = None if angle is None else _f64(angle)
+        self._chk("rp_set_grid_angle", self._L.rp_set_grid_angle(self.h, ng, int(spherical), _d(a)))
+
+    def locate_particles(self, lon, lat, z, t_b=1):
+        """ini_ROMSPath's placement: returns dict x, y, z, status, gid, isin."""
+        lo, la = _f64(lon), _f64(lat)
+        n = lo.size
+        out = {"x": np.empty(n), "y": np.empty(n), "z": np.array(z, dtype=np.float64),
+               "status": np.empty(n), "gid": np.empty(n, dtype=np.int32),
+               "isin": np.empty(n, dtype=np.uint8)}
+        self._chk("rp_locate_particles", self._L.rp_locate_particles(
+            self.h, n, _d(lo), _d(la), _d(out["z"]), int(t_b), _d(out["x"]), _d(out["y"]),
+            _d(out["status"]), out["gid"].ctypes.data_as(PI32), out["isin"].ctypes.data_as(PU8)))
+        return out
 
     def sample_lonlat(self):
         lon, lat = np.empty(self.n), np.empty(self.n)

@@ -14,7 +14,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "lib" / "librp.so"
-SOURCES = ["rp_step.hip", "rp_adjoint.hip", "rp_sort.hip", "rp_abi.hip"]
+SOURCES = ["rp_step.hip", "rp_adjoint.hip", "rp_sort.hip", "rp_locate.hip", "rp_abi.hip"]
 HEADERS = ["rp_device.h", "rp_kernels.h", "rp_dual.h", "rp_step_dev.h"]
 # host-only C++ (no device code): the trajectory file writer
 HOST_SOURCES = ["rp_ncout.cpp"]

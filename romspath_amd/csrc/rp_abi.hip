@@ -24,7 +24,8 @@ struct GridDev {
     int8_t *mrho = nullptr, *mu = nullptr, *mv = nullptr;
     void *zeta = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *aks = nullptr;
     float* cur[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // FP32 form: fields at ix(2)
-    double *lon = nullptr, *lat = nullptr;   // lon_rho / lat_rho (x_rho / y_rho), output only
+    double *lon = nullptr, *lat = nullptr;   // lon_rho / lat_rho (x_rho / y_rho)
+    double* angle = nullptr;                  // grid angle (spherical grids), may stay null
     double scl[kMaxGrid][2], off[kMaxGrid][2];
     bool set = false;
     bool slot_set[3] = {false, false, false};
@@ -33,6 +34,7 @@ struct GridDev {
 struct Ctx {
     rp_config cfg;
     int fbytes = 4;
+    int spherical = 0;           // GRIDS(1)%spherical, for LL2ij
     int device = 0;
     int ncu = 256;
     hipStream_t stream = nullptr;
@@ -277,7 +279,7 @@ void free_grid(GridDev& g)
     dfree(g.mrho); dfree(g.mu); dfree(g.mv);
     dfree(g.zeta); dfree(g.u); dfree(g.v); dfree(g.w); dfree(g.aks);
     for (auto& p : g.cur) dfree(p);
-    dfree(g.lon); dfree(g.lat);
+    dfree(g.lon); dfree(g.lat); dfree(g.angle);
     g.set = false;
 }
 
@@ -446,6 +448,29 @@ int check_step(Ctx* c, int t_b, int t_c, int t_f, int64_t step_index)
     return RP_OK;
 }
 
+// the device view of every grid, as the step kernels read it
+void fill_tab(const Ctx* c, DGrid tab[kMaxGrid])
+{
+    std::memset(tab, 0, sizeof(DGrid) * kMaxGrid);
+    for (int i = 0; i < c->cfg.ngrid; i++) {
+        const GridDev& g = c->grids[i];
+        DGrid& d = tab[i];
+        d.L = g.L; d.M = g.M; d.N = g.N; d.vtransform = g.vtransform; d.hc = g.hc;
+        d.s_rho = g.s_rho; d.cs_r = g.cs_r; d.s_w = g.s_w; d.cs_w = g.cs_w;
+        d.h = g.h; d.pm = g.pm; d.pn = g.pn;
+        d.mrho = g.mrho; d.mu = g.mu; d.mv = g.mv;
+        d.zeta = g.zeta; d.u = g.u; d.v = g.v; d.w = g.w; d.aks = g.aks;
+        d.czeta = g.cur[0]; d.cu = g.cur[1]; d.cv = g.cur[2]; d.cw = g.cur[3]; d.caks = g.cur[4];
+        d.nwc = cur_column(g);
+        d.kh = c->cfg.kh[i];
+        for (int a = 0; a < kMaxGrid; a++)
+            for (int k = 0; k < 2; k++) {
+                d.scl[a][k] = g.scl[a][k];
+                d.off[a][k] = g.off[a][k];
+            }
+    }
+}
+
 int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, int t_c, int t_f,
               int64_t step_index, int want_diag, int64_t q0, int64_t cnt, StepPlan& P)
 {
@@ -578,24 +603,7 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     }
     {
         DGrid tab[kMaxGrid];
-        std::memset(tab, 0, sizeof(tab));
-        for (int i = 0; i < c->cfg.ngrid; i++) {
-            const GridDev& g = c->grids[i];
-            DGrid& d = tab[i];
-            d.L = g.L; d.M = g.M; d.N = g.N; d.vtransform = g.vtransform; d.hc = g.hc;
-            d.s_rho = g.s_rho; d.cs_r = g.cs_r; d.s_w = g.s_w; d.cs_w = g.cs_w;
-            d.h = g.h; d.pm = g.pm; d.pn = g.pn;
-            d.mrho = g.mrho; d.mu = g.mu; d.mv = g.mv;
-            d.zeta = g.zeta; d.u = g.u; d.v = g.v; d.w = g.w; d.aks = g.aks;
-            d.czeta = g.cur[0]; d.cu = g.cur[1]; d.cv = g.cur[2]; d.cw = g.cur[3]; d.caks = g.cur[4];
-            d.nwc = cur_column(g);
-            d.kh = c->cfg.kh[i];
-            for (int a = 0; a < kMaxGrid; a++)
-                for (int k = 0; k < 2; k++) {
-                    d.scl[a][k] = g.scl[a][k];
-                    d.off[a][k] = g.off[a][k];
-                }
-        }
+        fill_tab(c, tab);
         if (std::memcmp(tab, c->tab_host, sizeof(tab)) != 0) {
             std::memcpy(c->tab_host, tab, sizeof(tab));
             HIPCHK(c, hipMemcpyAsync(c->dgrids, c->tab_host, sizeof(tab), hipMemcpyHostToDevice,
@@ -1426,6 +1434,89 @@ int rp_set_lonlat(void* ctx, int32_t ng, const double* lon, const double* lat)
     if ((rc = dalloc(c, &g.lon, nr)) || (rc = dalloc(c, &g.lat, nr))) return rc;
     if ((rc = put(c, g.lon, lon, nr * 8)) || (rc = put(c, g.lat, lat, nr * 8))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RP_OK;
+}
+
+int rp_set_grid_angle(void* ctx, int32_t ng, int32_t spherical, const double* angle)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (ng < 1 || ng > c->cfg.ngrid || !c->grids[ng - 1].set)
+        return fail(c, RP_E_STATE, "rp_set_grid_angle: grid not set");
+    if (spherical != 0 && spherical != 1) return fail(c, RP_E_ARG, "spherical must be 0 or 1");
+    GridDev& g = c->grids[ng - 1];
+    if (ng == 1) c->spherical = spherical;
+    int rc;
+    if (angle) {
+        const size_t nr = (size_t)g.L * g.M;
+        if ((rc = dalloc(c, &g.angle, nr)) || (rc = put(c, g.angle, angle, nr * 8))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else {
+        dfree(g.angle);
+    }
+    return RP_OK;
+}
+
+int rp_locate_particles(void* ctx, int64_t n, const double* lon, const double* lat, double* z,
+                        int32_t t_b, double* x, double* y, double* status, int32_t* gid,
+                        uint8_t* isin)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (n < 0 || (n > 0 && (!lon || !lat || !z || !x || !y || !status || !gid || !isin)))
+        return fail(c, RP_E_ARG, "rp_locate_particles: null column");
+    if (t_b < 1 || t_b > 3) return fail(c, RP_E_ARG, "slot numbers must be 1..3");
+    LocateArgs A;
+    std::memset(&A, 0, sizeof(A));
+    for (int i = 0; i < c->cfg.ngrid; i++) {
+        const GridDev& g = c->grids[i];
+        if (!g.set) return fail(c, RP_E_STATE, "rp_locate_particles: grid not set");
+        if (!g.lon || !g.lat) return fail(c, RP_E_STATE, "rp_locate_particles: rp_set_lonlat missing");
+        if (!g.slot_set[t_b - 1]) return fail(c, RP_E_STATE, "rp_locate_particles: hydro slot not loaded");
+        A.lon[i] = g.lon;
+        A.lat[i] = g.lat;
+        A.angle[i] = g.angle;
+    }
+    if (n == 0) return RP_OK;
+    A.ngrid = c->cfg.ngrid;
+    A.spherical = c->spherical;
+    A.tb_lane = t_b - 1;
+    A.n = n;
+    DGrid tab[kMaxGrid];
+    fill_tab(c, tab);
+    // inputs lon, lat, z; outputs z, x, y, status (FP64), gid, isin
+    double* d = nullptr;
+    DGrid* dtab = nullptr;
+    int32_t* dg = nullptr;
+    uint8_t* di = nullptr;
+    int rc = 0;
+    if ((rc = dalloc(c, &d, 7 * (size_t)n)) || (rc = dalloc(c, &dtab, kMaxGrid)) ||
+        (rc = dalloc(c, &dg, (size_t)n)) || (rc = dalloc(c, &di, (size_t)n))) {
+        dfree(d); dfree(dtab); dfree(dg); dfree(di);
+        return rc;
+    }
+    A.inlon = d; A.inlat = d + n; A.z = d + 2 * n;
+    A.x = d + 3 * n; A.y = d + 4 * n; A.status = d + 5 * n;
+    A.gid = dg; A.isin = di;
+    hipError_t e = hipMemcpyAsync(d, lon, n * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + n, lat, n * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d + 2 * n, z, n * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dtab, tab, sizeof(tab), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->ncu * 16);
+        if (c->fbytes == 4) hipLaunchKernelGGL(locate_kernel<float>, dim3(grid), dim3(256), 0, c->stream, dtab, A);
+        else hipLaunchKernelGGL(locate_kernel<double>, dim3(grid), dim3(256), 0, c->stream, dtab, A);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(z, d + 2 * n, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(x, d + 3 * n, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(y, d + 4 * n, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(status, d + 5 * n, n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(gid, dg, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(isin, di, n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(d); dfree(dtab); dfree(dg); dfree(di);
+    if (e != hipSuccess) return fail(c, RP_E_HIP, std::string("rp_locate_particles: ") + hipGetErrorString(e));
     return RP_OK;
 }
 

@@ -79,6 +79,21 @@ struct LonLatArgs {
 };
 __global__ void sample_lonlat(LonLatArgs A);
 
+// placement of released particles (ini_ROMSPath, ROMSPath.f90:322-403)
+struct LocateArgs {
+    const double* lon[kMaxGrid];
+    const double* lat[kMaxGrid];
+    const double* angle[kMaxGrid];   // null: 0
+    int ngrid, spherical, tb_lane;   // tb_lane: physical slot of t_b
+    int64_t n;
+    const double *inlon, *inlat;
+    double *z, *x, *y, *status;
+    int32_t* gid;
+    uint8_t* isin;
+};
+template <typename F>
+__global__ void locate_kernel(const DGrid* __restrict__ grids, LocateArgs A);
+
 // tables of the FP32-field form's p = 0 spline sweep; once per device, 0 on success
 int upload_p0_tables();
 

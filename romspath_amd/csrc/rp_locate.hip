@@ -1,0 +1,193 @@
+// rp_locate.hip -- placement of released particles on the grids
+// (ini_ROMSPath, ROMSPath.f90:322-403): LL2ij's bisection of each grid's
+// index box with inbox/inside (interpolation_module.f90:110-420), bounds
+// (boundary_module.f90:43-143) and zbounds (boundary_module.f90:145-179),
+// the first grid that holds a particle taking it.  One lane per particle;
+// the boundary polygons of inbox are walked vertex by vertex from the lon/lat
+// arrays instead of being copied out.
+#include "rp_step_dev.h"
+
+namespace rp {
+
+namespace {
+
+// vertex t (0-based, t == nb closes the polygon) of inbox's boundary of the
+// index box [Li,Ui] x [Lj,Uj], in the reference's order: the lower side
+// eastward, the right side northward, the upper side westward, the left side
+// southward (interpolation_module.f90:238-257)
+struct BoxWalk {
+    const double *lon, *lat;
+    int L, Li, Ui, Lj, Uj, a, b, nb;
+    __device__ __forceinline__ BoxWalk(const double* lo, const double* la, int L_, int li, int ui,
+                                       int lj, int uj)
+        : lon(lo), lat(la), L(L_), Li(li), Ui(ui), Lj(lj), Uj(uj)
+    {
+        a = Ui - Li;
+        b = Uj - Lj;
+        nb = 2 * (a + b);
+    }
+    __device__ __forceinline__ void vertex(int t, double& x, double& y) const
+    {
+        if (t >= nb) t = 0;
+        int i, j;
+        if (t < a) { i = Li + t; j = Lj; }
+        else if (t < a + b) { i = Ui; j = Lj + (t - a); }
+        else if (t < 2 * a + b) { i = Ui - (t - a - b); j = Uj; }
+        else { i = Li; j = Uj - (t - 2 * a - b); }
+        const int64_t o = (int64_t)(j - 1) * L + (i - 1);
+        x = lon[o];
+        y = lat[o];
+    }
+};
+
+// inside() (interpolation_module.f90:261-420) over the box boundary: the
+// edges are taken in order, so the crossing count and the early exit on a
+// vertex hit match the reference's chunked candidate list
+__device__ bool inbox(const BoxWalk& w, double xo, double yo)
+{
+    double xk, yk;
+    w.vertex(0, xk, yk);
+    int crossings = 0;
+    for (int k = 0; k < w.nb; k++) {
+        double x1, y1;
+        w.vertex(k + 1, x1, y1);
+        if (((x1 - xo) * (xo - xk) >= 0.0) && (xk != x1)) {
+            const double dx1 = xo - xk;
+            const double dx2 = x1 - xo;
+            const double dxy = dx2 * (yo - yk) - dx1 * (y1 - yo);
+            int inc = 0;
+            if ((xk == xo) && (yk == yo)) return true;
+            if (((dx1 == 0.0) && (yo >= yk)) || ((dx2 == 0.0) && (yo >= y1))) inc = 1;
+            else if ((dx1 * dx2 > 0.0) && ((x1 - xk) * dxy >= 0.0)) inc = 2;
+            if (x1 > xk) crossings += inc;
+            else crossings -= inc;
+        }
+        xk = x1;
+        yk = y1;
+    }
+    return crossings != 0;
+}
+
+__device__ __forceinline__ bool inbox(const double* lon, const double* lat, int L, int Li, int Ui,
+                                      int Lj, int Uj, double xo, double yo)
+{
+    return inbox(BoxWalk(lon, lat, L, Li, Ui, Lj, Uj), xo, yo);
+}
+
+// LL2ij (interpolation_module.f90:110-225) for one point
+__device__ void ll2ij(const double* lon, const double* lat, const double* angle, int L, int M,
+                      int spherical, double inlon, double inlat, double& Ipar, double& Jpar)
+{
+    const double pi = 3.14159265358979323846, deg2rad = pi / 180.0, Eradius = 6371315.0;
+    int Imin = 1, Imax = L, Jmin = 1, Jmax = M;
+    bool found = inbox(lon, lat, L, Imin, Imax, Jmin, Jmax, inlon, inlat);
+    if (found) {
+        while (((Imax - Imin) > 1) || ((Jmax - Jmin) > 1)) {
+            if ((Imax - Imin) > 1) {
+                const int i0 = (Imin + Imax) / 2;
+                if (inbox(lon, lat, L, Imin, i0, Jmin, Jmax, inlon, inlat)) Imax = i0;
+                else Imin = i0;
+            }
+            if ((Jmax - Jmin) > 1) {
+                const int j0 = (Jmin + Jmax) / 2;
+                if (inbox(lon, lat, L, Imin, Imax, Jmin, j0, inlon, inlat)) Jmax = j0;
+                else Jmin = j0;
+            }
+        }
+    }
+    found = inbox(lon, lat, L, Imin, Imax, Jmin, Jmax, inlon, inlat);
+    auto at = [&](const double* f, int i, int j) { return f[(int64_t)(j - 1) * L + (i - 1)]; };
+    if (spherical == 0) {
+        const double xmin = at(lon, Imin, Jmin), xmax = at(lon, Imax, Jmax);
+        const double ymin = at(lat, Imin, Jmin), ymax = at(lat, Imax, Jmax);
+        const double xfac = ((double)Imax - (double)Imin) / (xmax - xmin);
+        const double yfac = ((double)Jmax - (double)Jmin) / (ymax - ymin);
+        const double dx = (inlon - xmin) * xfac;
+        const double dy = (inlat - ymin) * yfac;
+        Ipar = (double)Imin + dx;
+        Jpar = (double)Jmin + dy;
+    } else if (found) {
+        const double yfac = Eradius * deg2rad;
+        const double xfac = yfac * cos(inlat * deg2rad);
+        const double xpp = (inlon - at(lon, Imin, Jmin)) * xfac;
+        const double ypp = (inlat - at(lat, Imin, Jmin)) * yfac;
+        double t1 = (at(lon, Imin + 1, Jmin) - at(lon, Imin, Jmin + 1)) * xfac;
+        double t2 = (at(lat, Imin + 1, Jmin) - at(lat, Imin, Jmin + 1)) * yfac;
+        const double diag2 = t1 * t1 + t2 * t2;
+        t1 = (at(lon, Imin, Jmin) - at(lon, Imin + 1, Jmin)) * xfac;
+        t2 = (at(lat, Imin, Jmin) - at(lat, Imin + 1, Jmin)) * yfac;
+        const double aa2 = t1 * t1 + t2 * t2;
+        t1 = (at(lon, Imin, Jmin) - at(lon, Imin, Jmin + 1)) * xfac;
+        t2 = (at(lat, Imin, Jmin) - at(lat, Imin, Jmin + 1)) * yfac;
+        const double bb2 = t1 * t1 + t2 * t2;
+        const double phi = asin((diag2 - aa2 - bb2) / (2.0 * sqrt(aa2 * bb2)));
+        const double ang = angle ? at(angle, Imin, Jmin) : 0.0;
+        double dx = xpp * cos(ang) + ypp * sin(ang);
+        double dy = ypp * cos(ang) - xpp * sin(ang);
+        dx = dx + dy * tan(phi);
+        dy = dy / cos(phi);
+        dx = fmin(fmax(0.0, dx / sqrt(aa2)), 1.0);
+        dy = fmin(fmax(0.0, dy / sqrt(bb2)), 1.0);
+        Ipar = (double)Imin + dx;
+        Jpar = (double)Jmin + dy;
+    } else {
+        Ipar = -1.0;
+        Jpar = -1.0;
+    }
+}
+
+}  // namespace
+
+template <typename F>
+__global__ void __launch_bounds__(256) locate_kernel(const DGrid* __restrict__ grids, LocateArgs A)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < A.n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        double x = 1.0, y = 1.0, z = A.z[q], st = 9.0;
+        int gid = A.ngrid;
+        bool isin = false;
+        for (int ng = 1; ng <= A.ngrid; ng++) {
+            const DGrid& g = grids[ng - 1];
+            double I, J;
+            ll2ij(A.lon[ng - 1], A.lat[ng - 1], A.angle[ng - 1], g.L, g.M, A.spherical, A.inlon[q],
+                  A.inlat[q], I, J);
+            bool ingrid, obound;
+            bounds(g, I, J, ingrid, obound);
+            if (!isin && ingrid) {
+                x = I;
+                y = J;
+                gid = ng;
+                st = 0.0;
+                isin = true;
+                // zbounds: depth bilinear, zeta (IDW) at slot t_b
+                bool inz = false;
+                if ((int)floor(I) > 0) {
+                    const RhoCell c = rho_cell(g, I, J);
+                    const double tdepth = -1.0 * bilinear2d(g.h, c);
+                    SlotSel S;
+                    S.s[0] = S.s[1] = S.s[2] = A.tb_lane;
+                    double z3[3];
+                    zeta3<F>(g, c, S, z3);
+                    const double tzeta = z3[0];
+                    if ((z < tzeta) && (z > tdepth)) inz = true;
+                    if (z >= tzeta) {
+                        z = tzeta - 0.01;
+                        inz = true;
+                    }
+                }
+                if (!inz) st = 9.0;
+            }
+        }
+        A.x[q] = x;
+        A.y[q] = y;
+        A.z[q] = z;
+        A.status[q] = st;
+        A.gid[q] = gid;
+        A.isin[q] = isin ? 1 : 0;
+    }
+}
+
+template __global__ void locate_kernel<float>(const DGrid*, LocateArgs);
+template __global__ void locate_kernel<double>(const DGrid*, LocateArgs);
+
+}  // namespace rp

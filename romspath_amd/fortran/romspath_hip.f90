@@ -53,6 +53,7 @@ module romspath_hip
             rp_destroy, rp_set_tuning, rp_set_scenarios, rp_step_scenario, rp_step_batch, &
             rp_set_perturbation, rp_adjoint_record, rp_adjoint, &
             rp_set_lonlat, rp_sample_lonlat, rp_nc_create, rp_nc_write, rp_nc_close, &
+            rp_set_grid_angle, rp_locate_particles, &
             rp_error_text
 
   interface
@@ -216,6 +217,28 @@ module romspath_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: ng
       real(c_double), intent(in) :: lon(*), lat(*)
+    end function
+
+    ! GRIDS(ng)%angle and GRIDS(1)%spherical, for LL2ij
+    integer(c_int) function rp_set_grid_angle(ctx, ng, spherical, angle) bind(C, name="rp_set_grid_angle")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ng, spherical
+      type(c_ptr), value :: angle
+    end function
+
+    ! particle placement of ini_ROMSPath (ROMSPath.f90:322-403): LL2ij, bounds, zbounds
+    integer(c_int) function rp_locate_particles(ctx, n, lon, lat, z, t_b, x, y, status, gid, isin) &
+        bind(C, name="rp_locate_particles")
+      import :: c_int, c_int32_t, c_int64_t, c_int8_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: lon(*), lat(*)
+      real(c_double), intent(inout) :: z(*)
+      integer(c_int32_t), value :: t_b
+      real(c_double), intent(out) :: x(*), y(*), status(*)
+      integer(c_int32_t), intent(out) :: gid(*)
+      integer(c_int8_t), intent(out) :: isin(*)
     end function
 
     ! getInterp2D("lon"/"lat") at every particle (dataOutput, ROMSPath.f90:1012-1025)

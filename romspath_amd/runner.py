@@ -50,6 +50,12 @@ class GpuRun:
         self.ctx.set_particles(cfg.particles)
         self.records = records or RecordSource(cfg)
 
+    def load_initial(self):
+        """Records 1, 2, 3 into slots 1, 2, 3, as ini_ROMSPath does (ROMSPath.f90:307-309)."""
+        for slot in (1, 2, 3):
+            for g in range(1, self.cfg.ngrid + 1):
+                self.ctx.set_hydro_slot(g, slot, self.records.get(g, slot))
+
     def load(self, st):
         for ld in st.loads:
             for g in range(1, self.cfg.ngrid + 1):
