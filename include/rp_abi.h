@@ -129,6 +129,18 @@ int rp_get_particles(void *ctx, int64_t n, double *x, double *y, double *z, doub
 int rp_set_lonlat(void *ctx, int32_t ng, const double *lon, const double *lat);
 int rp_sample_lonlat(void *ctx, int64_t n, double *lon, double *lat);
 
+/* SaltTempOn (ROMSPath.f90:958-981): rp_set_hydro_tracers stores salt and
+ * temp (xi_rho, eta_rho, s_rho) of time slot 1..3, as updateHydro reads them
+ * (hydrodynamic_module.f90:663-745); asynchronous like rp_set_hydro_slot.
+ * rp_sample_tracers gives P_salt and P_temp at every particle's current
+ * position (release order): zeta of slots b, c, f, getInterp3d("salt"/"temp")
+ * on rho levels with each slot's zeta (interpolation_module.f90:509-589,
+ * 769-830), polintd in time at ix(2).  Synchronises the stream. */
+int rp_set_hydro_tracers(void *ctx, int32_t ng, int32_t slot, const double *salt,
+                         const double *temp);
+int rp_sample_tracers(void *ctx, int64_t n, const double ex[3], const double ix[3], int32_t t_b,
+                      int32_t t_c, int32_t t_f, double *salt, double *temp);
+
 /* ini_ROMSPath's placement of the released particles (ROMSPath.f90:322-403):
  * for grid ng = 1..Ngrid in order, LL2ij (interpolation_module.f90:110-225:
  * bisection of the index box with inbox/inside, then the Cartesian linear

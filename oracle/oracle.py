@@ -29,7 +29,8 @@ class OgGrid(C.Structure):
                 ("h", PD), ("pm", PD), ("pn", PD),
                 ("mask_rho", PI32), ("mask_u", PI32), ("mask_v", PI32),
                 ("scl", (C.c_double * 2) * MAXGRID), ("off", (C.c_double * 2) * MAXGRID),
-                ("zeta", PD * 3), ("u", PD * 3), ("v", PD * 3), ("w", PD * 3), ("aks", PD * 3)]
+                ("zeta", PD * 3), ("u", PD * 3), ("v", PD * 3), ("w", PD * 3), ("aks", PD * 3),
+                ("salt", PD * 3), ("temp", PD * 3)]
 
 
 class OgParams(C.Structure):
@@ -95,6 +96,9 @@ def lib():
         L.og_locate.argtypes = [C.POINTER(OgGrid), C.c_int, C.POINTER(PD), C.POINTER(PD), C.POINTER(PD),
                                 C.c_int, C.c_int, C.c_int64, PD, PD, PD, PD, PD, PD, PI32, PU8]
         L.og_locate.restype = None
+        L.og_sample_tracers.argtypes = [C.POINTER(OgGrid), C.POINTER(OgTime), C.c_int64, PD, PD, PD,
+                                        PI32, PD, PD]
+        L.og_sample_tracers.restype = None
         L.og_set_threads.argtypes = [C.c_int]
         L.og_set_threads.restype = None
         L.og_update_particles.restype = C.c_int64
@@ -155,6 +159,11 @@ class OracleGrid:
                 a = np.ascontiguousarray(a + rec["sv"])
             keep[name] = a
             getattr(self.s, name)[slot - 1] = dptr(a)
+        for name in ("salt", "temp"):          # SaltTempOn fields, when the record has them
+            if name in rec:
+                a = np.ascontiguousarray(rec[name], dtype=np.float64)
+                keep[name] = a
+                getattr(self.s, name)[slot - 1] = dptr(a)
         self.slots[slot - 1] = keep
 
 
@@ -382,3 +391,16 @@ def locate(cfg, inlon, inlat, z, lonlat=None, spherical=0, angle=None, tb=1, rec
                     dptr(out["x"]), dptr(out["y"]), dptr(out["status"]), iptr(out["gid"]),
                     out["isin"].ctypes.data_as(PU8))
     return out
+
+
+def sample_tracers(grids, ex, ix, tb, tc, tf, x, y, z, gid):
+    """SaltTempOn's salt and temp at the particles (ROMSPath.f90:958-981); grids:
+    OracleGrid list with salt/temp in the slots."""
+    T = make_time(ex, ix, 0, tb, tc, tf)
+    arr = (OgGrid * len(grids))(*[g.s for g in grids])
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    gid = np.ascontiguousarray(gid, dtype=np.int32)
+    salt, temp = np.empty(x.size), np.empty(x.size)
+    lib().og_sample_tracers(arr, C.byref(T), x.size, dptr(x), dptr(y), dptr(z), iptr(gid),
+                            dptr(salt), dptr(temp))
+    return salt, temp

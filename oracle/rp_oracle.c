@@ -304,6 +304,21 @@ double og_interp3d(const og_grid *g, int var, double X, double Y, double Z, int 
         } else if (nwater == 0.0) {
             for (int c = 0; c < 4; c++) { cff[c] = 0.0; vb[c] = 0.0; vt[c] = 0.0; }
         }
+    } else if (var == 3 || var == 4) {
+        /* salt, temp: rho points and rho levels, the w/AKs mask rule
+         * (interpolation_module.f90:769-830) */
+        const double *f = var == 3 ? g->salt[s] : g->temp[s];
+        vb[0] = W3(f, Inode, Jnode, ibot);         vt[0] = W3(f, Inode, Jnode, itop);
+        vb[1] = W3(f, Inode + 1, Jnode, ibot);     vt[1] = W3(f, Inode + 1, Jnode, itop);
+        vb[2] = W3(f, Inode + 1, Jnode + 1, ibot); vt[2] = W3(f, Inode + 1, Jnode + 1, itop);
+        vb[3] = W3(f, Inode, Jnode + 1, ibot);     vt[3] = W3(f, Inode, Jnode + 1, itop);
+        m[0] = R2(g->mask_rho, Inode, Jnode);
+        m[1] = R2(g->mask_rho, Inode + 1, Jnode);
+        m[2] = R2(g->mask_rho, Inode + 1, Jnode + 1);
+        m[3] = R2(g->mask_rho, Inode, Jnode + 1);
+        double nwater = m[0] + m[1] + m[2] + m[3];
+        if (nwater < 4.0)
+            og_getcoeff(x, y, m, cff);
     } else {
         const double *f = g->w[s];
         vb[0] = W3(f, Inode, Jnode, ibot);         vt[0] = W3(f, Inode, Jnode, itop);
@@ -553,6 +568,35 @@ void og_locate(const og_grid *grids, int ngrid, const double *const *lon, const 
                 if (!inz) status[q] = 9.0;
             }
         }
+    }
+}
+
+/* The SaltTempOn block of update_particles (ROMSPath.f90:958-981) at the
+ * particles' positions: zeta of slots b, c, f (IDW), getInterp3d("salt"/
+ * "temp", ztype 1) with each slot's own zeta and the depth at the position,
+ * then polintd in time at ix(2). */
+void og_sample_tracers(const og_grid *grids, const og_time *T, int64_t n, const double *x,
+                       const double *y, const double *z, const int32_t *gid, double *salt,
+                       double *temp)
+{
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+    for (int64_t q = 0; q < n; q++) {
+        const og_grid *g = &grids[gid[q] - 1];
+        double tdepth = -1.0 * og_interp2d(g, 0, T->tc, x[q], y[q]);
+        double zb = og_interp2d(g, 3, T->tb, x[q], y[q]);
+        double zc = og_interp2d(g, 3, T->tc, x[q], y[q]);
+        double zf = og_interp2d(g, 3, T->tf, x[q], y[q]);
+        double ey[3];
+        ey[0] = og_interp3d(g, 3, x[q], y[q], z[q], T->tb, 1, zb, tdepth);
+        ey[1] = og_interp3d(g, 3, x[q], y[q], z[q], T->tc, 1, zc, tdepth);
+        ey[2] = og_interp3d(g, 3, x[q], y[q], z[q], T->tf, 1, zf, tdepth);
+        salt[q] = og_polintd(T->ex, ey, T->ix[1]);
+        ey[0] = og_interp3d(g, 4, x[q], y[q], z[q], T->tb, 1, zb, tdepth);
+        ey[1] = og_interp3d(g, 4, x[q], y[q], z[q], T->tc, 1, zc, tdepth);
+        ey[2] = og_interp3d(g, 4, x[q], y[q], z[q], T->tf, 1, zf, tdepth);
+        temp[q] = og_polintd(T->ex, ey, T->ix[1]);
     }
 }
 

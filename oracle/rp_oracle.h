@@ -40,6 +40,8 @@ typedef struct {
     const double *v[3];          /* L*(M-1)*N */
     const double *w[3];          /* L*M*(N+1) */
     const double *aks[3];        /* L*M*(N+1) */
+    const double *salt[3];       /* L*M*N, may be NULL (SaltTempOn off) */
+    const double *temp[3];       /* L*M*N */
 } og_grid;
 
 typedef struct {
@@ -93,6 +95,9 @@ double og_vturb(const og_grid *g, const og_params *P, const og_time *T, double X
  * columns 0,1 feed HTurb (x then y), columns 2.. feed VTurb's sub-steps.
  * Returns the number of particles that took the step (the active count). */
 void og_set_threads(int n);
+void og_sample_tracers(const og_grid *grids, const og_time *T, int64_t n, const double *x,
+                       const double *y, const double *z, const int32_t *gid, double *salt,
+                       double *temp);
 void og_ll2ij(const double *lon, const double *lat, const double *angle, int L, int M,
               int spherical, double inlon, double inlat, double *Ipar, double *Jpar);
 void og_zbounds(const og_grid *g, double Ipar, double Jpar, double *Zpar, int *inz, int t);

@@ -31,6 +31,7 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_set_tuning", "rp_set_scenarios", "rp_step_scenario", "rp_step_batch",
            "rp_set_perturbation", "rp_adjoint_record", "rp_adjoint",
            "rp_set_lonlat", "rp_sample_lonlat", "rp_set_grid_angle", "rp_locate_particles",
+           "rp_set_hydro_tracers", "rp_sample_tracers",
            "rp_nc_create", "rp_nc_write", "rp_nc_records", "rp_nc_error", "rp_nc_close",
            "rp_last_error", "rp_destroy"]
 # entry points whose result is not an int status
@@ -104,11 +105,13 @@ def lib():
         L.rp_adjoint.argtypes = [vp, C.c_int64, PD, PD, PD, PD, PD, PD, PD]
         L.rp_set_lonlat.argtypes = [vp, C.c_int32, PD, PD]
         L.rp_set_grid_angle.argtypes = [vp, C.c_int32, C.c_int32, PD]
+        L.rp_set_hydro_tracers.argtypes = [vp, C.c_int32, C.c_int32, PD, PD]
+        L.rp_sample_tracers.argtypes = [vp, C.c_int64, PD, PD, C.c_int32, C.c_int32, C.c_int32, PD, PD]
         L.rp_locate_particles.argtypes = [vp, C.c_int64, PD, PD, PD, C.c_int32, PD, PD, PD, PI32, PU8]
         L.rp_sample_lonlat.argtypes = [vp, C.c_int64, PD, PD]
         L.rp_nc_create.argtypes = [C.POINTER(vp), C.c_char_p, C.c_int64, C.c_int32, C.c_int32,
                                    C.POINTER(RpNcMeta), PD]
-        L.rp_nc_write.argtypes = [vp, C.c_double, PD, PD, PD, PD, PD, PI32, PD, PD, PD]
+        L.rp_nc_write.argtypes = [vp, C.c_double, PD, PD, PD, PD, PD, PI32, PD, PD, PD, PD, PD]
         L.rp_nc_records.argtypes = [vp]
         L.rp_nc_error.argtypes = [vp]
         L.rp_nc_close.argtypes = [vp]
@@ -301,6 +304,18 @@ class Context:
     def set_lonlat(self, ng, lon, lat):
         lo, la = _f64(lon), _f64(lat)
         self._chk("rp_set_lonlat", self._L.rp_set_lonlat(self.h, ng, _d(lo), _d(la)))
+
+    def set_hydro_tracers(self, ng, slot, salt, temp):
+        sa, te = _f64(salt), _f64(temp)
+        self._chk("rp_set_hydro_tracers", self._L.rp_set_hydro_tracers(self.h, ng, slot, _d(sa), _d(te)))
+
+    def sample_tracers(self, ex, ix, tb, tc, tf):
+        salt, temp = np.empty(self.n), np.empty(self.n)
+        exa = (C.c_double * 3)(*ex)
+        ixa = (C.c_double * 3)(*ix)
+        self._chk("rp_sample_tracers", self._L.rp_sample_tracers(
+            self.h, self.n, exa, ixa, tb, tc, tf, _d(salt), _d(temp)))
+        return salt, temp
 
     def set_grid_angle(self, ng, spherical, angle=None):
         a = None if angle is None else _f64(angle)

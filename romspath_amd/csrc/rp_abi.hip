@@ -26,6 +26,8 @@ struct GridDev {
     float* cur[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // FP32 form: fields at ix(2)
     double *lon = nullptr, *lat = nullptr;   // lon_rho / lat_rho (x_rho / y_rho)
     double* angle = nullptr;                  // grid angle (spherical grids), may stay null
+    void *salt = nullptr, *temp = nullptr;    // SaltTempOn fields, slot-interleaved (L, M, N)
+    bool tracer_set[3] = {false, false, false};
     double scl[kMaxGrid][2], off[kMaxGrid][2];
     bool set = false;
     bool slot_set[3] = {false, false, false};
@@ -73,7 +75,7 @@ struct Ctx {
         hipEvent_t packed = nullptr;    // on stream: the pack kernel has read it
         bool used = false;
     };
-    FieldStage fstage[kMaxGrid][5];
+    FieldStage fstage[kMaxGrid][7];   // zeta, u, v, w, aks, salt, temp
     // VTurb scratch
     double* scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -280,6 +282,8 @@ void free_grid(GridDev& g)
     dfree(g.zeta); dfree(g.u); dfree(g.v); dfree(g.w); dfree(g.aks);
     for (auto& p : g.cur) dfree(p);
     dfree(g.lon); dfree(g.lat); dfree(g.angle);
+    dfree(g.salt); dfree(g.temp);
+    for (auto& t : g.tracer_set) t = false;
     g.set = false;
 }
 
@@ -1434,6 +1438,84 @@ int rp_set_lonlat(void* ctx, int32_t ng, const double* lon, const double* lat)
     if ((rc = dalloc(c, &g.lon, nr)) || (rc = dalloc(c, &g.lat, nr))) return rc;
     if ((rc = put(c, g.lon, lon, nr * 8)) || (rc = put(c, g.lat, lat, nr * 8))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return RP_OK;
+}
+
+int rp_set_hydro_tracers(void* ctx, int32_t ng, int32_t slot, const double* salt, const double* temp)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (ng < 1 || ng > c->cfg.ngrid || !c->grids[ng - 1].set)
+        return fail(c, RP_E_STATE, "rp_set_hydro_tracers: grid not set");
+    if (slot < 1 || slot > 3) return fail(c, RP_E_ARG, "slot must be 1..3");
+    if (!salt || !temp) return fail(c, RP_E_ARG, "null field");
+    GridDev& g = c->grids[ng - 1];
+    const size_t elems = (size_t)g.L * g.M * g.N * 4 * c->fbytes;
+    int rc;
+    if (!g.salt && ((rc = dalloc(c, (char**)&g.salt, elems)) || (rc = dalloc(c, (char**)&g.temp, elems))))
+        return rc;
+    const int gi = ng - 1, s = slot - 1;
+    if ((rc = upload_field(c, gi, 5, g.salt, salt, nullptr, g.L, g.M, g.N, s)) ||
+        (rc = upload_field(c, gi, 6, g.temp, temp, nullptr, g.L, g.M, g.N, s)))
+        return rc;
+    g.tracer_set[s] = true;
+    return RP_OK;
+}
+
+int rp_sample_tracers(void* ctx, int64_t n, const double ex[3], const double ix[3], int32_t t_b,
+                      int32_t t_c, int32_t t_f, double* salt, double* temp)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !ex || !ix) return RP_E_ARG;
+    if (n != c->n) return fail(c, RP_E_ARG, "particle count differs from rp_set_particles");
+    if (n > 0 && (!salt || !temp)) return fail(c, RP_E_ARG, "null output");
+    if (t_b < 1 || t_b > 3 || t_c < 1 || t_c > 3 || t_f < 1 || t_f > 3)
+        return fail(c, RP_E_ARG, "slot numbers must be 1..3");
+    TracerArgs A;
+    std::memset(&A, 0, sizeof(A));
+    for (int i = 0; i < c->cfg.ngrid; i++) {
+        const GridDev& g = c->grids[i];
+        if (!g.set) return fail(c, RP_E_STATE, "rp_sample_tracers: grid not set");
+        if (!g.tracer_set[t_b - 1] || !g.tracer_set[t_c - 1] || !g.tracer_set[t_f - 1] ||
+            !g.slot_set[t_b - 1] || !g.slot_set[t_c - 1] || !g.slot_set[t_f - 1])
+            return fail(c, RP_E_STATE, "rp_sample_tracers: salt/temp or hydro slot not loaded");
+        A.salt[i] = g.salt;
+        A.temp[i] = g.temp;
+    }
+    if (n == 0) return RP_OK;
+    A.slots.s[0] = t_b - 1; A.slots.s[1] = t_c - 1; A.slots.s[2] = t_f - 1;
+    for (int k = 0; k < 3; k++) A.ex[k] = ex[k];
+    A.ix2 = ix[1];
+    A.n = n;
+    A.x = c->x; A.y = c->y; A.z = c->z; A.gid = c->gid;
+    DGrid tab[kMaxGrid];
+    fill_tab(c, tab);
+    double* d = nullptr;
+    DGrid* dtab = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &d, 2 * (size_t)n)) || (rc = dalloc(c, &dtab, kMaxGrid))) {
+        dfree(d); dfree(dtab);
+        return rc;
+    }
+    A.out_salt = d; A.out_temp = d + n;
+    hipError_t e = hipMemcpyAsync(dtab, tab, sizeof(tab), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->ncu * 16);
+        if (c->fbytes == 4) hipLaunchKernelGGL(tracer_kernel<float>, dim3(grid), dim3(256), 0, c->stream, dtab, A);
+        else hipLaunchKernelGGL(tracer_kernel<double>, dim3(grid), dim3(256), 0, c->stream, dtab, A);
+        e = hipGetLastError();
+    }
+    std::vector<double> h(2 * (size_t)n);
+    std::vector<uint32_t> ids(n);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, 2 * (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ids.data(), c->id, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(d); dfree(dtab);
+    if (e != hipSuccess) return fail(c, RP_E_HIP, std::string("rp_sample_tracers: ") + hipGetErrorString(e));
+    for (int64_t i = 0; i < n; i++) {
+        salt[ids[i]] = h[i];
+        temp[ids[i]] = h[n + i];
+    }
     return RP_OK;
 }
 

@@ -94,6 +94,20 @@ struct LocateArgs {
 template <typename F>
 __global__ void locate_kernel(const DGrid* __restrict__ grids, LocateArgs A);
 
+// salt and temp at the particles (SaltTempOn, ROMSPath.f90:958-981)
+struct TracerArgs {
+    const void* salt[kMaxGrid];   // slot-interleaved, field precision, (L, M, N)
+    const void* temp[kMaxGrid];
+    SlotSel slots;                 // physical slots of t_b, t_c, t_f
+    double ex[3], ix2;
+    int64_t n;
+    const double *x, *y, *z;
+    const int8_t* gid;
+    double *out_salt, *out_temp;
+};
+template <typename F>
+__global__ void tracer_kernel(const DGrid* __restrict__ grids, TracerArgs A);
+
 // tables of the FP32-field form's p = 0 spline sweep; once per device, 0 on success
 int upload_p0_tables();
 

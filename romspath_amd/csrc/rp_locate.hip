@@ -1,4 +1,7 @@
-// rp_locate.hip -- placement of released particles on the grids
+// rp_locate.hip -- placement of released particles on the grids, and the
+// optional salt/temp sampler at the particles.
+//
+// Placement of released particles on the grids
 // (ini_ROMSPath, ROMSPath.f90:322-403): LL2ij's bisection of each grid's
 // index box with inbox/inside (interpolation_module.f90:110-420), bounds
 // (boundary_module.f90:43-143) and zbounds (boundary_module.f90:145-179),
@@ -186,6 +189,44 @@ __global__ void __launch_bounds__(256) locate_kernel(const DGrid* __restrict__ g
         A.isin[q] = isin ? 1 : 0;
     }
 }
+
+// The SaltTempOn block of update_particles (ROMSPath.f90:958-981): zeta of
+// slots b, c, f, getInterp3d("salt"/"temp", ztype 1) with each slot's own
+// zeta and vertical bracket (interpolation_module.f90:509-589, 769-830),
+// polintd at ix(2).
+template <typename F>
+__global__ void __launch_bounds__(256) tracer_kernel(const DGrid* __restrict__ grids, TracerArgs A)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < A.n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int gi = (int)A.gid[q] - 1;
+        const DGrid& g = grids[gi];
+        const double X = A.x[q], Y = A.y[q], Z = A.z[q];
+        const RhoCell c = rho_cell(g, X, Y);
+        const double tdepth = -1.0 * bilinear2d(g.h, c);
+        double zt[3];
+        zeta3<F>(g, c, A.slots, zt);
+        double cff[4];
+        rho_weights(g, c, cff);
+        const bool zero[4] = {false, false, false, false};
+        double es[3], et[3];
+        for (int t = 0; t < 3; t++) {
+            const VBracket vb = bracket_rho<false>(g, Z, zt[t], tdepth);
+            SlotSel S;
+            S.s[0] = S.s[1] = S.s[2] = A.slots.s[t];
+            double o[3];
+            blend3<F, 1>(A.salt[gi], nullptr, c.o, g.N, cff, zero, vb, S, o);
+            es[t] = o[0];
+            blend3<F, 1>(A.temp[gi], nullptr, c.o, g.N, cff, zero, vb, S, o);
+            et[t] = o[0];
+        }
+        A.out_salt[q] = polintd(A.ex, es[0], es[1], es[2], A.ix2);
+        A.out_temp[q] = polintd(A.ex, et[0], et[1], et[2], A.ix2);
+    }
+}
+
+template __global__ void tracer_kernel<float>(const DGrid*, TracerArgs);
+template __global__ void tracer_kernel<double>(const DGrid*, TracerArgs);
 
 template __global__ void locate_kernel<float>(const DGrid*, LocateArgs);
 template __global__ void locate_kernel<double>(const DGrid*, LocateArgs);

@@ -207,6 +207,13 @@ int rp_nc_create(void** nc, const char* path, int64_t numpar, int32_t ngrid, int
         desc("identification number for particle behavior or status",
              "nondimensional, see ROMSPath User Guide", "status, scalar, series"));
     if (ngrid > 1) add("GID", NC_FLOAT, true, true, desc("Grid ID", "none", nullptr));
+    if (flags & RP_NC_SALTTEMP) {
+        add("salinity", NC_FLOAT, true, true,
+            desc("Salinity at the particle's location", nullptr, "salinity, scalar, series"));
+        add("temperature", NC_FLOAT, true, true,
+            desc("Temperature at the particle's location", "degree Celsius",
+                 "temperature, scalar, series"));
+    }
     if (flags & RP_NC_WATERDEPTH)
         add("WaterDepth", NC_FLOAT, true, true,
             desc("Local Water Depth at Particle Location", "m", "WaterDepth, scalar, series"));
@@ -316,13 +323,15 @@ int rp_nc_create(void** nc, const char* path, int64_t numpar, int32_t ngrid, int
 
 int rp_nc_write(void* nc, double model_time, const double* age, const double* lon,
                 const double* lat, const double* zp, const double* status, const int32_t* gid,
-                const double* wd, const double* zeta, const double* bath)
+                const double* wd, const double* zeta, const double* bath, const double* salt,
+                const double* temp)
 {
     Writer* w = (Writer*)nc;
     if (!w) return RP_E_ARG;
     if (!age || !lon || !lat || !zp || !status) return wfail(w, RP_E_ARG, "null particle column");
     if ((w->ngrid > 1 && !gid) || ((w->flags & RP_NC_WATERDEPTH) && !wd) ||
-        ((w->flags & RP_NC_ZETA) && !zeta) || ((w->flags & RP_NC_BATH) && !bath))
+        ((w->flags & RP_NC_ZETA) && !zeta) || ((w->flags & RP_NC_BATH) && !bath) ||
+        ((w->flags & RP_NC_SALTTEMP) && (!salt || !temp)))
         return wfail(w, RP_E_ARG, "a column the file defines is missing");
     if (w->nrec >= 0x7FFFFFFF) return wfail(w, RP_E_RANGE, "record count limit");
     const uint64_t base = w->rec_begin + (uint64_t)w->nrec * w->recsize;
@@ -341,6 +350,8 @@ int rp_nc_write(void* nc, double model_time, const double* age, const double* lo
         else if (v.name == "WaterDepth") src = wd;
         else if (v.name == "zeta") src = zeta;
         else if (v.name == "h") src = bath;
+        else if (v.name == "salinity") src = salt;
+        else if (v.name == "temperature") src = temp;
         encode(w, v.type, src, isrc, v.per_particle ? w->numpar : 1);
         if ((rc = put_at(w, base + (v.begin - w->rec_begin), w->buf.data(), w->buf.size()))) return rc;
     }

@@ -27,7 +27,8 @@ module romspath_hip
   end type rp_config
 
   ! rp_nc_meta (include/rp_ncout.h): global attributes of the trajectory file
-  integer(c_int32_t), parameter, public :: RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4
+  integer(c_int32_t), parameter, public :: RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4, &
+                                           RP_NC_SALTTEMP = 8
   type, bind(C), public :: rp_nc_meta
     type(c_ptr) :: time_units, run_name
     integer(c_int32_t) :: idt
@@ -53,7 +54,7 @@ module romspath_hip
             rp_destroy, rp_set_tuning, rp_set_scenarios, rp_step_scenario, rp_step_batch, &
             rp_set_perturbation, rp_adjoint_record, rp_adjoint, &
             rp_set_lonlat, rp_sample_lonlat, rp_nc_create, rp_nc_write, rp_nc_close, &
-            rp_set_grid_angle, rp_locate_particles, &
+            rp_set_grid_angle, rp_locate_particles, rp_set_hydro_tracers, rp_sample_tracers, &
             rp_error_text
 
   interface
@@ -263,13 +264,32 @@ module romspath_hip
 
     ! writeNetCDF (ROMSPath.f90:1870-2157): one record
     integer(c_int) function rp_nc_write(nc, model_time, age, lon, lat, zp, status, gid, wd, zeta, &
-        bath) bind(C, name="rp_nc_write")
+        bath, salt, temp) bind(C, name="rp_nc_write")
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: nc
       real(c_double), value :: model_time
       real(c_double), intent(in) :: age(*), lon(*), lat(*), zp(*), status(*)
       integer(c_int32_t), intent(in) :: gid(*)
-      type(c_ptr), value :: wd, zeta, bath
+      type(c_ptr), value :: wd, zeta, bath, salt, temp
+    end function
+
+    ! SaltTempOn: salt/temp of slot tind (updateHydro) and P_salt/P_temp at the particles
+    integer(c_int) function rp_set_hydro_tracers(ctx, ng, slot, salt, temp) &
+        bind(C, name="rp_set_hydro_tracers")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ng, slot
+      real(c_double), intent(in) :: salt(*), temp(*)
+    end function
+
+    integer(c_int) function rp_sample_tracers(ctx, n, ex, ix, t_b, t_c, t_f, salt, temp) &
+        bind(C, name="rp_sample_tracers")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: ex(3), ix(3)
+      integer(c_int32_t), value :: t_b, t_c, t_f
+      real(c_double), intent(out) :: salt(*), temp(*)
     end function
 
     integer(c_int) function rp_nc_close(nc) bind(C, name="rp_nc_close")

@@ -254,7 +254,8 @@ contains
     if (rc /= 0) call die('rp_get_particles', rc)
     rc = rp_sample_lonlat(ctx, nl, plon, plat)
     if (rc /= 0) call die('rp_sample_lonlat', rc)
-    rc = rp_nc_write(nc, dble(time), page, plon, plat, pz, pst, pgid, c_null_ptr, c_null_ptr, c_null_ptr)
+    rc = rp_nc_write(nc, dble(time), page, plon, plat, pz, pst, pgid, c_null_ptr, c_null_ptr, c_null_ptr, &
+                     c_null_ptr, c_null_ptr)
     if (rc /= 0) call die('rp_nc_write', rc)
   end subroutine data_output
 

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import abi
 
-FLAGS = {"WaterDepth": 1, "zeta": 2, "h": 4}
+FLAGS = {"WaterDepth": 1, "zeta": 2, "h": 4, "salttemp": 8}
 
 
 def meta_of(prm, time_units="seconds since model start", run_name=None, behavior=0):
@@ -54,17 +54,18 @@ class TrajectoryFile:
             raise abi.RpError("rp_nc_create", rc, self._L.rp_nc_error(None).decode())
 
     def write(self, model_time, p, lon, lat):
-        """p: the dict of rp_get_particles (release order); lon, lat likewise."""
+        """p: the dict of rp_get_particles (release order), plus "salt" and "temp"
+        when the file has them; lon, lat likewise."""
         cols = [np.ascontiguousarray(p[k], dtype=np.float64) for k in ("age", "z", "status")]
         lo = np.ascontiguousarray(lon, dtype=np.float64)
         la = np.ascontiguousarray(lat, dtype=np.float64)
         gid = np.ascontiguousarray(p["gid"], dtype=np.int32)
         opt = {k: None if p.get(k) is None else np.ascontiguousarray(p[k], dtype=np.float64)
-               for k in ("wd", "zeta", "bath")}
+               for k in ("wd", "zeta", "bath", "salt", "temp")}
         d = lambda a: None if a is None else a.ctypes.data_as(abi.PD)
         rc = self._L.rp_nc_write(self.h, float(model_time), d(cols[0]), d(lo), d(la), d(cols[1]),
                                  d(cols[2]), gid.ctypes.data_as(abi.PI32), d(opt["wd"]),
-                                 d(opt["zeta"]), d(opt["bath"]))
+                                 d(opt["zeta"]), d(opt["bath"]), d(opt["salt"]), d(opt["temp"]))
         if rc != 0:
             raise abi.RpError("rp_nc_write", rc, self._L.rp_nc_error(self.h).decode())
 

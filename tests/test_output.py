@@ -13,13 +13,14 @@ from romspath_amd.synth import RunParams
 
 
 def _cols(rng, n, ngrid):
-    p = {k: rng.standard_normal(n) * 100 for k in ("age", "z", "status", "wd", "zeta", "bath")}
+    p = {k: rng.standard_normal(n) * 100
+         for k in ("age", "z", "status", "wd", "zeta", "bath", "salt", "temp")}
     p["status"] = rng.choice([9.0, 0.0, 1.0, 1.1, 1.2, -8.0, -9.0], n)
     p["gid"] = rng.integers(1, ngrid + 1, n).astype(np.int32)
     return p, rng.uniform(-180, 180, n), rng.uniform(-90, 90, n)
 
 
-@pytest.mark.parametrize("ngrid,extra", [(1, ()), (2, ("WaterDepth", "zeta", "h"))])
+@pytest.mark.parametrize("ngrid,extra", [(1, ()), (2, ("WaterDepth", "zeta", "h", "salttemp"))])
 def test_records_read_back(tmp_path, ngrid, extra):
     rng = np.random.default_rng(ngrid)
     n = 1001
@@ -40,7 +41,8 @@ def test_records_read_back(tmp_path, ngrid, extra):
         assert f.dimensions == {"numpar": n, "time": None}
         names = ["model_time", "dob", "age", "lon", "lat", "zp", "status"]
         names += ["GID"] if ngrid > 1 else []
-        names += list(extra)
+        names += ["salinity", "temperature"] if "salttemp" in extra else []
+        names += [e for e in extra if e != "salttemp"]
         assert list(f.variables) == names
         v = f.variables
         assert v["model_time"].dimensions == ("time",)
@@ -63,6 +65,8 @@ def test_records_read_back(tmp_path, ngrid, extra):
                 assert np.array_equal(v["WaterDepth"][r], p["wd"].astype(np.float32))
                 assert np.array_equal(v["zeta"][r], p["zeta"].astype(np.float32))
                 assert np.array_equal(v["h"][r], p["bath"].astype(np.float32))
+                assert np.array_equal(v["salinity"][r], p["salt"].astype(np.float32))
+                assert np.array_equal(v["temperature"][r], p["temp"].astype(np.float32))
         a = f._attributes
         assert a["adv_time_step"] == 120 and a["Vertical_Turb"] == b"True"
         assert a["seed"] == 5 and a["turb_time_step"] == 12.0
