@@ -141,6 +141,27 @@ int rp_set_hydro_tracers(void *ctx, int32_t ng, int32_t slot, const double *salt
 int rp_sample_tracers(void *ctx, int64_t n, const double ex[3], const double ix[3], int32_t t_b,
                       int32_t t_c, int32_t t_f, double *salt, double *temp);
 
+/* WriteBottom (ROMSPath.f90:260-266, 353-355, 935-951, 2056-2078).
+ * rp_set_hydro_bstress stores bustr (xi_u, eta_u) and bvstr (xi_v, eta_v)
+ * of time slot 1..3; asynchronous like rp_set_hydro_slot.
+ * rp_bottom_begin (after rp_set_particles, before the first step): P_HOB =
+ * 9999, P_bustr = P_bvstr = 0 for every particle.
+ * rp_bottom_update (after each rp_step, with that step's ex, ix and slots;
+ * queued on the stream): for each particle that took the step (its age
+ * changed), btemp = -tdepth + Z, P_HOB = min(btemp, P_HOB), and where btemp
+ * equals the new P_HOB, P_bustr / P_bvstr = polintd at ix(2) of getInterpStr
+ * (interpolation_module.f90:964-1174) over slots t_b, t_c, t_f.
+ * rp_bottom_get copies P_HOB, P_bustr, P_bvstr out in release order and,
+ * when reset_hob is set, puts P_HOB back to 9999 as writeNetCDF does after
+ * writing it; synchronises the stream. */
+int rp_set_hydro_bstress(void *ctx, int32_t ng, int32_t slot, const double *bustr,
+                         const double *bvstr);
+int rp_bottom_begin(void *ctx);
+int rp_bottom_update(void *ctx, const double ex[3], const double ix[3], int32_t t_b, int32_t t_c,
+                     int32_t t_f);
+int rp_bottom_get(void *ctx, int64_t n, double *hob, double *bustr, double *bvstr,
+                  int32_t reset_hob);
+
 /* ini_ROMSPath's placement of the released particles (ROMSPath.f90:322-403):
  * for grid ng = 1..Ngrid in order, LL2ij (interpolation_module.f90:110-225:
  * bisection of the index box with inbox/inside, then the Cartesian linear

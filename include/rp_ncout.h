@@ -15,6 +15,7 @@
  *   status(time, numpar)        float   par(:,pStatus)
  *   GID(time, numpar)           float   par(:,pGID), only when Ngrid > 1
  *   salinity, temperature       float   P_salt, P_temp, when asked for (SaltTempOn)
+ *   HOB, bustr, bvstr           float   P_HOB, P_bustr, P_bvstr, when asked for (WriteBottom)
  *   WaterDepth, zeta, h         float   par(:,pWD/pZeta/pBath), when asked for
  * with the reference's long_name / units / field attributes and the global
  * attributes of createNetCDF that apply to the hot path.  Values are rounded
@@ -34,7 +35,7 @@
 extern "C" {
 #endif
 
-enum { RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4, RP_NC_SALTTEMP = 8 };
+enum { RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4, RP_NC_SALTTEMP = 8, RP_NC_BOTTOM = 16 };
 
 /* Global attributes (createNetCDF, ROMSPath.f90:1650-1760). */
 typedef struct {
@@ -56,12 +57,14 @@ int rp_nc_create(void **nc, const char *path, int64_t numpar, int32_t ngrid, int
                  const rp_nc_meta *meta, const double *dob);
 
 /* writeNetCDF (ROMSPath.f90:1870-2157): appends one time record.
- * model_time = DBLE(time) + reftime.  gid, wd, zeta, bath, salt and temp
- * may be NULL when the file has no such variable. */
+ * model_time = DBLE(time) + reftime.  gid, wd, zeta, bath, salt, temp, hob,
+ * bustr and bvstr may be NULL when the file has no such variable.  The
+ * reference resets P_HOB to 9999 after writing it (ROMSPath.f90:2063-2065);
+ * here that is rp_bottom_get's reset flag (rp_abi.h). */
 int rp_nc_write(void *nc, double model_time, const double *age, const double *lon,
                 const double *lat, const double *zp, const double *status, const int32_t *gid,
                 const double *wd, const double *zeta, const double *bath, const double *salt,
-                const double *temp);
+                const double *temp, const double *hob, const double *bustr, const double *bvstr);
 
 /* Records written so far. */
 int64_t rp_nc_records(void *nc);

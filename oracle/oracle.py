@@ -30,7 +30,7 @@ class OgGrid(C.Structure):
                 ("mask_rho", PI32), ("mask_u", PI32), ("mask_v", PI32),
                 ("scl", (C.c_double * 2) * MAXGRID), ("off", (C.c_double * 2) * MAXGRID),
                 ("zeta", PD * 3), ("u", PD * 3), ("v", PD * 3), ("w", PD * 3), ("aks", PD * 3),
-                ("salt", PD * 3), ("temp", PD * 3)]
+                ("salt", PD * 3), ("temp", PD * 3), ("bustr", PD * 3), ("bvstr", PD * 3)]
 
 
 class OgParams(C.Structure):
@@ -99,6 +99,11 @@ def lib():
         L.og_sample_tracers.argtypes = [C.POINTER(OgGrid), C.POINTER(OgTime), C.c_int64, PD, PD, PD,
                                         PI32, PD, PD]
         L.og_sample_tracers.restype = None
+        L.og_interp_str.argtypes = [C.POINTER(OgGrid), C.c_int, C.c_int, C.c_double, C.c_double]
+        L.og_interp_str.restype = C.c_double
+        L.og_bottom_update.argtypes = [C.POINTER(OgGrid), C.POINTER(OgTime), C.c_int64, PD, PD, PD,
+                                       PI32, PD, PD, PD, PD, PD, PD]
+        L.og_bottom_update.restype = None
         L.og_set_threads.argtypes = [C.c_int]
         L.og_set_threads.restype = None
         L.og_update_particles.restype = C.c_int64
@@ -159,7 +164,7 @@ class OracleGrid:
                 a = np.ascontiguousarray(a + rec["sv"])
             keep[name] = a
             getattr(self.s, name)[slot - 1] = dptr(a)
-        for name in ("salt", "temp"):          # SaltTempOn fields, when the record has them
+        for name in ("salt", "temp", "bustr", "bvstr"):   # optional fields, when the record has them
             if name in rec:
                 a = np.ascontiguousarray(rec[name], dtype=np.float64)
                 keep[name] = a
@@ -404,3 +409,12 @@ def sample_tracers(grids, ex, ix, tb, tc, tf, x, y, z, gid):
     lib().og_sample_tracers(arr, C.byref(T), x.size, dptr(x), dptr(y), dptr(z), iptr(gid),
                             dptr(salt), dptr(temp))
     return salt, temp
+
+
+def bottom_update(grids, T, state, last_age, hob, bustr, bvstr):
+    """WriteBottom's running minimum height above the bed and the stresses
+    there (ROMSPath.f90:935-951) for the particles that took the last step."""
+    arr = (OgGrid * len(grids))(*[g.s for g in grids])
+    lib().og_bottom_update(arr, C.byref(T), state.n, dptr(state.x), dptr(state.y), dptr(state.z),
+                           iptr(state.gid), dptr(state.age), dptr(state.bath), dptr(last_age),
+                           dptr(hob), dptr(bustr), dptr(bvstr))

@@ -600,6 +600,104 @@ void og_sample_tracers(const og_grid *grids, const og_time *T, int64_t n, const 
     }
 }
 
+/* getInterpStr (interpolation_module.f90:964-1174) without WETDRY: var 0
+ * bustr at u points, var 1 bvstr at v points, slot 1..3.  The Jnode < 1
+ * branch of bvstr is unreachable after the reference's clamp and is left
+ * out.  Node indices are clamped into the arrays for the reads only (no
+ * effect for a particle inside the grid; the reference would read outside). */
+static int og_clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
+double og_interp_str(const og_grid *g, int var, int slot, double Ipar, double Jpar)
+{
+    const int s = slot - 1;
+    int Inode = (int)floor(Ipar), Jnode = (int)floor(Jpar);
+    double X = Ipar - Inode, Y = Jpar - Jnode;
+    double cff[4], v[4], m[4];
+    if (var == 0) {
+        const int xiu = g->L - 1;
+        const double *f = g->bustr[s];
+#define BU(a, i, j) ((double)(a)[((i) - 1) + (int64_t)xiu * ((j) - 1)])
+        double uvI = Ipar - 0.5;
+        Inode = (int)floor(uvI);
+        X = uvI - Inode;
+        const int j0 = og_clampi(Jnode, 1, g->M), j1 = og_clampi(Jnode + 1, 1, g->M);
+        if (Inode >= xiu) {
+            v[0] = BU(f, xiu, j0); v[1] = BU(f, xiu, j0);
+            v[2] = BU(f, xiu, j1); v[3] = BU(f, xiu, j1);
+            m[0] = BU(g->mask_u, xiu, j0); m[1] = 0; m[2] = 0; m[3] = BU(g->mask_u, xiu, j1);
+        } else if (Inode < 1) {
+            v[0] = BU(f, 1, j0); v[1] = BU(f, 1, j0);
+            v[2] = BU(f, 1, j1); v[3] = BU(f, 1, j1);
+            m[0] = 0; m[1] = BU(g->mask_u, 1, j0); m[2] = BU(g->mask_u, 1, j1); m[3] = 0;
+        } else {
+            v[0] = BU(f, Inode, j0); v[1] = BU(f, Inode + 1, j0);
+            v[2] = BU(f, Inode + 1, j1); v[3] = BU(f, Inode, j1);
+            m[0] = BU(g->mask_u, Inode, j0); m[1] = BU(g->mask_u, Inode + 1, j0);
+            m[2] = BU(g->mask_u, Inode + 1, j1); m[3] = BU(g->mask_u, Inode, j1);
+        }
+#undef BU
+    } else {
+        const int etav = g->M - 1;
+        const double *f = g->bvstr[s];
+#define BV(a, i, j) ((double)(a)[((i) - 1) + (int64_t)g->L * ((j) - 1)])
+        double uvJ = Jpar - 0.5;
+        Jnode = (int)floor(uvJ);
+        if (Jnode < 1) Jnode = 1;
+        if (Jnode > etav) Inode = etav;      /* interpolation_module.f90:1089 assigns Inode */
+        Y = uvJ - Jnode;
+        const int i0 = og_clampi(Inode, 1, g->L), i1 = og_clampi(Inode + 1, 1, g->L);
+        if (Jnode >= etav) {
+            v[0] = BV(f, i0, etav); v[1] = BV(f, i1, etav);
+            v[2] = BV(f, i1, etav); v[3] = BV(f, i0, etav);
+            m[0] = BV(g->mask_v, i0, etav); m[1] = BV(g->mask_v, i1, etav); m[2] = 0; m[3] = 0;
+        } else {
+            v[0] = BV(f, i0, Jnode); v[1] = BV(f, i1, Jnode);
+            v[2] = BV(f, i1, Jnode + 1); v[3] = BV(f, i0, Jnode + 1);
+            m[0] = BV(g->mask_v, i0, Jnode); m[1] = BV(g->mask_v, i1, Jnode);
+            m[2] = BV(g->mask_v, i1, Jnode + 1); m[3] = BV(g->mask_v, i0, Jnode + 1);
+        }
+#undef BV
+    }
+    cff[0] = (1 - X) * (1 - Y); cff[1] = X * (1 - Y); cff[2] = X * Y; cff[3] = (1 - X) * Y;
+    double nwater = m[0] + m[1] + m[2] + m[3];
+    if (nwater < 4.0 && nwater > 0.0) {
+        for (int c = 0; c < 4; c++)
+            if (m[c] == 0.0) v[c] = 0.0;
+    } else if (nwater == 0.0) {
+        for (int c = 0; c < 4; c++) { cff[c] = 0.0; v[c] = 0.0; }
+    }
+    return cff[0] * v[0] + cff[1] * v[1] + cff[2] * v[2] + cff[3] * v[3];
+}
+
+/* The WriteBottom block of update_particles (ROMSPath.f90:935-951) for the
+ * particles that took the last step (their age changed): height above the
+ * bed from the post-step depth (bath = -tdepth), its running minimum, and
+ * bustr/bvstr (polintd at ix(2)) where the minimum was just reached. */
+void og_bottom_update(const og_grid *grids, const og_time *T, int64_t n, const double *x,
+                      const double *y, const double *z, const int32_t *gid, const double *age,
+                      const double *bath, double *last_age, double *hob, double *bustr,
+                      double *bvstr)
+{
+    for (int64_t q = 0; q < n; q++) {
+        if (age[q] == last_age[q]) continue;
+        last_age[q] = age[q];
+        const og_grid *g = &grids[gid[q] - 1];
+        const double btemp = bath[q] + z[q];
+        hob[q] = fmin(btemp, hob[q]);
+        if (btemp == hob[q]) {
+            double ey[3];
+            ey[0] = og_interp_str(g, 0, T->tb, x[q], y[q]);
+            ey[1] = og_interp_str(g, 0, T->tc, x[q], y[q]);
+            ey[2] = og_interp_str(g, 0, T->tf, x[q], y[q]);
+            bustr[q] = og_polintd(T->ex, ey, T->ix[1]);
+            ey[0] = og_interp_str(g, 1, T->tb, x[q], y[q]);
+            ey[1] = og_interp_str(g, 1, T->tc, x[q], y[q]);
+            ey[2] = og_interp_str(g, 1, T->tf, x[q], y[q]);
+            bvstr[q] = og_polintd(T->ex, ey, T->ix[1]);
+        }
+    }
+}
+
 /* boundary_module.f90:43-143 */
 void og_bounds(const og_grid *g, double X, double Y, int *nmask, int *ingrid, int *obound)
 {

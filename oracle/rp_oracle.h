@@ -42,6 +42,8 @@ typedef struct {
     const double *aks[3];        /* L*M*(N+1) */
     const double *salt[3];       /* L*M*N, may be NULL (SaltTempOn off) */
     const double *temp[3];       /* L*M*N */
+    const double *bustr[3];      /* (L-1)*M, may be NULL (WriteBottom off) */
+    const double *bvstr[3];      /* L*(M-1) */
 } og_grid;
 
 typedef struct {
@@ -95,6 +97,11 @@ double og_vturb(const og_grid *g, const og_params *P, const og_time *T, double X
  * columns 0,1 feed HTurb (x then y), columns 2.. feed VTurb's sub-steps.
  * Returns the number of particles that took the step (the active count). */
 void og_set_threads(int n);
+double og_interp_str(const og_grid *g, int var, int slot, double Ipar, double Jpar);
+void og_bottom_update(const og_grid *grids, const og_time *T, int64_t n, const double *x,
+                      const double *y, const double *z, const int32_t *gid, const double *age,
+                      const double *bath, double *last_age, double *hob, double *bustr,
+                      double *bvstr);
 void og_sample_tracers(const og_grid *grids, const og_time *T, int64_t n, const double *x,
                        const double *y, const double *z, const int32_t *gid, double *salt,
                        double *temp);

@@ -32,6 +32,7 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_set_perturbation", "rp_adjoint_record", "rp_adjoint",
            "rp_set_lonlat", "rp_sample_lonlat", "rp_set_grid_angle", "rp_locate_particles",
            "rp_set_hydro_tracers", "rp_sample_tracers",
+           "rp_set_hydro_bstress", "rp_bottom_begin", "rp_bottom_update", "rp_bottom_get",
            "rp_nc_create", "rp_nc_write", "rp_nc_records", "rp_nc_error", "rp_nc_close",
            "rp_last_error", "rp_destroy"]
 # entry points whose result is not an int status
@@ -107,11 +108,15 @@ def lib():
         L.rp_set_grid_angle.argtypes = [vp, C.c_int32, C.c_int32, PD]
         L.rp_set_hydro_tracers.argtypes = [vp, C.c_int32, C.c_int32, PD, PD]
         L.rp_sample_tracers.argtypes = [vp, C.c_int64, PD, PD, C.c_int32, C.c_int32, C.c_int32, PD, PD]
+        L.rp_set_hydro_bstress.argtypes = [vp, C.c_int32, C.c_int32, PD, PD]
+        L.rp_bottom_begin.argtypes = [vp]
+        L.rp_bottom_update.argtypes = [vp, PD, PD, C.c_int32, C.c_int32, C.c_int32]
+        L.rp_bottom_get.argtypes = [vp, C.c_int64, PD, PD, PD, C.c_int32]
         L.rp_locate_particles.argtypes = [vp, C.c_int64, PD, PD, PD, C.c_int32, PD, PD, PD, PI32, PU8]
         L.rp_sample_lonlat.argtypes = [vp, C.c_int64, PD, PD]
         L.rp_nc_create.argtypes = [C.POINTER(vp), C.c_char_p, C.c_int64, C.c_int32, C.c_int32,
                                    C.POINTER(RpNcMeta), PD]
-        L.rp_nc_write.argtypes = [vp, C.c_double, PD, PD, PD, PD, PD, PI32, PD, PD, PD, PD, PD]
+        L.rp_nc_write.argtypes = [vp, C.c_double, PD, PD, PD, PD, PD, PI32, PD, PD, PD, PD, PD, PD, PD, PD]
         L.rp_nc_records.argtypes = [vp]
         L.rp_nc_error.argtypes = [vp]
         L.rp_nc_close.argtypes = [vp]
@@ -316,6 +321,25 @@ class Context:
         self._chk("rp_sample_tracers", self._L.rp_sample_tracers(
             self.h, self.n, exa, ixa, tb, tc, tf, _d(salt), _d(temp)))
         return salt, temp
+
+    def set_hydro_bstress(self, ng, slot, bustr, bvstr):
+        bu, bv = _f64(bustr), _f64(bvstr)
+        self._chk("rp_set_hydro_bstress", self._L.rp_set_hydro_bstress(self.h, ng, slot, _d(bu), _d(bv)))
+
+    def bottom_begin(self):
+        self._chk("rp_bottom_begin", self._L.rp_bottom_begin(self.h))
+
+    def bottom_update(self, ex, ix, tb, tc, tf):
+        exa = (C.c_double * 3)(*ex)
+        ixa = (C.c_double * 3)(*ix)
+        self._chk("rp_bottom_update", self._L.rp_bottom_update(self.h, exa, ixa, tb, tc, tf))
+
+    def bottom_get(self, reset_hob=False):
+        """P_HOB, P_bustr, P_bvstr in release order."""
+        hob, bu, bv = np.empty(self.n), np.empty(self.n), np.empty(self.n)
+        self._chk("rp_bottom_get", self._L.rp_bottom_get(self.h, self.n, _d(hob), _d(bu), _d(bv),
+                                                         int(bool(reset_hob))))
+        return hob, bu, bv
 
     def set_grid_angle(self, ng, spherical, angle=None):
         a = None if angle is None else _f64(angle)

@@ -28,6 +28,8 @@ struct GridDev {
     double* angle = nullptr;                  // grid angle (spherical grids), may stay null
     void *salt = nullptr, *temp = nullptr;    // SaltTempOn fields, slot-interleaved (L, M, N)
     bool tracer_set[3] = {false, false, false};
+    void *bustr = nullptr, *bvstr = nullptr;  // WriteBottom fields, slot-interleaved (L-1, M), (L, M-1)
+    bool bstress_set[3] = {false, false, false};
     double scl[kMaxGrid][2], off[kMaxGrid][2];
     bool set = false;
     bool slot_set[3] = {false, false, false};
@@ -75,7 +77,10 @@ struct Ctx {
         hipEvent_t packed = nullptr;    // on stream: the pack kernel has read it
         bool used = false;
     };
-    FieldStage fstage[kMaxGrid][7];   // zeta, u, v, w, aks, salt, temp
+    FieldStage fstage[kMaxGrid][9];   // zeta, u, v, w, aks, salt, temp, bustr, bvstr
+    // WriteBottom (rp_bottom_begin): release order, bot_n particles
+    double *bot_last_age = nullptr, *bot_hob = nullptr, *bot_bustr = nullptr, *bot_bvstr = nullptr;
+    int64_t bot_n = -1;
     // VTurb scratch
     double* scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -284,6 +289,8 @@ void free_grid(GridDev& g)
     dfree(g.lon); dfree(g.lat); dfree(g.angle);
     dfree(g.salt); dfree(g.temp);
     for (auto& t : g.tracer_set) t = false;
+    dfree(g.bustr); dfree(g.bvstr);
+    for (auto& t : g.bstress_set) t = false;
     g.set = false;
 }
 
@@ -475,6 +482,19 @@ void fill_tab(const Ctx* c, DGrid tab[kMaxGrid])
     }
 }
 
+// the device grid table (c->dgrids) brought up to date with the grids
+int refresh_tab(Ctx* c)
+{
+    DGrid tab[kMaxGrid];
+    fill_tab(c, tab);
+    if (std::memcmp(tab, c->tab_host, sizeof(tab)) != 0) {
+        std::memcpy(c->tab_host, tab, sizeof(tab));
+        HIPCHK(c, hipMemcpyAsync(c->dgrids, c->tab_host, sizeof(tab), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return RP_OK;
+}
+
 int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, int t_c, int t_f,
               int64_t step_index, int want_diag, int64_t q0, int64_t cnt, StepPlan& P)
 {
@@ -605,17 +625,7 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
             }
         }
     }
-    {
-        DGrid tab[kMaxGrid];
-        fill_tab(c, tab);
-        if (std::memcmp(tab, c->tab_host, sizeof(tab)) != 0) {
-            std::memcpy(c->tab_host, tab, sizeof(tab));
-            HIPCHK(c, hipMemcpyAsync(c->dgrids, c->tab_host, sizeof(tab), hipMemcpyHostToDevice,
-                                     c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-        }
-    }
-    return RP_OK;
+    return refresh_tab(c);
 }
 
 // events for one timed entry, taken from the pool; the entry goes back to
@@ -1013,6 +1023,7 @@ int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, con
     rc |= dalloc(c, &c->mid, 4 * (size_t)n);
     if (rc) return RP_E_ALLOC;
     c->n = n;
+    c->bot_n = -1;
     c->steps_done = 0;
     c->ranges_intact = true;
     free_recording(c);
@@ -1519,6 +1530,126 @@ int rp_sample_tracers(void* ctx, int64_t n, const double ex[3], const double ix[
     return RP_OK;
 }
 
+int rp_set_hydro_bstress(void* ctx, int32_t ng, int32_t slot, const double* bustr, const double* bvstr)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (ng < 1 || ng > c->cfg.ngrid || !c->grids[ng - 1].set)
+        return fail(c, RP_E_STATE, "rp_set_hydro_bstress: grid not set");
+    if (slot < 1 || slot > 3) return fail(c, RP_E_ARG, "slot must be 1..3");
+    if (!bustr || !bvstr) return fail(c, RP_E_ARG, "null field");
+    GridDev& g = c->grids[ng - 1];
+    const size_t nu = (size_t)(g.L - 1) * g.M, nv = (size_t)g.L * (g.M - 1);
+    int rc;
+    if (!g.bustr && ((rc = dalloc(c, (char**)&g.bustr, nu * 4 * c->fbytes)) ||
+                     (rc = dalloc(c, (char**)&g.bvstr, nv * 4 * c->fbytes))))
+        return rc;
+    const int gi = ng - 1, s = slot - 1;
+    if ((rc = upload_field(c, gi, 7, g.bustr, bustr, nullptr, g.L - 1, g.M, 1, s)) ||
+        (rc = upload_field(c, gi, 8, g.bvstr, bvstr, nullptr, g.L, g.M - 1, 1, s)))
+        return rc;
+    g.bstress_set[s] = true;
+    return RP_OK;
+}
+
+namespace {
+__global__ void bottom_init(int64_t n, const double* __restrict__ age, const uint32_t* __restrict__ id,
+                            double* last_age, double* hob, double* bu, double* bv)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = id[q];
+        last_age[r] = age[q];
+        hob[r] = 9999.0;
+        bu[r] = 0.0;
+        bv[r] = 0.0;
+    }
+}
+__global__ void bottom_reset_hob(int64_t n, double* hob)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x)
+        hob[q] = 9999.0;
+}
+}  // namespace
+
+int rp_bottom_begin(void* ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    const int64_t n = c->n;
+    int rc;
+    if ((rc = dalloc(c, &c->bot_last_age, (size_t)n)) || (rc = dalloc(c, &c->bot_hob, (size_t)n)) ||
+        (rc = dalloc(c, &c->bot_bustr, (size_t)n)) || (rc = dalloc(c, &c->bot_bvstr, (size_t)n)))
+        return rc;
+    if (n > 0) {
+        const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->ncu * 16);
+        hipLaunchKernelGGL(bottom_init, dim3(grid), dim3(256), 0, c->stream, n, c->age, c->id,
+                           c->bot_last_age, c->bot_hob, c->bot_bustr, c->bot_bvstr);
+        HIPCHK(c, hipGetLastError());
+    }
+    c->bot_n = n;
+    return RP_OK;
+}
+
+int rp_bottom_update(void* ctx, const double ex[3], const double ix[3], int32_t t_b, int32_t t_c,
+                     int32_t t_f)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !ex || !ix) return RP_E_ARG;
+    if (c->bot_n != c->n) return fail(c, RP_E_STATE, "rp_bottom_update: rp_bottom_begin not called for these particles");
+    if (t_b < 1 || t_b > 3 || t_c < 1 || t_c > 3 || t_f < 1 || t_f > 3)
+        return fail(c, RP_E_ARG, "slot numbers must be 1..3");
+    BottomArgs A;
+    std::memset(&A, 0, sizeof(A));
+    for (int i = 0; i < c->cfg.ngrid; i++) {
+        const GridDev& g = c->grids[i];
+        if (!g.set) return fail(c, RP_E_STATE, "rp_bottom_update: grid not set");
+        if (!g.bstress_set[t_b - 1] || !g.bstress_set[t_c - 1] || !g.bstress_set[t_f - 1])
+            return fail(c, RP_E_STATE, "rp_bottom_update: bustr/bvstr slot not loaded");
+        A.bustr[i] = g.bustr;
+        A.bvstr[i] = g.bvstr;
+    }
+    if (c->n == 0) return RP_OK;
+    A.slots.s[0] = t_b - 1; A.slots.s[1] = t_c - 1; A.slots.s[2] = t_f - 1;
+    for (int k = 0; k < 3; k++) A.ex[k] = ex[k];
+    A.ix2 = ix[1];
+    A.n = c->n;
+    A.x = c->x; A.y = c->y; A.z = c->z; A.age = c->age; A.bath = c->bath;
+    A.gid = c->gid; A.id = c->id;
+    A.last_age = c->bot_last_age; A.hob = c->bot_hob;
+    A.bustr_out = c->bot_bustr; A.bvstr_out = c->bot_bvstr;
+    int rc;
+    if ((rc = refresh_tab(c))) return rc;
+    const DGrid* dtab = c->dgrids;
+    const int grid = (int)std::min<int64_t>((c->n + 255) / 256, (int64_t)c->ncu * 16);
+    if (c->fbytes == 4) hipLaunchKernelGGL(bottom_kernel<float>, dim3(grid), dim3(256), 0, c->stream, dtab, A);
+    else hipLaunchKernelGGL(bottom_kernel<double>, dim3(grid), dim3(256), 0, c->stream, dtab, A);
+    HIPCHK(c, hipGetLastError());
+    return RP_OK;
+}
+
+int rp_bottom_get(void* ctx, int64_t n, double* hob, double* bustr, double* bvstr, int32_t reset_hob)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (c->bot_n != c->n) return fail(c, RP_E_STATE, "rp_bottom_get: rp_bottom_begin not called for these particles");
+    if (n != c->n) return fail(c, RP_E_ARG, "particle count differs from rp_set_particles");
+    if (n > 0 && (!hob || !bustr || !bvstr)) return fail(c, RP_E_ARG, "null output");
+    if (n == 0) return RP_OK;
+    hipError_t e = hipMemcpyAsync(hob, c->bot_hob, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(bustr, c->bot_bustr, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(bvstr, c->bot_bvstr, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && reset_hob) {
+        const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->ncu * 16);
+        hipLaunchKernelGGL(bottom_reset_hob, dim3(grid), dim3(256), 0, c->stream, n, c->bot_hob);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, RP_E_HIP, std::string("rp_bottom_get: ") + hipGetErrorString(e));
+    return RP_OK;
+}
+
 int rp_set_grid_angle(void* ctx, int32_t ng, int32_t spherical, const double* angle)
 {
     Ctx* c = (Ctx*)ctx;
@@ -1739,6 +1870,7 @@ int rp_destroy(void* ctx)
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->age); dfree(c->dob);
     dfree(c->wd); dfree(c->zeta); dfree(c->bath);
     dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
+    dfree(c->bot_last_age); dfree(c->bot_hob); dfree(c->bot_bustr); dfree(c->bot_bvstr);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     free_recording(c);
     dfree(c->adj.active);

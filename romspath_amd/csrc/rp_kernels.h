@@ -108,6 +108,23 @@ struct TracerArgs {
 template <typename F>
 __global__ void tracer_kernel(const DGrid* __restrict__ grids, TracerArgs A);
 
+// WriteBottom (ROMSPath.f90:935-951): the running minimum height above the
+// bed and the bottom stresses where it was reached; the per-particle columns
+// are in release order (indexed through id)
+struct BottomArgs {
+    const void* bustr[kMaxGrid];   // slot-interleaved, field precision, (L-1, M)
+    const void* bvstr[kMaxGrid];   // (L, M-1)
+    SlotSel slots;
+    double ex[3], ix2;
+    int64_t n;
+    const double *x, *y, *z, *age, *bath;
+    const int8_t* gid;
+    const uint32_t* id;
+    double *last_age, *hob, *bustr_out, *bvstr_out;
+};
+template <typename F>
+__global__ void bottom_kernel(const DGrid* __restrict__ grids, BottomArgs A);
+
 // tables of the FP32-field form's p = 0 spline sweep; once per device, 0 on success
 int upload_p0_tables();
 

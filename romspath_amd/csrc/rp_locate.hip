@@ -228,6 +228,111 @@ __global__ void __launch_bounds__(256) tracer_kernel(const DGrid* __restrict__ g
 template __global__ void tracer_kernel<float>(const DGrid*, TracerArgs);
 template __global__ void tracer_kernel<double>(const DGrid*, TracerArgs);
 
+// getInterpStr (interpolation_module.f90:964-1174, without WETDRY) at slot s:
+// var 0 bustr on u points (I shifted by 1/2), var 1 bvstr on v points (J
+// shifted, Jnode clamped to >= 1, and the reference's Inode = eta_v when
+// Jnode > eta_v kept).  Masked corners give 0, an all-land cell 0.  Node
+// indices are clamped into the arrays for the reads only, which changes
+// nothing for a particle inside the grid.
+template <typename F>
+__device__ double interp_str(const DGrid& g, const F* __restrict__ f, int var, int s, double Ipar,
+                             double Jpar)
+{
+    int Inode = (int)floor(Ipar), Jnode = (int)floor(Jpar);
+    double X = Ipar - Inode, Y = Jpar - Jnode;
+    double v[4], m[4];
+    if (var == 0) {
+        const int xiu = g.L - 1;
+        const double uvI = Ipar - 0.5;
+        Inode = (int)floor(uvI);
+        X = uvI - Inode;
+        const int j0 = min(max(Jnode, 1), g.M), j1 = min(max(Jnode + 1, 1), g.M);
+        auto F_ = [&](int i, int j) { return (double)f[4 * ((int64_t)(i - 1) + (int64_t)xiu * (j - 1)) + s]; };
+        auto M_ = [&](int i, int j) { return (double)g.mu[(int64_t)(i - 1) + (int64_t)xiu * (j - 1)]; };
+        if (Inode >= xiu) {
+            v[0] = v[1] = F_(xiu, j0);
+            v[2] = v[3] = F_(xiu, j1);
+            m[0] = M_(xiu, j0); m[1] = 0.0; m[2] = 0.0; m[3] = M_(xiu, j1);
+        } else if (Inode < 1) {
+            v[0] = v[1] = F_(1, j0);
+            v[2] = v[3] = F_(1, j1);
+            m[0] = 0.0; m[1] = M_(1, j0); m[2] = M_(1, j1); m[3] = 0.0;
+        } else {
+            v[0] = F_(Inode, j0); v[1] = F_(Inode + 1, j0);
+            v[2] = F_(Inode + 1, j1); v[3] = F_(Inode, j1);
+            m[0] = M_(Inode, j0); m[1] = M_(Inode + 1, j0);
+            m[2] = M_(Inode + 1, j1); m[3] = M_(Inode, j1);
+        }
+    } else {
+        const int etav = g.M - 1;
+        const double uvJ = Jpar - 0.5;
+        Jnode = (int)floor(uvJ);
+        if (Jnode < 1) Jnode = 1;
+        if (Jnode > etav) Inode = etav;
+        Y = uvJ - Jnode;
+        const int i0 = min(max(Inode, 1), g.L), i1 = min(max(Inode + 1, 1), g.L);
+        auto F_ = [&](int i, int j) { return (double)f[4 * ((int64_t)(i - 1) + (int64_t)g.L * (j - 1)) + s]; };
+        auto M_ = [&](int i, int j) { return (double)g.mv[(int64_t)(i - 1) + (int64_t)g.L * (j - 1)]; };
+        if (Jnode >= etav) {
+            v[0] = v[3] = F_(i0, etav);
+            v[1] = v[2] = F_(i1, etav);
+            m[0] = M_(i0, etav); m[1] = M_(i1, etav); m[2] = 0.0; m[3] = 0.0;
+        } else {
+            v[0] = F_(i0, Jnode); v[1] = F_(i1, Jnode);
+            v[2] = F_(i1, Jnode + 1); v[3] = F_(i0, Jnode + 1);
+            m[0] = M_(i0, Jnode); m[1] = M_(i1, Jnode);
+            m[2] = M_(i1, Jnode + 1); m[3] = M_(i0, Jnode + 1);
+        }
+    }
+    double cff[4] = {(1 - X) * (1 - Y), X * (1 - Y), X * Y, (1 - X) * Y};
+    const double nwater = m[0] + m[1] + m[2] + m[3];
+    if (nwater < 4.0 && nwater > 0.0) {
+        for (int k = 0; k < 4; k++)
+            if (m[k] == 0.0) v[k] = 0.0;
+    } else if (nwater == 0.0) {
+        for (int k = 0; k < 4; k++) { cff[k] = 0.0; v[k] = 0.0; }
+    }
+    return cff[0] * v[0] + cff[1] * v[1] + cff[2] * v[2] + cff[3] * v[3];
+}
+
+// The WriteBottom block of update_particles (ROMSPath.f90:935-951) for the
+// particles that took the last step, i.e. whose age changed since the last
+// call (the loop adds idt to the age of every particle it does not skip and
+// skips none after that): btemp = -tdepth + Z from the step's depth and
+// position, P_HOB = min(btemp, P_HOB), and where btemp is the new minimum
+// P_bustr / P_bvstr = polintd over slots b, c, f of getInterpStr at ix(2).
+template <typename F>
+__global__ void __launch_bounds__(256) bottom_kernel(const DGrid* __restrict__ grids, BottomArgs A)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < A.n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = A.id[q];
+        const double age = A.age[q];
+        if (age == A.last_age[r]) continue;
+        A.last_age[r] = age;
+        const double btemp = A.bath[q] + A.z[q];
+        const double hob = fmin(btemp, A.hob[r]);
+        A.hob[r] = hob;
+        if (btemp != hob) continue;
+        const int gi = (int)A.gid[q] - 1;
+        const DGrid& g = grids[gi];
+        const double X = A.x[q], Y = A.y[q];
+        const F* bu = (const F*)A.bustr[gi];
+        const F* bv = (const F*)A.bvstr[gi];
+        const double u0 = interp_str(g, bu, 0, A.slots.s[0], X, Y);
+        const double u1 = interp_str(g, bu, 0, A.slots.s[1], X, Y);
+        const double u2 = interp_str(g, bu, 0, A.slots.s[2], X, Y);
+        A.bustr_out[r] = polintd(A.ex, u0, u1, u2, A.ix2);
+        const double v0 = interp_str(g, bv, 1, A.slots.s[0], X, Y);
+        const double v1 = interp_str(g, bv, 1, A.slots.s[1], X, Y);
+        const double v2 = interp_str(g, bv, 1, A.slots.s[2], X, Y);
+        A.bvstr_out[r] = polintd(A.ex, v0, v1, v2, A.ix2);
+    }
+}
+
+template __global__ void bottom_kernel<float>(const DGrid*, BottomArgs);
+template __global__ void bottom_kernel<double>(const DGrid*, BottomArgs);
+
 template __global__ void locate_kernel<float>(const DGrid*, LocateArgs);
 template __global__ void locate_kernel<double>(const DGrid*, LocateArgs);
 

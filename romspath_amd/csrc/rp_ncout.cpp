@@ -214,6 +214,13 @@ int rp_nc_create(void** nc, const char* path, int64_t numpar, int32_t ngrid, int
             desc("Temperature at the particle's location", "degree Celsius",
                  "temperature, scalar, series"));
     }
+    if (flags & RP_NC_BOTTOM) {
+        add("HOB", NC_FLOAT, true, true,
+            desc("Particle Minimum height above bottom since last output time step", "m",
+                 "HOB, scalar, series"));
+        add("bustr", NC_FLOAT, true, true, desc("U bottom stress at minimum HOB", "newton meter-2", nullptr));
+        add("bvstr", NC_FLOAT, true, true, desc("V bottom stress at minimum HOB", "newton meter-2", nullptr));
+    }
     if (flags & RP_NC_WATERDEPTH)
         add("WaterDepth", NC_FLOAT, true, true,
             desc("Local Water Depth at Particle Location", "m", "WaterDepth, scalar, series"));
@@ -324,14 +331,15 @@ int rp_nc_create(void** nc, const char* path, int64_t numpar, int32_t ngrid, int
 int rp_nc_write(void* nc, double model_time, const double* age, const double* lon,
                 const double* lat, const double* zp, const double* status, const int32_t* gid,
                 const double* wd, const double* zeta, const double* bath, const double* salt,
-                const double* temp)
+                const double* temp, const double* hob, const double* bustr, const double* bvstr)
 {
     Writer* w = (Writer*)nc;
     if (!w) return RP_E_ARG;
     if (!age || !lon || !lat || !zp || !status) return wfail(w, RP_E_ARG, "null particle column");
     if ((w->ngrid > 1 && !gid) || ((w->flags & RP_NC_WATERDEPTH) && !wd) ||
         ((w->flags & RP_NC_ZETA) && !zeta) || ((w->flags & RP_NC_BATH) && !bath) ||
-        ((w->flags & RP_NC_SALTTEMP) && (!salt || !temp)))
+        ((w->flags & RP_NC_SALTTEMP) && (!salt || !temp)) ||
+        ((w->flags & RP_NC_BOTTOM) && (!hob || !bustr || !bvstr)))
         return wfail(w, RP_E_ARG, "a column the file defines is missing");
     if (w->nrec >= 0x7FFFFFFF) return wfail(w, RP_E_RANGE, "record count limit");
     const uint64_t base = w->rec_begin + (uint64_t)w->nrec * w->recsize;
@@ -352,6 +360,9 @@ int rp_nc_write(void* nc, double model_time, const double* age, const double* lo
         else if (v.name == "h") src = bath;
         else if (v.name == "salinity") src = salt;
         else if (v.name == "temperature") src = temp;
+        else if (v.name == "HOB") src = hob;
+        else if (v.name == "bustr") src = bustr;
+        else if (v.name == "bvstr") src = bvstr;
         encode(w, v.type, src, isrc, v.per_particle ? w->numpar : 1);
         if ((rc = put_at(w, base + (v.begin - w->rec_begin), w->buf.data(), w->buf.size()))) return rc;
     }

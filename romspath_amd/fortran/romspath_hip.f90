@@ -28,7 +28,7 @@ module romspath_hip
 
   ! rp_nc_meta (include/rp_ncout.h): global attributes of the trajectory file
   integer(c_int32_t), parameter, public :: RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4, &
-                                           RP_NC_SALTTEMP = 8
+                                           RP_NC_SALTTEMP = 8, RP_NC_BOTTOM = 16
   type, bind(C), public :: rp_nc_meta
     type(c_ptr) :: time_units, run_name
     integer(c_int32_t) :: idt
@@ -55,6 +55,7 @@ module romspath_hip
             rp_set_perturbation, rp_adjoint_record, rp_adjoint, &
             rp_set_lonlat, rp_sample_lonlat, rp_nc_create, rp_nc_write, rp_nc_close, &
             rp_set_grid_angle, rp_locate_particles, rp_set_hydro_tracers, rp_sample_tracers, &
+            rp_set_hydro_bstress, rp_bottom_begin, rp_bottom_update, rp_bottom_get, &
             rp_error_text
 
   interface
@@ -264,13 +265,13 @@ module romspath_hip
 
     ! writeNetCDF (ROMSPath.f90:1870-2157): one record
     integer(c_int) function rp_nc_write(nc, model_time, age, lon, lat, zp, status, gid, wd, zeta, &
-        bath, salt, temp) bind(C, name="rp_nc_write")
+        bath, salt, temp, hob, bustr, bvstr) bind(C, name="rp_nc_write")
       import :: c_int, c_int32_t, c_ptr, c_double
       type(c_ptr), value :: nc
       real(c_double), value :: model_time
       real(c_double), intent(in) :: age(*), lon(*), lat(*), zp(*), status(*)
       integer(c_int32_t), intent(in) :: gid(*)
-      type(c_ptr), value :: wd, zeta, bath, salt, temp
+      type(c_ptr), value :: wd, zeta, bath, salt, temp, hob, bustr, bvstr
     end function
 
     ! SaltTempOn: salt/temp of slot tind (updateHydro) and P_salt/P_temp at the particles
@@ -290,6 +291,37 @@ module romspath_hip
       real(c_double), intent(in) :: ex(3), ix(3)
       integer(c_int32_t), value :: t_b, t_c, t_f
       real(c_double), intent(out) :: salt(*), temp(*)
+    end function
+
+    ! WriteBottom: bustr/bvstr of slot tind, P_HOB/P_bustr/P_bvstr at the particles
+    integer(c_int) function rp_set_hydro_bstress(ctx, ng, slot, bustr, bvstr) &
+        bind(C, name="rp_set_hydro_bstress")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ng, slot
+      real(c_double), intent(in) :: bustr(*), bvstr(*)
+    end function
+
+    integer(c_int) function rp_bottom_begin(ctx) bind(C, name="rp_bottom_begin")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function rp_bottom_update(ctx, ex, ix, t_b, t_c, t_f) &
+        bind(C, name="rp_bottom_update")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ex(3), ix(3)
+      integer(c_int32_t), value :: t_b, t_c, t_f
+    end function
+
+    integer(c_int) function rp_bottom_get(ctx, n, hob, bustr, bvstr, reset_hob) &
+        bind(C, name="rp_bottom_get")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(out) :: hob(*), bustr(*), bvstr(*)
+      integer(c_int32_t), value :: reset_hob
     end function
 
     integer(c_int) function rp_nc_close(nc) bind(C, name="rp_nc_close")

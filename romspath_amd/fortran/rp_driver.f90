@@ -255,7 +255,7 @@ contains
     rc = rp_sample_lonlat(ctx, nl, plon, plat)
     if (rc /= 0) call die('rp_sample_lonlat', rc)
     rc = rp_nc_write(nc, dble(time), page, plon, plat, pz, pst, pgid, c_null_ptr, c_null_ptr, c_null_ptr, &
-                     c_null_ptr, c_null_ptr)
+                     c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr)
     if (rc /= 0) call die('rp_nc_write', rc)
   end subroutine data_output
 

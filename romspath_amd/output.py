@@ -15,7 +15,7 @@ import numpy as np
 
 from . import abi
 
-FLAGS = {"WaterDepth": 1, "zeta": 2, "h": 4, "salttemp": 8}
+FLAGS = {"WaterDepth": 1, "zeta": 2, "h": 4, "salttemp": 8, "bottom": 16}
 
 
 def meta_of(prm, time_units="seconds since model start", run_name=None, behavior=0):
@@ -55,17 +55,19 @@ class TrajectoryFile:
 
     def write(self, model_time, p, lon, lat):
         """p: the dict of rp_get_particles (release order), plus "salt" and "temp"
-        when the file has them; lon, lat likewise."""
+        (SaltTempOn) and "hob", "bustr", "bvstr" (WriteBottom) when the file
+        has them; lon, lat likewise."""
         cols = [np.ascontiguousarray(p[k], dtype=np.float64) for k in ("age", "z", "status")]
         lo = np.ascontiguousarray(lon, dtype=np.float64)
         la = np.ascontiguousarray(lat, dtype=np.float64)
         gid = np.ascontiguousarray(p["gid"], dtype=np.int32)
         opt = {k: None if p.get(k) is None else np.ascontiguousarray(p[k], dtype=np.float64)
-               for k in ("wd", "zeta", "bath", "salt", "temp")}
+               for k in ("wd", "zeta", "bath", "salt", "temp", "hob", "bustr", "bvstr")}
         d = lambda a: None if a is None else a.ctypes.data_as(abi.PD)
         rc = self._L.rp_nc_write(self.h, float(model_time), d(cols[0]), d(lo), d(la), d(cols[1]),
                                  d(cols[2]), gid.ctypes.data_as(abi.PI32), d(opt["wd"]),
-                                 d(opt["zeta"]), d(opt["bath"]), d(opt["salt"]), d(opt["temp"]))
+                                 d(opt["zeta"]), d(opt["bath"]), d(opt["salt"]), d(opt["temp"]),
+                                 d(opt["hob"]), d(opt["bustr"]), d(opt["bvstr"]))
         if rc != 0:
             raise abi.RpError("rp_nc_write", rc, self._L.rp_nc_error(self.h).decode())
 
@@ -83,8 +85,12 @@ class TrajectoryFile:
 
 def data_output(ctx, nc, model_time):
     """dataOutput (ROMSPath.f90:1012-1025): lon/lat at every particle, then
-    one record.  ctx: abi.Context after rp_set_lonlat for every grid."""
+    one record.  ctx: abi.Context after rp_set_lonlat for every grid (and
+    rp_bottom_begin when the file has the WriteBottom variables: P_HOB is
+    read and put back to 9999, ROMSPath.f90:2056-2065)."""
     p = ctx.get_particles()
+    if nc.flags & FLAGS["bottom"]:
+        p["hob"], p["bustr"], p["bvstr"] = ctx.bottom_get(reset_hob=True)
     lon, lat = ctx.sample_lonlat()
     nc.write(model_time, p, lon, lat)
     return p, lon, lat

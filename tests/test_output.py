@@ -14,13 +14,13 @@ from romspath_amd.synth import RunParams
 
 def _cols(rng, n, ngrid):
     p = {k: rng.standard_normal(n) * 100
-         for k in ("age", "z", "status", "wd", "zeta", "bath", "salt", "temp")}
+         for k in ("age", "z", "status", "wd", "zeta", "bath", "salt", "temp", "hob", "bustr", "bvstr")}
     p["status"] = rng.choice([9.0, 0.0, 1.0, 1.1, 1.2, -8.0, -9.0], n)
     p["gid"] = rng.integers(1, ngrid + 1, n).astype(np.int32)
     return p, rng.uniform(-180, 180, n), rng.uniform(-90, 90, n)
 
 
-@pytest.mark.parametrize("ngrid,extra", [(1, ()), (2, ("WaterDepth", "zeta", "h", "salttemp"))])
+@pytest.mark.parametrize("ngrid,extra", [(1, ()), (2, ("WaterDepth", "zeta", "h", "salttemp", "bottom"))])
 def test_records_read_back(tmp_path, ngrid, extra):
     rng = np.random.default_rng(ngrid)
     n = 1001
@@ -42,7 +42,8 @@ def test_records_read_back(tmp_path, ngrid, extra):
         names = ["model_time", "dob", "age", "lon", "lat", "zp", "status"]
         names += ["GID"] if ngrid > 1 else []
         names += ["salinity", "temperature"] if "salttemp" in extra else []
-        names += [e for e in extra if e != "salttemp"]
+        names += ["HOB", "bustr", "bvstr"] if "bottom" in extra else []
+        names += [e for e in extra if e not in ("salttemp", "bottom")]
         assert list(f.variables) == names
         v = f.variables
         assert v["model_time"].dimensions == ("time",)
@@ -67,10 +68,15 @@ def test_records_read_back(tmp_path, ngrid, extra):
                 assert np.array_equal(v["h"][r], p["bath"].astype(np.float32))
                 assert np.array_equal(v["salinity"][r], p["salt"].astype(np.float32))
                 assert np.array_equal(v["temperature"][r], p["temp"].astype(np.float32))
+                for k in ("hob", "bustr", "bvstr"):
+                    assert np.array_equal(v["HOB" if k == "hob" else k][r], p[k].astype(np.float32))
         a = f._attributes
         assert a["adv_time_step"] == 120 and a["Vertical_Turb"] == b"True"
         assert a["seed"] == 5 and a["turb_time_step"] == 12.0
         assert a["OpenOceanBoundary"] == b"TRUE" and a["Horiz_Turb"] == b"False"
+        if "bottom" in extra:
+            assert v["HOB"].units == b"m" and v["bustr"].units == b"newton meter-2"
+            assert v["bvstr"].long_name == b"V bottom stress at minimum HOB"
     finally:
         f.close()
 
