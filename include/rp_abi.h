@@ -132,14 +132,33 @@ int rp_sample_lonlat(void *ctx, int64_t n, double *lon, double *lat);
 /* SaltTempOn (ROMSPath.f90:958-981): rp_set_hydro_tracers stores salt and
  * temp (xi_rho, eta_rho, s_rho) of time slot 1..3, as updateHydro reads them
  * (hydrodynamic_module.f90:663-745); asynchronous like rp_set_hydro_slot.
- * rp_sample_tracers gives P_salt and P_temp at every particle's current
+ * rp_sample_tracers gives salt and temp at every particle's current
  * position (release order): zeta of slots b, c, f, getInterp3d("salt"/"temp")
  * on rho levels with each slot's zeta (interpolation_module.f90:509-589,
- * 769-830), polintd in time at ix(2).  Synchronises the stream. */
+ * 769-830), polintd in time at ix(2).  Synchronises the stream.  The
+ * reference's P_salt keeps a particle's value from the last step it took;
+ * rp_tracers_* below keep that state. */
 int rp_set_hydro_tracers(void *ctx, int32_t ng, int32_t slot, const double *salt,
                          const double *temp);
 int rp_sample_tracers(void *ctx, int64_t n, const double ex[3], const double ix[3], int32_t t_b,
                       int32_t t_c, int32_t t_f, double *salt, double *temp);
+
+/* P_Salt / P_Temp kept as the reference keeps them, per step.
+ * rp_tracers_begin (after rp_set_particles and rp_locate_particles' isIn,
+ * with slot t_c's salt/temp and hydro loaded; ROMSPath.f90:247-258,
+ * 408-424): P = 0, then for the particles in a grid P = getInterp3d at t_c;
+ * with mean set (SaltTempMean) also mean = P and mI = 1 when any particle
+ * is in a grid.  rp_tracers_update (after each rp_step, with that step's
+ * ex, ix and slots; queued on the stream; ROMSPath.f90:958-981, 533): for
+ * each particle that took the step (its age changed) P = the sample
+ * rp_sample_tracers would give, mean += P; then mI += 1.
+ * rp_tracers_get: P (or, with mean, mean / mI as writeNetCDF writes it,
+ * ROMSPath.f90:2010-2014) in release order; reset_mean puts the means and mI
+ * back to 0 (ROMSPath.f90:2030-2035).  Synchronises the stream. */
+int rp_tracers_begin(void *ctx, int32_t mean, int32_t t_c);
+int rp_tracers_update(void *ctx, const double ex[3], const double ix[3], int32_t t_b, int32_t t_c,
+                      int32_t t_f);
+int rp_tracers_get(void *ctx, int64_t n, double *salt, double *temp, int32_t reset_mean);
 
 /* WriteBottom (ROMSPath.f90:260-266, 353-355, 935-951, 2056-2078).
  * rp_set_hydro_bstress stores bustr (xi_u, eta_u) and bvstr (xi_v, eta_v)

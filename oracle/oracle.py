@@ -418,3 +418,32 @@ def bottom_update(grids, T, state, last_age, hob, bustr, bvstr):
     lib().og_bottom_update(arr, C.byref(T), state.n, dptr(state.x), dptr(state.y), dptr(state.z),
                            iptr(state.gid), dptr(state.age), dptr(state.bath), dptr(last_age),
                            dptr(hob), dptr(bustr), dptr(bvstr))
+
+
+def tracers_begin(grids, tc, state, mean):
+    """Release-time P_Salt/P_Temp (ROMSPath.f90:247-258, 408-424): 0, then for
+    the particles in a grid getInterp3d at slot t_c (polintd over three equal
+    samples returns the sample).  Returns (last_age, P_salt, P_temp,
+    mean_salt, mean_temp, mI); the means are None without SaltTempMean."""
+    n = state.n
+    ps, pt = np.zeros(n), np.zeros(n)
+    s, t = sample_tracers(grids, (0.0, 1.0, 2.0), (0.0, 1.0, 2.0), tc, tc, tc, state.x, state.y,
+                          state.z, state.gid)
+    isin = np.asarray(state.isin).astype(bool)
+    ps[isin], pt[isin] = s[isin], t[isin]
+    if not mean:
+        return state.age.copy(), ps, pt, None, None, 0
+    return state.age.copy(), ps, pt, ps.copy(), pt.copy(), int(isin.any())
+
+
+def tracers_update(grids, st, state, last_age, ps, pt, ms=None, mt=None):
+    """The SaltTempOn block of update_particles (ROMSPath.f90:958-981) for the
+    particles that took the step (age changed); the caller adds 1 to mI."""
+    s, t = sample_tracers(grids, st.ex, st.ix, st.tb, st.tc, st.tf, state.x, state.y, state.z,
+                          state.gid)
+    act = state.age != last_age
+    last_age[act] = state.age[act]
+    ps[act], pt[act] = s[act], t[act]
+    if ms is not None:
+        ms[act] += s[act]
+        mt[act] += t[act]

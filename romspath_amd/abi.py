@@ -33,6 +33,7 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_set_lonlat", "rp_sample_lonlat", "rp_set_grid_angle", "rp_locate_particles",
            "rp_set_hydro_tracers", "rp_sample_tracers",
            "rp_set_hydro_bstress", "rp_bottom_begin", "rp_bottom_update", "rp_bottom_get",
+           "rp_tracers_begin", "rp_tracers_update", "rp_tracers_get",
            "rp_nc_create", "rp_nc_write", "rp_nc_records", "rp_nc_error", "rp_nc_close",
            "rp_last_error", "rp_destroy"]
 # entry points whose result is not an int status
@@ -108,6 +109,9 @@ def lib():
         L.rp_set_grid_angle.argtypes = [vp, C.c_int32, C.c_int32, PD]
         L.rp_set_hydro_tracers.argtypes = [vp, C.c_int32, C.c_int32, PD, PD]
         L.rp_sample_tracers.argtypes = [vp, C.c_int64, PD, PD, C.c_int32, C.c_int32, C.c_int32, PD, PD]
+        L.rp_tracers_begin.argtypes = [vp, C.c_int32, C.c_int32]
+        L.rp_tracers_update.argtypes = [vp, PD, PD, C.c_int32, C.c_int32, C.c_int32]
+        L.rp_tracers_get.argtypes = [vp, C.c_int64, PD, PD, C.c_int32]
         L.rp_set_hydro_bstress.argtypes = [vp, C.c_int32, C.c_int32, PD, PD]
         L.rp_bottom_begin.argtypes = [vp]
         L.rp_bottom_update.argtypes = [vp, PD, PD, C.c_int32, C.c_int32, C.c_int32]
@@ -320,6 +324,21 @@ class Context:
         ixa = (C.c_double * 3)(*ix)
         self._chk("rp_sample_tracers", self._L.rp_sample_tracers(
             self.h, self.n, exa, ixa, tb, tc, tf, _d(salt), _d(temp)))
+        return salt, temp
+
+    def tracers_begin(self, mean, t_c):
+        self._chk("rp_tracers_begin", self._L.rp_tracers_begin(self.h, int(bool(mean)), t_c))
+
+    def tracers_update(self, ex, ix, tb, tc, tf):
+        exa = (C.c_double * 3)(*ex)
+        ixa = (C.c_double * 3)(*ix)
+        self._chk("rp_tracers_update", self._L.rp_tracers_update(self.h, exa, ixa, tb, tc, tf))
+
+    def tracers_get(self, reset_mean=False):
+        """P_Salt, P_Temp (or their means since the last reset) in release order."""
+        salt, temp = np.empty(self.n), np.empty(self.n)
+        self._chk("rp_tracers_get", self._L.rp_tracers_get(self.h, self.n, _d(salt), _d(temp),
+                                                           int(bool(reset_mean))))
         return salt, temp
 
     def set_hydro_bstress(self, ng, slot, bustr, bvstr):

@@ -78,6 +78,12 @@ struct Ctx {
         bool used = false;
     };
     FieldStage fstage[kMaxGrid][9];   // zeta, u, v, w, aks, salt, temp, bustr, bvstr
+    // SaltTempOn per step (rp_tracers_begin): release order, trc_n particles
+    double *trc_last_age = nullptr, *trc_salt = nullptr, *trc_temp = nullptr;
+    double *trc_mean_salt = nullptr, *trc_mean_temp = nullptr;   // SaltTempMean, else null
+    int64_t trc_n = -1;
+    bool trc_mean = false;
+    int64_t trc_mI = 0;
     // WriteBottom (rp_bottom_begin): release order, bot_n particles
     double *bot_last_age = nullptr, *bot_hob = nullptr, *bot_bustr = nullptr, *bot_bvstr = nullptr;
     int64_t bot_n = -1;
@@ -1024,6 +1030,7 @@ int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, con
     if (rc) return RP_E_ALLOC;
     c->n = n;
     c->bot_n = -1;
+    c->trc_n = -1;
     c->steps_done = 0;
     c->ranges_intact = true;
     free_recording(c);
@@ -1473,6 +1480,43 @@ int rp_set_hydro_tracers(void* ctx, int32_t ng, int32_t slot, const double* salt
     return RP_OK;
 }
 
+namespace {
+// checks and fills what every tracer launch shares: the slots' salt/temp
+// and hydro fields on every grid, the stored particle columns
+int tracer_args(Ctx* c, int32_t t_b, int32_t t_c, int32_t t_f, const char* who, TracerArgs& A)
+{
+    if (t_b < 1 || t_b > 3 || t_c < 1 || t_c > 3 || t_f < 1 || t_f > 3)
+        return fail(c, RP_E_ARG, "slot numbers must be 1..3");
+    std::memset(&A, 0, sizeof(A));
+    for (int i = 0; i < c->cfg.ngrid; i++) {
+        const GridDev& g = c->grids[i];
+        if (!g.set) return fail(c, RP_E_STATE, std::string(who) + ": grid not set");
+        if (!g.tracer_set[t_b - 1] || !g.tracer_set[t_c - 1] || !g.tracer_set[t_f - 1] ||
+            !g.slot_set[t_b - 1] || !g.slot_set[t_c - 1] || !g.slot_set[t_f - 1])
+            return fail(c, RP_E_STATE, std::string(who) + ": salt/temp or hydro slot not loaded");
+        A.salt[i] = g.salt;
+        A.temp[i] = g.temp;
+    }
+    A.slots.s[0] = t_b - 1; A.slots.s[1] = t_c - 1; A.slots.s[2] = t_f - 1;
+    A.n = c->n;
+    A.x = c->x; A.y = c->y; A.z = c->z; A.gid = c->gid;
+    A.id = c->id; A.age = c->age; A.isin = c->isin;
+    return RP_OK;
+}
+
+int launch_tracers(Ctx* c, const TracerArgs& A)
+{
+    int rc;
+    if ((rc = refresh_tab(c))) return rc;
+    if (A.n == 0) return RP_OK;
+    const int grid = (int)std::min<int64_t>((A.n + 255) / 256, (int64_t)c->ncu * 16);
+    if (c->fbytes == 4) hipLaunchKernelGGL(tracer_kernel<float>, dim3(grid), dim3(256), 0, c->stream, c->dgrids, A);
+    else hipLaunchKernelGGL(tracer_kernel<double>, dim3(grid), dim3(256), 0, c->stream, c->dgrids, A);
+    HIPCHK(c, hipGetLastError());
+    return RP_OK;
+}
+}  // namespace
+
 int rp_sample_tracers(void* ctx, int64_t n, const double ex[3], const double ix[3], int32_t t_b,
                       int32_t t_c, int32_t t_f, double* salt, double* temp)
 {
@@ -1480,52 +1524,124 @@ int rp_sample_tracers(void* ctx, int64_t n, const double ex[3], const double ix[
     if (!c || !ex || !ix) return RP_E_ARG;
     if (n != c->n) return fail(c, RP_E_ARG, "particle count differs from rp_set_particles");
     if (n > 0 && (!salt || !temp)) return fail(c, RP_E_ARG, "null output");
-    if (t_b < 1 || t_b > 3 || t_c < 1 || t_c > 3 || t_f < 1 || t_f > 3)
-        return fail(c, RP_E_ARG, "slot numbers must be 1..3");
     TracerArgs A;
-    std::memset(&A, 0, sizeof(A));
-    for (int i = 0; i < c->cfg.ngrid; i++) {
-        const GridDev& g = c->grids[i];
-        if (!g.set) return fail(c, RP_E_STATE, "rp_sample_tracers: grid not set");
-        if (!g.tracer_set[t_b - 1] || !g.tracer_set[t_c - 1] || !g.tracer_set[t_f - 1] ||
-            !g.slot_set[t_b - 1] || !g.slot_set[t_c - 1] || !g.slot_set[t_f - 1])
-            return fail(c, RP_E_STATE, "rp_sample_tracers: salt/temp or hydro slot not loaded");
-        A.salt[i] = g.salt;
-        A.temp[i] = g.temp;
-    }
+    int rc;
+    if ((rc = tracer_args(c, t_b, t_c, t_f, "rp_sample_tracers", A))) return rc;
     if (n == 0) return RP_OK;
-    A.slots.s[0] = t_b - 1; A.slots.s[1] = t_c - 1; A.slots.s[2] = t_f - 1;
     for (int k = 0; k < 3; k++) A.ex[k] = ex[k];
     A.ix2 = ix[1];
-    A.n = n;
-    A.x = c->x; A.y = c->y; A.z = c->z; A.gid = c->gid;
-    DGrid tab[kMaxGrid];
-    fill_tab(c, tab);
     double* d = nullptr;
-    DGrid* dtab = nullptr;
-    int rc;
-    if ((rc = dalloc(c, &d, 2 * (size_t)n)) || (rc = dalloc(c, &dtab, kMaxGrid))) {
-        dfree(d); dfree(dtab);
-        return rc;
-    }
+    if ((rc = dalloc(c, &d, 2 * (size_t)n))) return rc;
     A.out_salt = d; A.out_temp = d + n;
-    hipError_t e = hipMemcpyAsync(dtab, tab, sizeof(tab), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) {
-        const int grid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)c->ncu * 16);
-        if (c->fbytes == 4) hipLaunchKernelGGL(tracer_kernel<float>, dim3(grid), dim3(256), 0, c->stream, dtab, A);
-        else hipLaunchKernelGGL(tracer_kernel<double>, dim3(grid), dim3(256), 0, c->stream, dtab, A);
-        e = hipGetLastError();
+    A.mode = 0;
+    if ((rc = launch_tracers(c, A))) {
+        dfree(d);
+        return rc;
     }
     std::vector<double> h(2 * (size_t)n);
     std::vector<uint32_t> ids(n);
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, 2 * (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipMemcpyAsync(h.data(), d, 2 * (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(ids.data(), c->id, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    dfree(d); dfree(dtab);
+    dfree(d);
     if (e != hipSuccess) return fail(c, RP_E_HIP, std::string("rp_sample_tracers: ") + hipGetErrorString(e));
     for (int64_t i = 0; i < n; i++) {
         salt[ids[i]] = h[i];
         temp[ids[i]] = h[n + i];
+    }
+    return RP_OK;
+}
+
+int rp_tracers_begin(void* ctx, int32_t mean, int32_t t_c)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    TracerArgs A;
+    int rc;
+    if ((rc = tracer_args(c, t_c, t_c, t_c, "rp_tracers_begin", A))) return rc;
+    const size_t n = (size_t)c->n;
+    c->trc_n = -1;
+    if ((rc = dalloc(c, &c->trc_last_age, n)) || (rc = dalloc(c, &c->trc_salt, n)) ||
+        (rc = dalloc(c, &c->trc_temp, n)))
+        return rc;
+    dfree(c->trc_mean_salt); dfree(c->trc_mean_temp);
+    if (mean && ((rc = dalloc(c, &c->trc_mean_salt, n)) || (rc = dalloc(c, &c->trc_mean_temp, n))))
+        return rc;
+    // P_Salt = P_Temp = 0 and the means 0 (ROMSPath.f90:247-258), then the
+    // particles in a grid sampled at t_c
+    HIPCHK(c, hipMemsetAsync(c->trc_salt, 0, n * 8, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->trc_temp, 0, n * 8, c->stream));
+    if (mean) {
+        HIPCHK(c, hipMemsetAsync(c->trc_mean_salt, 0, n * 8, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->trc_mean_temp, 0, n * 8, c->stream));
+    }
+    A.ex[0] = 0.0; A.ex[1] = 1.0; A.ex[2] = 2.0;   // three equal samples: polintd returns the sample
+    A.ix2 = 1.0;
+    A.mode = 1;
+    A.last_age = c->trc_last_age;
+    A.p_salt = c->trc_salt; A.p_temp = c->trc_temp;
+    A.mean_salt = c->trc_mean_salt; A.mean_temp = c->trc_mean_temp;
+    if ((rc = launch_tracers(c, A))) return rc;
+    // mI = 1 once a particle in a grid has been sampled (ROMSPath.f90:422)
+    c->trc_mI = 0;
+    if (mean && n > 0) {
+        std::vector<uint8_t> isin(n);
+        HIPCHK(c, hipMemcpyAsync(isin.data(), c->isin, n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (uint8_t v : isin)
+            if (v) { c->trc_mI = 1; break; }
+    }
+    c->trc_n = c->n;
+    c->trc_mean = mean != 0;
+    return RP_OK;
+}
+
+int rp_tracers_update(void* ctx, const double ex[3], const double ix[3], int32_t t_b, int32_t t_c,
+                      int32_t t_f)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !ex || !ix) return RP_E_ARG;
+    if (c->trc_n != c->n) return fail(c, RP_E_STATE, "rp_tracers_update: rp_tracers_begin not called for these particles");
+    TracerArgs A;
+    int rc;
+    if ((rc = tracer_args(c, t_b, t_c, t_f, "rp_tracers_update", A))) return rc;
+    for (int k = 0; k < 3; k++) A.ex[k] = ex[k];
+    A.ix2 = ix[1];
+    A.mode = 2;
+    A.last_age = c->trc_last_age;
+    A.p_salt = c->trc_salt; A.p_temp = c->trc_temp;
+    A.mean_salt = c->trc_mean_salt; A.mean_temp = c->trc_mean_temp;
+    if ((rc = launch_tracers(c, A))) return rc;
+    c->trc_mI++;                      // mI = mI + 1 after every update_particles (ROMSPath.f90:533)
+    return RP_OK;
+}
+
+int rp_tracers_get(void* ctx, int64_t n, double* salt, double* temp, int32_t reset_mean)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (c->trc_n != c->n) return fail(c, RP_E_STATE, "rp_tracers_get: rp_tracers_begin not called for these particles");
+    if (n != c->n) return fail(c, RP_E_ARG, "particle count differs from rp_set_particles");
+    if (n > 0 && (!salt || !temp)) return fail(c, RP_E_ARG, "null output");
+    if (n == 0) return RP_OK;
+    const double* ds = c->trc_mean ? c->trc_mean_salt : c->trc_salt;
+    const double* dt = c->trc_mean ? c->trc_mean_temp : c->trc_temp;
+    hipError_t e = hipMemcpyAsync(salt, ds, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(temp, dt, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && c->trc_mean && reset_mean) {
+        e = hipMemsetAsync(c->trc_mean_salt, 0, (size_t)n * 8, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->trc_mean_temp, 0, (size_t)n * 8, c->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, RP_E_HIP, std::string("rp_tracers_get: ") + hipGetErrorString(e));
+    if (c->trc_mean) {
+        // mean_salt(n) / mI (ROMSPath.f90:2010-2014)
+        const double mI = (double)c->trc_mI;
+        for (int64_t i = 0; i < n; i++) {
+            salt[i] = salt[i] / mI;
+            temp[i] = temp[i] / mI;
+        }
+        if (reset_mean) c->trc_mI = 0;
     }
     return RP_OK;
 }
@@ -1871,6 +1987,8 @@ int rp_destroy(void* ctx)
     dfree(c->wd); dfree(c->zeta); dfree(c->bath);
     dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
     dfree(c->bot_last_age); dfree(c->bot_hob); dfree(c->bot_bustr); dfree(c->bot_bvstr);
+    dfree(c->trc_last_age); dfree(c->trc_salt); dfree(c->trc_temp);
+    dfree(c->trc_mean_salt); dfree(c->trc_mean_temp);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     free_recording(c);
     dfree(c->adj.active);

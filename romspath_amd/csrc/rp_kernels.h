@@ -94,7 +94,12 @@ struct LocateArgs {
 template <typename F>
 __global__ void locate_kernel(const DGrid* __restrict__ grids, LocateArgs A);
 
-// salt and temp at the particles (SaltTempOn, ROMSPath.f90:958-981)
+// salt and temp at the particles (SaltTempOn, ROMSPath.f90:958-981).
+// mode 0: every particle, into out_salt/out_temp in stored order; mode 1
+// (release, ROMSPath.f90:408-424): the particles in a grid, P = the sample,
+// mean = P, and last_age = age for all; mode 2 (after a step): the particles
+// whose age changed, P = the sample, mean += P.  Modes 1 and 2 index the
+// release-order columns through id; mean_salt/mean_temp may be null.
 struct TracerArgs {
     const void* salt[kMaxGrid];   // slot-interleaved, field precision, (L, M, N)
     const void* temp[kMaxGrid];
@@ -104,6 +109,11 @@ struct TracerArgs {
     const double *x, *y, *z;
     const int8_t* gid;
     double *out_salt, *out_temp;
+    int mode;
+    const uint32_t* id;
+    const double* age;
+    const uint8_t* isin;
+    double *last_age, *p_salt, *p_temp, *mean_salt, *mean_temp;
 };
 template <typename F>
 __global__ void tracer_kernel(const DGrid* __restrict__ grids, TracerArgs A);

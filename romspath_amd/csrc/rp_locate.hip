@@ -199,6 +199,17 @@ __global__ void __launch_bounds__(256) tracer_kernel(const DGrid* __restrict__ g
 {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < A.n;
          q += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t r = 0;
+        if (A.mode == 1) {
+            r = A.id[q];
+            A.last_age[r] = A.age[q];
+            if (!A.isin[q]) continue;
+        } else if (A.mode == 2) {
+            r = A.id[q];
+            const double age = A.age[q];
+            if (age == A.last_age[r]) continue;
+            A.last_age[r] = age;
+        }
         const int gi = (int)A.gid[q] - 1;
         const DGrid& g = grids[gi];
         const double X = A.x[q], Y = A.y[q], Z = A.z[q];
@@ -220,8 +231,19 @@ __global__ void __launch_bounds__(256) tracer_kernel(const DGrid* __restrict__ g
             blend3<F, 1>(A.temp[gi], nullptr, c.o, g.N, cff, zero, vb, S, o);
             et[t] = o[0];
         }
-        A.out_salt[q] = polintd(A.ex, es[0], es[1], es[2], A.ix2);
-        A.out_temp[q] = polintd(A.ex, et[0], et[1], et[2], A.ix2);
+        const double ps = polintd(A.ex, es[0], es[1], es[2], A.ix2);
+        const double pt = polintd(A.ex, et[0], et[1], et[2], A.ix2);
+        if (A.mode == 0) {
+            A.out_salt[q] = ps;
+            A.out_temp[q] = pt;
+            continue;
+        }
+        A.p_salt[r] = ps;
+        A.p_temp[r] = pt;
+        if (A.mean_salt) {
+            A.mean_salt[r] = A.mode == 1 ? ps : A.mean_salt[r] + ps;
+            A.mean_temp[r] = A.mode == 1 ? pt : A.mean_temp[r] + pt;
+        }
     }
 }
 

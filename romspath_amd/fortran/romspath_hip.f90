@@ -56,6 +56,7 @@ module romspath_hip
             rp_set_lonlat, rp_sample_lonlat, rp_nc_create, rp_nc_write, rp_nc_close, &
             rp_set_grid_angle, rp_locate_particles, rp_set_hydro_tracers, rp_sample_tracers, &
             rp_set_hydro_bstress, rp_bottom_begin, rp_bottom_update, rp_bottom_get, &
+            rp_tracers_begin, rp_tracers_update, rp_tracers_get, &
             rp_error_text
 
   interface
@@ -291,6 +292,30 @@ module romspath_hip
       real(c_double), intent(in) :: ex(3), ix(3)
       integer(c_int32_t), value :: t_b, t_c, t_f
       real(c_double), intent(out) :: salt(*), temp(*)
+    end function
+
+    ! SaltTempOn / SaltTempMean kept per step: P_Salt, P_Temp (or their means)
+    integer(c_int) function rp_tracers_begin(ctx, mean, t_c) bind(C, name="rp_tracers_begin")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mean, t_c
+    end function
+
+    integer(c_int) function rp_tracers_update(ctx, ex, ix, t_b, t_c, t_f) &
+        bind(C, name="rp_tracers_update")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ex(3), ix(3)
+      integer(c_int32_t), value :: t_b, t_c, t_f
+    end function
+
+    integer(c_int) function rp_tracers_get(ctx, n, salt, temp, reset_mean) &
+        bind(C, name="rp_tracers_get")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(out) :: salt(*), temp(*)
+      integer(c_int32_t), value :: reset_mean
     end function
 
     ! WriteBottom: bustr/bvstr of slot tind, P_HOB/P_bustr/P_bvstr at the particles

@@ -46,3 +46,34 @@ def test_affine_tracer_is_reproduced():
     want = A0 + BX * x + CZ * zz + DT * ix[1]
     assert np.abs(salt - want).max() < 1e-9
     assert np.abs(temp - (2.0 * want - 40.0)).max() < 1e-8
+
+
+def test_per_step_state_and_mean():
+    """rp_tracers_* restated: release-time values at t_c for the particles in a
+    grid, per-step values only for the particles that stepped, and the
+    SaltTempMean sums."""
+    import types
+    cfg, slot_recs, ex, x, y, z, zk = tracer_case()
+    og = O.OracleGrid(cfg.grids[0])
+    for s, rec in enumerate(slot_recs):
+        og.set_slot(s + 1, rec)
+    n = len(x)
+    rng = np.random.default_rng(4)
+    isin = rng.random(n) < 0.8
+    st = types.SimpleNamespace(n=n, x=x, y=y, z=z, gid=np.ones(n, np.int32), isin=isin,
+                               age=rng.uniform(0, 1e4, n))
+    zz = np.where(z > zk[-1], zk[-1], z)
+    la, ps, pt, ms, mt, mI = O.tracers_begin([og], 2, st, mean=True)
+    want0 = A0 + BX * x + CZ * zz + DT * ex[1]
+    assert mI == 1 and np.array_equal(la, st.age)
+    assert np.abs(ps[isin] - want0[isin]).max() < 1e-9 and np.all(ps[~isin] == 0.0)
+    assert np.array_equal(ms, ps) and np.abs(pt[isin] - (2 * want0[isin] - 40)).max() < 1e-8
+    step = types.SimpleNamespace(ex=ex, ix=(0.0, 600.0, 720.0), tb=1, tc=2, tf=3)
+    act = np.arange(n) % 3 != 0
+    st.age = st.age + np.where(act, 120.0, 0.0)
+    p0, m0 = ps.copy(), ms.copy()
+    O.tracers_update([og], step, st, la, ps, pt, ms, mt)
+    want1 = A0 + BX * x + CZ * zz + DT * 600.0
+    assert np.abs(ps[act] - want1[act]).max() < 1e-9 and np.array_equal(ps[~act], p0[~act])
+    assert np.array_equal(ms[act], m0[act] + ps[act]) and np.array_equal(ms[~act], m0[~act])
+    assert np.array_equal(la, st.age)
