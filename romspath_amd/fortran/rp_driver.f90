@@ -5,8 +5,11 @@
 ! with `call update_particles()` replaced by rp_step.  Namelist groups and
 ! names are the reference's (ROMSPath.h:69-258).  Grid and hydro records are
 ! the analytic flat-bottom double gyre of the CPU-runnable case (no netCDF
-! library exists in this environment); particles come from a text file of
-! "X Y Z DOB" lines (fractional rho indices, metres, seconds).
+! library exists in this environment); particles come from the reference's
+! release file, "lon lat Z DOB" per line read list-directed
+! (ROMSPath.f90:319-332), placed as ini_ROMSPath places them (LL2ij, bounds,
+! zbounds: rp_locate_particles); on this Cartesian grid lon/lat are x_rho/y_rho
+! in metres, as InitGrid reads them.
 !
 ! usage: rp_driver <namelist file> <particle file> <output file> [<netCDF file>]
 ! With a fourth argument the run also writes the reference's trajectory file,
@@ -154,30 +157,34 @@ program rp_driver
   ! --- particles (par(:,pX..pDOB) of ini_ROMSPath) ---
   allocate (px(numpar), py(numpar), pz(numpar), page(numpar), pdob(numpar), pst(numpar), pgid(numpar), &
             pin(numpar), wd(numpar), pzeta(numpar), bath(numpar))
+  allocate (plon(numpar), plat(numpar), xr(L, M), yr(L, M))
   open (newunit=iu, file=trim(parfile), status='old', action='read')
   do ip = 1, numpar
-    read (iu, *) px(ip), py(ip), pz(ip), pdob(ip)
+    pdob(ip) = -9.d0
+    read (iu, *) plon(ip), plat(ip), pz(ip), pdob(ip)
   end do
   close (iu)
-  page = 0.d0
-  pst = 0.d0
-  pgid = 1
-  pin = 1_c_int8_t
+  ! lon_rho/lat_rho of the grid (x_rho/y_rho on a Cartesian grid, InitGrid)
+  do j = 1, M
+    do i = 1, L
+      xr(i, j) = (i - 1) * dx
+      yr(i, j) = (j - 1) * dx
+    end do
+  end do
+  rc = rp_set_lonlat(ctx, 1_c_int32_t, xr, yr)
+  if (rc /= 0) call die('rp_set_lonlat', rc)
+  rc = rp_set_grid_angle(ctx, 1_c_int32_t, 0_c_int32_t, c_null_ptr)
+  if (rc /= 0) call die('rp_set_grid_angle', rc)
   nl = numpar
+  ! ini_ROMSPath's placement with the hydro of slot t_b = 1 (ROMSPath.f90:366-403)
+  rc = rp_locate_particles(ctx, nl, plon, plat, pz, 1_c_int32_t, px, py, pst, pgid, pin)
+  if (rc /= 0) call die('rp_locate_particles', rc)
+  page = 0.d0
   rc = rp_set_particles(ctx, nl, px, py, pz, page, pdob, pst, pgid, pin)
   if (rc /= 0) call die('rp_set_particles', rc)
 
   ! --- trajectory file (initNetCDF / createNetCDF, ROMSPath.f90:446-449) ---
   if (write_nc) then
-    allocate (plon(numpar), plat(numpar), xr(L, M), yr(L, M))
-    do j = 1, M
-      do i = 1, L
-        xr(i, j) = (i - 1) * dx       ! Cartesian: InitGrid reads x_rho/y_rho as lon/lat
-        yr(i, j) = (j - 1) * dx
-      end do
-    end do
-    rc = rp_set_lonlat(ctx, 1_c_int32_t, xr, yr)
-    if (rc /= 0) call die('rp_set_lonlat', rc)
     tunits = 'seconds since model start' // c_null_char
     meta%time_units = c_loc(tunits)
     meta%run_name = c_null_ptr

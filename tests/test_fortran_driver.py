@@ -27,7 +27,10 @@ def test_fortran_driver_matches_oracle(tmp_path):
     cfg = synth.config1(n_particles=n, days=days)
     p = cfg.particles
     (tmp_path / "run.nml").write_text(NML.format(n=n, days=days))
-    np.savetxt(tmp_path / "par.txt", np.c_[p.x, p.y, p.z, p.dob], fmt="%.17e")
+    # the reference's release file: lon lat z dob, here x_rho / y_rho in metres
+    g = cfg.grids[0]
+    lon, lat = (p.x - 1.0) * g.dx, (p.y - 1.0) * g.dy
+    np.savetxt(tmp_path / "par.txt", np.c_[lon, lat, p.z, p.dob], fmt="%.17e")
     print("case fortran driver C1", n, "particles", flush=True)
     r = subprocess.run([str(DRIVER), str(tmp_path / "run.nml"), str(tmp_path / "par.txt"),
                         str(tmp_path / "out.txt"), str(tmp_path / "traj.nc")],
@@ -35,7 +38,12 @@ def test_fortran_driver_matches_oracle(tmp_path):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     got = np.loadtxt(tmp_path / "out.txt")
-    o = O.run_oracle(cfg)
+    # the oracle from the oracle's own placement of the same release file
+    loc = O.locate(cfg, lon, lat, p.z)
+    assert loc["isin"].all() and np.abs(loc["x"] - p.x).max() < 1e-9
+    placed = synth.Particles(loc["x"], loc["y"], loc["z"], p.dob, gid=loc["gid"],
+                             status=loc["status"], isin=loc["isin"])
+    o = O.run_oracle(cfg, particles=placed)
     assert np.array_equal(got[:, 3], o.status)
     assert np.abs(got[:, 0] - o.x).max() <= 1e-9
     assert np.abs(got[:, 1] - o.y).max() <= 1e-9
@@ -49,7 +57,6 @@ def test_fortran_driver_matches_oracle(tmp_path):
         assert np.array_equal(v["dob"][:], p.dob.astype(np.float32))
         assert np.array_equal(v["zp"][2], got[:, 2].astype(np.float32))
         assert np.array_equal(v["status"][2], got[:, 3].astype(np.float32))
-        g = cfg.grids[0]
         assert np.array_equal(v["lon"][2], interp2d_rho(g.x_rho, got[:, 0], got[:, 1]).astype(np.float32))
         assert np.array_equal(v["lat"][2], interp2d_rho(g.y_rho, got[:, 0], got[:, 1]).astype(np.float32))
         assert "GID" not in v
