@@ -289,10 +289,11 @@ def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None, r
     ndev = ndev_of(prm)
     active = 0
     step_seconds = 0.0
-    for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps):
+    hydro = recs if hasattr(recs, "tdim") else None
+    for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps, hydro=hydro):
         for ld in st.loads:
             for i, g in enumerate(grids):
-                g.set_slot(ld.slot, recs.get(i + 1, ld.record))
+                g.set_slot(ld.slot, recs.get(i + 1, ld.record, ld.filenum))
         T = make_time(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf)
         if inject:
             dev = injected_deviates(prm.seed, st.step_index, state.n, ndev)

@@ -35,9 +35,12 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_set_hydro_bstress", "rp_bottom_begin", "rp_bottom_update", "rp_bottom_get",
            "rp_tracers_begin", "rp_tracers_update", "rp_tracers_get",
            "rp_nc_create", "rp_nc_write", "rp_nc_records", "rp_nc_error", "rp_nc_close",
+           "rp_ncin_open", "rp_ncin_dim", "rp_ncin_var", "rp_ncin_get", "rp_ncin_att_text",
+           "rp_ncin_error", "rp_ncin_close",
            "rp_last_error", "rp_destroy"]
 # entry points whose result is not an int status
-_RESTYPE = {"rp_last_error": C.c_char_p, "rp_nc_error": C.c_char_p, "rp_nc_records": C.c_int64}
+_RESTYPE = {"rp_last_error": C.c_char_p, "rp_nc_error": C.c_char_p, "rp_nc_records": C.c_int64,
+            "rp_ncin_error": C.c_char_p}
 
 
 class RpNcMeta(C.Structure):
@@ -120,6 +123,13 @@ def lib():
         L.rp_sample_lonlat.argtypes = [vp, C.c_int64, PD, PD]
         L.rp_nc_create.argtypes = [C.POINTER(vp), C.c_char_p, C.c_int64, C.c_int32, C.c_int32,
                                    C.POINTER(RpNcMeta), PD]
+        L.rp_ncin_open.argtypes = [C.POINTER(vp), C.c_char_p]
+        L.rp_ncin_dim.argtypes = [vp, C.c_char_p, PI64]
+        L.rp_ncin_var.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int32), PI64, C.POINTER(C.c_int32)]
+        L.rp_ncin_get.argtypes = [vp, C.c_char_p, C.c_int64, PD, C.c_int64]
+        L.rp_ncin_att_text.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64, PI64]
+        L.rp_ncin_error.argtypes = [vp]
+        L.rp_ncin_close.argtypes = [vp]
         L.rp_nc_write.argtypes = [vp, C.c_double, PD, PD, PD, PD, PD, PI32, PD, PD, PD, PD, PD, PD, PD, PD]
         L.rp_nc_records.argtypes = [vp]
         L.rp_nc_error.argtypes = [vp]

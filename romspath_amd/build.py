@@ -16,8 +16,9 @@ CSRC = PKG / "csrc"
 LIB = PKG / "lib" / "librp.so"
 SOURCES = ["rp_step.hip", "rp_adjoint.hip", "rp_sort.hip", "rp_locate.hip", "rp_abi.hip"]
 HEADERS = ["rp_device.h", "rp_kernels.h", "rp_dual.h", "rp_step_dev.h"]
-# host-only C++ (no device code): the trajectory file writer
-HOST_SOURCES = ["rp_ncout.cpp"]
+# host-only C++ (no device code): the trajectory file writer and the
+# netCDF classic reader of the hydrodynamic input
+HOST_SOURCES = ["rp_ncout.cpp", "rp_ncin.cpp"]
 CXX = os.environ.get("CXX", "g++")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -16,7 +16,8 @@ class RecordSource:
         self.cfg = cfg
         self.cache = {} if cache else None
 
-    def get(self, ng, record):
+    def get(self, ng, record, filenum=None):
+        assert filenum is None, "in-memory records have no files"
         key = (ng, record)
         if self.cache is not None and key in self.cache:
             return self.cache[key]
@@ -45,8 +46,12 @@ class GpuRun:
         scl, off = scale_offsets(cfg.grids, cfg.refine) if ng > 1 else (None, None)
         for i, g in enumerate(cfg.grids):
             self.ctx.set_grid(i + 1, g, None if scl is None else scl[i], None if off is None else off[i])
-            # Cartesian grids: InitGrid reads x_rho / y_rho into lon_rho / lat_rho
-            self.ctx.set_lonlat(i + 1, g.x_rho, g.y_rho)
+            if hasattr(g, "lon"):            # a grid read from a history file (roms.RomsGrid)
+                self.ctx.set_lonlat(i + 1, g.lon, g.lat)
+                self.ctx.set_grid_angle(i + 1, g.spherical, g.angle)
+            else:
+                # Cartesian grids: InitGrid reads x_rho / y_rho into lon_rho / lat_rho
+                self.ctx.set_lonlat(i + 1, g.x_rho, g.y_rho)
         self.ctx.set_particles(cfg.particles)
         self.records = records or RecordSource(cfg)
 
@@ -59,7 +64,7 @@ class GpuRun:
     def load(self, st):
         for ld in st.loads:
             for g in range(1, self.cfg.ngrid + 1):
-                self.ctx.set_hydro_slot(g, ld.slot, self.records.get(g, ld.record))
+                self.ctx.set_hydro_slot(g, ld.slot, self.records.get(g, ld.record, ld.filenum))
 
     def run(self, max_steps=None, inject=False, on_step=None, diag_every_step=False,
             mode="step", counts=None, use_graph=True, output=None):
@@ -71,7 +76,8 @@ class GpuRun:
         prm = self.cfg.params
         n = self.cfg.particles.n
         ndev = ndev_of(prm)
-        steps = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps))
+        hydro = self.records if hasattr(self.records, "tdim") else None
+        steps = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps, hydro=hydro))
         if diag_every_step:
             for st in steps:
                 st.output = True

@@ -15,7 +15,8 @@ from dataclasses import dataclass
 @dataclass
 class Load:
     slot: int       # 1-based slot to fill
-    record: int     # 1-based record number
+    record: int     # 1-based record number (tstep: within the file when read from files)
+    filenum: int | None = None   # history file number (HydroFiles), None for in-memory records
 
 
 @dataclass
@@ -32,11 +33,19 @@ class InternalStep:
     output: bool              # dataOutput after this step (printdt >= iprint)
 
 
-def schedule(dt: int, idt: int, days: float, iprint: int | None = None, max_steps=None):
-    """Yield the internal steps of a run in order (run_ROMSPath and below)."""
+def schedule(dt: int, idt: int, days: float, iprint: int | None = None, max_steps=None,
+             hydro=None):
+    """Yield the internal steps of a run in order (run_ROMSPath and below).
+
+    hydro: records read from history files (anything with filenum0,
+    multifile and tdim(filenum), e.g. roms.HydroFiles).  Loads then name the
+    file, and at the top of each external step a tstep past the records of
+    the current file moves to record 1 of the next file, or stops the run
+    when multifile is off (ROMSPath.f90:473-480)."""
     t_b, t_c, t_f = 1, 2, 3
     tstep = 1
-    pending = [Load(1, 1), Load(2, 2), Load(3, 3)]
+    filenum = hydro.filenum0 if hydro is not None else None
+    pending = [Load(1, 1, filenum), Load(2, 2, filenum), Load(3, 3, filenum)]
     seconds = int(days * 86400.0)
     stepT = seconds // dt
     stepIT = int(dt / idt)
@@ -44,11 +53,16 @@ def schedule(dt: int, idt: int, days: float, iprint: int | None = None, max_step
     iprint = iprint if iprint is not None else dt
     k = 0
     for ets in range(1, stepT + 1):
+        if hydro is not None and tstep > hydro.tdim(filenum):
+            tstep = 1
+            filenum += 1
+            if not hydro.multifile:
+                raise RuntimeError("NOT ENOUGH TIMES STEP TO CONTINUE (multifile is off)")
         if ets > 2:
             t_b = t_b % 3 + 1
             t_c = t_c % 3 + 1
             t_f = t_f % 3 + 1
-            pending.append(Load(t_f, tstep))
+            pending.append(Load(t_f, tstep, filenum))
         ex = (float((ets - 2) * dt), float((ets - 1) * dt), float(ets * dt))
         for its in range(1, stepIT + 1):
             ix = (ex[1] + float((its - 2) * idt), ex[1] + float((its - 1) * idt),

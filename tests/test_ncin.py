@@ -1,0 +1,272 @@
+"""netCDF classic input (include/rp_ncin.h, romspath_amd/roms.py): files
+written by scipy's independent netCDF writer (CDF-1 and CDF-2) and a CDF-5
+file built byte by byte from the format specification read back value for
+value; InitGrid's grid, reftime and time_units; getFileNames; the
+run_External_Timestep file roll-over; and an oracle run whose records come
+from a set of history files equals the run on the same records in memory,
+bit for bit.  No GPU.
+"""
+import struct
+
+import numpy as np
+import pytest
+from scipy.io import netcdf_file
+
+from oracle import oracle as O
+from romspath_amd import abi, synth
+from romspath_amd.roms import HydroFiles, NcFile, file_name, read_grid, seconds_units
+from romspath_amd.runner import RecordSource
+from romspath_amd.schedule import schedule
+
+
+def write_history(path, g, recs, times, version=2, units="seconds since 2000-01-01 00:00:00",
+                  ftype="d"):
+    """A ROMS-like history file: grid variables and one record per entry of
+    recs (dicts of zeta, u, v, w, aks and optional salt, temp, bustr, bvstr)."""
+    L, M, N = g.L, g.M, g.N
+    f = netcdf_file(str(path), "w", version=version)
+    for name, n in (("ocean_time", None), ("xi_rho", L), ("eta_rho", M), ("xi_u", L - 1),
+                    ("eta_u", M), ("xi_v", L), ("eta_v", M - 1), ("s_rho", N), ("s_w", N + 1)):
+        f.createDimension(name, n)
+
+    def put(name, dims, data, t="d"):
+        v = f.createVariable(name, t, dims)
+        v[...] = data
+        return v
+
+    # scalars as one-element variables: scipy's writer places rank-0 variables
+    # inside the record area of a file that has record variables
+    f.createDimension("one", 1)
+    put("spherical", ("one",), 0, "i")
+    put("Vtransform", ("one",), g.vtransform, "i")
+    put("Vstretching", ("one",), 4, "i")
+    put("theta_s", ("one",), g.theta_s)
+    put("theta_b", ("one",), g.theta_b)
+    put("hc", ("one",), g.hc)
+    rho = ("eta_rho", "xi_rho")
+    for name, a in (("h", g.h), ("pm", g.pm), ("pn", g.pn), ("angle", np.zeros((M, L))),
+                    ("x_rho", g.x_rho), ("y_rho", g.y_rho), ("mask_rho", g.mask_rho)):
+        put(name, rho, a)
+    put("mask_u", ("eta_u", "xi_u"), g.mask_u)
+    put("mask_v", ("eta_v", "xi_v"), g.mask_v)
+    put("s_rho", ("s_rho",), g.s_rho)
+    put("Cs_r", ("s_rho",), g.cs_r)
+    put("s_w", ("s_w",), g.s_w)
+    put("Cs_w", ("s_w",), g.cs_w)
+    t = f.createVariable("ocean_time", "d", ("ocean_time",))
+    t.units = units
+    dims = {"zeta": ("ocean_time",) + rho, "u": ("ocean_time", "s_rho", "eta_u", "xi_u"),
+            "v": ("ocean_time", "s_rho", "eta_v", "xi_v"), "w": ("ocean_time", "s_w") + rho,
+            "AKs": ("ocean_time", "s_w") + rho, "salt": ("ocean_time", "s_rho") + rho,
+            "temp": ("ocean_time", "s_rho") + rho, "bustr": ("ocean_time", "eta_u", "xi_u"),
+            "bvstr": ("ocean_time", "eta_v", "xi_v")}
+    keys = [k for k in ("zeta", "u", "v", "w", "aks", "salt", "temp", "bustr", "bvstr") if k in recs[0]]
+    vars_ = {k: f.createVariable("AKs" if k == "aks" else k, ftype, dims["AKs" if k == "aks" else k])
+             for k in keys}
+    for r, (rec, tt) in enumerate(zip(recs, times)):
+        t[r] = tt
+        for k in keys:
+            vars_[k][r] = rec[k]
+    f.close()
+
+
+@pytest.mark.parametrize("version", [1, 2])
+def test_reads_what_scipy_wrote(tmp_path, version):
+    rng = np.random.default_rng(version)
+    path = tmp_path / "a.nc"
+    f = netcdf_file(str(path), "w", version=version)
+    f.createDimension("t", None)       # scipy wants the unlimited dimension first
+    f.createDimension("x", 5)
+    f.createDimension("y", 3)
+    f.title = "probe"
+    data = {}
+    for name, t in (("b", "b"), ("h", "h"), ("i", "i"), ("f", "f"), ("d", "d")):
+        v = f.createVariable("fix_" + name, t, ("y", "x"))
+        a = rng.integers(-100, 100, (3, 5)).astype(t) if t in "bhi" else rng.standard_normal((3, 5)).astype(t)
+        v[...] = a
+        data["fix_" + name] = a
+        r = f.createVariable("rec_" + name, t, ("t", "x"))
+        r.units = "m s-1"
+        a = rng.integers(-100, 100, (4, 5)).astype(t) if t in "bhi" else rng.standard_normal((4, 5)).astype(t)
+        r[:] = a
+        data["rec_" + name] = a
+    f.close()
+    with NcFile(path) as g:
+        assert g.dim("x") == 5 and g.dim("t") == 4
+        assert g.var("rec_f") == ((4, 5), True) and g.var("fix_h") == ((3, 5), False)
+        assert g.att_text(None, "title") == "probe" and g.att_text("rec_d", "units") == "m s-1"
+        for name, a in data.items():
+            if name.startswith("fix_"):
+                assert np.array_equal(g.read(name), a.astype(np.float64)), name
+            else:
+                for r in range(4):
+                    assert np.array_equal(g.read(name, r), a[r].astype(np.float64)), (name, r)
+
+
+@pytest.mark.parametrize("t", ["h", "b"])
+def test_single_record_variable_unpadded(tmp_path, t):
+    # one record variable of a 2- or 1-byte type: records are not padded to 4 bytes
+    path = tmp_path / "s.nc"
+    f = netcdf_file(str(path), "w", version=1)
+    f.createDimension("t", None)
+    f.createDimension("x", 3)
+    v = f.createVariable("r", t, ("t", "x"))
+    a = np.arange(15).reshape(5, 3).astype(t) - 7
+    v[:] = a
+    f.close()
+    ref = netcdf_file(str(path), "r", mmap=False)
+    want = ref.variables["r"][:].copy()
+    ref.close()
+    with NcFile(path) as g:
+        got = np.array([g.read("r", r) for r in range(g.dim("t"))])
+    assert np.array_equal(got, want.astype(np.float64))
+
+
+def _cdf5_bytes():
+    """A CDF-5 file by the format specification: dims x = 3, t unlimited;
+    fixed int64 a(x), uint16 b(x) and a rank-0 double s; record double
+    r(t, x), two records."""
+    be = lambda fmt, *v: struct.pack(">" + fmt, *v)
+
+    def name(s):
+        b = s.encode()
+        return be("Q", len(b)) + b + b"\0" * ((4 - len(b) % 4) % 4)
+
+    hdr = b"CDF\x05" + be("Q", 2)
+    hdr += be("I", 0x0A) + be("Q", 2) + name("x") + be("Q", 3) + name("t") + be("Q", 0)
+    hdr += be("I", 0) + be("Q", 0)                       # no global attributes
+    hdr += be("I", 0x0B) + be("Q", 4)
+    vars_ = [("a", [0], 10, 24), ("b", [0], 8, 8), ("s", [], 6, 8), ("r", [1, 0], 6, 24)]
+    sizes = []
+    for nm, dims, typ, vsize in vars_:
+        sizes.append(name(nm) + be("Q", len(dims)) + b"".join(be("Q", d) for d in dims)
+                     + be("I", 0) + be("Q", 0) + be("I", typ) + be("Q", vsize))
+    head_len = len(hdr) + sum(len(s) + 8 for s in sizes)
+    begin_a = head_len
+    begin_b = begin_a + 24
+    begin_s = begin_b + 8
+    begin_r = begin_s + 8
+    body = b""
+    for s, beg in zip(sizes, (begin_a, begin_b, begin_s, begin_r)):
+        body += s + be("Q", beg)
+    data = be("3q", -5, 2**40, 7) + be("3H", 1, 65535, 3) + b"\0\0" + be("d", 3.25)
+    data += be("3d", 0.5, -1.5, 2.25) + be("3d", 1e300, -0.0, 3.0)
+    return hdr + body + data
+
+
+def test_cdf5(tmp_path):
+    path = tmp_path / "c5.nc"
+    path.write_bytes(_cdf5_bytes())
+    with NcFile(path) as g:
+        assert g.dim("t") == 2 and g.var("r") == ((2, 3), True)
+        assert np.array_equal(g.read("a"), [-5.0, 2.0**40, 7.0])
+        assert np.array_equal(g.read("b"), [1.0, 65535.0, 3.0])
+        assert g.read("s") == 3.25 and g.var("s") == ((), False)
+        assert np.array_equal(g.read("r", 0), [0.5, -1.5, 2.25])
+        assert np.array_equal(g.read("r", 1), [1e300, -0.0, 3.0])
+
+
+def test_errors(tmp_path):
+    h5 = tmp_path / "h5.nc"
+    h5.write_bytes(b"\x89HDF\r\n\x1a\n" + b"\0" * 100)
+    with pytest.raises(abi.RpError) as e:
+        NcFile(h5)
+    assert e.value.code == -7 and "HDF5" in str(e.value)
+    bad = tmp_path / "bad.nc"
+    bad.write_bytes(_cdf5_bytes()[:40])
+    with pytest.raises(abi.RpError) as e:
+        NcFile(bad)
+    assert e.value.code == -7
+    with pytest.raises(abi.RpError) as e:
+        NcFile(tmp_path / "missing.nc")
+    assert e.value.code == -7
+    ok = tmp_path / "c5.nc"
+    ok.write_bytes(_cdf5_bytes())
+    with NcFile(ok) as g:
+        with pytest.raises(abi.RpError) as e:
+            g.read("nope")
+        assert e.value.code == -1
+        with pytest.raises(abi.RpError) as e:
+            g.read("r", 2)
+        assert e.value.code == -5                      # RP_E_RANGE
+        with pytest.raises(abi.RpError) as e:
+            g.dim("y")
+        assert e.value.code == -1
+
+
+def test_file_names_and_time_units():
+    assert file_name("ocean_his_", 7, 4, ".nc") == "ocean_his_0007.nc"
+    assert file_name("his", 123, 2, ".nc") == "his**.nc"          # Fortran I2.2 overflow
+    assert file_name("/d/one.nc ", 9, 4, ".nc", multifile=False) == "/d/one.nc"
+    assert seconds_units("days since 2000-01-01 00:00:00") == (86400.0, "seconds since 2000-01-01 00:00:00")
+    assert seconds_units("hours since 1990-1-1") == (3600.0, "seconds since 1990-1-1")
+    assert seconds_units("minutes since 1990-1-1") == (60.0, "seconds since 1990-1-1")
+    assert seconds_units("seconds since 1970-01-01") == (1.0, "seconds since 1970-01-01")
+    with pytest.raises(ValueError):
+        seconds_units("fortnights")
+
+
+class _Files:
+    def __init__(self, tdim, multifile=True, filenum0=3):
+        self.t, self.multifile, self.filenum0 = tdim, multifile, filenum0
+
+    def tdim(self, filenum):
+        return self.t
+
+
+def test_schedule_rolls_over_files():
+    # dt = idt: one internal step per external step; 12 external steps
+    plain = [ld for st in schedule(3600, 3600, 0.5, 3600) for ld in st.loads]
+    rolled = [ld for st in schedule(3600, 3600, 0.5, 3600, hydro=_Files(4)) for ld in st.loads]
+    assert [ld.slot for ld in plain] == [ld.slot for ld in rolled]
+    # tstep moves to record 1 of the next file once it passes 4
+    assert [(ld.filenum, ld.record) for ld in rolled] == [
+        (3, 1), (3, 2), (3, 3), (3, 3), (3, 4), (4, 1), (4, 2), (4, 3), (4, 4), (5, 1), (5, 2),
+        (5, 3), (5, 4)]
+    assert [ld.record for ld in plain] == [(f - 3) * 4 + r for f, r in
+                                          ((ld.filenum, ld.record) for ld in rolled)]
+    with pytest.raises(RuntimeError):
+        list(schedule(3600, 3600, 0.5, 3600, hydro=_Files(4, multifile=False)))
+
+
+def history_set(tmp_path, cfg, tdim, nfiles, ftype="d"):
+    """cfg's records 1 .. tdim*nfiles in files his_0001.nc, ...; returns the prefix."""
+    recs = RecordSource(cfg)
+    g = cfg.grids[0]
+    for k in range(nfiles):
+        ids = range(k * tdim + 1, (k + 1) * tdim + 1)
+        write_history(tmp_path / f"his_{k + 1:04d}.nc", g, [recs.get(1, r) for r in ids],
+                      [(r - 1) * cfg.params.dt for r in ids], ftype=ftype)
+    return str(tmp_path / "his_")
+
+
+def test_grid_and_records_from_files(tmp_path):
+    cfg = synth.config_mini(n_particles=10)
+    prefix = history_set(tmp_path, cfg, 4, 2)
+    g, rg = cfg.grids[0], read_grid(prefix + "0001.nc")
+    assert (rg.L, rg.M, rg.N, rg.vtransform, rg.spherical) == (g.L, g.M, g.N, g.vtransform, 0)
+    for k in ("h", "pm", "pn", "mask_rho", "mask_u", "mask_v", "s_rho", "cs_r", "s_w", "cs_w"):
+        assert np.array_equal(getattr(rg, k), getattr(g, k)), k
+    assert np.array_equal(rg.lon, g.x_rho) and rg.reftime == 0.0
+    assert rg.time_units == "seconds since 2000-01-01 00:00:00"
+    hf = HydroFiles([prefix], filenum=1, multifile=True)
+    recs = RecordSource(cfg)
+    for filenum, tstep, glob in ((1, 1, 1), (1, 4, 4), (2, 1, 5), (2, 3, 7)):
+        a, b = hf.get(1, tstep, filenum), recs.get(1, glob)
+        for k in ("zeta", "u", "v", "w", "aks"):
+            assert np.array_equal(a[k], b[k]), (filenum, tstep, k)
+    assert hf.tdim(1) == 4
+    hf.close()
+
+
+def test_oracle_run_from_files_equals_in_memory(tmp_path):
+    cfg = synth.config_mini(n_particles=300, days=0.5)
+    prefix = history_set(tmp_path, cfg, 4, 4)
+    rg = read_grid(prefix + "0001.nc")
+    cfg2 = synth.Config(cfg.name, [rg], cfg.hydro, cfg.params, cfg.particles, cfg.refine)
+    hf = HydroFiles([prefix], filenum=1, multifile=True)
+    a = O.run_oracle(cfg)
+    b = O.run_oracle(cfg2, records=hf)
+    hf.close()
+    for k in ("x", "y", "z", "status", "age"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
