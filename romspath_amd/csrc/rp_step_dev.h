@@ -811,10 +811,41 @@ __device__ __forceinline__ Band band_of(double p, double h, double serr)
     return B;
 }
 
+// The stored 1/d and l1 rows in FP32.  The LDL' pivots of the Toeplitz
+// band converge along the rows, and their FP32 roundings stop changing
+// after a few to a few dozen rows (earlier the larger p is).  A storing
+// sweep notes the first row m from which every stored pair equals the one
+// before it; rows from m on then all hold the last row's pair, and the back
+// substitution takes it from registers instead of reading those rows.
+struct FactorTail {
+    int m;
+    float rd, l1;
+};
+struct TailTrack {
+    int m;
+    float rd, l1;
+    __device__ __forceinline__ void init()
+    {
+        m = 0; rd = 0.0f; l1 = 0.0f;
+    }
+    __device__ __forceinline__ void put(const Scr32& S, const FastLayout& Lo, int r, double rdv,
+                                        double l1v)
+    {
+        const float frd = (float)rdv, fl1 = (float)l1v;
+        S.st_nt(Lo.rd + r, frd);
+        S.st_nt(Lo.l1 + r, fl1);
+        m = (frd == rd && fl1 == l1) ? m : r + 1;
+        rd = frd;
+        l1 = fl1;
+    }
+    __device__ __forceinline__ FactorTail tail() const { return FactorTail{m, rd, l1}; }
+};
+
 // One row of the sweep for G(p) and dG/dp (qn: qty of the next row)
 struct GSweep {
     double d, dd, e, de, dn, ddn, en, den;
     double w, dw, wp, dwp, l2p, dl2p, g, gp;
+    TailTrack tt;
     // STORE: also keep 1/d, w and l1 of the row, exactly as the forward half
     // of the full solve at this p would (same operations, same order)
     template <bool STORE = false>
@@ -824,9 +855,8 @@ struct GSweep {
 #pragma clang fp contract(fast)
         const double rd = rcp64(d);
         if constexpr (STORE) {
-            S->st_nt(Lo->rd + m, (float)rd);
             S->st_nt(Lo->w + m, (float)w);
-            S->st_nt(Lo->l1 + m, (float)(e * rd));
+            tt.put(*S, *Lo, m, rd, e * rd);
         }
         const double wrd = w * rd;
         g += w * wrd;
@@ -848,9 +878,10 @@ struct GSweep {
 
 // G(p) and dG/dp by one LDL' sweep over the n interior unknowns.  qty row n
 // holds 0, so every chunk of rows reads the next row's qty without a guard.
+// STORE: keep the factor rows for back_u (tail: where they stop changing).
 template <bool STORE = false>
 __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, const FastLayout& Lo,
-                                         double& G, double& Gp)
+                                         double& G, double& Gp, FactorTail* tail = nullptr)
 {
     const int oq = Lo.q;
     GSweep sw;
@@ -858,6 +889,7 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, c
     sw.dn = B.a; sw.ddn = B.da; sw.en = B.b; sw.den = B.db;
     sw.w = (double)S(oq); sw.dw = 0.0; sw.wp = 0.0; sw.dwp = 0.0; sw.l2p = 0.0; sw.dl2p = 0.0;
     sw.g = 0.0; sw.gp = 0.0;
+    if constexpr (STORE) sw.tt.init();
     int m0 = 0;
     for (; m0 + kCh <= n; m0 += kCh) {
         float qn[kCh];
@@ -869,6 +901,7 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, c
     for (; m0 < n; m0++) sw.template row<STORE>(B, (double)S(oq + m0 + 1), &S, &Lo, m0);
     G = sw.g;
     Gp = sw.gp;
+    if constexpr (STORE) *tail = sw.tt.tail();
 }
 
 // The p = 0 sweep from universal tables.  At p = 0 the band is
@@ -923,15 +956,15 @@ __device__ __forceinline__ void stream_G_p0(double h, double serr, int n, const 
 // the way back.
 struct FSweep {
     double d, e, dn, en, w, wp, l2p;
+    TailTrack tt;
     __device__ __forceinline__ void row(const Band& B, const FastLayout& Lo, const Scr32& S, int m,
                                         double qn)
     {
 #pragma clang fp contract(fast)
         const double rd = rcp64(d);
         const double rat = e * rd, rat2 = B.c * rd;
-        S.st_nt(Lo.rd + m, (float)rd);
         S.st_nt(Lo.w + m, (float)w);
-        S.st_nt(Lo.l1 + m, (float)rat);
+        tt.put(S, Lo, m, rd, rat);
         const double d1 = dn - rat * e;
         const double e1 = en - rat * B.c;
         const double d2 = B.a - rat2 * B.c;
@@ -957,12 +990,14 @@ struct BSweep {
     }
 };
 
-__device__ __forceinline__ void back_u(const Band& B, const FastLayout& Lo, const Scr32& S);
+__device__ __forceinline__ void back_u(const Band& B, const FastLayout& Lo, const Scr32& S,
+                                       const FactorTail& tail);
 
 __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, const Scr32& S)
 {
     const int n = Lo.np - 2;
     FSweep fw;
+    fw.tt.init();
     fw.d = B.a; fw.e = B.b; fw.dn = B.a; fw.en = B.b;
     fw.w = (double)S(Lo.q); fw.wp = 0.0; fw.l2p = 0.0;
     int m0 = 0;
@@ -974,16 +1009,52 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
         for (int k = 0; k < kCh; k++) fw.row(B, Lo, S, m0 + k, (double)qn[k]);
     }
     for (; m0 < n; m0++) fw.row(B, Lo, S, m0, (double)S(Lo.q + m0 + 1));
-    back_u(B, Lo, S);
+    back_u(B, Lo, S, fw.tt.tail());
 }
 
-// back substitution from the stored 1/d, w, l1 rows: u(2..np-1) into Lo.u
-__device__ __forceinline__ void back_u(const Band& B, const FastLayout& Lo, const Scr32& S)
+// smallest x in [0, hi] with pred(x) true in every active lane (pred
+// monotone in x, true at hi), by bisection on wave votes: wave-uniform
+template <class P>
+__device__ __forceinline__ int wave_first(int hi, P pred)
+{
+    int lo = 0;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (__all(pred(mid))) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// back substitution from the stored 1/d, w, l1 rows: u(2..np-1) into Lo.u.
+// Rows from tail.m on hold the tail's pair, taken from registers.  The
+// lanes of a wave have different m: rows from the largest m on take the
+// tail in every lane, rows below the smallest m are read in every lane, and
+// rows in between are read and then replaced by the tail (the same value)
+// where the lane's tail starts at or before them.
+__device__ __forceinline__ void back_u(const Band& B, const FastLayout& Lo, const Scr32& S,
+                                       const FactorTail& tail)
 {
     const int n = Lo.np - 2;
     BSweep bw;
     bw.up1 = 0.0; bw.up2 = 0.0; bw.c = B.c; bw.n = n;
+    const int tm = tail.m;
+    const int mhi = wave_first(n, [&](int x) { return tm <= x; });
+    const int mlo = n - wave_first(n, [&](int x) { return tm >= n - x; });
     int m1 = n - 1;
+    for (; m1 - kCh + 1 >= mhi; m1 -= kCh) {
+        float wv[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; k++) wv[k] = S.ld_nt(Lo.w + m1 - k);
+#pragma unroll
+        for (int k = 0; k < kCh; k++) bw.row(Lo, S, m1 - k, wv[k], tail.rd, tail.l1);
+    }
+    for (; m1 >= mhi; m1--) bw.row(Lo, S, m1, S.ld_nt(Lo.w + m1), tail.rd, tail.l1);
+    auto mixed = [&](int m, float r, float l) {
+        const bool st = m < tm;
+        bw.row(Lo, S, m, S.ld_nt(Lo.w + m), st ? r : tail.rd, st ? l : tail.l1);
+    };
+    for (; m1 >= mlo; m1--) mixed(m1, S.ld_nt(Lo.rd + m1), S.ld_nt(Lo.l1 + m1));
     for (; m1 - kCh + 1 >= 0; m1 -= kCh) {
         float wv[kCh], rv[kCh], a1[kCh];
 #pragma unroll
@@ -1173,6 +1244,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     S(Lo.q + n) = 0.0f;   // read as the next row's qty by the last row of a sweep
     double p = 0.0, q = 0.0, sfq;
     bool stored = false;
+    FactorTail tail{0, 0.0f, 0.0f};
     if (s <= 0.0) {
         p = 1.0;
     } else {
@@ -1193,7 +1265,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
                 // factor rows, so that the accepted p needs only the back
                 // substitution
                 stored = it >= kStoreFrom;
-                if (stored) stream_G<true>(band_of(pq, h, serr), n, S, Lo, G, Gp);
+                if (stored) stream_G<true>(band_of(pq, h, serr), n, S, Lo, G, Gp, &tail);
                 else stream_G(band_of(pq, h, serr), n, S, Lo, G, Gp);
                 sfq = 6.0 * (G + pq * Gp) / ((1.0 + q) * (1.0 + q));
                 if (fabs(sfq - s) <= 0.01 * s) break;
@@ -1210,7 +1282,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
             p = q / (1.0 + q);
         }
     }
-    if (stored) back_u(band_of(p, h, serr), Lo, S);
+    if (stored) back_u(band_of(p, h, serr), Lo, S, tail);
     else solve_u(band_of(p, h, serr), Lo, S);
     const double k2 = 6.0 * (1.0 - p) * (serr * serr);
     PieceCache pc;
