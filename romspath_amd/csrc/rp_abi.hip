@@ -579,7 +579,8 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
         int& bpc = c->vwalk_blocks_per_cu[f32 ? 0 : 1][nested ? 1 : 0];
         if (bpc == 0) {
             int nb = 0;
-            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, P.k[2], block, 0));
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, P.k[2], block,
+                                                                   f32 ? (size_t)block * kLdsQ * 4 : 0));
             bpc = std::max(nb, 1);
         }
         P.grid_vt = std::min<int64_t>(P.grid_all, (int64_t)c->ncu * bpc);
@@ -675,8 +676,9 @@ int enqueue_step(Ctx* c, StepPlan& P, const Ctx::StepEvents* ev)
             for (int k = 0; k < 3; k++) c->cur_slots[k] = A.slots.s[k];
         }
     }
+    const size_t lds = c->fbytes == 4 ? (size_t)P.block * kLdsQ * 4 : 0;   // qty rows (FP32 form)
     if (!P.vt) {
-        hipLaunchKernelGGL(P.k[0], dim3((unsigned)P.grid_all), bd, 0, c->stream, c->dgrids, A);
+        hipLaunchKernelGGL(P.k[0], dim3((unsigned)P.grid_all), bd, lds, c->stream, c->dgrids, A);
         HIPCHK(c, hipGetLastError());
         if (ev) HIPCHK(c, hipEventRecord(ev->e[1], c->stream));
     } else {
@@ -684,7 +686,7 @@ int enqueue_step(Ctx* c, StepPlan& P, const Ctx::StepEvents* ev)
         HIPCHK(c, hipGetLastError());
         if (ev) HIPCHK(c, hipEventRecord(ev->e[1], c->stream));
         hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, c->stream, c->work);
-        hipLaunchKernelGGL(P.k[2], dim3((unsigned)P.grid_vt), bd, 0, c->stream, c->dgrids, A);
+        hipLaunchKernelGGL(P.k[2], dim3((unsigned)P.grid_vt), bd, lds, c->stream, c->dgrids, A);
     }
     HIPCHK(c, hipGetLastError());
     if (ev) HIPCHK(c, hipEventRecord(ev->e[2], c->stream));

@@ -11,6 +11,10 @@ namespace rp {
 // the stop is counted (rp_secant_capped reports it).
 constexpr int kSecantCap = 64;
 
+// qty rows of the FP32-field vertical walk kept in LDS per lane: the FP32
+// step kernels take blockDim x kLdsQ x 4 bytes of dynamic LDS
+constexpr int kLdsQ = 25;
+
 struct SlotSel {
     int s[3];   // physical slot (0..3) holding t_b, t_c, t_f
 };

@@ -386,15 +386,23 @@ using Scr = ScrT<double>;
 // row's byte offset is a scalar offset, so a row access needs no address
 // arithmetic per lane.  S.at(row) takes a per-lane row.  Accesses beyond
 // the descriptor's extent read 0 and are not written.
+// The first kLdsQ rows of the FP32 work rows (the qty rows, which every
+// secant sweep reads) are also kept in LDS, kLdsQ consecutive words per
+// lane of the block (dynamic LDS of blockDim x kLdsQ x 4 bytes; the odd
+// stride puts the 64 lanes of a row in distinct banks).
+extern __shared__ float rp_qlds[];
+
 struct Scr32 {
     __amdgpu_buffer_rsrc_t r;
     uint32_t voff;   // lane * 4
     uint32_t rowb;   // T * 4
+    float* lq;       // this lane's LDS rows: qty row j < kLdsQ at lq[j]
     __device__ __forceinline__ Scr32(float* base, uint32_t bytes, uint32_t lane, uint32_t T)
         : r(__builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, 0x00020000)), voff(lane * 4u),
-          rowb(T * 4u)
+          rowb(T * 4u), lq(rp_qlds + threadIdx.x * kLdsQ)
     {
     }
+    __device__ __forceinline__ float lq_ld(int j) const { return lq[j]; }
     struct Ref {
         const Scr32& s;
         uint32_t vo, so;
@@ -876,6 +884,12 @@ struct GSweep {
     }
 };
 
+// qty row j (wave-uniform j) from LDS or global memory
+__device__ __forceinline__ float qty_at(const Scr32& S, int oq, int j)
+{
+    return j < kLdsQ ? S.lq_ld(j) : (float)S(oq + j);
+}
+
 // G(p) and dG/dp by one LDL' sweep over the n interior unknowns.  qty row n
 // holds 0, so every chunk of rows reads the next row's qty without a guard.
 // STORE: keep the factor rows for back_u (tail: where they stop changing).
@@ -887,10 +901,17 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, c
     GSweep sw;
     sw.d = B.a; sw.dd = B.da; sw.e = B.b; sw.de = B.db;
     sw.dn = B.a; sw.ddn = B.da; sw.en = B.b; sw.den = B.db;
-    sw.w = (double)S(oq); sw.dw = 0.0; sw.wp = 0.0; sw.dwp = 0.0; sw.l2p = 0.0; sw.dl2p = 0.0;
+    sw.w = (double)S.lq_ld(0); sw.dw = 0.0; sw.wp = 0.0; sw.dwp = 0.0; sw.l2p = 0.0; sw.dl2p = 0.0;
     sw.g = 0.0; sw.gp = 0.0;
     if constexpr (STORE) sw.tt.init();
     int m0 = 0;
+    for (; m0 + kCh < kLdsQ && m0 + kCh <= n; m0 += kCh) {
+        float qn[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; k++) qn[k] = S.lq_ld(m0 + k + 1);
+#pragma unroll
+        for (int k = 0; k < kCh; k++) sw.template row<STORE>(B, (double)qn[k], &S, &Lo, m0 + k);
+    }
     for (; m0 + kCh <= n; m0 += kCh) {
         float qn[kCh];
 #pragma unroll
@@ -898,7 +919,7 @@ __device__ __forceinline__ void stream_G(const Band& B, int n, const Scr32& S, c
 #pragma unroll
         for (int k = 0; k < kCh; k++) sw.template row<STORE>(B, (double)qn[k], &S, &Lo, m0 + k);
     }
-    for (; m0 < n; m0++) sw.template row<STORE>(B, (double)S(oq + m0 + 1), &S, &Lo, m0);
+    for (; m0 < n; m0++) sw.template row<STORE>(B, (double)qty_at(S, oq, m0 + 1), &S, &Lo, m0);
     G = sw.g;
     Gp = sw.gp;
     if constexpr (STORE) *tail = sw.tt.tail();
@@ -925,7 +946,7 @@ __device__ __forceinline__ void stream_G_p0(double h, double serr, int n, const 
     const double r = serr / h;
     const double s = 6.0 * r * r;
     const double kap = h / s;
-    double w = (double)S(oq), om = 0.0, wp = 0.0, omp = 0.0;
+    double w = (double)S.lq_ld(0), om = 0.0, wp = 0.0, omp = 0.0;
     double A = 0.0, B = 0.0, C = 0.0, r2p = 0.0, dr2p = 0.0;
     auto row = [&](int m, double qn) {
         const double* T = kP0Tab + 6 * m;
@@ -939,6 +960,13 @@ __device__ __forceinline__ void stream_G_p0(double h, double serr, int n, const 
         r2p = T[3]; dr2p = T[5];
     };
     int m0 = 0;
+    for (; m0 + kCh < kLdsQ && m0 + kCh <= n; m0 += kCh) {
+        float qn[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; k++) qn[k] = S.lq_ld(m0 + k + 1);
+#pragma unroll
+        for (int k = 0; k < kCh; k++) row(m0 + k, (double)qn[k]);
+    }
     for (; m0 + kCh <= n; m0 += kCh) {
         float qn[kCh];
 #pragma unroll
@@ -946,7 +974,7 @@ __device__ __forceinline__ void stream_G_p0(double h, double serr, int n, const 
 #pragma unroll
         for (int k = 0; k < kCh; k++) row(m0 + k, (double)qn[k]);
     }
-    for (; m0 < n; m0++) row(m0, (double)S(oq + m0 + 1));
+    for (; m0 < n; m0++) row(m0, (double)qty_at(S, oq, m0 + 1));
     G = A / s;
     Gp = (A + kap * (2.0 * B - C)) / s;
 }
@@ -999,8 +1027,15 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
     FSweep fw;
     fw.tt.init();
     fw.d = B.a; fw.e = B.b; fw.dn = B.a; fw.en = B.b;
-    fw.w = (double)S(Lo.q); fw.wp = 0.0; fw.l2p = 0.0;
+    fw.w = (double)S.lq_ld(0); fw.wp = 0.0; fw.l2p = 0.0;
     int m0 = 0;
+    for (; m0 + kCh < kLdsQ && m0 + kCh <= n; m0 += kCh) {
+        float qn[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; k++) qn[k] = S.lq_ld(m0 + k + 1);
+#pragma unroll
+        for (int k = 0; k < kCh; k++) fw.row(B, Lo, S, m0 + k, (double)qn[k]);
+    }
     for (; m0 + kCh <= n; m0 += kCh) {
         float qn[kCh];
 #pragma unroll
@@ -1008,7 +1043,7 @@ __device__ __forceinline__ void solve_u(const Band& B, const FastLayout& Lo, con
 #pragma unroll
         for (int k = 0; k < kCh; k++) fw.row(B, Lo, S, m0 + k, (double)qn[k]);
     }
-    for (; m0 < n; m0++) fw.row(B, Lo, S, m0, (double)S(Lo.q + m0 + 1));
+    for (; m0 < n; m0++) fw.row(B, Lo, S, m0, (double)qty_at(S, Lo.q, m0 + 1));
     back_u(B, Lo, S, fw.tt.tail());
 }
 
@@ -1242,6 +1277,15 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     const double serr = A.serr, s = A.smth;
     const int n = np - 2;
     S(Lo.q + n) = 0.0f;   // read as the next row's qty by the last row of a sweep
+    // the column walk wrote every qty row to global memory; the first kLdsQ
+    // go to LDS too, where the sweeps read them
+    for (int j0 = 0; j0 < kLdsQ; j0 += 5) {
+        float t[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) t[j] = S(Lo.q + j0 + j);
+#pragma unroll
+        for (int j = 0; j < 5; j++) S.lq[j0 + j] = t[j];
+    }
     double p = 0.0, q = 0.0, sfq;
     bool stored = false;
     FactorTail tail{0, 0.0f, 0.0f};
