@@ -327,7 +327,9 @@ int64_t sort_keys(const Ctx* c, SortArgs& K)
 int sort_buffers(Ctx* c, int64_t bins)
 {
     if (c->sort_nb < bins) {
-        if (dalloc(c, &c->sort_cnt, (size_t)bins)) return RP_E_ALLOC;
+        // the counts, then (from a multiple of 4) one total per scan tile
+        const int64_t tiles = (bins + kScanTile - 1) / kScanTile;
+        if (dalloc(c, &c->sort_cnt, (size_t)(((bins + 3) & ~3LL) + tiles))) return RP_E_ALLOC;
         c->sort_nb = bins;
     }
     if (c->alt_n != c->n) {
@@ -391,7 +393,11 @@ int sort_range(Ctx* c, int64_t q0, int64_t cnt)
     const unsigned gp = (unsigned)std::min<int64_t>((cnt + 255) / 256, 8192);
     hipLaunchKernelGGL(sort_fill, dim3(gb), dim3(256), 0, c->stream, c->sort_cnt, bins, 0u);
     hipLaunchKernelGGL(sort_count, dim3(gp), dim3(256), 0, c->stream, K);
-    hipLaunchKernelGGL(sort_scan, dim3(1), dim3(1024), 0, c->stream, c->sort_cnt, bins);
+    const int64_t tiles = (bins + kScanTile - 1) / kScanTile;
+    uint32_t* sums = c->sort_cnt + ((bins + 3) & ~3LL);
+    hipLaunchKernelGGL(sort_scan_tiles, dim3((unsigned)tiles), dim3(1024), 0, c->stream, c->sort_cnt, bins, sums);
+    hipLaunchKernelGGL(sort_scan, dim3(1), dim3(1024), 0, c->stream, sums, tiles);
+    hipLaunchKernelGGL(sort_scan_add, dim3((unsigned)tiles), dim3(1024), 0, c->stream, c->sort_cnt, bins, sums);
     hipLaunchKernelGGL(sort_scatter, dim3(gp), dim3(256), 0, c->stream, K);
     double** cur64[PermArgs::kF64];
     uint8_t** cur8[PermArgs::kU8];

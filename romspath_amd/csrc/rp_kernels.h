@@ -190,6 +190,9 @@ struct PermArgs {
 __global__ void sort_fill(uint32_t* a, int64_t n, uint32_t v);
 __global__ void sort_count(SortArgs K);
 __global__ void sort_scan(uint32_t* cnt, int64_t nb);
+constexpr int kScanTile = 4096;
+__global__ void sort_scan_tiles(uint32_t* cnt, int64_t nb, uint32_t* sums);
+__global__ void sort_scan_add(uint32_t* cnt, int64_t nb, const uint32_t* sums);
 __global__ void sort_scatter(SortArgs K);
 __global__ void sort_permute(PermArgs P);
 __global__ void sort_copy_back(PermArgs P);

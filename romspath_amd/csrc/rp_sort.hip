@@ -75,6 +75,59 @@ __global__ void __launch_bounds__(1024) sort_scan(uint32_t* cnt, int64_t nb)
     }
 }
 
+// Exclusive scan of cnt[0..nb) in tiles of kScanTile values (1024 threads x
+// 4 consecutive values), three launches: sort_scan_tiles scans each tile in
+// place and writes its total to sums[tile]; sort_scan (one block) scans the
+// tile totals; sort_scan_add adds each tile's offset.
+__device__ __forceinline__ uint32_t block_excl(uint32_t s, uint32_t* wsum, uint32_t& total)
+{
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint32_t v = s;                                   // inclusive scan within the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) wsum[wv] = v;
+    __syncthreads();
+    if (wv == 0) {
+        uint32_t w = lane < 16 ? wsum[lane] : 0u;
+        for (int o = 1; o < 16; o <<= 1) {
+            const uint32_t u = __shfl_up(w, o, 64);
+            if (lane >= o) w += u;
+        }
+        if (lane < 16) wsum[lane] = w;
+    }
+    __syncthreads();
+    total = wsum[15];
+    return (v - s) + (wv > 0 ? wsum[wv - 1] : 0u);
+}
+
+__global__ void __launch_bounds__(1024) sort_scan_tiles(uint32_t* cnt, int64_t nb, uint32_t* sums)
+{
+    __shared__ uint32_t wsum[16];
+    const int64_t i0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * 4;
+    uint32_t x[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = i0 + k < nb ? cnt[i0 + k] : 0u;
+    uint32_t total;
+    uint32_t run = block_excl(x[0] + x[1] + x[2] + x[3], wsum, total);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (i0 + k < nb) cnt[i0 + k] = run;
+        run += x[k];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) sort_scan_add(uint32_t* cnt, int64_t nb, const uint32_t* sums)
+{
+    const uint32_t off = sums[blockIdx.x];
+    const int64_t i0 = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * 4;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (i0 + k < nb) cnt[i0 + k] += off;
+}
+
 __global__ void __launch_bounds__(256) sort_scatter(SortArgs K)
 {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < K.n;
