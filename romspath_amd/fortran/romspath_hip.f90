@@ -57,6 +57,7 @@ module romspath_hip
             rp_set_grid_angle, rp_locate_particles, rp_set_hydro_tracers, rp_sample_tracers, &
             rp_set_hydro_bstress, rp_bottom_begin, rp_bottom_update, rp_bottom_get, &
             rp_tracers_begin, rp_tracers_update, rp_tracers_get, &
+            rp_ncin_open, rp_ncin_dim, rp_ncin_var, rp_ncin_get, rp_ncin_close, &
             rp_error_text
 
   interface
@@ -292,6 +293,42 @@ module romspath_hip
       real(c_double), intent(in) :: ex(3), ix(3)
       integer(c_int32_t), value :: t_b, t_c, t_f
       real(c_double), intent(out) :: salt(*), temp(*)
+    end function
+
+    ! netCDF classic input (include/rp_ncin.h): NF90_OPEN / NF90_INQUIRE_DIMENSION /
+    ! NF90_INQUIRE_VARIABLE / NF90_GET_VAR of one record / NF90_CLOSE without netCDF-Fortran
+    integer(c_int) function rp_ncin_open(f, path) bind(C, name="rp_ncin_open")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), intent(out) :: f
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+
+    integer(c_int) function rp_ncin_dim(f, name, len) bind(C, name="rp_ncin_dim")
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: f
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int64_t), intent(out) :: len
+    end function
+
+    integer(c_int) function rp_ncin_var(f, name, ndims, shape, is_record) bind(C, name="rp_ncin_var")
+      import :: c_int, c_ptr, c_char, c_int32_t, c_int64_t
+      type(c_ptr), value :: f
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int32_t), intent(out) :: ndims, is_record
+      integer(c_int64_t), intent(out) :: shape(8)
+    end function
+
+    integer(c_int) function rp_ncin_get(f, name, rec, out, count) bind(C, name="rp_ncin_get")
+      import :: c_int, c_ptr, c_char, c_int64_t, c_double
+      type(c_ptr), value :: f
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int64_t), value :: rec, count
+      real(c_double), intent(out) :: out(*)
+    end function
+
+    integer(c_int) function rp_ncin_close(f) bind(C, name="rp_ncin_close")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: f
     end function
 
     ! SaltTempOn / SaltTempMean kept per step: P_Salt, P_Temp (or their means)

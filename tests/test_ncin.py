@@ -270,3 +270,46 @@ def test_oracle_run_from_files_equals_in_memory(tmp_path):
     hf.close()
     for k in ("x", "y", "z", "status", "age"):
         assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+def test_fortran_binding_reads_a_record(tmp_path):
+    """The ISO_C_BINDING interfaces of rp_ncin_* (romspath_hip.f90) used from
+    Fortran as updateHydro would: record 2 of zeta into a (xi_rho, eta_rho)
+    array, checked against scipy's reading of the same file."""
+    import subprocess
+    from romspath_amd.build import FC, PKG
+    cfg = synth.config_mini(n_particles=10)
+    prefix = history_set(tmp_path, cfg, 3, 1)
+    g = cfg.grids[0]
+    src = tmp_path / "rd.f90"
+    src.write_text(f"""program rd
+  use iso_c_binding
+  use romspath_hip
+  implicit none
+  type(c_ptr) :: f
+  integer(c_int64_t) :: nt
+  double precision :: z({g.L}, {g.M})
+  integer :: rc
+  rc = rp_ncin_open(f, '{prefix}0001.nc' // c_null_char)
+  if (rc /= 0) stop 1
+  rc = rp_ncin_dim(f, 'ocean_time' // c_null_char, nt)
+  rc = rp_ncin_get(f, 'zeta' // c_null_char, 1_c_int64_t, z, int({g.L * g.M}, c_int64_t))
+  if (rc /= 0) stop 2
+  write (*, '(I4, 3ES25.17)') nt, z(1, 1), z({g.L}, 1), z(3, {g.M})
+  rc = rp_ncin_close(f)
+end program rd
+""")
+    exe = tmp_path / "rd"
+    lib = PKG / "lib"
+    r = subprocess.run([FC, "-O1", "-I", str(lib / "fmod"), str(src), str(lib / "fmod" / "romspath_hip.o"),
+                        "-L", str(lib), "-lrp", f"-Wl,-rpath,{lib}", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = r.stdout.split()
+    ref = netcdf_file(prefix + "0001.nc", "r", mmap=False)
+    zeta = ref.variables["zeta"][1].copy()          # (eta_rho, xi_rho)
+    ref.close()
+    assert int(vals[0]) == 3
+    assert [float(v) for v in vals[1:]] == [zeta[0, 0], zeta[0, g.L - 1], zeta[g.M - 1, 2]]
