@@ -17,7 +17,10 @@
  *     reference reads out of bounds after a failed nested hand-off,
  *     ROMSPath.f90:837-843);
  *   - the w-level scan guards its lower index (interpolation_module.f90:572
- *     would read s_w(0) if z sat exactly on the bed).
+ *     would read s_w(0) if z sat exactly on the bed);
+ *   - getInterpStr clamps its node reads into the bustr/bvstr arrays
+ *     (interpolation_module.f90:1012-1174 reads outside them only for a
+ *     particle on the grid's last row or column).
  */
 #ifdef _OPENMP
 #include <omp.h>
