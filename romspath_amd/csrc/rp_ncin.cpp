@@ -111,7 +111,7 @@ bool read_atts(Cur& c, int nn, std::vector<Att>& out)
         a.type = (int32_t)c.be(4);
         a.n = (int64_t)c.be(nn);
         const int ts = type_size(a.type);
-        if (c.bad || ts == 0 || a.n < 0) return false;
+        if (c.bad || ts == 0 || a.n < 0 || a.n > (1LL << 31)) return false;
         const uint64_t bytes = (uint64_t)a.n * ts;
         if (c.p + bytes > c.b.size()) return false;
         a.raw.assign(c.b.begin() + (long)c.p, c.b.begin() + (long)(c.p + bytes));
@@ -280,10 +280,10 @@ int rp_ncin_open(void** out, const char* path)
     r->f = f;
     r->version = magic[3];
     // the header ends before the first variable's data; read it in growing
-    // chunks until it parses
+    // chunks until it parses (headers beyond 256 MiB are taken as malformed)
     std::vector<unsigned char> buf;
     int rc = -1;
-    for (long long want = 1 << 16; rc != 0; want *= 4) {
+    for (long long want = 1 << 16; rc != 0 && want <= (1LL << 28); want *= 4) {
         const long long n = want < size ? want : size;
         buf.resize((size_t)n);
         std::fseek(f, 0, SEEK_SET);
