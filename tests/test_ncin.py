@@ -313,3 +313,38 @@ end program rd
     ref.close()
     assert int(vals[0]) == 3
     assert [float(v) for v in vals[1:]] == [zeta[0, 0], zeta[0, g.L - 1], zeta[g.M - 1, 2]]
+
+
+def test_read_flags_constants_and_offsets(tmp_path):
+    """updateHydro's read*/const* switches, TempOffset, and the SaltTempOn /
+    WriteBottom fields (hydrodynamic_module.f90:232-512)."""
+    from romspath_amd.roms import ReadFlags
+    cfg = synth.config_mini(n_particles=10)
+    g = cfg.grids[0]
+    recs = RecordSource(cfg)
+    rng = np.random.default_rng(5)
+    rows = []
+    for r in (1, 2):
+        rec = dict(recs.get(1, r))
+        rec["salt"] = 34.0 + rng.standard_normal((g.N, g.M, g.L))
+        rec["temp"] = 12.0 + rng.standard_normal((g.N, g.M, g.L))
+        rec["bustr"] = rng.standard_normal((g.M, g.L - 1))
+        rec["bvstr"] = rng.standard_normal((g.M - 1, g.L))
+        rows.append(rec)
+    write_history(tmp_path / "his.nc", g, rows, [0.0, 3600.0])
+    fl = ReadFlags(readZeta=False, constZeta=0.25, readW=False, constW=-1e-4, readSalt=True,
+                   readTemp=True, TempOffset=1.5, SaltTempOn=True, WriteBottom=True)
+    hf = HydroFiles([str(tmp_path / "his.nc")], multifile=False, flags=fl)
+    got = hf.get(1, 2)
+    assert np.all(got["zeta"] == 0.25) and got["zeta"].shape == (g.M, g.L)
+    assert np.all(got["w"] == -1e-4) and got["w"].shape == (g.N + 1, g.M, g.L)
+    assert np.array_equal(got["u"], rows[1]["u"]) and np.array_equal(got["aks"], rows[1]["aks"])
+    assert np.array_equal(got["salt"], rows[1]["salt"])
+    assert np.array_equal(got["temp"], rows[1]["temp"] + 1.5)
+    assert np.array_equal(got["bustr"], rows[1]["bustr"]) and np.array_equal(got["bvstr"], rows[1]["bvstr"])
+    fl2 = ReadFlags(SaltTempOn=True, readSalt=False, constSalt=35.0, readTemp=False, constTemp=10.0)
+    got = HydroFiles([str(tmp_path / "his.nc")], multifile=False, flags=fl2).get(1, 1)
+    assert np.all(got["salt"] == 35.0) and np.all(got["temp"] == 10.0)
+    with pytest.raises(IndexError):
+        hf.get(1, 3)
+    hf.close()
