@@ -88,3 +88,30 @@ def test_bottom_errors():
         assert np.all(hob == 9999.0) and np.all(bu == 0.0) and np.all(bv == 0.0)
     finally:
         run.close()
+
+
+def test_samplers_with_no_particles():
+    """Every sampler entry point accepts an empty particle set (n = 0) and
+    returns empty columns without launching anything."""
+    cfg = synth.config1(n_particles=0)
+    g = cfg.grids[0]
+    print("case samplers with no particles", flush=True)
+    run = GpuRun(cfg, precision="fp64")
+    try:
+        st = next(iter(schedule(cfg.params.dt, cfg.params.idt, cfg.params.days, cfg.params.iprint, 1)))
+        run.load(st)
+        rng = np.random.default_rng(0)
+        for slot in (1, 2, 3):
+            run.ctx.set_hydro_bstress(1, slot, *stress_fields(g))
+            run.ctx.set_hydro_tracers(1, slot, 34.0 + rng.standard_normal((g.N, g.M, g.L)),
+                                      rng.standard_normal((g.N, g.M, g.L)))
+        run.ctx.bottom_begin()
+        run.ctx.tracers_begin(True, 2)
+        run.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index)
+        run.ctx.bottom_update(st.ex, st.ix, st.tb, st.tc, st.tf)
+        run.ctx.tracers_update(st.ex, st.ix, st.tb, st.tc, st.tf)
+        for cols in (run.ctx.bottom_get(reset_hob=True), run.ctx.tracers_get(reset_mean=True),
+                     run.ctx.sample_tracers(st.ex, st.ix, st.tb, st.tc, st.tf)):
+            assert all(len(c) == 0 for c in cols)
+    finally:
+        run.close()
