@@ -343,3 +343,27 @@ def test_interv_matches_searchsorted():
     for xx in np.concatenate([rng.uniform(-60, 10, 500), xt]):
         want = int(np.clip(np.searchsorted(xt, xx, side="right"), 1, xt.size - 1))
         assert L.og_interv(O.dptr(xt), xt.size, xx, C.byref(mf)) == want
+
+
+def test_smooth_matches_committed_reference_vectors():
+    """The reference PPPACK's own outputs, committed as fixtures by
+    tests/golden/make_pppack_vectors.py: the restatement reproduces every
+    smoothing-spline case (sfp and coefficients) and every ppvalu value bit
+    for bit, also where the reference sources are absent."""
+    from pathlib import Path
+    gold = np.load(Path(__file__).parent / "golden" / "pppack_smooth.npz")
+    offs = gold["offs"]
+    for c, n in enumerate(gold["n"]):
+        sl = slice(offs[c], offs[c + 1])
+        x, y, dy = (np.ascontiguousarray(gold[k][sl]) for k in ("x", "y", "dy"))
+        v = np.zeros(n * 7)
+        a = np.zeros(n * 4)
+        sf = L.og_smooth(O.dptr(x), O.dptr(y), O.dptr(dy), int(n), float(gold["s"][c]), O.dptr(v), O.dptr(a))
+        assert sf == gold["sf"][c], c
+        assert np.array_equal(a, gold["a"][4 * offs[c]:4 * offs[c + 1]]), c
+    n = int(gold["n"][-1])
+    x = np.ascontiguousarray(gold["x"][offs[-2]:offs[-1]])
+    coef = np.ascontiguousarray(gold["a"][4 * offs[-2]:4 * offs[-1]])
+    for i, xx in enumerate(gold["ppvalu_x"]):
+        for jd in (0, 1):
+            assert L.og_ppvalu(O.dptr(x), O.dptr(coef), n - 1, 4, float(xx), jd) == gold["ppvalu"][i, jd]
