@@ -148,6 +148,16 @@ class RomsGrid:
     reftime: float = 0.0     # first value of the time variable, in seconds
     time_units: str = "seconds"
 
+    # the coordinates InitGrid keeps in lon_rho / lat_rho, which the nested
+    # grids' offsets are matched on (grid_module.f90:437-481)
+    @property
+    def x_rho(self):
+        return self.lon
+
+    @property
+    def y_rho(self):
+        return self.lat
+
 
 def _spherical(v):
     # ROMS writes spherical as an int flag; older files as the character T/F
@@ -216,8 +226,12 @@ class HydroFiles:
     against (ROMSPath.f90:473-480)."""
 
     def __init__(self, prefixes, filenum=1, numdigits=4, suffix=".nc", multifile=False,
-                 flags=None, time_dname="ocean_time", cache=True):
+                 flags=None, time_dname="ocean_time", cache=True, stokes_prefixes=None):
         self.prefixes = list(prefixes)
+        # -DSTOKES: ustokes / vstokes of the same record from a second file
+        # series (stokesprefix, hydrodynamic_module.f90:573-660), handed on
+        # as su / sv for the library to add to u / v
+        self.stokes_prefixes = None if stokes_prefixes is None else list(stokes_prefixes)
         self.filenum0 = int(filenum)
         self.numdigits = numdigits
         self.suffix = suffix
@@ -230,10 +244,12 @@ class HydroFiles:
     def path(self, ng, filenum):
         return file_name(self.prefixes[ng - 1], filenum, self.numdigits, self.suffix, self.multifile)
 
-    def _file(self, ng, filenum):
-        key = (ng, filenum)
+    def _file(self, ng, filenum, stokes=False):
+        key = (ng, filenum, stokes)
         if key not in self._files:
-            self._files[key] = NcFile(self.path(ng, filenum))
+            prefix = (self.stokes_prefixes if stokes else self.prefixes)[ng - 1]
+            self._files[key] = NcFile(file_name(prefix, filenum, self.numdigits, self.suffix,
+                                                self.multifile))
         return self._files[key]
 
     def tdim(self, filenum):
@@ -271,6 +287,10 @@ class HydroFiles:
         if fl.WriteBottom:
             rec["bustr"] = f.read("bustr", r)
             rec["bvstr"] = f.read("bvstr", r)
+        if self.stokes_prefixes is not None:
+            fs = self._file(ng, filenum, stokes=True)
+            rec["su"] = fs.read("ustokes", r)
+            rec["sv"] = fs.read("vstokes", r)
         if self.cache is not None:
             self.cache[key] = rec
         return rec

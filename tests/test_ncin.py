@@ -348,3 +348,52 @@ def test_read_flags_constants_and_offsets(tmp_path):
     with pytest.raises(IndexError):
         hf.get(1, 3)
     hf.close()
+
+
+def nested_history_set(tmp_path, cfg, nrec=6):
+    """One history series and one Stokes series (-DSTOKES) per nest of cfg,
+    records 1 .. nrec in file 0001; returns (prefixes, stokes prefixes)."""
+    recs = RecordSource(cfg)
+    prefixes, sprefixes = [], []
+    for gi, g in enumerate(cfg.grids):
+        d = tmp_path / f"g{gi}"
+        d.mkdir()
+        ids = range(1, nrec + 1)
+        rows = [recs.get(gi + 1, r) for r in ids]
+        write_history(d / "his_0001.nc", g, rows, [(r - 1) * cfg.params.dt for r in ids])
+        prefixes.append(str(d / "his_"))
+        f = netcdf_file(str(d / "stk_0001.nc"), "w", version=2)
+        f.createDimension("ocean_time", None)
+        for name, n in (("xi_u", g.L - 1), ("eta_u", g.M), ("xi_v", g.L), ("eta_v", g.M - 1),
+                        ("s_rho", g.N)):
+            f.createDimension(name, n)
+        su = f.createVariable("ustokes", "d", ("ocean_time", "s_rho", "eta_u", "xi_u"))
+        sv = f.createVariable("vstokes", "d", ("ocean_time", "s_rho", "eta_v", "xi_v"))
+        for r, row in enumerate(rows):
+            su[r] = row["su"]
+            sv[r] = row["sv"]
+        f.close()
+        sprefixes.append(str(d / "stk_"))
+    return prefixes, sprefixes
+
+
+def test_nested_oracle_run_from_files_equals_in_memory(tmp_path):
+    """Two nests, one history series each (child first, as the reference
+    orders them) and a Stokes series each: grids read back give the same
+    scl/off, and the oracle run from the files equals the in-memory run bit
+    for bit."""
+    from romspath_amd.nesting import scale_offsets
+    cfg = synth.config_mini_nested(n_particles=300)
+    prefixes, sprefixes = nested_history_set(tmp_path, cfg)
+    rgs = [read_grid(p + "0001.nc") for p in prefixes]
+    s1, o1 = scale_offsets(cfg.grids, cfg.refine)
+    s2, o2 = scale_offsets(rgs, cfg.refine)
+    assert np.array_equal(np.asarray(s1), np.asarray(s2)) and np.array_equal(np.asarray(o1), np.asarray(o2))
+    cfg2 = synth.Config(cfg.name, rgs, cfg.hydro, cfg.params, cfg.particles, cfg.refine)
+    hf = HydroFiles(prefixes, filenum=1, multifile=True, stokes_prefixes=sprefixes)
+    a = O.run_oracle(cfg)
+    b = O.run_oracle(cfg2, records=hf)
+    hf.close()
+    assert set(np.unique(a.gid)) == {1, 2}
+    for k in ("x", "y", "z", "status", "gid"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
