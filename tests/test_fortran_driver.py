@@ -66,3 +66,37 @@ def test_fortran_driver_matches_oracle(tmp_path):
 
 def test_fortran_driver_is_built():
     assert DRIVER.exists()
+
+
+@pytest.mark.gpu
+def test_fortran_driver_reads_history_files(tmp_path):
+    """The driver's $romsoutput mode: grid and records from ROMS history files
+    (netCDF classic, 4 records a file, rolled over to the second file at the
+    fifth external step) through rp_ncin; the result equals the oracle's on
+    the same records."""
+    from test_ncin import history_set
+    n, days = 300, 6 / 24.0
+    cfg = synth.config1(n_particles=n, days=days)
+    p = cfg.particles
+    prefix = history_set(tmp_path, cfg, 4, 2)
+    nml = NML.format(n=n, days=days) + (
+        f"&romsoutput prefix='{prefix}', suffix='.nc', filenum=1, numdigits=4, multifile=.TRUE.,\n"
+        "  time_vname='ocean_time', time_dname='ocean_time' /\n")
+    (tmp_path / "run.nml").write_text(nml)
+    g = cfg.grids[0]
+    lon, lat = (p.x - 1.0) * g.dx, (p.y - 1.0) * g.dy
+    np.savetxt(tmp_path / "par.txt", np.c_[lon, lat, p.z, p.dob], fmt="%.17e")
+    print("case fortran driver from history files", n, "particles", flush=True)
+    r = subprocess.run([str(DRIVER), str(tmp_path / "run.nml"), str(tmp_path / "par.txt"),
+                        str(tmp_path / "out.txt")], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    got = np.loadtxt(tmp_path / "out.txt")
+    loc = O.locate(cfg, lon, lat, p.z)
+    placed = synth.Particles(loc["x"], loc["y"], loc["z"], p.dob, gid=loc["gid"],
+                             status=loc["status"], isin=loc["isin"])
+    o = O.run_oracle(cfg, particles=placed)
+    assert np.array_equal(got[:, 3], o.status)
+    assert np.abs(got[:, 0] - o.x).max() <= 1e-9
+    assert np.abs(got[:, 1] - o.y).max() <= 1e-9
+    assert np.abs(got[:, 2] - o.z).max() <= 1e-9 * 50.0 / 20
