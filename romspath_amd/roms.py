@@ -239,17 +239,22 @@ class HydroFiles:
         self.flags = flags or ReadFlags()
         self.time_dname = time_dname
         self._files = {}
+        # the last few records read (a record is read again only for the
+        # repeated load of the schedule's third external step)
         self.cache = {} if cache else None
+        self.cache_size = 6
 
-    def path(self, ng, filenum):
-        return file_name(self.prefixes[ng - 1], filenum, self.numdigits, self.suffix, self.multifile)
+    def path(self, ng, filenum, stokes=False):
+        prefix = (self.stokes_prefixes if stokes else self.prefixes)[ng - 1]
+        return file_name(prefix, filenum, self.numdigits, self.suffix, self.multifile)
 
     def _file(self, ng, filenum, stokes=False):
         key = (ng, filenum, stokes)
         if key not in self._files:
-            prefix = (self.stokes_prefixes if stokes else self.prefixes)[ng - 1]
-            self._files[key] = NcFile(file_name(prefix, filenum, self.numdigits, self.suffix,
-                                                self.multifile))
+            # files of this series before the previous one are done with
+            for k in [k for k in self._files if k[0] == ng and k[2] == stokes and k[1] < filenum - 1]:
+                self._files.pop(k).close()
+            self._files[key] = NcFile(self.path(ng, filenum, stokes))
         return self._files[key]
 
     def tdim(self, filenum):
@@ -292,6 +297,8 @@ class HydroFiles:
             rec["su"] = fs.read("ustokes", r)
             rec["sv"] = fs.read("vstokes", r)
         if self.cache is not None:
+            if len(self.cache) >= self.cache_size:
+                self.cache.pop(next(iter(self.cache)))
             self.cache[key] = rec
         return rec
 
