@@ -364,8 +364,9 @@ def main():
         log("copied 1 GiB")
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(every GPU box here has a single MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(the particles would shard by release id, one process per GPU); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "c3":
