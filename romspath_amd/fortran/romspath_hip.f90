@@ -58,6 +58,7 @@ module romspath_hip
             rp_set_hydro_bstress, rp_bottom_begin, rp_bottom_update, rp_bottom_get, &
             rp_tracers_begin, rp_tracers_update, rp_tracers_get, &
             rp_ncin_open, rp_ncin_dim, rp_ncin_var, rp_ncin_get, rp_ncin_close, &
+            rp_ncin_att_text, rp_ncin_error, rp_nc_records, rp_nc_error, &
             rp_error_text
 
   interface
@@ -329,6 +330,33 @@ module romspath_hip
     integer(c_int) function rp_ncin_close(f) bind(C, name="rp_ncin_close")
       import :: c_int, c_ptr
       type(c_ptr), value :: f
+    end function
+
+    ! NF90_GET_ATT of a text attribute (var: c_null_ptr for a global one)
+    integer(c_int) function rp_ncin_att_text(f, var, att, buf, size, len) bind(C, name="rp_ncin_att_text")
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: f, var
+      character(kind=c_char), intent(in) :: att(*)
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_int64_t), value :: size
+      integer(c_int64_t), intent(out) :: len
+    end function
+
+    ! message of the reader's last failure (f: c_null_ptr for rp_ncin_open), a C string
+    type(c_ptr) function rp_ncin_error(f) bind(C, name="rp_ncin_error")
+      import :: c_ptr
+      type(c_ptr), value :: f
+    end function
+
+    ! records written so far, and the writer's last failure (a C string)
+    integer(c_int64_t) function rp_nc_records(nc) bind(C, name="rp_nc_records")
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: nc
+    end function
+
+    type(c_ptr) function rp_nc_error(nc) bind(C, name="rp_nc_error")
+      import :: c_ptr
+      type(c_ptr), value :: nc
     end function
 
     ! SaltTempOn / SaltTempMean kept per step: P_Salt, P_Temp (or their means)

@@ -40,3 +40,11 @@ def test_bad_arguments_return_codes():
     rc = L.rp_create(C.byref(h), C.byref(cfg))
     assert rc in (-1, -6)          # RP_E_ARG with a GPU, RP_E_NODEV without
     assert not h.value
+
+
+def test_fortran_module_binds_every_entry_point():
+    """romspath_hip.f90, the ISO_C_BINDING module a Fortran host uses, has an
+    interface for every entry point the headers declare."""
+    f90 = (ROOT / "romspath_amd" / "fortran" / "romspath_hip.f90").read_text()
+    bound = set(re.findall(r'bind\(C,\s*name="(rp_[a-z_0-9]+)"\)', f90))
+    assert declared() == bound
