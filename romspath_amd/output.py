@@ -86,9 +86,12 @@ class TrajectoryFile:
 def data_output(ctx, nc, model_time):
     """dataOutput (ROMSPath.f90:1012-1025): lon/lat at every particle, then
     one record.  ctx: abi.Context after rp_set_lonlat for every grid (and
-    rp_bottom_begin when the file has the WriteBottom variables: P_HOB is
-    read and put back to 9999, ROMSPath.f90:2056-2065)."""
+    rp_tracers_begin / rp_bottom_begin when the file has the SaltTempOn /
+    WriteBottom variables: the means and P_HOB are read and reset,
+    ROMSPath.f90:2009-2035, 2056-2065)."""
     p = ctx.get_particles()
+    if nc.flags & FLAGS["salttemp"]:
+        p["salt"], p["temp"] = ctx.tracers_get(reset_mean=True)
     if nc.flags & FLAGS["bottom"]:
         p["hob"], p["bustr"], p["bvstr"] = ctx.bottom_get(reset_hob=True)
     lon, lat = ctx.sample_lonlat()

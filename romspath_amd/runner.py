@@ -62,20 +62,37 @@ class GpuRun:
                 self.ctx.set_hydro_slot(g, slot, self.records.get(g, slot))
 
     def load(self, st):
+        """updateHydro for the step's loads: the slot's hydro fields, and salt/temp
+        (SaltTempOn) and bustr/bvstr (WriteBottom) when the record has them."""
         for ld in st.loads:
             for g in range(1, self.cfg.ngrid + 1):
-                self.ctx.set_hydro_slot(g, ld.slot, self.records.get(g, ld.record, ld.filenum))
+                rec = self.records.get(g, ld.record, ld.filenum)
+                self.ctx.set_hydro_slot(g, ld.slot, rec)
+                if "salt" in rec and "temp" in rec:
+                    self.ctx.set_hydro_tracers(g, ld.slot, rec["salt"], rec["temp"])
+                if "bustr" in rec and "bvstr" in rec:
+                    self.ctx.set_hydro_bstress(g, ld.slot, rec["bustr"], rec["bvstr"])
 
     def run(self, max_steps=None, inject=False, on_step=None, diag_every_step=False,
-            mode="step", counts=None, use_graph=True, output=None):
+            mode="step", counts=None, use_graph=True, output=None, salttemp_mean=False):
         """mode "step": one rp_step per internal step; "batch": the steps of each
         external step in one rp_step_batch (one HIP graph when use_graph);
         "scenario": rp_step_scenario for each of the scenarios given by counts.
         output: a TrajectoryFile that gets a record whenever the reference's
-        printdt reaches iprint (ROMSPath.f90:532-543), at model time ix(3)."""
+        printdt reaches iprint (ROMSPath.f90:532-543), at model time ix(3).
+        When the file has the salinity/temperature (SaltTempOn; salttemp_mean
+        for SaltTempMean) or HOB/bustr/bvstr (WriteBottom) variables, the
+        per-particle samplers start after the first loads, as ini_ROMSPath
+        starts them, and are updated after every step (mode "step" only)."""
+        from .output import FLAGS
         prm = self.cfg.params
         n = self.cfg.particles.n
         ndev = ndev_of(prm)
+        want_tr = output is not None and bool(output.flags & FLAGS["salttemp"])
+        want_bot = output is not None and bool(output.flags & FLAGS["bottom"])
+        if (want_tr or want_bot) and mode != "step":
+            raise ValueError("the SaltTempOn / WriteBottom samplers update after every rp_step: "
+                             "use mode='step'")
         hydro = self.records if hasattr(self.records, "tdim") else None
         steps = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps, hydro=hydro))
         if diag_every_step:
@@ -98,8 +115,13 @@ class GpuRun:
                         on_step(st, self)
                 i = j
             return self.ctx.get_particles()
-        for st in steps:
+        for k, st in enumerate(steps):
             self.load(st)
+            if k == 0:
+                if want_bot:
+                    self.ctx.bottom_begin()
+                if want_tr:
+                    self.ctx.tracers_begin(salttemp_mean, st.tc)
             if inject:
                 self.ctx.set_deviates(injected_deviates(prm.seed, st.step_index, n, ndev))
             if mode == "scenario":
@@ -109,6 +131,10 @@ class GpuRun:
             else:
                 self.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index,
                               want_diag=st.output)
+                if want_bot:
+                    self.ctx.bottom_update(st.ex, st.ix, st.tb, st.tc, st.tf)
+                if want_tr:
+                    self.ctx.tracers_update(st.ex, st.ix, st.tb, st.tc, st.tf)
             if output is not None and st.output:
                 from .output import data_output
                 data_output(self.ctx, output, float(int(st.ix[2])))

@@ -74,3 +74,84 @@ def test_run_writes_records_of_particle_columns(tmp_path):
             assert np.array_equal(lon, want_lon)
     finally:
         f.close()
+
+
+def test_run_writes_sampler_variables(tmp_path):
+    """GpuRun.run with a trajectory file that has the SaltTempOn (mean) and
+    WriteBottom variables: the runner loads salt/temp and bustr/bvstr with
+    each record, starts the samplers after the first loads, updates them
+    after every step, and every record holds the oracle's values (its
+    restatements applied to the GPU's own state each step), in FP64."""
+    import types
+    from oracle import oracle as O
+    from romspath_amd.output import TrajectoryFile, meta_of
+    from romspath_amd.runner import GpuRun, RecordSource
+    from scipy.io import netcdf_file
+    from test_bottom import stress_fields
+
+    cfg = synth.config_mini_nested(n_particles=400)
+
+    class Recs(RecordSource):
+        def get(self, ng, record, filenum=None):
+            rec = dict(super().get(ng, record, filenum))
+            g = cfg.grids[ng - 1]
+            rng = np.random.default_rng(100 * ng + record)
+            rec["salt"] = 34.0 + rng.standard_normal((g.N, g.M, g.L))
+            rec["temp"] = 12.0 + rng.standard_normal((g.N, g.M, g.L))
+            rec["bustr"], rec["bvstr"] = stress_fields(g, 3600.0 * record, rng)
+            return rec
+
+    recs = Recs(cfg)
+    print("case run with sampler variables, mini-nested fp64", flush=True)
+    run = GpuRun(cfg, precision="fp64", records=recs, sort_interval=4)
+    nc = TrajectoryFile(tmp_path / "t.nc", cfg.particles.dob, cfg.ngrid, meta_of(cfg.params),
+                        ("salttemp", "bottom"))
+    ogs = [O.OracleGrid(g) for g in cfg.grids]
+    state = {}
+    snaps = []
+
+    def on_step(st, r):
+        p = r.ctx.get_particles()
+        s = types.SimpleNamespace(n=len(p["x"]), x=p["x"], y=p["y"], z=p["z"], gid=p["gid"],
+                                  age=p["age"], bath=p["bath"], isin=p["isin"])
+        for ld in st.loads:
+            for gi in range(cfg.ngrid):
+                ogs[gi].set_slot(ld.slot, recs.get(gi + 1, ld.record))
+        if not state:
+            # the samplers began after the first loads, from the release state
+            s0 = types.SimpleNamespace(n=s.n, x=cfg.particles.x, y=cfg.particles.y,
+                                       z=cfg.particles.z, gid=cfg.particles.gid,
+                                       age=cfg.particles.age, isin=cfg.particles.isin)
+            la, ps, pt, ms, mt, mI = O.tracers_begin(ogs, st.tc, s0, True)
+            state.update(la=la, ps=ps, pt=pt, ms=ms, mt=mt, mI=mI, bla=cfg.particles.age.copy(),
+                         hob=np.full(s.n, 9999.0), bu=np.zeros(s.n), bv=np.zeros(s.n))
+        T = O.make_time(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf)
+        O.tracers_update(ogs, st, s, state["la"], state["ps"], state["pt"], state["ms"], state["mt"])
+        state["mI"] += 1
+        O.bottom_update(ogs, T, s, state["bla"], state["hob"], state["bu"], state["bv"])
+        if st.output:
+            snaps.append((state["ms"] / state["mI"], state["mt"] / state["mI"], state["hob"].copy(),
+                          state["bu"].copy(), state["bv"].copy()))
+            state["ms"][:] = 0.0
+            state["mt"][:] = 0.0
+            state["mI"] = 0
+            state["hob"][:] = 9999.0
+
+    try:
+        run.run(output=nc, on_step=on_step, salttemp_mean=True)
+    finally:
+        nc.close()
+        run.close()
+    f = netcdf_file(str(tmp_path / "t.nc"), "r", mmap=False)
+    try:
+        v = f.variables
+        assert len(snaps) == v["model_time"].shape[0] >= 2
+        for r, (sa, te, hob, bu, bv) in enumerate(snaps):
+            assert np.array_equal(v["salinity"][r], sa.astype(np.float32))
+            assert np.array_equal(v["temperature"][r], te.astype(np.float32))
+            assert np.array_equal(v["HOB"][r], hob.astype(np.float32))
+            assert np.array_equal(v["bustr"][r], bu.astype(np.float32))
+            assert np.array_equal(v["bvstr"][r], bv.astype(np.float32))
+        assert (v["HOB"][-1] < 9999.0).sum() > 200
+    finally:
+        f.close()
