@@ -66,12 +66,13 @@ __global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ALL)
 }
 
 // The FP32-field advection kernel is held to 3 waves per SIMD (168 VGPRs and
-// a small spill): its RK stages wait on field gathers, and the extra waves
-// hide them better than the spill costs.
+// a small spill), the FP64 one to 2: their RK stages wait on field gathers,
+// and the extra waves hide them better than the spill costs (measured on C3:
+// FP64 form 2.47 ms at 2 waves against 3.63 at 1 and 2.66 at 3).
 template <typename F>
 constexpr int advect_waves()
 {
-    return RP_WAVES_ADVECT > 0 ? RP_WAVES_ADVECT : (is_fast<F>() ? 3 : 1);
+    return RP_WAVES_ADVECT > 0 ? RP_WAVES_ADVECT : (is_fast<F>() ? 3 : 2);
 }
 
 template <typename F, bool NESTED>
@@ -82,12 +83,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(advect
 }
 
 // The FP32-field vertical walk is held to 5 waves per SIMD (96 VGPRs, a
-// spill of about 250 B): its sweeps and gathers wait on memory, and more
-// resident waves hide more of that than the spill costs.
+// spill of about 250 B), the FP64 one to 6: their sweeps and gathers wait on
+// memory, and more resident waves hide more of that than the spill costs
+// (measured on C3: FP64 form 98.8 ms at 6 waves against 143.7 at the
+// compiler's 2, 109.4 at 4 and 100.6 at 8).
 template <typename F>
 constexpr int vwalk_waves()
 {
-    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (is_fast<F>() ? 5 : 1);
+    return RP_WAVES_VWALK > 0 ? RP_WAVES_VWALK : (is_fast<F>() ? 5 : 6);
 }
 
 template <typename F, bool NESTED>
