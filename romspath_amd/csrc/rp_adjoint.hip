@@ -23,10 +23,12 @@ namespace rp {
 using D3 = Dual<3>;
 
 // forward step of a recorded run: the whole step in one kernel, spline on
-// the reference-order path; persistent grid with the vertical walk's queue
+// the reference-order path; persistent grid with the vertical walk's queue.
+// Held to 2 waves per SIMD (the compiler's choice, 1, waits on memory more
+// than the spill costs: C5 recorded forward 1.57 s against 2.28 s).
 template <typename F, bool NESTED>
-__global__ void __launch_bounds__(256) adj_forward_kernel(const DGrid* __restrict__ grids,
-                                                          StepArgs A)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    adj_forward_kernel(const DGrid* __restrict__ grids, StepArgs A)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
@@ -173,9 +175,11 @@ __device__ __forceinline__ void adj_particle(const DGrid* __restrict__ grids, co
 
 // One recorded step backwards: lambda <- J^T lambda for every particle that
 // took the step (the others keep lambda), and the perturbation gradient.
+// Held to 2 waves per SIMD like the forward (C5 adjoint 5.04 s against 6.23
+// at the compiler's 1; 3 waves spill too much).
 template <typename F, bool NESTED>
-__global__ void __launch_bounds__(256) adj_step_kernel(const DGrid* __restrict__ grids, StepArgs A,
-                                                       AdjArgs J)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    adj_step_kernel(const DGrid* __restrict__ grids, StepArgs A, AdjArgs J)
 {
     extern __shared__ double gp_lds[];
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
