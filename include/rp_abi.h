@@ -109,7 +109,10 @@ int rp_set_particles(void *ctx, int64_t n, const double *x, const double *y, con
  * values (ROMSPath.f90:488-524); step_index counts internal steps from 0 and
  * keys the random deviates.  want_diag != 0 also stores the water depth,
  * zeta and bathymetry columns (pWD, pZeta, pBath; ROMSPath.f90:845-847),
- * which only the output reads.  Asynchronous. */
+ * which only the output reads.  While the WriteBottom or SaltTempOn sampler
+ * is active (rp_bottom_begin, rp_tracers_begin) pBath, the step's own
+ * -tdepth, is stored on every step, since those samplers read it.
+ * Asynchronous. */
 int rp_step(void *ctx, const double ex[3], const double ix[3], int32_t ets, int32_t t_b,
             int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag);
 

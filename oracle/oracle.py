@@ -97,7 +97,7 @@ def lib():
                                 C.c_int, C.c_int, C.c_int64, PD, PD, PD, PD, PD, PD, PI32, PU8]
         L.og_locate.restype = None
         L.og_sample_tracers.argtypes = [C.POINTER(OgGrid), C.POINTER(OgTime), C.c_int64, PD, PD, PD,
-                                        PI32, PD, PD]
+                                        PI32, PD, PD, PD]
         L.og_sample_tracers.restype = None
         L.og_interp_str.argtypes = [C.POINTER(OgGrid), C.c_int, C.c_int, C.c_double, C.c_double]
         L.og_interp_str.restype = C.c_double
@@ -262,11 +262,14 @@ def ref_pppack():
 
 # ---- whole runs on the oracle (same schedule as the product driver) ----
 
-def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None, records=None):
+def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None, records=None,
+               before_first=None):
     """Run cfg on the oracle; deviates are the same seeded per-step blocks the
     GPU tests inject.  Returns the OracleState; its step_seconds is the time
     spent in the particle-step calls alone (records, deviates and slot copies
-    excluded).  records: a RecordSource to share (None = a new one)."""
+    excluded).  records: a RecordSource to share (None = a new one).
+    before_first(st, state, grids): called after the first loads, before the
+    first step (where ini_ROMSPath samples and writes the release record)."""
     import time
     import sys
     root = str(HERE.parent)
@@ -290,10 +293,14 @@ def run_oracle(cfg, max_steps=None, inject=True, on_step=None, particles=None, r
     active = 0
     step_seconds = 0.0
     hydro = recs if hasattr(recs, "tdim") else None
+    first = True
     for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, max_steps, hydro=hydro):
         for ld in st.loads:
             for i, g in enumerate(grids):
                 g.set_slot(ld.slot, recs.get(i + 1, ld.record, ld.filenum))
+        if first and before_first is not None:
+            before_first(st, state, grids)
+        first = False
         T = make_time(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf)
         if inject:
             dev = injected_deviates(prm.seed, st.step_index, state.n, ndev)
@@ -399,15 +406,21 @@ def locate(cfg, inlon, inlat, z, lonlat=None, spherical=0, angle=None, tb=1, rec
     return out
 
 
-def sample_tracers(grids, ex, ix, tb, tc, tf, x, y, z, gid):
+def sample_tracers(grids, ex, ix, tb, tc, tf, x, y, z, gid, bath=None):
     """SaltTempOn's salt and temp at the particles (ROMSPath.f90:958-981); grids:
-    OracleGrid list with salt/temp in the slots."""
+    OracleGrid list with salt/temp in the slots.  bath: the step's pBath
+    column (-tdepth of the step, which getInterp3d receives); None: the depth
+    at (x, y), as the release-time sample uses."""
     T = make_time(ex, ix, 0, tb, tc, tf)
     arr = (OgGrid * len(grids))(*[g.s for g in grids])
     x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
     gid = np.ascontiguousarray(gid, dtype=np.int32)
+    pb = None
+    if bath is not None:
+        bath = np.ascontiguousarray(bath, dtype=np.float64)
+        pb = dptr(bath)
     salt, temp = np.empty(x.size), np.empty(x.size)
-    lib().og_sample_tracers(arr, C.byref(T), x.size, dptr(x), dptr(y), dptr(z), iptr(gid),
+    lib().og_sample_tracers(arr, C.byref(T), x.size, dptr(x), dptr(y), dptr(z), iptr(gid), pb,
                             dptr(salt), dptr(temp))
     return salt, temp
 
@@ -425,13 +438,24 @@ def tracers_begin(grids, tc, state, mean):
     """Release-time P_Salt/P_Temp (ROMSPath.f90:247-258, 408-424): 0, then for
     the particles in a grid getInterp3d at slot t_c (polintd over three equal
     samples returns the sample).  Returns (last_age, P_salt, P_temp,
-    mean_salt, mean_temp, mI); the means are None without SaltTempMean."""
+    mean_salt, mean_temp, mI); the means are None without SaltTempMean.
+    When state has wd/zeta/bath columns, the particles in a grid get pWD,
+    pZeta and pBath at t_c as well (ROMSPath.f90:415-417)."""
     n = state.n
     ps, pt = np.zeros(n), np.zeros(n)
     s, t = sample_tracers(grids, (0.0, 1.0, 2.0), (0.0, 1.0, 2.0), tc, tc, tc, state.x, state.y,
                           state.z, state.gid)
     isin = np.asarray(state.isin).astype(bool)
     ps[isin], pt[isin] = s[isin], t[isin]
+    if all(hasattr(state, k) for k in ("wd", "zeta", "bath")):
+        for q in np.nonzero(isin)[0]:
+            g = grids[int(state.gid[q]) - 1]
+            X, Y = float(state.x[q]), float(state.y[q])
+            tdepth = -1.0 * lib().og_interp2d(C.byref(g.s), 0, tc, X, Y)
+            zeta = lib().og_interp2d(C.byref(g.s), 3, tc, X, Y)
+            state.wd[q] = (-1.0 * tdepth) + zeta
+            state.zeta[q] = zeta
+            state.bath[q] = -1.0 * tdepth
     if not mean:
         return state.age.copy(), ps, pt, None, None, 0
     return state.age.copy(), ps, pt, ps.copy(), pt.copy(), int(isin.any())
@@ -439,9 +463,10 @@ def tracers_begin(grids, tc, state, mean):
 
 def tracers_update(grids, st, state, last_age, ps, pt, ms=None, mt=None):
     """The SaltTempOn block of update_particles (ROMSPath.f90:958-981) for the
-    particles that took the step (age changed); the caller adds 1 to mI."""
+    particles that took the step (age changed), with the step's depth
+    (state.bath); the caller adds 1 to mI."""
     s, t = sample_tracers(grids, st.ex, st.ix, st.tb, st.tc, st.tf, state.x, state.y, state.z,
-                          state.gid)
+                          state.gid, bath=state.bath)
     act = state.age != last_age
     last_age[act] = state.age[act]
     ps[act], pt[act] = s[act], t[act]

@@ -576,18 +576,23 @@ void og_locate(const og_grid *grids, int ngrid, const double *const *lon, const 
 
 /* The SaltTempOn block of update_particles (ROMSPath.f90:958-981) at the
  * particles' positions: zeta of slots b, c, f (IDW), getInterp3d("salt"/
- * "temp", ztype 1) with each slot's own zeta and the depth at the position,
- * then polintd in time at ix(2). */
+ * "temp", ztype 1) with each slot's own zeta, then polintd in time at ix(2).
+ * The depth passed to getInterp3d is the step's local tdepth
+ * (ROMSPath.f90:849, 964-971): bath (= -tdepth, the pBath column the step
+ * wrote) when given.  For a rejected move that depth belongs to the
+ * attempted position (tempX, tempY) on the old grid while X, Y stay.
+ * bath == NULL: the depth at (x, y) at slot t_c, as the release-time
+ * sample computes it (ROMSPath.f90:411-414). */
 void og_sample_tracers(const og_grid *grids, const og_time *T, int64_t n, const double *x,
-                       const double *y, const double *z, const int32_t *gid, double *salt,
-                       double *temp)
+                       const double *y, const double *z, const int32_t *gid, const double *bath,
+                       double *salt, double *temp)
 {
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 64)
 #endif
     for (int64_t q = 0; q < n; q++) {
         const og_grid *g = &grids[gid[q] - 1];
-        double tdepth = -1.0 * og_interp2d(g, 0, T->tc, x[q], y[q]);
+        double tdepth = bath ? -1.0 * bath[q] : -1.0 * og_interp2d(g, 0, T->tc, x[q], y[q]);
         double zb = og_interp2d(g, 3, T->tb, x[q], y[q]);
         double zc = og_interp2d(g, 3, T->tc, x[q], y[q]);
         double zf = og_interp2d(g, 3, T->tf, x[q], y[q]);
@@ -1171,8 +1176,9 @@ int64_t og_update_particles(const og_grid *grids, const og_params *P, const og_t
             p->y[q] = tempY;
             p->z[q] = newZ;
             p->status[q] = 1.0;
-            if (htop) p->status[q] = 1.1;
-            if (hbot) p->status[q] = 1.2;
+            /* default-REAL literals 1.1 and 1.2, widened (ROMSPath.f90:880-881) */
+            if (htop) p->status[q] = (double)1.1f;
+            if (hbot) p->status[q] = (double)1.2f;
         } else if (obound) {
             p->isin[q] = 0;
             gid = P->ngrid;

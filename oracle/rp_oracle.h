@@ -103,8 +103,8 @@ void og_bottom_update(const og_grid *grids, const og_time *T, int64_t n, const d
                       const double *bath, double *last_age, double *hob, double *bustr,
                       double *bvstr);
 void og_sample_tracers(const og_grid *grids, const og_time *T, int64_t n, const double *x,
-                       const double *y, const double *z, const int32_t *gid, double *salt,
-                       double *temp);
+                       const double *y, const double *z, const int32_t *gid, const double *bath,
+                       double *salt, double *temp);
 void og_ll2ij(const double *lon, const double *lat, const double *angle, int L, int M,
               int spherical, double inlon, double inlat, double *Ipar, double *Jpar);
 void og_zbounds(const og_grid *g, double Ipar, double Jpar, double *Zpar, int *inz, int t);

@@ -190,13 +190,19 @@ int put(Ctx* c, void* dst, const void* src, size_t bytes)
     return RP_OK;
 }
 
+// The reference's status literals are default REAL (single precision):
+// par(n,pStatus)=1.1 and =1.2 (ROMSPath.f90:880-881) store 1.1f and 1.2f
+// widened to double.  Those are the values returned; the double 1.1 and 1.2
+// are accepted on input too.
+constexpr double kStTop = (double)1.1f, kStBot = (double)1.2f;
+
 int status_code(double s, int8_t* out)
 {
     if (s == 9.0) *out = ST_UNSET;
     else if (s == 0.0) *out = ST_NEW;
     else if (s == 1.0) *out = ST_MOVED;
-    else if (s == 1.1) *out = ST_TOP;
-    else if (s == 1.2) *out = ST_BOT;
+    else if (s == kStTop || s == 1.1) *out = ST_TOP;
+    else if (s == kStBot || s == 1.2) *out = ST_BOT;
     else if (s == -8.0) *out = ST_EXIT;
     else if (s == -9.0) *out = ST_LAND;
     else return -1;
@@ -209,8 +215,8 @@ double status_value(int8_t c)
     case ST_UNSET: return 9.0;
     case ST_NEW: return 0.0;
     case ST_MOVED: return 1.0;
-    case ST_TOP: return 1.1;
-    case ST_BOT: return 1.2;
+    case ST_TOP: return kStTop;
+    case ST_BOT: return kStBot;
     case ST_EXIT: return -8.0;
     case ST_LAND: return -9.0;
     default: return NAN;
@@ -550,7 +556,8 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     A.nmid = c->n;
     A.step_index = step_index;
     A.seed = c->cfg.seed;
-    A.want_diag = want_diag != 0;
+    // the per-step samplers read the step's own depth (pBath) every step
+    A.want_diag = want_diag != 0 ? 1 : ((c->bot_n == c->n || c->trc_n == c->n) ? 2 : 0);
     A.dev = c->use_dev ? c->dev : nullptr;
     A.ndev = c->ndev;
     A.active = c->active;
@@ -1509,6 +1516,9 @@ int tracer_args(Ctx* c, int32_t t_b, int32_t t_c, int32_t t_f, const char* who, 
     A.n = c->n;
     A.x = c->x; A.y = c->y; A.z = c->z; A.gid = c->gid;
     A.id = c->id; A.age = c->age; A.isin = c->isin;
+    A.bath = c->bath;
+    A.wd = c->wd;
+    A.pzeta = c->zeta;
     return RP_OK;
 }
 

@@ -17,7 +17,9 @@ namespace rp {
 constexpr int kMaxGrid = 5;
 
 // Status codes held on the device (int8); the host maps them to the
-// reference's FP64 par(:,pStatus) values {9, 0, 1, 1.1, 1.2, -8, -9}.
+// reference's FP64 par(:,pStatus) values {9, 0, 1, 1.1, 1.2, -8, -9}; 1.1
+// and 1.2 are single-precision literals there, so they read back as
+// (double)1.1f and (double)1.2f.
 enum : int8_t {
     ST_NEW = 0,       // 0.0  released, not yet moved
     ST_UNSET = 9,     // 9.0  below the bed at release: never moves

@@ -104,6 +104,9 @@ __global__ void locate_kernel(const DGrid* __restrict__ grids, LocateArgs A);
 // mean = P, and last_age = age for all; mode 2 (after a step): the particles
 // whose age changed, P = the sample, mean += P.  Modes 1 and 2 index the
 // release-order columns through id; mean_salt/mean_temp may be null.
+// Modes 0 and 1 use the depth at the particle; mode 2 the step's own depth
+// (bath), which for a rejected move is the attempted position's
+// (ROMSPath.f90:849, 964-971).
 struct TracerArgs {
     const void* salt[kMaxGrid];   // slot-interleaved, field precision, (L, M, N)
     const void* temp[kMaxGrid];
@@ -117,6 +120,9 @@ struct TracerArgs {
     const uint32_t* id;
     const double* age;
     const uint8_t* isin;
+    // mode 2 reads bath, the step's pBath (-tdepth of the step); mode 1
+    // writes pWD, pZeta, pBath of the particles in a grid (ROMSPath.f90:415-417)
+    double *bath, *wd, *pzeta;
     double *last_age, *p_salt, *p_temp, *mean_salt, *mean_temp;
 };
 template <typename F>

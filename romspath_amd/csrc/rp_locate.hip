@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(256) tracer_kernel(const DGrid* __restrict__ g
         const DGrid& g = grids[gi];
         const double X = A.x[q], Y = A.y[q], Z = A.z[q];
         const RhoCell c = rho_cell(g, X, Y);
-        const double tdepth = -1.0 * bilinear2d(g.h, c);
+        const double tdepth = A.mode == 2 ? -1.0 * A.bath[q] : -1.0 * bilinear2d(g.h, c);
         double zt[3];
         zeta3<F>(g, c, A.slots, zt);
         double cff[4];
@@ -233,6 +233,12 @@ __global__ void __launch_bounds__(256) tracer_kernel(const DGrid* __restrict__ g
         }
         const double ps = polintd(A.ex, es[0], es[1], es[2], A.ix2);
         const double pt = polintd(A.ex, et[0], et[1], et[2], A.ix2);
+        if (A.mode == 1) {
+            // the release sample also sets pWD, pZeta, pBath at t_c
+            A.wd[q] = (-1.0 * tdepth) + zt[1];
+            A.pzeta[q] = zt[1];
+            A.bath[q] = -1.0 * tdepth;
+        }
         if (A.mode == 0) {
             A.out_salt[q] = ps;
             A.out_temp[q] = pt;

@@ -1599,9 +1599,13 @@ __device__ __forceinline__ void finish(const DGrid* __restrict__ grids, const St
 {
     const StepOut<double> o = finish_math<F, NESTED>(grids, A, gid, A.x[q], A.y[q], Z0, newX, newY,
                                                      AdvectZ, TurbV);
+    // want_diag 1: pWD, pZeta and pBath (dataOutput reads them); 2: pBath
+    // only, which the per-step samplers read (the step's own tdepth)
     if (A.want_diag) {
-        A.wd[q] = (-1.0 * o.tdepth) + o.zeta;
-        A.zeta[q] = o.zeta;
+        if (A.want_diag == 1) {
+            A.wd[q] = (-1.0 * o.tdepth) + o.zeta;
+            A.zeta[q] = o.zeta;
+        }
         A.bath[q] = -1.0 * o.tdepth;
     }
     A.isin[q] = o.obound && !o.ingrid ? 0 : 1;

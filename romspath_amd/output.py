@@ -83,17 +83,21 @@ class TrajectoryFile:
                 raise abi.RpError("rp_nc_close", rc, "close failed")
 
 
-def data_output(ctx, nc, model_time):
+def data_output(ctx, nc, model_time, lonlat=None):
     """dataOutput (ROMSPath.f90:1012-1025): lon/lat at every particle, then
     one record.  ctx: abi.Context after rp_set_lonlat for every grid (and
     rp_tracers_begin / rp_bottom_begin when the file has the SaltTempOn /
     WriteBottom variables: the means and P_HOB are read and reset,
-    ROMSPath.f90:2009-2035, 2056-2065)."""
+    ROMSPath.f90:2009-2035, 2056-2065).  model_time is the record's
+    model_time value, time + reftime (ROMSPath.f90:1931).  lonlat: (lon,
+    lat) to write instead of sampling them, as the release record
+    writeNetCDF(0, pLon, pLat) writes the release file's values
+    (ROMSPath.f90:453)."""
     p = ctx.get_particles()
     if nc.flags & FLAGS["salttemp"]:
         p["salt"], p["temp"] = ctx.tracers_get(reset_mean=True)
     if nc.flags & FLAGS["bottom"]:
         p["hob"], p["bustr"], p["bvstr"] = ctx.bottom_get(reset_hob=True)
-    lon, lat = ctx.sample_lonlat()
+    lon, lat = ctx.sample_lonlat() if lonlat is None else lonlat
     nc.write(model_time, p, lon, lat)
     return p, lon, lat

@@ -74,7 +74,8 @@ class GpuRun:
                     self.ctx.set_hydro_bstress(g, ld.slot, rec["bustr"], rec["bvstr"])
 
     def run(self, max_steps=None, inject=False, on_step=None, diag_every_step=False,
-            mode="step", counts=None, use_graph=True, output=None, salttemp_mean=False):
+            mode="step", counts=None, use_graph=True, output=None, salttemp_mean=False,
+            release_lonlat=None):
         """mode "step": one rp_step per internal step; "batch": the steps of each
         external step in one rp_step_batch (one HIP graph when use_graph);
         "scenario": rp_step_scenario for each of the scenarios given by counts.
@@ -83,7 +84,13 @@ class GpuRun:
         When the file has the salinity/temperature (SaltTempOn; salttemp_mean
         for SaltTempMean) or HOB/bustr/bvstr (WriteBottom) variables, the
         per-particle samplers start after the first loads, as ini_ROMSPath
-        starts them, and are updated after every step (mode "step" only)."""
+        starts them, and are updated after every step (mode "step" only).
+        The file gets the release record at time 0 before the first step
+        (writeNetCDF(0, pLon, pLat), ROMSPath.f90:453), with release_lonlat
+        as its lon/lat when given (the release file's values) and sampled
+        at the release positions otherwise; every record's model_time is
+        time + reftime, reftime being the last grid's (InitGrid,
+        grid_module.f90:374-376; 0 for grids not read from a file)."""
         from .output import FLAGS
         prm = self.cfg.params
         n = self.cfg.particles.n
@@ -115,6 +122,7 @@ class GpuRun:
                         on_step(st, self)
                 i = j
             return self.ctx.get_particles()
+        reftime = float(getattr(self.cfg.grids[-1], "reftime", 0.0))
         for k, st in enumerate(steps):
             self.load(st)
             if k == 0:
@@ -122,6 +130,9 @@ class GpuRun:
                     self.ctx.bottom_begin()
                 if want_tr:
                     self.ctx.tracers_begin(salttemp_mean, st.tc)
+                if output is not None:
+                    from .output import data_output
+                    data_output(self.ctx, output, 0.0 + reftime, lonlat=release_lonlat)
             if inject:
                 self.ctx.set_deviates(injected_deviates(prm.seed, st.step_index, n, ndev))
             if mode == "scenario":
@@ -137,7 +148,7 @@ class GpuRun:
                     self.ctx.tracers_update(st.ex, st.ix, st.tb, st.tc, st.tf)
             if output is not None and st.output:
                 from .output import data_output
-                data_output(self.ctx, output, float(int(st.ix[2])))
+                data_output(self.ctx, output, float(int(st.ix[2])) + reftime)
             if on_step is not None:
                 on_step(st, self)
         return self.ctx.get_particles()

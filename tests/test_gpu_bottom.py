@@ -15,6 +15,7 @@ from romspath_amd import synth
 from romspath_amd.runner import GpuRun
 from romspath_amd.schedule import schedule
 
+from lockstep import Lockstep, check_same_state
 from test_bottom import stress_fields
 
 pytestmark = pytest.mark.gpu
@@ -22,50 +23,65 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("precision,sort_interval", [("fp64", 0), ("fp64", 4), ("fp32", 4)])
 def test_bottom_matches_oracle_each_step(precision, sort_interval):
+    """The GPU and the oracle step side by side with the same injected
+    deviates, no output steps (want_diag = 0, so pBath comes from the
+    sampler's own per-step store).  FP64: the oracle's update runs on the
+    oracle's own stepped state and every column agrees bit for bit, the
+    rejected moves (status -9) included, whose depth is the attempted
+    point's.  FP32 fields: the oracle's update applied to the GPU's state
+    (the stepped states differ at 1e-7 cells, which can move a running
+    minimum by a step), equal minimum heights and stresses within FP32
+    rounding of the fields."""
     cfg = synth.config_mini_nested(n_particles=600)
     prm = cfg.params
     steps = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, 24))
     print(f"case bottom mini-nested {precision} sort {sort_interval}", flush=True)
-    run = GpuRun(cfg, precision=precision, sort_interval=sort_interval)
-    ogs = [O.OracleGrid(g) for g in cfg.grids]
-    dummy = {k: np.zeros(1) for k in ("zeta", "u", "v", "w", "aks")}
-    run.ctx.bottom_begin()
+
+    def extra(gi, rec_no):
+        g = cfg.grids[gi]
+        f = stress_fields(g, 3600.0 * rec_no, np.random.default_rng(100 * gi + rec_no))
+        return dict(bustr=f[0], bvstr=f[1])
+
+    ls = Lockstep(cfg, precision=precision, sort_interval=sort_interval, extra=extra)
     n = cfg.particles.n
     la = np.ascontiguousarray(cfg.particles.age, dtype=np.float64).copy()
     hob, bu, bv = np.full(n, 9999.0), np.zeros(n), np.zeros(n)
     tol = 0.0 if precision == "fp64" else 2e-7
-    checked = 0
+    checked = rejected = 0
     try:
+        ls.ctx.bottom_begin()
         for k, st in enumerate(steps):
-            run.load(st)
-            for ld in st.loads:
-                for gi, g in enumerate(cfg.grids):
-                    f = stress_fields(g, 3600.0 * ld.record, np.random.default_rng(100 * gi + ld.record))
-                    run.ctx.set_hydro_bstress(gi + 1, ld.slot, *f)
-                    ogs[gi].set_slot(ld.slot, dict(dummy, bustr=f[0], bvstr=f[1]))
-            run.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index)
-            run.ctx.bottom_update(st.ex, st.ix, st.tb, st.tc, st.tf)
-            p = run.ctx.get_particles()
-            state = types.SimpleNamespace(n=n, x=p["x"], y=p["y"], z=p["z"], gid=p["gid"],
-                                          age=p["age"], bath=p["bath"])
-            T = O.make_time(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf)
-            O.bottom_update(ogs, T, state, la, hob, bu, bv)
+            ls.load(st)
+            T = ls.step(st)
+            ls.ctx.bottom_update(st.ex, st.ix, st.tb, st.tc, st.tf)
+            rejected += int((ls.state.status == -9.0).sum())
+            if precision == "fp64":
+                O.bottom_update(ls.ogs, T, ls.state, la, hob, bu, bv)
+            else:
+                p = ls.ctx.get_particles()
+                gstate = types.SimpleNamespace(n=n, x=p["x"], y=p["y"], z=p["z"], gid=p["gid"],
+                                               age=p["age"], bath=p["bath"])
+                O.bottom_update(ls.ogs, T, gstate, la, hob, bu, bv)
             if k in (9, len(steps) - 1):
                 reset = k == 9
-                gh, gu, gv = run.ctx.bottom_get(reset_hob=reset)
-                assert np.array_equal(gh, hob)
+                gh, gu, gv = ls.ctx.bottom_get(reset_hob=reset)
                 if tol == 0.0:
+                    check_same_state(ls.ctx.get_particles(), ls.state, f"step {k}")
+                    assert np.array_equal(gh, hob)
                     assert np.array_equal(gu, bu) and np.array_equal(gv, bv)
                 else:
+                    assert np.array_equal(gh, hob)
                     assert np.abs(gu - bu).max() < tol and np.abs(gv - bv).max() < tol
                 if reset:
                     hob[:] = 9999.0
                 checked += 1
+        p = ls.ctx.get_particles()
         assert set(np.unique(p["gid"])) == {1, 2}
         assert (hob < 9999.0).sum() > n // 2 and (bu != 0.0).sum() > n // 2
     finally:
-        run.close()
+        ls.close()
     assert checked == 2
+    assert rejected > 0, "the case has no rejected moves"
 
 
 def test_bottom_errors():

@@ -38,7 +38,7 @@ def test_lonlat_sampler_matches_restatement():
 
 
 def test_run_writes_records_of_particle_columns(tmp_path):
-    cfg = synth.config_mini_nested(n_particles=600)     # 2 h: records at 1 h and 2 h
+    cfg = synth.config_mini_nested(n_particles=600)     # 2 h: records at 0, 1 h and 2 h
     print("case nested mini fp32 60 steps with trajectory file", flush=True)
     run = GpuRun(cfg, precision="fp32", sort_interval=5)
     path = tmp_path / "traj.nc"
@@ -58,9 +58,18 @@ def test_run_writes_records_of_particle_columns(tmp_path):
     f = netcdf_file(str(path), "r", mmap=False)
     try:
         v = f.variables
-        assert np.array_equal(v["model_time"][:], [float(int(s[0])) for s in seen])
+        # the release record at time 0 (writeNetCDF(0, ...), ROMSPath.f90:453),
+        # then one per iprint at int(ix(3)); reftime is 0 for these grids
+        assert np.array_equal(v["model_time"][:], [0.0] + [float(int(s[0])) for s in seen])
         assert np.array_equal(v["dob"][:], cfg.particles.dob.astype(np.float32))
-        for r, (_, p, lon, lat) in enumerate(seen):
+        rel = cfg.particles
+        assert np.array_equal(v["age"][0], rel.age)
+        assert np.array_equal(v["zp"][0], rel.z.astype(np.float32))
+        assert np.array_equal(v["status"][0], rel.status.astype(np.float32))
+        lon0, lat0 = _want_lonlat(cfg, {"x": rel.x, "y": rel.y, "gid": rel.gid})
+        assert np.array_equal(v["lon"][0], lon0.astype(np.float32))
+        assert np.all(v["h"][0] == np.float32(9999.0))     # pBath before any step
+        for r, (_, p, lon, lat) in enumerate(seen, start=1):
             assert np.array_equal(v["age"][r], p["age"])
             assert np.array_equal(v["zp"][r], p["z"].astype(np.float32))
             assert np.array_equal(v["status"][r], p["status"].astype(np.float32))
@@ -76,12 +85,41 @@ def test_run_writes_records_of_particle_columns(tmp_path):
         f.close()
 
 
+def test_release_record_lonlat_and_reftime(tmp_path):
+    """The release record writes the lon/lat it is given (the release file's
+    values, writeNetCDF(0, pLon, pLat)), and every model_time is time +
+    reftime of the grid (ROMSPath.f90:1931)."""
+    cfg = synth.config_mini(n_particles=200, days=2 / 24.0)
+    cfg.grids[-1].reftime = 86400.0 * 3
+    print("case mini fp64 release record with reftime", flush=True)
+    run = GpuRun(cfg, precision="fp64")
+    nc = TrajectoryFile(tmp_path / "r.nc", cfg.particles.dob, cfg.ngrid, meta_of(cfg.params))
+    rng = np.random.default_rng(5)
+    lon = rng.uniform(-70, -60, cfg.particles.n)
+    lat = rng.uniform(30, 40, cfg.particles.n)
+    try:
+        run.run(output=nc, release_lonlat=(lon, lat))
+    finally:
+        nc.close()
+        run.close()
+        del cfg.grids[-1].reftime
+    f = netcdf_file(str(tmp_path / "r.nc"), "r", mmap=False)
+    try:
+        v = f.variables
+        assert np.array_equal(v["model_time"][:], 86400.0 * 3 + np.array([0.0, 3600.0, 7200.0]))
+        assert np.array_equal(v["lon"][0], lon.astype(np.float32))
+        assert np.array_equal(v["lat"][0], lat.astype(np.float32))
+    finally:
+        f.close()
+
+
 def test_run_writes_sampler_variables(tmp_path):
     """GpuRun.run with a trajectory file that has the SaltTempOn (mean) and
     WriteBottom variables: the runner loads salt/temp and bustr/bvstr with
     each record, starts the samplers after the first loads, updates them
     after every step, and every record holds the oracle's values (its
-    restatements applied to the GPU's own state each step), in FP64."""
+    restatements on the oracle's own run with the same injected deviates),
+    in FP64, from the release record at time 0 on."""
     import types
     from oracle import oracle as O
     from romspath_amd.output import TrajectoryFile, meta_of
@@ -106,42 +144,41 @@ def test_run_writes_sampler_variables(tmp_path):
     run = GpuRun(cfg, precision="fp64", records=recs, sort_interval=4)
     nc = TrajectoryFile(tmp_path / "t.nc", cfg.particles.dob, cfg.ngrid, meta_of(cfg.params),
                         ("salttemp", "bottom"))
-    ogs = [O.OracleGrid(g) for g in cfg.grids]
     state = {}
     snaps = []
 
-    def on_step(st, r):
-        p = r.ctx.get_particles()
-        s = types.SimpleNamespace(n=len(p["x"]), x=p["x"], y=p["y"], z=p["z"], gid=p["gid"],
-                                  age=p["age"], bath=p["bath"], isin=p["isin"])
-        for ld in st.loads:
-            for gi in range(cfg.ngrid):
-                ogs[gi].set_slot(ld.slot, recs.get(gi + 1, ld.record))
-        if not state:
-            # the samplers began after the first loads, from the release state
-            s0 = types.SimpleNamespace(n=s.n, x=cfg.particles.x, y=cfg.particles.y,
-                                       z=cfg.particles.z, gid=cfg.particles.gid,
-                                       age=cfg.particles.age, isin=cfg.particles.isin)
-            la, ps, pt, ms, mt, mI = O.tracers_begin(ogs, st.tc, s0, True)
-            state.update(la=la, ps=ps, pt=pt, ms=ms, mt=mt, mI=mI, bla=cfg.particles.age.copy(),
-                         hob=np.full(s.n, 9999.0), bu=np.zeros(s.n), bv=np.zeros(s.n))
+    def snapshot():
+        snaps.append((state["ms"] / state["mI"], state["mt"] / state["mI"], state["hob"].copy(),
+                      state["bu"].copy(), state["bv"].copy()))
+        state["ms"][:] = 0.0
+        state["mt"][:] = 0.0
+        state["mI"] = 0
+        state["hob"][:] = 9999.0
+
+    def before_first(st, s, grids):
+        # ini_ROMSPath: the release samples, then the release record
+        state["grids"] = grids
+        la, ps, pt, ms, mt, mI = O.tracers_begin(grids, st.tc, s, True)
+        state.update(la=la, ps=ps, pt=pt, ms=ms, mt=mt, mI=mI, bla=s.age.copy(),
+                     hob=np.full(s.n, 9999.0), bu=np.zeros(s.n), bv=np.zeros(s.n))
+        snapshot()
+
+    def on_step(st, s):
+        grids = state["grids"]
         T = O.make_time(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf)
-        O.tracers_update(ogs, st, s, state["la"], state["ps"], state["pt"], state["ms"], state["mt"])
+        O.tracers_update(grids, st, s, state["la"], state["ps"], state["pt"], state["ms"], state["mt"])
         state["mI"] += 1
-        O.bottom_update(ogs, T, s, state["bla"], state["hob"], state["bu"], state["bv"])
+        O.bottom_update(grids, T, s, state["bla"], state["hob"], state["bu"], state["bv"])
         if st.output:
-            snaps.append((state["ms"] / state["mI"], state["mt"] / state["mI"], state["hob"].copy(),
-                          state["bu"].copy(), state["bv"].copy()))
-            state["ms"][:] = 0.0
-            state["mt"][:] = 0.0
-            state["mI"] = 0
-            state["hob"][:] = 9999.0
+            snapshot()
 
     try:
-        run.run(output=nc, on_step=on_step, salttemp_mean=True)
+        run.run(output=nc, inject=True, salttemp_mean=True)
     finally:
         nc.close()
         run.close()
+    # the oracle's samplers on the oracle's own run with the same deviates
+    O.run_oracle(cfg, inject=True, records=recs, on_step=on_step, before_first=before_first)
     f = netcdf_file(str(tmp_path / "t.nc"), "r", mmap=False)
     try:
         v = f.variables

@@ -2,8 +2,10 @@
 
 FP64 mode: status, grid id and isIn must match exactly and positions to
 1e-9 of a cell (x, y in cell units; z in units of the local mean layer
-thickness h/N).  FP32-field mode: position error against the FP64 oracle
-within 1e-4 of a cell.  Random walks use identical injected deviates.
+thickness h/N).  FP32-field mode: the largest position error against the
+FP64 oracle within 1e-4 of a cell, over the particles whose status (and grid)
+agree, which must be at least 99% of them.  Random walks use identical
+injected deviates.
 """
 import numpy as np
 import pytest
@@ -84,10 +86,10 @@ def test_mini_fp32_turbulence_error():
     g, o, _, _ = _run_both(cfg, 60, "fp32", inject=True)
     ex, ey, ez = _err(g, o, cfg)
     same = g["status"] == o.status
-    print("mini fp32 err p95", np.percentile(ex, 95), np.percentile(ez, 95), "max", ex.max(), ez.max(),
+    print("mini fp32 err max", ex[same].max(), ey[same].max(), ez[same].max(),
           "status agree", same.mean(), flush=True)
     assert same.mean() > 0.99
-    assert np.percentile(ex[same], 95) <= 1e-4 and np.percentile(ez[same], 95) <= 1e-4
+    assert ex[same].max() <= 1e-4 and ey[same].max() <= 1e-4 and ez[same].max() <= 1e-4
 
 
 def test_rng_matches_philox_reference():
@@ -131,10 +133,10 @@ def test_nested_fp32_fields_error():
     g, o, _, _ = _run_both(cfg, 60, "fp32", inject=True)
     same = (g["status"] == o.status) & (g["gid"] == o.gid)
     ex, ey, ez = _err(g, o, cfg)
-    print("nested fp32 err p95", np.percentile(ex[same], 95), np.percentile(ez[same], 95),
+    print("nested fp32 err max", ex[same].max(), ey[same].max(), ez[same].max(),
           "agree", same.mean(), flush=True)
     assert same.mean() > 0.99
-    assert np.percentile(ex[same], 95) <= 1e-4 and np.percentile(ez[same], 95) <= 1e-4
+    assert ex[same].max() <= 1e-4 and ey[same].max() <= 1e-4 and ez[same].max() <= 1e-4
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -151,7 +153,7 @@ def test_thirty_levels_spline_path(precision):
     else:
         same = g["status"] == o.status
         assert same.mean() > 0.99
-        assert np.percentile(ez[same], 95) <= 1e-4
+        assert ex[same].max() <= 1e-4 and ey[same].max() <= 1e-4 and ez[same].max() <= 1e-4
 
 
 @pytest.mark.parametrize("case", ["c1", "mini", "nested"])

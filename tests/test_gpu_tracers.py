@@ -8,6 +8,8 @@ import pytest
 from oracle import oracle as O
 from romspath_amd import synth
 from romspath_amd.runner import GpuRun
+
+from lockstep import Lockstep
 from romspath_amd.schedule import schedule
 
 from test_tracers import A0, BX, CZ, DT, tracer_case
@@ -81,53 +83,41 @@ def test_nested_tracers_after_steps_match_oracle():
 
 @pytest.mark.parametrize("mean", [False, True])
 def test_per_step_tracers_match_oracle(mean):
-    """rp_tracers_begin/update/get against the oracle's restatement applied to
-    the GPU's own particle state after every step, bit for bit in FP64:
-    release-time values, values kept for particles that did not step, and
-    SaltTempMean's mean / mI with a reset after a read; re-sorting on."""
-    import types
+    """rp_tracers_begin/update/get against the oracle's restatement, the GPU
+    and the oracle stepping side by side with the same injected deviates,
+    each sampler on its own code's state, bit for bit in FP64: release-time
+    values, values kept for particles that did not step, rejected moves
+    sampled with the attempted point's depth, and SaltTempMean's mean / mI
+    with a reset after a read; re-sorting on, no output steps."""
     cfg = synth.config_mini_nested(n_particles=600)
     prm = cfg.params
     steps = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, 24))
     print(f"case per-step tracers mini-nested fp64 mean={mean}", flush=True)
-    run = GpuRun(cfg, precision="fp64", sort_interval=4)
-    ogs = [O.OracleGrid(g) for g in cfg.grids]
 
-    def fields(gi, rec_no):
+    def extra(gi, rec_no):
         g = cfg.grids[gi]
         rng = np.random.default_rng(1000 * gi + rec_no)
         salt = 34.0 + rng.standard_normal((g.N, g.M, g.L))
-        return salt, 10.0 + 3.0 * rng.standard_normal((g.N, g.M, g.L))
+        return dict(salt=salt, temp=10.0 + 3.0 * rng.standard_normal((g.N, g.M, g.L)))
 
-    def load(st):
-        run.load(st)
-        for ld in st.loads:
-            for gi in range(cfg.ngrid):
-                salt, temp = fields(gi, ld.record)
-                run.ctx.set_hydro_tracers(gi + 1, ld.slot, salt, temp)
-                rec = dict(run.records.get(gi + 1, ld.record), salt=salt, temp=temp)
-                ogs[gi].set_slot(ld.slot, rec)
-
-    def state():
-        p = run.ctx.get_particles()
-        return types.SimpleNamespace(n=len(p["x"]), x=p["x"], y=p["y"], z=p["z"], gid=p["gid"],
-                                     age=p["age"], isin=p["isin"])
-
+    ls = Lockstep(cfg, precision="fp64", sort_interval=4, extra=extra)
+    rejected = 0
     try:
-        load(steps[0])
-        run.ctx.tracers_begin(mean, steps[0].tc)
-        la, ps, pt, ms, mt, mI = O.tracers_begin(ogs, steps[0].tc, state(), mean)
+        ls.load(steps[0])
+        ls.ctx.tracers_begin(mean, steps[0].tc)
+        la, ps, pt, ms, mt, mI = O.tracers_begin(ls.ogs, steps[0].tc, ls.state, mean)
         assert ps.any()
         checked = 0
         for k, st in enumerate(steps):
             if k > 0:
-                load(st)
-            run.ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index)
-            run.ctx.tracers_update(st.ex, st.ix, st.tb, st.tc, st.tf)
-            O.tracers_update(ogs, st, state(), la, ps, pt, ms, mt)
+                ls.load(st)
+            ls.step(st)
+            ls.ctx.tracers_update(st.ex, st.ix, st.tb, st.tc, st.tf)
+            O.tracers_update(ls.ogs, st, ls.state, la, ps, pt, ms, mt)
+            rejected += int((ls.state.status == -9.0).sum())
             mI += 1
             if k in (9, len(steps) - 1):
-                gs, gt = run.ctx.tracers_get(reset_mean=True)
+                gs, gt = ls.ctx.tracers_get(reset_mean=True)
                 if mean:
                     assert np.array_equal(gs, ms / mI) and np.array_equal(gt, mt / mI)
                     ms[:] = 0.0
@@ -137,5 +127,6 @@ def test_per_step_tracers_match_oracle(mean):
                     assert np.array_equal(gs, ps) and np.array_equal(gt, pt)
                 checked += 1
         assert checked == 2
+        assert rejected > 0, "the case has no rejected moves"
     finally:
-        run.close()
+        ls.close()

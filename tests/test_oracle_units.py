@@ -273,9 +273,11 @@ def _og_smooth(x, y, dy, s):
     return sf, a.reshape(4, n).T.copy()
 
 
-@pytest.mark.skipif(O.ref_pppack() is None, reason="reference PPPACK not built (oracle/_ref)")
 def test_smooth_bit_exact_vs_reference_pppack():
+    # loaded inside the test, never at collection: a GPU-only run never maps it
     R = O.ref_pppack()
+    if R is None:
+        pytest.skip("reference PPPACK not built (oracle/_ref)")
     rng = np.random.default_rng(7)
     for trial in range(150):
         n = int(rng.integers(6, 130))
@@ -291,9 +293,11 @@ def test_smooth_bit_exact_vs_reference_pppack():
         assert np.array_equal(a1, a2.reshape(4, n).T)
 
 
-@pytest.mark.skipif(O.ref_pppack() is None, reason="reference PPPACK not built (oracle/_ref)")
 def test_ppvalu_bit_exact_vs_reference_pppack():
+    # loaded inside the test, never at collection: a GPU-only run never maps it
     R = O.ref_pppack()
+    if R is None:
+        pytest.skip("reference PPPACK not built (oracle/_ref)")
     rng = np.random.default_rng(8)
     n = 40
     x, y = _profile(n, rng)

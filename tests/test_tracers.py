@@ -61,7 +61,7 @@ def test_per_step_state_and_mean():
     rng = np.random.default_rng(4)
     isin = rng.random(n) < 0.8
     st = types.SimpleNamespace(n=n, x=x, y=y, z=z, gid=np.ones(n, np.int32), isin=isin,
-                               age=rng.uniform(0, 1e4, n))
+                               age=rng.uniform(0, 1e4, n), bath=np.full(n, 50.0))
     zz = np.where(z > zk[-1], zk[-1], z)
     la, ps, pt, ms, mt, mI = O.tracers_begin([og], 2, st, mean=True)
     want0 = A0 + BX * x + CZ * zz + DT * ex[1]
