@@ -116,6 +116,24 @@ int rp_set_particles(void *ctx, int64_t n, const double *x, const double *y, con
 int rp_step(void *ctx, const double ex[3], const double ix[3], int32_t ets, int32_t t_b,
             int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag);
 
+/* No counterpart in the reference.  rp_step for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): same arguments, same results, and no
+ * hipMalloc, hipFree or allocating library call.  Its scratch (the hand-over
+ * rows between the step's two kernels, the vertical walk's work-queue
+ * counter and per-lane work rows) lives in one 64 MiB device array that
+ * every GPU the library runs on holds until the process ends, instead of
+ * the context's own buffers.  Particles are stepped in launch pairs of at
+ * most 131,072; the array's 60 MiB row region bounds the vertical walk's
+ * lanes (fewer resident lanes than rp_step uses), and a spline whose work
+ * rows for one block of lanes exceed it returns RP_E_ARG.  No re-sort
+ * happens (results never depend on the order), the step is not timed, and a
+ * recorded run (rp_adjoint_record) returns RP_E_STATE.  All calls on one
+ * GPU share the array: calls that overlap on one GPU (other contexts,
+ * other streams) race.  Asynchronous. */
+int rp_step_noalloc(void *ctx, const double ex[3], const double ix[3], int32_t ets, int32_t t_b,
+                    int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag);
+
 /* dataOutput (ROMSPath.f90:1012-1072): copies the particle columns back in
  * release order.  Any pointer may be NULL.  Synchronises the stream. */
 int rp_get_particles(void *ctx, int64_t n, double *x, double *y, double *z, double *age,

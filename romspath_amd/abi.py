@@ -26,6 +26,7 @@ PU8 = C.POINTER(C.c_uint8)
 PI64 = C.POINTER(C.c_int64)
 
 EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", "rp_step",
+           "rp_step_noalloc",
            "rp_get_particles", "rp_set_deviates", "rp_sample_deviates", "rp_active_steps",
            "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_phase_timing",
            "rp_set_tuning", "rp_set_scenarios", "rp_step_scenario", "rp_step_batch",
@@ -92,6 +93,8 @@ def lib():
         L.rp_set_particles.argtypes = [vp, C.c_int64, PD, PD, PD, PD, PD, PD, PI32, PU8]
         L.rp_step.argtypes = [vp, PD, PD, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
                               C.c_int32]
+        L.rp_step_noalloc.argtypes = [vp, PD, PD, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int64, C.c_int32]
         L.rp_get_particles.argtypes = [vp, C.c_int64, PD, PD, PD, PD, PD, PI32, PU8, PD, PD, PD]
         L.rp_set_deviates.argtypes = [vp, PD, C.c_int64, C.c_int32]
         L.rp_sample_deviates.argtypes = [vp, C.c_int64, C.c_int64, C.c_int32, PD]
@@ -247,6 +250,12 @@ class Context:
         ixa = (C.c_double * 3)(*ix)
         self._chk("rp_step", self._L.rp_step(self.h, exa, ixa, ets, tb, tc, tf, step_index,
                                              int(want_diag)))
+
+    def step_noalloc(self, ex, ix, ets, tb, tc, tf, step_index, want_diag=False):
+        exa = (C.c_double * 3)(*ex)
+        ixa = (C.c_double * 3)(*ix)
+        self._chk("rp_step_noalloc", self._L.rp_step_noalloc(self.h, exa, ixa, ets, tb, tc, tf,
+                                                             step_index, int(want_diag)))
 
     def set_tuning(self, block=0, sort_interval=0, tile=(1, 1)):
         self._chk("rp_set_tuning", self._L.rp_set_tuning(self.h, int(block), int(sort_interval),

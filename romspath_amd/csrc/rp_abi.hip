@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -15,6 +16,46 @@
 using namespace rp;
 
 namespace {
+
+// ---------------------------------------------------------------------------
+// Scratch of rp_step_noalloc, which allocates nothing: one file-scope device
+// array, held by every GPU the library runs on until the process ends.
+//   [0, kNaHead)                  the vertical walk's work-queue counter
+//   [kNaHead, +kNaMidBytes)       hand-over rows of one launch pair:
+//                                 4 FP64 rows of kNaChunk particles
+//   [kNaHead + kNaMidBytes, end)  the vertical walk's per-lane work rows
+// Larger particle sets are stepped in launch pairs of kNaChunk particles;
+// the row region bounds the lanes of the persistent vertical walk.  Every
+// call on a GPU shares that GPU's array: calls that overlap on one GPU race.
+// Host code uses the address hipGetSymbolAddress returned (na_ptr), never
+// the array's name.
+// ---------------------------------------------------------------------------
+constexpr size_t kNaBytes = 64ull << 20;
+constexpr size_t kNaHead = 256;
+constexpr int64_t kNaChunk = 131072;
+constexpr size_t kNaMidBytes = 4ull * sizeof(double) * kNaChunk;
+constexpr size_t kNaRowBytes = kNaBytes - kNaHead - kNaMidBytes;
+static_assert(kNaRowBytes < 0x7FFFFFFFull, "row region addressed through one buffer descriptor");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaMaxDev = 64;
+std::mutex na_mu;
+char* na_ptr[kNaMaxDev] = {};
+
+// the array's device address on the current device, looked up once per device
+int na_setup(int dev, char** out)
+{
+    if (dev < 0 || dev >= kNaMaxDev) return -1;
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        na_ptr[dev] = (char*)p;
+    }
+    *out = na_ptr[dev];
+    return 0;
+}
 
 struct GridDev {
     int L = 0, M = 0, N = 0, vtransform = 0;
@@ -37,6 +78,7 @@ struct GridDev {
 
 struct Ctx {
     rp_config cfg;
+    char* na = nullptr;          // this device's rp_step_noalloc scratch (na_setup)
     int fbytes = 4;
     int spherical = 0;           // GRIDS(1)%spherical, for LL2ij
     int device = 0;
@@ -443,6 +485,7 @@ struct StepPlan {
     StepArgs A;
     int block = 256;
     int64_t grid_all = 0, grid_vt = 0;
+    int64_t vt_cap = 0;                        // blocks of the persistent vertical walk at most
     bool vt = false;
     KFn k[3] = {nullptr, nullptr, nullptr};   // step / advect / vertical walk
     bool cur = false;                          // FP32 form: build the current fields first
@@ -513,8 +556,12 @@ int refresh_tab(Ctx* c)
     return RP_OK;
 }
 
+// na: the no-allocation scratch (rp_step_noalloc), or null.  With it the
+// hand-over rows, the work-queue counter and the vertical walk's work rows
+// come from that array, nothing is allocated and nothing is re-sorted.
 int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, int t_c, int t_f,
-              int64_t step_index, int want_diag, int64_t q0, int64_t cnt, StepPlan& P)
+              int64_t step_index, int want_diag, int64_t q0, int64_t cnt, StepPlan& P,
+              char* na = nullptr)
 {
     int rc = check_step(c, t_b, t_c, t_f, step_index);
     if (rc) return rc;
@@ -553,7 +600,7 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     A.loop = vt ? c->cfg.idt / (int)c->cfg.deltat : 0;
     A.q0 = q0;
     A.n = cnt;
-    A.nmid = c->n;
+    A.nmid = na ? kNaChunk : c->n;
     A.step_index = step_index;
     A.seed = c->cfg.seed;
     // the per-step samplers read the step's own depth (pBath) every step
@@ -561,9 +608,9 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     A.dev = c->use_dev ? c->dev : nullptr;
     A.ndev = c->ndev;
     A.active = c->active;
-    A.work = c->work;
+    A.work = na ? (unsigned long long*)na : c->work;
     A.capped = c->capped;
-    A.mid = c->mid;
+    A.mid = na ? (double*)(na + kNaHead) : c->mid;
     A.pert = c->pert;
     A.pert_nx = c->pert_nx;
     A.pert_ny = c->pert_ny;
@@ -596,23 +643,38 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
                                                                    f32 ? (size_t)block * kLdsQ * 4 : 0));
             bpc = std::max(nb, 1);
         }
-        P.grid_vt = std::min<int64_t>(P.grid_all, (int64_t)c->ncu * bpc);
+        P.vt_cap = (int64_t)c->ncu * bpc;
         if (f32) {
             // the FP32 rows (6 np per lane) are addressed through one buffer
             // descriptor, whose extent is below 2^31 bytes
             const int64_t per_block = 6LL * A.np * block * 4;
-            P.grid_vt = std::max<int64_t>(1, std::min<int64_t>(P.grid_vt, 0x7FFFFFFFLL / per_block));
+            P.vt_cap = std::max<int64_t>(1, std::min<int64_t>(P.vt_cap, 0x7FFFFFFFLL / per_block));
         }
-        const int64_t T = P.grid_vt * block;
-        const size_t need = (size_t)vturb_rows(A.np, maxN + 1) * T * sizeof(double);
-        if (c->scratch_bytes < need) {
-            if ((rc = dalloc(c, (char**)&c->scratch, need))) return rc;
-            c->scratch_bytes = need;
+        if (na) {
+            // as many blocks as the array's row region holds
+            const int64_t lane_bytes = f32 ? 6LL * A.np * 4 : (int64_t)vturb_rows(A.np, maxN + 1) * 8;
+            const int64_t fit = (int64_t)kNaRowBytes / (lane_bytes * block);
+            if (fit < 1)
+                return fail(c, RP_E_ARG, "rp_step_noalloc: the vertical walk's work rows of one block (" +
+                                             std::to_string(lane_bytes * block) + " bytes) exceed the " +
+                                             std::to_string(kNaRowBytes) + "-byte row region");
+            P.vt_cap = std::min(P.vt_cap, fit);
+            A.scratch = (double*)(na + kNaHead + kNaMidBytes);
+            A.scratch_bytes = (uint32_t)kNaRowBytes;
         }
-        A.scratch = c->scratch;
-        A.scratch_bytes = (uint32_t)std::min<size_t>(c->scratch_bytes, 0x7FFFFFFF);
+        P.grid_vt = std::min<int64_t>(P.grid_all, P.vt_cap);
+        if (!na) {
+            const int64_t T = P.grid_vt * block;
+            const size_t need = (size_t)vturb_rows(A.np, maxN + 1) * T * sizeof(double);
+            if (c->scratch_bytes < need) {
+                if ((rc = dalloc(c, (char**)&c->scratch, need))) return rc;
+                c->scratch_bytes = need;
+            }
+            A.scratch = c->scratch;
+            A.scratch_bytes = (uint32_t)std::min<size_t>(c->scratch_bytes, 0x7FFFFFFF);
+        }
     }
-    if (c->cfg.sort_interval > 0) {
+    if (c->cfg.sort_interval > 0 && !na) {
         SortArgs K;
         if ((rc = sort_buffers(c, sort_keys(c, K)))) return rc;
     }
@@ -631,6 +693,7 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
                 const int kin = f == 4 ? g.N + 1 : 1;
                 const int kout = f == 4 ? cur_column(g) : 1;
                 const int64_t npt = field_points(g, f) / kin * kout;
+                if (!g.cur[f] && na) return fail(c, RP_E_STATE, "rp_step_noalloc: current fields not allocated");
                 if (!g.cur[f] && (rc = dalloc(c, &g.cur[f], (size_t)npt))) return rc;
                 const int j = J.njobs++;
                 J.src[j] = (const float*)src[f];
@@ -698,7 +761,7 @@ int enqueue_step(Ctx* c, StepPlan& P, const Ctx::StepEvents* ev)
         hipLaunchKernelGGL(P.k[1], dim3((unsigned)P.grid_all), bd, 0, c->stream, c->dgrids, A);
         HIPCHK(c, hipGetLastError());
         if (ev) HIPCHK(c, hipEventRecord(ev->e[1], c->stream));
-        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, c->stream, c->work);
+        hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, c->stream, A.work);
         hipLaunchKernelGGL(P.k[2], dim3((unsigned)P.grid_vt), bd, lds, c->stream, c->dgrids, A);
     }
     HIPCHK(c, hipGetLastError());
@@ -909,7 +972,7 @@ int rp_create(void** out, const rp_config* cfg)
         delete c;
         return RP_E_HIP;
     }
-    if (upload_p0_tables() != 0) {
+    if (upload_p0_tables() != 0 || na_setup(c->device, &c->na) != 0) {
         rp_destroy(c);
         return RP_E_HIP;
     }
@@ -954,6 +1017,15 @@ int rp_set_grid(void* ctx, int32_t ng, const rp_grid* gh)
     rc |= dalloc(c, (char**)&g.v, nv * g.N * 4 * fb);
     rc |= dalloc(c, (char**)&g.w, nr * (g.N + 1) * 4 * fb);
     rc |= dalloc(c, (char**)&g.aks, nr * (g.N + 1) * 4 * fb);
+    if (c->fbytes == 4) {
+        // the FP32 form's current fields (one value per point at ix(2)),
+        // allocated with the grid so that no step needs to
+        for (int f = 0; f < 5; f++) {
+            const int kin = f == 4 ? g.N + 1 : 1;
+            const int kout = f == 4 ? cur_column(g) : 1;
+            rc |= dalloc(c, &g.cur[f], (size_t)(field_points(g, f) / kin * kout));
+        }
+    }
     if (rc) {
         free_grid(g);
         return RP_E_ALLOC;
@@ -1117,6 +1189,34 @@ int rp_step(void* ctx, const double ex[3], const double ix[3], int32_t ets, int3
     if ((rc = enqueue_step(c, P, &ev))) return rc;
     c->ev_free.pop_back();
     c->ev_used.push_back(ev);
+    return RP_OK;
+}
+
+int rp_step_noalloc(void* ctx, const double ex[3], const double ix[3], int32_t ets, int32_t t_b,
+                    int32_t t_c, int32_t t_f, int64_t step_index, int32_t want_diag)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !ex || !ix) return RP_E_ARG;
+    if (!c->na) return fail(c, RP_E_STATE, "rp_step_noalloc: no scratch address for this device");
+    if (c->adj.K > 0)
+        return fail(c, RP_E_STATE, "rp_step_noalloc: a recorded run (rp_adjoint_record) stores "
+                                   "checkpoints; use rp_step");
+    if (c->n == 0) return check_step(c, t_b, t_c, t_f, step_index);
+    StepPlan P;
+    const int64_t first = std::min<int64_t>(c->n, kNaChunk);
+    int rc = plan_step(c, ex, ix, ets, t_b, t_c, t_f, step_index, want_diag, 0, first, P, c->na);
+    if (rc) return rc;
+    // launch pairs of at most kNaChunk particles; each zeroes the shared
+    // work counter before its vertical walk and writes the hand-over rows
+    // before reading them
+    for (int64_t q0 = 0; q0 < c->n; q0 += kNaChunk) {
+        P.A.q0 = q0;
+        P.A.n = std::min<int64_t>(kNaChunk, c->n - q0);
+        P.grid_all = (P.A.n + P.block - 1) / P.block;
+        P.grid_vt = std::min<int64_t>(P.grid_all, P.vt_cap);
+        if ((rc = enqueue_step(c, P, nullptr))) return rc;
+    }
+    c->steps_done++;
     return RP_OK;
 }
 

@@ -31,7 +31,7 @@ struct StepArgs {
     int open_boundary, ngrid;
     int np, loop;
     int64_t q0, n;               // stored particles q0 .. q0+n-1 take this launch
-    int64_t nmid;                // row stride of mid (all stored particles)
+    int64_t nmid;                // row stride of mid (>= n); row r holds particle q0 + r
     int64_t step_index;
     uint64_t seed;
     double *x, *y, *z, *age;
@@ -46,7 +46,7 @@ struct StepArgs {
     int ndev;
     double* scratch;             // VTurb work arrays, column per lane
     uint32_t scratch_bytes;      // extent of scratch addressed as FP32 rows (< 2^31)
-    double* mid;                 // [4][n] newX, newY, AdvectZ, clamped z between phases
+    double* mid;                 // [4][nmid] newX, newY, AdvectZ, clamped z between phases
     unsigned long long* active;  // += particles that took the step
     unsigned long long* work;    // vwalk_kernel's next unclaimed particle (zeroed per launch)
     unsigned* capped;            // += secant searches stopped at the cap

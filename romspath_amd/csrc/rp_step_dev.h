@@ -1635,17 +1635,19 @@ __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const 
             if (A.ets == 1) advect<F, 2>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
             else advect<F, 3>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
             if (PHASE == 1) {
-                A.mid[q] = newX;
-                A.mid[A.nmid + q] = newY;
-                A.mid[2 * A.nmid + q] = AdvectZ;
-                A.mid[3 * A.nmid + q] = Zpar;
+                const int64_t r = q - A.q0;   // hand-over rows are per launch
+                A.mid[r] = newX;
+                A.mid[A.nmid + r] = newY;
+                A.mid[2 * A.nmid + r] = AdvectZ;
+                A.mid[3 * A.nmid + r] = Zpar;
                 return true;
             }
         } else {
-            newX = A.mid[q];
-            newY = A.mid[A.nmid + q];
-            AdvectZ = A.mid[2 * A.nmid + q];
-            Zpar = A.mid[3 * A.nmid + q];
+            const int64_t r = q - A.q0;
+            newX = A.mid[r];
+            newY = A.mid[A.nmid + r];
+            AdvectZ = A.mid[2 * A.nmid + r];
+            Zpar = A.mid[3 * A.nmid + r];
         }
         if (A.vturb_on) {
             const RhoCell c0 = rho_cell(g, X0, Y0);

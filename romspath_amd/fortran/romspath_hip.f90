@@ -104,6 +104,18 @@ module romspath_hip
       integer(c_int32_t), value :: want_diag
     end function
 
+    ! rp_step without any device allocation (no reference counterpart): for
+    ! hosts that cannot allocate, e.g. inside a captured graph
+    integer(c_int) function rp_step_noalloc(ctx, ex, ix, ets, t_b, t_c, t_f, step_index, &
+        want_diag) bind(C, name="rp_step_noalloc")
+      import :: c_int, c_int32_t, c_int64_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ex(3), ix(3)
+      integer(c_int32_t), value :: ets, t_b, t_c, t_f
+      integer(c_int64_t), value :: step_index
+      integer(c_int32_t), value :: want_diag
+    end function
+
     integer(c_int) function rp_get_particles(ctx, n, x, y, z, age, status, gid, isin, wd, zeta, &
         bath) bind(C, name="rp_get_particles")
       import :: c_int, c_int32_t, c_int64_t, c_int8_t, c_ptr, c_double
