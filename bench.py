@@ -57,9 +57,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def window_start(prm, steps, warmup):
+    """First timed step: at least `warmup` untimed steps, and late enough that
+    the timed window holds a hydro record load after the initial ones (the
+    first comes at external step 3, ROMSPath.f90:488-493), so uploads are
+    always inside the timed region: the window is centred on that load when
+    it is shorter than the run-up to it."""
+    first_load = 2 * (prm.dt // prm.idt)
+    return max(warmup, first_load - steps // 2)
+
+
 def run_form(cfg, records, steps, warmup, precision, sort_interval=16):
     prm = cfg.params
-    sched = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, warmup + steps))
+    start = window_start(prm, steps, warmup)
+    sched = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, start + steps))
     run = GpuRun(cfg, precision=precision, records=records, sort_interval=sort_interval)
     ctx = run.ctx
 
@@ -69,13 +80,14 @@ def run_form(cfg, records, steps, warmup, precision, sort_interval=16):
                 ctx.set_hydro_slot(g, ld.slot, records.get(g, ld.record))
         ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index, want_diag=st.output)
 
-    for st in sched[:warmup]:
+    for st in sched[:start]:
         do(st)
     ctx.synchronize()
     a0 = ctx.active_steps()
     ctx.kernel_timing()
+    uploads = sum(len(st.loads) for st in sched[start:]) * cfg.ngrid
     t0 = time.perf_counter()
-    for k, st in enumerate(sched[warmup:]):
+    for k, st in enumerate(sched[start:]):
         do(st)
         if (k + 1) % 60 == 0:
             log(f"  {precision}: {k + 1}/{steps} steps")
@@ -88,7 +100,8 @@ def run_form(cfg, records, steps, warmup, precision, sort_interval=16):
     out = ctx.get_particles()
     run.close()
     return dict(active=a1 - a0, seconds=t1 - t0, avg_kernel_ms=avg_ms, launches=launches,
-                phase_ms=phase_ms, capped=capped, state=out)
+                phase_ms=phase_ms, capped=capped, state=out, window=(start, start + steps),
+                uploads=uploads)
 
 
 def batched_ensembles(runup, window, tune_steps, log_every=120):
@@ -350,8 +363,10 @@ def main():
     ap.add_argument("--c5-steps", type=int, default=128)
     ap.add_argument("--c5-ckpt", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy 1 GiB device-to-device (reference reading for the HBM "
-                         "byte counters) and exit")
+                    help="only run three kernels of known traffic and exit: fill 1 GiB "
+                         "(writes 1 GiB), copy 1 GiB device-to-device (reads and writes "
+                         "1 GiB), sum 1 GiB (reads 1 GiB): reference readings for the HBM "
+                         "byte counters")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -361,7 +376,9 @@ def main():
         torch.cuda.synchronize()
         b.copy_(a)
         torch.cuda.synchronize()
-        log("copied 1 GiB")
+        s = a.sum()
+        torch.cuda.synchronize()
+        log(f"filled, copied and summed 1 GiB (sum {float(s):.0f})")
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path "
@@ -386,8 +403,8 @@ def main():
                 f"{npart} particles")
         dims = {"grid": [400, 300, 30]}
     prm = cfg.params
-    total = args.warmup + args.steps
     need = set()
+    total = window_start(prm, args.steps, args.warmup) + args.steps
     for st in schedule(prm.dt, prm.idt, prm.days, prm.iprint, total):
         for ld in st.loads:
             need.add(ld.record)
@@ -406,11 +423,18 @@ def main():
     per_launch = res["active"] / max(res["launches"], 1)
     achieved = balg * per_launch / (res["avg_kernel_ms"] * 1e-3) / 1e9 if res["avg_kernel_ms"] else 0.0
     adv_ms, walk_ms = res["phase_ms"]
-    traffic = None
+    # HBM bytes and L2 hit rate of this command's step kernels, from the
+    # stored rocprofv3 profile of the same command (32-byte request counters,
+    # calibrated on a 1 GiB read-only kernel and a 1 GiB copy): the counters
+    # cannot be read from inside the run itself
+    traffic = l2_hit = measured_gbs = None
     tr_file = ROOT / "profiles" / "hbm_traffic_c3_fp32.json"
     if args.workload == "c3" and args.precision == "fp32" and tr_file.exists():
         tr = json.loads(tr_file.read_text())
         traffic = (tr["read_bytes_per_particle_step"] + tr["write_bytes_per_particle_step"]) * per_launch
+        l2_hit = tr.get("l2_hit_rate")
+        if res["avg_kernel_ms"]:
+            measured_gbs = traffic / (res["avg_kernel_ms"] * 1e-3) / 1e9
     log(f"  kernels: advect {adv_ms:.3f} ms, vertical walk {walk_ms:.3f} ms per step")
     # FP64 vector roofline of the same kernels: FLOP and VALU instructions per
     # particle-step from rocprofv3 SQ counters of this workload
@@ -486,16 +510,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": ("fp32 fields / fp64 state" if f == 4 else "fp64"),
         "data": "synthetic",
         "config": dict({"workload": desc + f", {'FP32' if f == 4 else 'FP64'} fields / FP64 state",
                         "particles": npart, "idt_s": prm.idt, "dt_turb_s": prm.deltat,
                         "field_precision": args.precision, "sort_interval": args.sort_interval,
+                        "timed_steps": list(res["window"]),
+                        "hydro_uploads_timed": res["uploads"],
                         "parallelism": "single GPU"}, **dims),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "traffic_source": "rocprofv3 FETCH_SIZE x2 + WRITE_SIZE per step, "
-                                       "profiles/hbm_traffic_c3_fp32.md" if traffic else None,
+                     "achieved_is": "algorithmic bytes (b_alg x active particles per launch) "
+                                    "per second of step kernel time",
+                     "traffic_source": ("stored profile of this command: 32 x rocprofv3 "
+                                        "TCC_EA0_RDREQ_DRAM_32B + TCC_EA0_WRREQ_WRITE_DRAM_32B per "
+                                        "particle-step x this run's particles per launch, "
+                                        "profiles/hbm_traffic_c3_fp32.md") if traffic else None,
+                     "l2_hit_rate": l2_hit,
+                     "l2_hit_rate_source": ("stored profile: TCC_HIT / (TCC_HIT + TCC_MISS) of the "
+                                            "step kernels, profiles/hbm_traffic_c3_fp32.md")
+                     if l2_hit is not None else None,
+                     "measured_traffic_gbs": measured_gbs,
                      "kernel": "one step = advect_kernel + vwalk_kernel",
                      "b_alg_bytes_per_particle_step": balg,
                      "avg_kernel_ms": res["avg_kernel_ms"],
