@@ -130,29 +130,25 @@ def batched_ensembles(runup, window, tune_steps, log_every=120):
             yield i, j
             i = j
 
-    # run-up in graph batches, with the autotuner on its last steps
+    # run-up in graph batches; the autotuner replays the last tune_steps
+    # steps of the run-up from a snapshot once per candidate, then the
+    # run-up's last steps run with the chosen knobs
     tune_from = max(0, runup - tune_steps)
     for i, j in groups(0, tune_from):
         run.load(steps[i])
         ctx.step_batch(steps[i:j], use_graph=True)
         if i // log_every != j // log_every:
             log(f"  run-up {j}/{runup}")
-    it = iter(range(tune_from, runup))
+    snap0 = run.snapshot()
 
-    def next_step():
-        k = next(it, None)
-        if k is None:
-            return False
-        st = steps[k]
+    def run_step(st):
         run.load(st)
         ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index, want_diag=st.output)
-        return True
 
-    tuned = autotune(ctx, next_step, trial_steps=8, log=log)
-    for k in it:                       # run-up steps the tuner left over
-        st = steps[k]
-        run.load(st)
-        ctx.step(st.ex, st.ix, st.ets, st.tb, st.tc, st.tf, st.step_index, want_diag=st.output)
+    tuned = autotune(ctx, steps[tune_from:runup], run_step,
+                     lambda: run.restore(snap0, steps, tune_from), log=log)
+    for st in steps[tune_from:runup]:       # the window again, now with the tuned knobs
+        run_step(st)
     snap = ctx.get_particles()
     state = synth.Particles(snap["x"], snap["y"], snap["z"], cfg.particles.dob.copy(),
                             gid=snap["gid"], status=snap["status"], age=snap["age"],
@@ -357,7 +353,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the C4 ensemble comparison")
     ap.add_argument("--c4-runup", type=int, default=720, help="steps before the C4 window (24 h)")
     ap.add_argument("--c4-window", type=int, default=30)
-    ap.add_argument("--c4-tune-steps", type=int, default=96)
+    ap.add_argument("--c4-tune-steps", type=int, default=64,
+                    help="steps each tuning candidate replays (>= 2 x the largest sort interval)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the C5 adjoint measurement")
     ap.add_argument("--c5-particles", type=int, default=262_144)
     ap.add_argument("--c5-steps", type=int, default=128)

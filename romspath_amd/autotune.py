@@ -3,10 +3,15 @@ and sort tile shape (rp_set_tuning).
 
 None of the knobs changes a result (each particle's step reads only its own
 columns and its deviates are keyed by its release id), so the tuner times
-them on the run itself: every candidate takes the next steps of the
-schedule, starting with a fresh re-sort, and is scored by device time per
-active particle-step (HIP events on the context stream, rp_kernel_timing).
-The steps it spends are ordinary steps of the run.
+them on the run itself: every candidate replays the same window of steps
+from the same state (the caller's restore() puts the particles and hydro
+slots back to the window's start), starting with a fresh re-sort, and is
+scored by device time of those steps (HIP events on the context stream,
+rp_kernel_timing).  The window spans at least twice the largest re-sort
+interval among the candidates, so each candidate's interval shows in its
+time; the active particle-steps are the same for every candidate, which the
+tuner checks.  After the trials the state is restored once more and the
+fastest candidate's knobs are left set.
 """
 from __future__ import annotations
 
@@ -50,32 +55,41 @@ def candidates(blocks=BLOCKS, intervals=SORT_INTERVALS, tiles=TILES):
     return [Trial(b, s, t) for b, s, t in itertools.product(blocks, intervals, tiles)]
 
 
-def autotune(ctx, next_step, trial_steps=8, trials=None, log=None) -> TuneResult:
-    """Time each candidate on the next trial_steps steps of the run.
+def autotune(ctx, window, run_step, restore, trials=None, log=None) -> TuneResult:
+    """Time each candidate on the same window of steps.
 
-    ctx: abi.Context with grids, particles and the first hydro records loaded.
-    next_step(): performs the run's next internal step (hydro loads included)
-    and returns False when the schedule is exhausted.
-    Leaves ctx tuned to the fastest candidate."""
+    ctx: abi.Context (or any object with set_tuning, synchronize,
+    kernel_timing and active_steps).  window: the schedule steps every
+    candidate replays; run_step(st) performs one of them (hydro loads
+    included); restore() returns particles and hydro slots to the state
+    before window[0].  Leaves the state restored and ctx tuned to the
+    fastest candidate."""
     trials = trials or candidates()
+    longest = max(t.sort_interval for t in trials)
+    if len(window) < 2 * longest:
+        raise ValueError(f"a tuning window of {len(window)} steps cannot tell sort intervals up to "
+                         f"{longest} apart: it needs at least {2 * longest}")
     done = []
     for t in trials:
+        restore()
         ctx.set_tuning(t.block, t.sort_interval, t.tile)
+        ctx.synchronize()
         ctx.kernel_timing()
         a0 = ctx.active_steps()
-        k = 0
-        while k < trial_steps and next_step():
-            k += 1
+        for st in window:
+            run_step(st)
         ms, launches = ctx.kernel_timing()
-        t.steps = k
+        t.steps = len(window)
         t.active = ctx.active_steps() - a0
         t.ms = ms * launches
         done.append(t)
         if log:
             log(f"  tune block={t.block} sort={t.sort_interval} tile={t.tile}: "
-                f"{t.ns_per_particle_step:.3f} ns/particle-step over {k} steps")
-        if k < trial_steps:
-            break
-    best = min(done, key=lambda t: t.ns_per_particle_step)
+                f"{t.ms:.2f} ms over {t.steps} steps, {t.ns_per_particle_step:.3f} ns/particle-step")
+    if len({t.active for t in done}) != 1:
+        raise RuntimeError("tuning trials stepped different particle counts: restore() did not "
+                           "return to the same state")
+    best = min(done, key=lambda t: t.ms)
+    restore()
     ctx.set_tuning(best.block, best.sort_interval, best.tile)
     return TuneResult(best, done)

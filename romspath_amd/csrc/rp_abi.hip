@@ -93,6 +93,9 @@ struct Ctx {
     uint8_t* isin = nullptr;
     uint32_t* id = nullptr;
     double* mid = nullptr;       // [4][n] hand-over between the step's two kernels
+    uint32_t* act = nullptr;     // [n] the active set of a launch (compact_scatter)
+    uint32_t* act_blk = nullptr; // [(n + 255) / 256] block counts, then offsets
+    unsigned long long* act_n = nullptr;   // its count
     // injected deviates
     double* dev = nullptr;
     int ndev = 0;
@@ -614,6 +617,11 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     A.pert = c->pert;
     A.pert_nx = c->pert_nx;
     A.pert_ny = c->pert_ny;
+    // the step kernels run over the launch's active set, built by
+    // compact_count / sort_scan / compact_scatter (the no-allocation form
+    // runs over all particles)
+    A.act = na ? nullptr : c->act;
+    A.act_n = na ? nullptr : c->act_n;
     if (vt && A.np < 3) return fail(c, RP_E_ARG, "smoothing spline needs sub*(N+1) >= 3");
 
     const int block = c->cfg.block > 0 ? c->cfg.block : 256;
@@ -753,6 +761,14 @@ int enqueue_step(Ctx* c, StepPlan& P, const Ctx::StepEvents* ev)
         }
     }
     const size_t lds = c->fbytes == 4 ? (size_t)P.block * kLdsQ * 4 : 0;   // qty rows (FP32 form)
+    if (A.act) {
+        const int64_t nb = (A.n + 255) / 256;
+        hipLaunchKernelGGL(compact_count, dim3((unsigned)nb), dim3(256), 0, c->stream, A, c->act_blk);
+        hipLaunchKernelGGL(sort_scan, dim3(1), dim3(1024), 0, c->stream, c->act_blk, nb);
+        hipLaunchKernelGGL(compact_scatter, dim3((unsigned)nb), dim3(256), 0, c->stream, A,
+                           (const uint32_t*)c->act_blk, c->act, c->act_n);
+        HIPCHK(c, hipGetLastError());
+    }
     if (!P.vt) {
         hipLaunchKernelGGL(P.k[0], dim3((unsigned)P.grid_all), bd, lds, c->stream, c->dgrids, A);
         HIPCHK(c, hipGetLastError());
@@ -977,7 +993,7 @@ int rp_create(void** out, const rp_config* cfg)
         return RP_E_HIP;
     }
     if (dalloc(c, &c->active, 1) || dalloc(c, &c->capped, 1) || dalloc(c, &c->dgrids, kMaxGrid) ||
-        dalloc(c, &c->work, 1)) {
+        dalloc(c, &c->work, 1) || dalloc(c, &c->act_n, 1)) {
         rp_destroy(c);
         return RP_E_ALLOC;
     }
@@ -1114,6 +1130,8 @@ int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, con
     rc |= dalloc(c, &c->status, n); rc |= dalloc(c, &c->gid, n); rc |= dalloc(c, &c->isin, n);
     rc |= dalloc(c, &c->id, n);
     rc |= dalloc(c, &c->mid, 4 * (size_t)n);
+    rc |= dalloc(c, &c->act, (size_t)n);
+    rc |= dalloc(c, &c->act_blk, (size_t)((n + 255) / 256 + 1));
     if (rc) return RP_E_ALLOC;
     c->n = n;
     c->bot_n = -1;
@@ -1333,12 +1351,8 @@ int rp_step_batch(void* ctx, int32_t nsteps, const double* ex, const double* ix,
         return RP_OK;
     };
     if (use_graph) {
-        if (c->gexec) {
-            // the previous batch may still run from it
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            (void)hipGraphExecDestroy(c->gexec);
-            c->gexec = nullptr;
-        }
+        // the previous batch may still run from the executable graph
+        if (c->gexec) HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_all();
         hipGraph_t graph = nullptr;
@@ -1348,7 +1362,23 @@ int rp_step_batch(void* ctx, int32_t nsteps, const double* ex, const double* ix,
             return rc;
         }
         if (e != hipSuccess) return fail(c, RP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        const hipError_t ei = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+        // batches of the same shape (steps, re-sorts) differ only in kernel
+        // arguments: the executable graph is updated in place; otherwise it
+        // is instantiated anew
+        bool updated = false;
+        if (c->gexec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult ur = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(c->gexec, graph, &bad, &ur) == hipSuccess &&
+                      ur == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(c->gexec);
+                c->gexec = nullptr;
+            }
+        }
+        hipError_t ei = hipSuccess;
+        if (!updated) ei = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (ei != hipSuccess) {
             c->gexec = nullptr;
@@ -2103,7 +2133,7 @@ int rp_destroy(void* ctx)
     for (auto& g : c->grids) free_grid(g);
     dfree(c->x); dfree(c->y); dfree(c->z); dfree(c->age); dfree(c->dob);
     dfree(c->wd); dfree(c->zeta); dfree(c->bath);
-    dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid);
+    dfree(c->status); dfree(c->gid); dfree(c->isin); dfree(c->id); dfree(c->mid); dfree(c->act); dfree(c->act_blk); dfree(c->act_n);
     dfree(c->bot_last_age); dfree(c->bot_hob); dfree(c->bot_bustr); dfree(c->bot_bvstr);
     dfree(c->trc_last_age); dfree(c->trc_salt); dfree(c->trc_temp);
     dfree(c->trc_mean_salt); dfree(c->trc_mean_temp);

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         b = __shfl(b, 0, 64);
         if (b >= (unsigned long long)A.n) break;
         const int64_t r = (int64_t)b + lane;
-        if (r < A.n && step_one<F, NESTED, 0, true>(grids, A, A.q0 + r, tid, T)) nact++;
+        if (r < A.n && step_one<F, NESTED, 0, true>(grids, A, A.q0 + r, r, tid, T)) nact++;
     }
     for (int off = 32; off > 0; off >>= 1) nact += __shfl_down(nact, off, 64);
     if (lane == 0 && nact) atomicAdd(A.active, nact);

@@ -52,6 +52,10 @@ struct StepArgs {
     unsigned* capped;            // += secant searches stopped at the cap
     const double* pert;          // velocity perturbation coefficients [2][ny][nx] or null
     int pert_nx, pert_ny;
+    // the launch's active set (compact_active): particle q0 + act[r] for
+    // r < *act_n; null: every particle q0 .. q0+n-1, inactive ones skipped
+    const uint32_t* act;
+    const unsigned long long* act_n;
 };
 
 // NESTED=false: one grid, all lanes read grid 0 (uniform, scalar loads).
@@ -63,6 +67,15 @@ template <typename F, bool NESTED>
 __global__ void advect_kernel(const DGrid* __restrict__ grids, StepArgs A);
 template <typename F, bool NESTED>
 __global__ void vwalk_kernel(const DGrid* __restrict__ grids, StepArgs A);
+
+// The active set of a launch (the particles that pass the skip tests of
+// ROMSPath.f90:672-690), in particle order: compact_count writes each
+// 256-particle block's count, an exclusive scan of the counts (sort_scan)
+// gives the block offsets, compact_scatter writes act[] by wave votes and
+// prefixes and the count to act_n.  Blocks of 256 threads.
+__global__ void compact_count(StepArgs A, uint32_t* bcnt);
+__global__ void compact_scatter(StepArgs A, const uint32_t* boff, uint32_t* act,
+                                unsigned long long* act_n);
 
 template <typename F>
 __global__ void pack_field(const double* __restrict__ src, const double* __restrict__ add, F* dst,

@@ -12,14 +12,61 @@ __device__ __forceinline__ void count_active(const StepArgs& A, unsigned long lo
     if ((threadIdx.x & 63) == 0 && nact) atomicAdd(A.active, nact);
 }
 
-// one lane per particle (grid covers n)
+// particles in the launch: the compacted active set, or all n.  When every
+// particle is active the list is the identity and is not read.
+__device__ __forceinline__ int64_t launch_count(const StepArgs& A)
+{
+    return A.act ? (int64_t)*A.act_n : A.n;
+}
+__device__ __forceinline__ int64_t launch_particle(const StepArgs& A, int64_t ne, int64_t r)
+{
+    return A.q0 + ((A.act && ne != A.n) ? (int64_t)A.act[r] : r);
+}
+
+// one lane per particle (the grid covers n, the upper bound of the count)
 template <typename F, bool NESTED, int PHASE>
 __device__ __forceinline__ void step_body(const DGrid* __restrict__ grids, const StepArgs& A)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ne = launch_count(A);
     unsigned long long nact = 0;
-    if (r < A.n && step_one<F, NESTED, PHASE>(grids, A, A.q0 + r, r, 0)) nact = 1;
+    if (r < ne && step_one<F, NESTED, PHASE>(grids, A, launch_particle(A, ne, r), r, r, 0)) nact = 1;
     if (PHASE != 1) count_active(A, nact);
+}
+
+// The launch's active set in particle order, in three launches:
+// compact_count counts each block's active particles, sort_scan turns the
+// counts into block offsets, compact_scatter writes the list and its length.
+__device__ __forceinline__ bool active_in_launch(const StepArgs& A, int64_t r)
+{
+    return r < A.n && !skips(A, A.q0 + r);
+}
+
+__global__ void __launch_bounds__(256) compact_count(StepArgs A, uint32_t* bcnt)
+{
+    __shared__ uint32_t wsum[4];
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const unsigned long long m = __ballot(active_in_launch(A, r));
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ void __launch_bounds__(256) compact_scatter(StepArgs A, const uint32_t* boff, uint32_t* act,
+                                                       unsigned long long* act_n)
+{
+    __shared__ uint32_t wsum[4];
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool a = active_in_launch(A, r);
+    const unsigned long long m = __ballot(a);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wsum[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = boff[blockIdx.x];
+    for (int k = 0; k < w; k++) off += wsum[k];
+    if (a) act[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)r;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        *act_n = (unsigned long long)boff[blockIdx.x] + wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
 // Persistent form for the vertical walk: a grid sized to the resident
@@ -34,13 +81,14 @@ __device__ __forceinline__ void vwalk_body(const DGrid* __restrict__ grids, cons
     const int64_t T = (int64_t)gridDim.x * blockDim.x;
     const int lane = threadIdx.x & 63;
     unsigned long long nact = 0;
+    const int64_t ne = launch_count(A);
     for (;;) {
         unsigned long long b = 0;
         if (lane == 0) b = atomicAdd(A.work, 64ull);
         b = __shfl(b, 0, 64);
-        if (b >= (unsigned long long)A.n) break;
+        if (b >= (unsigned long long)ne) break;
         const int64_t r = (int64_t)b + lane;
-        if (r < A.n && step_one<F, NESTED, 2>(grids, A, A.q0 + r, tid, T)) nact++;
+        if (r < ne && step_one<F, NESTED, 2>(grids, A, launch_particle(A, ne, r), r, tid, T)) nact++;
     }
     count_active(A, nact);
 }

@@ -1618,10 +1618,12 @@ __device__ __forceinline__ void finish(const DGrid* __restrict__ grids, const St
     A.gid[q] = (int8_t)o.gid;
 }
 
-// One particle of the step.  Returns whether it took the step.
+// One particle of the step.  Returns whether it took the step.  mrow: the
+// particle's row of the hand-over rows (its position in the launch's list);
+// tid, T: its vertical-walk scratch column and the row stride.
 template <typename F, bool NESTED, int PHASE, bool FAITHFUL = false>
 __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const StepArgs& A,
-                                         int64_t q, int64_t tid, int64_t T)
+                                         int64_t q, int64_t mrow, int64_t tid, int64_t T)
 {
     {
         if (skips(A, q)) return false;
@@ -1635,7 +1637,7 @@ __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const 
             if (A.ets == 1) advect<F, 2>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
             else advect<F, 3>(g, A, dv, X0, Y0, Zpar, newX, newY, AdvectZ);
             if (PHASE == 1) {
-                const int64_t r = q - A.q0;   // hand-over rows are per launch
+                const int64_t r = mrow;
                 A.mid[r] = newX;
                 A.mid[A.nmid + r] = newY;
                 A.mid[2 * A.nmid + r] = AdvectZ;
@@ -1643,7 +1645,7 @@ __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const 
                 return true;
             }
         } else {
-            const int64_t r = q - A.q0;
+            const int64_t r = mrow;
             newX = A.mid[r];
             newY = A.mid[A.nmid + r];
             AdvectZ = A.mid[2 * A.nmid + r];

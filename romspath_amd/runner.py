@@ -153,5 +153,22 @@ class GpuRun:
                 on_step(st, self)
         return self.ctx.get_particles()
 
+    def snapshot(self):
+        """The particle columns now, as a Particles set rp_set_particles takes
+        back (release order)."""
+        from .synth import Particles
+        p = self.ctx.get_particles()
+        return Particles(p["x"], p["y"], p["z"], self.cfg.particles.dob.copy(), gid=p["gid"],
+                         status=p["status"], age=p["age"], isin=p["isin"])
+
+    def restore(self, snap, steps, k):
+        """Particles from a snapshot taken before steps[k], and the hydro slots
+        as they were then (the records that last filled them)."""
+        from .schedule import slots_before
+        self.ctx.set_particles(snap)
+        for ld in slots_before(steps, k).values():
+            for g in range(1, self.cfg.ngrid + 1):
+                self.ctx.set_hydro_slot(g, ld.slot, self.records.get(g, ld.record, ld.filenum))
+
     def close(self):
         self.ctx.close()

@@ -79,5 +79,16 @@ def schedule(dt: int, idt: int, days: float, iprint: int | None = None, max_step
         tstep += 1
 
 
+def slots_before(steps, k):
+    """The load that last filled each slot before steps[k] runs its own
+    loads: {slot: Load}.  Replaying steps[k:] from a saved particle state
+    needs the hydro slots put back to these records first."""
+    last = {}
+    for st in steps[:k]:
+        for ld in st.loads:
+            last[ld.slot] = ld
+    return last
+
+
 def record_time(record: int, dt: int) -> float:
     return float((record - 1) * dt)
