@@ -14,6 +14,7 @@
  *   zp(time, numpar)            float   par(:,pZ)
  *   status(time, numpar)        float   par(:,pStatus)
  *   GID(time, numpar)           float   par(:,pGID), only when Ngrid > 1
+ *   hitBottom, hitLand          float   0, when asked for (TrackCollisions)
  *   salinity, temperature       float   P_salt, P_temp, when asked for (SaltTempOn)
  *   HOB, bustr, bvstr           float   P_HOB, P_bustr, P_bvstr, when asked for (WriteBottom)
  *   WaterDepth, zeta, h         float   par(:,pWD/pZeta/pBath), when asked for
@@ -35,9 +36,22 @@
 extern "C" {
 #endif
 
-enum { RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4, RP_NC_SALTTEMP = 8, RP_NC_BOTTOM = 16 };
+/* Optional variables: WriteWaterDepth, WriteZeta, WriteBath, SaltTempOn,
+ * WriteBottom, TrackCollisions (hitBottom / hitLand, written as zeros: the
+ * reference never increments them, advection_module.f90:173). */
+enum { RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4, RP_NC_SALTTEMP = 8, RP_NC_BOTTOM = 16,
+       RP_NC_COLLISIONS = 32 };
 
-/* Global attributes (createNetCDF, ROMSPath.f90:1650-1760). */
+/* Global attributes (createNetCDF, ROMSPath.f90:1676-1836), written in the
+ * reference's order: title, author, adv_time_step, days_run, Behavior,
+ * OpenOceanBoundary (only when set), swimfast, Vertical_Turb (+ seed,
+ * turb_time_step, spline_*, subset_scale, AKs_bakcground when on),
+ * Horiz_Turb (+ seed), run_name, executable, output_loc, administrator,
+ * institution, Start_datetime (local wall clock at creation, the
+ * reference's "YYYY/MM/DD hh:mm:ss"), SaltTempMean, TempOffset.  String
+ * fields may be NULL: run_name is then left out, the others are written
+ * empty.  The behaviour-10 / Process_VA and GROWTH attributes belong to
+ * modules outside the particle step and are not written. */
 typedef struct {
     const char *time_units;      /* model_time units (GRID_MOD time_units) */
     const char *run_name;        /* RunName, may be NULL */
@@ -48,6 +62,13 @@ typedef struct {
     int32_t vturb_on, hturb_on;  /* Vertical_Turb / Horiz_Turb */
     int32_t seed;
     double deltat, serr, smth, sub, aksback;
+    double swimfast;             /* $behavparam swimfast */
+    double temp_offset;          /* $other TempOffset */
+    int32_t salt_temp_mean;      /* $other SaltTempMean -> "TRUE" / "FALSE" */
+    const char *executable;      /* $output ExeDir */
+    const char *output_loc;      /* $output OutDir */
+    const char *administrator;   /* $output RunBy */
+    const char *institution;     /* $output Institution */
 } rp_nc_meta;
 
 /* createNetCDF(par(:,pDOB)) (ROMSPath.f90:449, 1140-1868): creates the file

@@ -50,7 +50,10 @@ class RpNcMeta(C.Structure):
                 ("days", C.c_float), ("behavior", C.c_int32), ("open_boundary", C.c_int32),
                 ("vturb_on", C.c_int32), ("hturb_on", C.c_int32), ("seed", C.c_int32),
                 ("deltat", C.c_double), ("serr", C.c_double), ("smth", C.c_double),
-                ("sub", C.c_double), ("aksback", C.c_double)]
+                ("sub", C.c_double), ("aksback", C.c_double), ("swimfast", C.c_double),
+                ("temp_offset", C.c_double), ("salt_temp_mean", C.c_int32),
+                ("executable", C.c_char_p), ("output_loc", C.c_char_p),
+                ("administrator", C.c_char_p), ("institution", C.c_char_p)]
 
 
 class RpConfig(C.Structure):

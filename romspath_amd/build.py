@@ -58,6 +58,27 @@ def build_fortran(verbose: bool = True) -> Path:
     return DRIVER
 
 
+NC_CHECK = PKG / "lib" / "rp_nc_check_san"
+
+
+def build_nc_check(verbose: bool = True) -> Path:
+    """The netCDF reader and writer (host C++) with a small driver, built with
+    AddressSanitizer and UndefinedBehaviorSanitizer for the tests that feed
+    them damaged files.  Host code only: g++, no device code."""
+    NC_CHECK.parent.mkdir(parents=True, exist_ok=True)
+    cmd = [CXX, "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer",
+           "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+           "-I", str(ROOT / "include"), "-o", str(NC_CHECK),
+           str(CSRC / "rp_nc_check.cpp"), str(CSRC / "rp_ncin.cpp"), str(CSRC / "rp_ncout.cpp")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"sanitizer build failed ({r.returncode})")
+    return NC_CHECK
+
+
 def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -101,6 +122,7 @@ def build(verbose: bool = True) -> Path:
                 verbose), "link", verbose)
     os.replace(tmp, LIB)
     build_fortran(verbose)
+    build_nc_check(verbose)
     return LIB
 
 

@@ -2,8 +2,10 @@
 // the extern "C" entry points declared in include/rp_abi.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -78,6 +80,10 @@ struct GridDev {
 
 struct Ctx {
     rp_config cfg;
+    // allocation-failure test knob: RP_FAIL_ALLOC_AFTER=k in the environment
+    // at rp_create lets the context's first k device or pinned allocations
+    // succeed and refuses the rest with RP_E_ALLOC; -1 (unset): no limit
+    int64_t alloc_budget = -1;
     char* na = nullptr;          // this device's rp_step_noalloc scratch (na_setup)
     int fbytes = 4;
     int spherical = 0;           // GRIDS(1)%spherical, for LL2ij
@@ -213,6 +219,10 @@ int dalloc(Ctx* c, T** p, size_t count)
         *p = nullptr;
     }
     if (count == 0) count = 1;
+    if (c->alloc_budget == 0)
+        return fail(c, RP_E_ALLOC, "allocation of " + std::to_string(count * sizeof(T)) +
+                                       " bytes refused (RP_FAIL_ALLOC_AFTER)");
+    if (c->alloc_budget > 0) c->alloc_budget--;
     hipError_t e = hipMalloc((void**)p, count * sizeof(T));
     if (e != hipSuccess) {
         *p = nullptr;
@@ -279,12 +289,30 @@ int make_event(Ctx* c, hipEvent_t& e)
 // Field f of grid gi has its own device staging buffer.  The call returns
 // once the caller's arrays have been read; the conversion is ordered on the
 // step stream, after the steps already queued there and before later ones.
+const char* const kFieldName[9] = {"zeta", "u", "v", "w", "AKs", "salt", "temp", "bustr", "bvstr"};
+
+// the first non-finite value of a caller's array, or -1
+int64_t first_nonfinite(const double* a, size_t n)
+{
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(a[i])) return (int64_t)i;
+    return -1;
+}
+
 int upload_field(Ctx* c, int gi, int f, void* dst, const double* src, const double* add, int nx,
                  int ny, int nk, int slot)
 {
     const size_t elems = (size_t)nx * ny * nk;
     const size_t need = add ? 2 * elems : elems;
     int rc;
+    // a NaN or Inf in a hydro record is rejected before anything is queued
+    for (int k = 0; k < (add ? 2 : 1); k++) {
+        const int64_t bad = first_nonfinite(k ? add : src, elems);
+        if (bad >= 0)
+            return fail(c, RP_E_ARG, std::string("non-finite value in the ") + (k ? "Stokes drift of " : "") +
+                                         kFieldName[f] + " field of grid " + std::to_string(gi + 1) +
+                                         " at element " + std::to_string(bad));
+    }
     if (!c->up_stream) HIPCHK(c, hipStreamCreateWithFlags(&c->up_stream, hipStreamNonBlocking));
     Ctx::FieldStage& fs = c->fstage[gi][f];
     if ((rc = make_event(c, fs.copied)) || (rc = make_event(c, fs.packed))) return rc;
@@ -307,6 +335,9 @@ int upload_field(Ctx* c, int gi, int f, void* dst, const double* src, const doub
             if (h.p) (void)hipHostFree(h.p);
             h.p = nullptr;
             h.elems = 0;
+            if (c->alloc_budget == 0)
+                return fail(c, RP_E_ALLOC, "pinned staging allocation refused (RP_FAIL_ALLOC_AFTER)");
+            if (c->alloc_budget > 0) c->alloc_budget--;
             HIPCHK(c, hipHostMalloc((void**)&h.p, elems * sizeof(double), hipHostMallocDefault));
             h.elems = elems;
         }
@@ -970,6 +1001,7 @@ int rp_create(void** out, const rp_config* cfg)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return RP_E_NODEV;
     Ctx* c = new Ctx();
     c->cfg = *cfg;
+    if (const char* k = std::getenv("RP_FAIL_ALLOC_AFTER")) c->alloc_budget = std::max<int64_t>(0, std::atoll(k));
     std::memset(c->tab_host, 0, sizeof(c->tab_host));
     if (cfg->precision != RP_PREC_FP32_FIELDS && cfg->precision != RP_PREC_FP64) {
         delete c;
@@ -1118,6 +1150,9 @@ int rp_set_particles(void* ctx, int64_t n, const double* x, const double* y, con
     std::vector<int8_t> st(n), gd(n);
     std::vector<uint32_t> ids(n);
     for (int64_t i = 0; i < n; i++) {
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]) ||
+            !std::isfinite(age[i]) || !std::isfinite(dob[i]))
+            return fail(c, RP_E_ARG, "non-finite particle column value at particle " + std::to_string(i));
         if (status_code(status[i], &st[i])) return fail(c, RP_E_ARG, "unknown status value");
         if (gid[i] < 1 || gid[i] > c->cfg.ngrid) return fail(c, RP_E_ARG, "grid id out of range");
         gd[i] = (int8_t)gid[i];
@@ -1167,6 +1202,7 @@ int rp_set_deviates(void* ctx, const double* normals, int64_t n, int32_t ndev)
     }
     const int need = 2 + (c->cfg.vturb_on ? c->cfg.idt / (int)c->cfg.deltat : 0);
     if (n != c->n || ndev < need) return fail(c, RP_E_ARG, "deviate block has the wrong shape");
+    if (first_nonfinite(normals, (size_t)n * ndev) >= 0) return fail(c, RP_E_ARG, "non-finite deviate");
     int rc = dalloc(c, &c->dev, (size_t)n * ndev);
     if (rc) return rc;
     if ((rc = put(c, c->dev, normals, (size_t)n * ndev * 8))) return rc;

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -207,6 +208,14 @@ int rp_nc_create(void** nc, const char* path, int64_t numpar, int32_t ngrid, int
         desc("identification number for particle behavior or status",
              "nondimensional, see ROMSPath User Guide", "status, scalar, series"));
     if (ngrid > 1) add("GID", NC_FLOAT, true, true, desc("Grid ID", "none", nullptr));
+    if (flags & RP_NC_COLLISIONS) {
+        add("hitBottom", NC_FLOAT, true, true,
+            desc("# of times Particle Collided with Bottom", "Number of Collisions",
+                 "hitBottom, scalar, series"));
+        add("hitLand", NC_FLOAT, true, true,
+            desc("# of times Particle Collided with Land", "Number of Collisions",
+                 "hitLand, scalar, series"));
+    }
     if (flags & RP_NC_SALTTEMP) {
         add("salinity", NC_FLOAT, true, true,
             desc("Salinity at the particle's location", nullptr, "salinity, scalar, series"));
@@ -231,11 +240,14 @@ int rp_nc_create(void** nc, const char* path, int64_t numpar, int32_t ngrid, int
         add("h", NC_FLOAT, true, true, desc("bathymetry at particle locations", "m", "H, scalar, series"));
 
     std::vector<Att> g;
+    auto str = [](const char* s) { return std::string(s ? s : ""); };
     g.push_back(text("title", "Particle tracking output using ROMS data."));
+    g.push_back(text("author", "Elias Hunter (hunter@marine.rutgers.edu)"));
     g.push_back(num("adv_time_step", NC_INT, meta->idt));
     g.push_back(num("days_run", NC_FLOAT, meta->days));
     g.push_back(num("Behavior", NC_INT, meta->behavior));
     if (meta->open_boundary) g.push_back(text("OpenOceanBoundary", "TRUE"));
+    g.push_back(num("swimfast", NC_DOUBLE, meta->swimfast));
     if (meta->vturb_on) {
         g.push_back(text("Vertical_Turb", "True"));
         g.push_back(num("seed", NC_INT, meta->seed));
@@ -254,7 +266,22 @@ int rp_nc_create(void** nc, const char* path, int64_t numpar, int32_t ngrid, int
         g.push_back(text("Horiz_Turb", "False"));
     }
     if (meta->run_name) g.push_back(text("run_name", meta->run_name));
-    g.push_back(text("SaltTempMean", "FALSE"));
+    g.push_back(text("executable", str(meta->executable)));
+    g.push_back(text("output_loc", str(meta->output_loc)));
+    g.push_back(text("administrator", str(meta->administrator)));
+    g.push_back(text("institution", str(meta->institution)));
+    {
+        // date_and_time -> "(I4'/'I0.2'/'I0.2' 'I0.2':'I0.2':'I0.2)"
+        char when[32] = "";
+        const time_t now = time(nullptr);
+        struct tm lt;
+        if (localtime_r(&now, &lt))
+            std::snprintf(when, sizeof when, "%4d/%02d/%02d %02d:%02d:%02d", lt.tm_year + 1900,
+                          lt.tm_mon + 1, lt.tm_mday, lt.tm_hour, lt.tm_min, lt.tm_sec);
+        g.push_back(text("Start_datetime", when));
+    }
+    g.push_back(text("SaltTempMean", meta->salt_temp_mean ? "TRUE" : "FALSE"));
+    g.push_back(num("TempOffset", NC_DOUBLE, meta->temp_offset));
     g.push_back(text("source", "romspath_amd (MI355X particle step), netCDF classic 64-bit offset"));
 
     // slab sizes, then the header with placeholder offsets to learn its length
@@ -363,7 +390,13 @@ int rp_nc_write(void* nc, double model_time, const double* age, const double* lo
         else if (v.name == "HOB") src = hob;
         else if (v.name == "bustr") src = bustr;
         else if (v.name == "bvstr") src = bvstr;
-        encode(w, v.type, src, isrc, v.per_particle ? w->numpar : 1);
+        if (!src && !isrc) {
+            // hitBottom / hitLand: the reference's counters stay at their
+            // initial 0 (the increment, advection_module.f90:173, is commented out)
+            w->buf.assign((size_t)w->numpar * 4, 0);
+        } else {
+            encode(w, v.type, src, isrc, v.per_particle ? w->numpar : 1);
+        }
         if ((rc = put_at(w, base + (v.begin - w->rec_begin), w->buf.data(), w->buf.size()))) return rc;
     }
     w->nrec++;

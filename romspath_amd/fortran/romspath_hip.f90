@@ -28,13 +28,17 @@ module romspath_hip
 
   ! rp_nc_meta (include/rp_ncout.h): global attributes of the trajectory file
   integer(c_int32_t), parameter, public :: RP_NC_WATERDEPTH = 1, RP_NC_ZETA = 2, RP_NC_BATH = 4, &
-                                           RP_NC_SALTTEMP = 8, RP_NC_BOTTOM = 16
+                                           RP_NC_SALTTEMP = 8, RP_NC_BOTTOM = 16, &
+                                           RP_NC_COLLISIONS = 32
   type, bind(C), public :: rp_nc_meta
     type(c_ptr) :: time_units, run_name
     integer(c_int32_t) :: idt
     real(c_float) :: days
     integer(c_int32_t) :: behavior, open_boundary, vturb_on, hturb_on, seed
     real(c_double) :: deltat, serr, smth, sub, aksback
+    real(c_double) :: swimfast, temp_offset
+    integer(c_int32_t) :: salt_temp_mean
+    type(c_ptr) :: executable, output_loc, administrator, institution
   end type rp_nc_meta
 
   ! rp_grid: GRIDDATA after InitGrid.  C's scl[5][2] is Fortran scl(2,5):

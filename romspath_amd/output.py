@@ -15,14 +15,27 @@ import numpy as np
 
 from . import abi
 
-FLAGS = {"WaterDepth": 1, "zeta": 2, "h": 4, "salttemp": 8, "bottom": 16}
+FLAGS = {"WaterDepth": 1, "zeta": 2, "h": 4, "salttemp": 8, "bottom": 16, "collisions": 32}
 
 
-def meta_of(prm, time_units="seconds since model start", run_name=None, behavior=0):
-    """Global attributes from the namelist values (createNetCDF, ROMSPath.f90:1650-1760)."""
+def meta_of(prm, time_units="seconds since model start", run_name=None, behavior=0,
+            executable=None, output_loc=None, administrator=None, institution=None,
+            salt_temp_mean=None):
+    """Global attributes from the namelist values (createNetCDF,
+    ROMSPath.f90:1676-1836).  salt_temp_mean defaults to prm.SaltTempMean;
+    the $output strings (ExeDir, OutDir, RunBy, Institution) are written
+    empty when not given."""
     m = abi.RpNcMeta()
+    enc = lambda s: s.encode() if s else None
     m.time_units = time_units.encode()
-    m.run_name = run_name.encode() if run_name else None
+    m.run_name = enc(run_name)
+    m.executable, m.output_loc = enc(executable), enc(output_loc)
+    m.administrator, m.institution = enc(administrator), enc(institution)
+    if salt_temp_mean is None:
+        salt_temp_mean = getattr(prm, "SaltTempMean", False)
+    m.salt_temp_mean = int(bool(salt_temp_mean))
+    m.swimfast = float(getattr(prm, "swimfast", 0.0))
+    m.temp_offset = float(getattr(prm, "TempOffset", 0.0))
     m.idt = int(prm.idt)
     m.days = float(prm.days)
     m.behavior = int(behavior)

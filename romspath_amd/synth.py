@@ -308,7 +308,10 @@ class RunParams:
     nsb: int = 0
     vertdist: float = 0.0
     OpenOceanBoundary: bool = True   # behavparam
-    seed: int = 1
+    swimfast: float = 0.0            # behavparam (written to the file only)
+    seed: int = 1                    # other
+    SaltTempMean: bool = False       # other: tracer means between outputs
+    TempOffset: float = 0.0          # other: added to the temperature records
 
     @property
     def n_internal(self):

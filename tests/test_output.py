@@ -20,14 +20,19 @@ def _cols(rng, n, ngrid):
     return p, rng.uniform(-180, 180, n), rng.uniform(-90, 90, n)
 
 
-@pytest.mark.parametrize("ngrid,extra", [(1, ()), (2, ("WaterDepth", "zeta", "h", "salttemp", "bottom"))])
+@pytest.mark.parametrize("ngrid,extra", [(1, ()),
+                                         (2, ("WaterDepth", "zeta", "h", "salttemp", "bottom",
+                                              "collisions"))])
 def test_records_read_back(tmp_path, ngrid, extra):
     rng = np.random.default_rng(ngrid)
     n = 1001
-    prm = RunParams(idt=120, days=2.0, VTurbOn=True, deltat=12.0, seed=5)
+    prm = RunParams(idt=120, days=2.0, VTurbOn=True, deltat=12.0, seed=5, swimfast=0.0005,
+                    SaltTempMean=ngrid > 1, TempOffset=1.5)
     dob = rng.uniform(0, 86400, n)
     path = tmp_path / "traj.nc"
-    nc = TrajectoryFile(path, dob, ngrid, meta_of(prm, "seconds since 2020-01-01"), extra)
+    meta = meta_of(prm, "seconds since 2020-01-01", run_name="gyre", executable="/opt/rp",
+                   institution="Rutgers")
+    nc = TrajectoryFile(path, dob, ngrid, meta, extra)
     recs = []
     for r in range(4):
         p, lon, lat = _cols(rng, n, ngrid)
@@ -41,9 +46,10 @@ def test_records_read_back(tmp_path, ngrid, extra):
         assert f.dimensions == {"numpar": n, "time": None}
         names = ["model_time", "dob", "age", "lon", "lat", "zp", "status"]
         names += ["GID"] if ngrid > 1 else []
+        names += ["hitBottom", "hitLand"] if "collisions" in extra else []
         names += ["salinity", "temperature"] if "salttemp" in extra else []
         names += ["HOB", "bustr", "bvstr"] if "bottom" in extra else []
-        names += [e for e in extra if e not in ("salttemp", "bottom")]
+        names += [e for e in extra if e not in ("salttemp", "bottom", "collisions")]
         assert list(f.variables) == names
         v = f.variables
         assert v["model_time"].dimensions == ("time",)
@@ -70,10 +76,30 @@ def test_records_read_back(tmp_path, ngrid, extra):
                 assert np.array_equal(v["temperature"][r], p["temp"].astype(np.float32))
                 for k in ("hob", "bustr", "bvstr"):
                     assert np.array_equal(v["HOB" if k == "hob" else k][r], p[k].astype(np.float32))
+            if "collisions" in extra:
+                # the reference's counters are never incremented
+                assert not v["hitBottom"][r].any() and not v["hitLand"][r].any()
+        if "collisions" in extra:
+            assert v["hitBottom"].long_name == b"# of times Particle Collided with Bottom"
+            assert v["hitLand"].units == b"Number of Collisions"
         a = f._attributes
+        # createNetCDF's global attributes in its order (ROMSPath.f90:1681-1834)
+        assert list(a)[:21] == [
+            "title", "author", "adv_time_step", "days_run", "Behavior", "OpenOceanBoundary",
+            "swimfast", "Vertical_Turb", "seed", "turb_time_step", "spline_error_cutoff",
+            "spline_smooth_param", "subset_scale", "AKs_bakcground", "Horiz_Turb", "run_name",
+            "executable", "output_loc", "administrator", "institution", "Start_datetime"]
+        assert list(a)[21:23] == ["SaltTempMean", "TempOffset"]
+        assert a["author"] == b"Elias Hunter (hunter@marine.rutgers.edu)"
         assert a["adv_time_step"] == 120 and a["Vertical_Turb"] == b"True"
         assert a["seed"] == 5 and a["turb_time_step"] == 12.0
         assert a["OpenOceanBoundary"] == b"TRUE" and a["Horiz_Turb"] == b"False"
+        assert a["swimfast"] == 0.0005 and a["TempOffset"] == 1.5
+        assert a["SaltTempMean"] == (b"TRUE" if ngrid > 1 else b"FALSE")
+        assert a["run_name"] == b"gyre" and a["executable"] == b"/opt/rp"
+        assert a["institution"] == b"Rutgers" and a["output_loc"] == b""
+        import re
+        assert re.fullmatch(rb"\d{4}/\d\d/\d\d \d\d:\d\d:\d\d", a["Start_datetime"])
         if "bottom" in extra:
             assert v["HOB"].units == b"m" and v["bustr"].units == b"newton meter-2"
             assert v["bvstr"].long_name == b"V bottom stress at minimum HOB"
