@@ -350,17 +350,23 @@ def test_read_flags_constants_and_offsets(tmp_path):
     hf.close()
 
 
-def nested_history_set(tmp_path, cfg, nrec=6):
+def nested_history_set(tmp_path, cfg, nrec=6, recs=None, units=None, t0_days=None):
     """One history series and one Stokes series (-DSTOKES) per nest of cfg,
-    records 1 .. nrec in file 0001; returns (prefixes, stokes prefixes)."""
-    recs = RecordSource(cfg)
+    records 1 .. nrec in file 0001; returns (prefixes, stokes prefixes).
+    recs: the record source (default cfg's models); t0_days: write the time
+    variable in days since the epoch of `units`, starting at t0_days."""
+    recs = recs or RecordSource(cfg)
     prefixes, sprefixes = [], []
     for gi, g in enumerate(cfg.grids):
         d = tmp_path / f"g{gi}"
         d.mkdir()
         ids = range(1, nrec + 1)
         rows = [recs.get(gi + 1, r) for r in ids]
-        write_history(d / "his_0001.nc", g, rows, [(r - 1) * cfg.params.dt for r in ids])
+        if t0_days is None:
+            write_history(d / "his_0001.nc", g, rows, [(r - 1) * cfg.params.dt for r in ids])
+        else:
+            write_history(d / "his_0001.nc", g, rows,
+                          [t0_days + (r - 1) * cfg.params.dt / 86400.0 for r in ids], units=units)
         prefixes.append(str(d / "his_"))
         f = netcdf_file(str(d / "stk_0001.nc"), "w", version=2)
         f.createDimension("ocean_time", None)
