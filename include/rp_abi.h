@@ -311,6 +311,19 @@ int rp_kernel_timing(void *ctx, double *avg_ms, int64_t *launches);
  * grid hand-off).  No reference counterpart. */
 int rp_phase_timing(void *ctx, double *advect_ms, double *walk_ms);
 
+/* The eigenbasis tables of the FP32-field vertical walk for n interior
+ * unknowns of the resampled AKs column (n = int((N+1)*sub) - 2, even, 64..128):
+ * the generalized eigenvectors of chol1d's matrix pencil at unit spacing
+ * (pentadiag(1,-4,6,-4,1), tridiag(1,4,1); pppack.f90:1509-1512), split into
+ * n/2 symmetric (par 0) and n/2 antisymmetric (par 1) ones.
+ * basis[par][k][j] (2 x 64 x 64 floats): component j of eigenvector k over
+ * sqrt(2) (zero past n/2); lam[par][k] (2 x 64 doubles): its eigenvalue,
+ * ascending (1 past n/2).  Host only, no context.  Returns RP_E_ARG for n
+ * outside the range.  No reference counterpart: it replaces the pentadiagonal
+ * eliminations of the reference's secant search (pppack.f90:5236-5298) with
+ * a change of basis computed once. */
+int rp_modal_basis(int32_t n, float *basis, double *lam);
+
 const char *rp_last_error(void *ctx);
 int rp_destroy(void *ctx);
 

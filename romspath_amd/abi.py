@@ -21,6 +21,7 @@ ERRORS = {-1: "RP_E_ARG", -2: "RP_E_STATE", -3: "RP_E_HIP", -4: "RP_E_ALLOC",
           -5: "RP_E_RANGE", -6: "RP_E_NODEV", -7: "RP_E_IO"}
 
 PD = C.POINTER(C.c_double)
+PF = C.POINTER(C.c_float)
 PI32 = C.POINTER(C.c_int32)
 PU8 = C.POINTER(C.c_uint8)
 PI64 = C.POINTER(C.c_int64)
@@ -29,6 +30,7 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_step_noalloc",
            "rp_get_particles", "rp_set_deviates", "rp_sample_deviates", "rp_active_steps",
            "rp_secant_capped", "rp_synchronize", "rp_kernel_timing", "rp_phase_timing",
+           "rp_modal_basis",
            "rp_set_tuning", "rp_set_scenarios", "rp_step_scenario", "rp_step_batch",
            "rp_set_perturbation", "rp_adjoint_record", "rp_adjoint",
            "rp_set_lonlat", "rp_sample_lonlat", "rp_set_grid_angle", "rp_locate_particles",
@@ -106,6 +108,7 @@ def lib():
         L.rp_synchronize.argtypes = [vp]
         L.rp_kernel_timing.argtypes = [vp, PD, PI64]
         L.rp_phase_timing.argtypes = [vp, PD, PD]
+        L.rp_modal_basis.argtypes = [C.c_int32, PF, PD]
         L.rp_set_tuning.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.rp_set_scenarios.argtypes = [vp, C.c_int32, PI64]
         L.rp_step_scenario.argtypes = [vp, C.c_int32, PD, PD, C.c_int32, C.c_int32, C.c_int32,

@@ -18,7 +18,7 @@ SOURCES = ["rp_step.hip", "rp_adjoint.hip", "rp_sort.hip", "rp_locate.hip", "rp_
 HEADERS = ["rp_device.h", "rp_kernels.h", "rp_dual.h", "rp_step_dev.h"]
 # host-only C++ (no device code): the trajectory file writer and the
 # netCDF classic reader of the hydrodynamic input
-HOST_SOURCES = ["rp_ncout.cpp", "rp_ncin.cpp"]
+HOST_SOURCES = ["rp_ncout.cpp", "rp_ncin.cpp", "rp_modal_host.cpp"]
 CXX = os.environ.get("CXX", "g++")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

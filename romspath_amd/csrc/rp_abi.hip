@@ -67,6 +67,7 @@ struct GridDev {
     int8_t *mrho = nullptr, *mu = nullptr, *mv = nullptr;
     void *zeta = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *aks = nullptr;
     float* cur[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // FP32 form: fields at ix(2)
+    ModalTab* modal = nullptr;                // FP32 form: eigenbasis tables of this grid's n
     double *lon = nullptr, *lat = nullptr;   // lon_rho / lat_rho (x_rho / y_rho)
     double* angle = nullptr;                  // grid angle (spherical grids), may stay null
     void *salt = nullptr, *temp = nullptr;    // SaltTempOn fields, slot-interleaved (L, M, N)
@@ -148,7 +149,11 @@ struct Ctx {
     unsigned long long* active = nullptr;
     unsigned long long* work = nullptr;
     unsigned* capped = nullptr;
-    int vwalk_blocks_per_cu[3][2] = {{0, 0}, {0, 0}, {0, 0}};   // [form][nested], 0 = not queried
+    // [form: FP32 band sweeps, FP64, FP32 eigenbasis][nested], 0 = not queried
+    int vwalk_blocks_per_cu[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    // RP_VWALK_FORM=band at rp_create: the FP32 form keeps its band-sweep
+    // vertical walk even where the eigenbasis one applies
+    bool vwalk_band = false;
     // per step: events before the step, between its two phases, after it;
     // a captured batch of steps has one entry covering all of its steps
     struct StepEvents {
@@ -367,6 +372,18 @@ int upload_field(Ctx* c, int gi, int f, void* dst, const double* src, const doub
 
 int np_of(const Ctx* c, int N) { return (int)((double)(N + 1) * c->cfg.sub); }
 
+// The eigenbasis tables (rp_modal.h, rp_modal_basis) of n interior
+// unknowns; the form applies to even n from kMh to 2 kMh (the kernel reads
+// rows j and n-1-j, j < kMh, without a branch).
+bool modal_applies(int n) { return n >= kMh && n % 2 == 0 && n / 2 <= kMh; }
+int build_modal_tab(int n, ModalTab& T)
+{
+    std::memset(&T, 0, sizeof T);
+    T.n = n;
+    T.half = n / 2;
+    return rp_modal_basis(n, &T.basis[0][0][0], &T.lam[0][0]);
+}
+
 void free_grid(GridDev& g)
 {
     dfree(g.s_rho); dfree(g.cs_r); dfree(g.s_w); dfree(g.cs_w);
@@ -374,6 +391,7 @@ void free_grid(GridDev& g)
     dfree(g.mrho); dfree(g.mu); dfree(g.mv);
     dfree(g.zeta); dfree(g.u); dfree(g.v); dfree(g.w); dfree(g.aks);
     for (auto& p : g.cur) dfree(p);
+    dfree(g.modal);
     dfree(g.lon); dfree(g.lat); dfree(g.angle);
     dfree(g.salt); dfree(g.temp);
     for (auto& t : g.tracer_set) t = false;
@@ -523,8 +541,16 @@ struct StepPlan {
     bool vt = false;
     KFn k[3] = {nullptr, nullptr, nullptr};   // step / advect / vertical walk
     bool cur = false;                          // FP32 form: build the current fields first
+    bool modal = false;                        // vertical walk: vwalk_modal_kernel
     CurJobs jobs;
 };
+
+// dynamic LDS of the vertical walk: the band-sweep FP32 form keeps qty rows
+// there; the eigenbasis form's tables are static LDS
+size_t vwalk_lds(const StepPlan& P)
+{
+    return P.cur && !P.modal ? (size_t)P.block * kLdsQ * 4 : 0;
+}
 
 // points of field f (zeta, u, v, w, aks) of a grid
 int64_t field_points(const GridDev& g, int f)
@@ -568,6 +594,7 @@ void fill_tab(const Ctx* c, DGrid tab[kMaxGrid])
         d.zeta = g.zeta; d.u = g.u; d.v = g.v; d.w = g.w; d.aks = g.aks;
         d.czeta = g.cur[0]; d.cu = g.cur[1]; d.cv = g.cur[2]; d.cw = g.cur[3]; d.caks = g.cur[4];
         d.nwc = cur_column(g);
+        d.np = np_of(c, g.N);
         d.kh = c->cfg.kh[i];
         for (int a = 0; a < kMaxGrid; a++)
             for (int k = 0; k < 2; k++) {
@@ -631,6 +658,8 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     A.open_boundary = c->cfg.open_boundary != 0;
     A.ngrid = c->cfg.ngrid;
     A.np = np_of(c, maxN);
+    A.np_mixed = 0;
+    for (int i = 0; i < c->cfg.ngrid; i++) A.np_mixed |= np_of(c, c->grids[i].N) != A.np;
     A.loop = vt ? c->cfg.idt / (int)c->cfg.deltat : 0;
     A.q0 = q0;
     A.n = cnt;
@@ -671,15 +700,29 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     } else {
         P.k[0] = step_kernel<double, true>; P.k[1] = advect_kernel<double, true>; P.k[2] = vwalk_kernel<double, true>;
     }
+    // the FP32 form's vertical walk in the eigenbasis when every grid has
+    // the same resampling size (a wave's transforms use one table)
+    P.cur = f32;
+    A.modal = nullptr;
+    P.modal = false;
+    if (f32 && vt && !c->vwalk_band && c->grids[0].modal) {
+        bool same = true;
+        for (int i = 1; i < c->cfg.ngrid; i++)
+            same = same && c->grids[i].modal && c->grids[i].N == c->grids[0].N;
+        if (same) {
+            A.modal = c->grids[0].modal;
+            P.modal = true;
+            P.k[2] = nested ? vwalk_modal_kernel<true> : vwalk_modal_kernel<false>;
+        }
+    }
     P.grid_vt = P.grid_all;
     A.scratch = nullptr;
     if (vt) {
         // persistent grid for the vertical walk: exactly the resident blocks
-        int& bpc = c->vwalk_blocks_per_cu[f32 ? 0 : 1][nested ? 1 : 0];
+        int& bpc = c->vwalk_blocks_per_cu[P.modal ? 2 : (f32 ? 0 : 1)][nested ? 1 : 0];
         if (bpc == 0) {
             int nb = 0;
-            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, P.k[2], block,
-                                                                   f32 ? (size_t)block * kLdsQ * 4 : 0));
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, P.k[2], block, vwalk_lds(P)));
             bpc = std::max(nb, 1);
         }
         P.vt_cap = (int64_t)c->ncu * bpc;
@@ -809,7 +852,7 @@ int enqueue_step(Ctx* c, StepPlan& P, const Ctx::StepEvents* ev)
         HIPCHK(c, hipGetLastError());
         if (ev) HIPCHK(c, hipEventRecord(ev->e[1], c->stream));
         hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, c->stream, A.work);
-        hipLaunchKernelGGL(P.k[2], dim3((unsigned)P.grid_vt), bd, lds, c->stream, c->dgrids, A);
+        hipLaunchKernelGGL(P.k[2], dim3((unsigned)P.grid_vt), bd, vwalk_lds(P), c->stream, c->dgrids, A);
     }
     HIPCHK(c, hipGetLastError());
     if (ev) HIPCHK(c, hipEventRecord(ev->e[2], c->stream));
@@ -1002,6 +1045,7 @@ int rp_create(void** out, const rp_config* cfg)
     Ctx* c = new Ctx();
     c->cfg = *cfg;
     if (const char* k = std::getenv("RP_FAIL_ALLOC_AFTER")) c->alloc_budget = std::max<int64_t>(0, std::atoll(k));
+    if (const char* f = std::getenv("RP_VWALK_FORM")) c->vwalk_band = std::strcmp(f, "band") == 0;
     std::memset(c->tab_host, 0, sizeof(c->tab_host));
     if (cfg->precision != RP_PREC_FP32_FIELDS && cfg->precision != RP_PREC_FP64) {
         delete c;
@@ -1107,8 +1151,14 @@ int rp_set_grid(void* ctx, int32_t ng, const rp_grid* gh)
             g.off[i][k] = gh->off[i][k];
         }
     for (int s = 0; s < 3; s++) g.slot_set[s] = false;
+    dfree(g.modal);
+    if (c->fbytes == 4 && c->cfg.vturb_on && modal_applies(np_of(c, g.N) - 2)) {
+        ModalTab T;
+        if (build_modal_tab(np_of(c, g.N) - 2, T) != RP_OK) return fail(c, RP_E_ARG, "eigenbasis tables");
+        if ((rc = dalloc(c, &g.modal, 1)) || (rc = put(c, g.modal, &T, sizeof T))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     g.set = true;
-    // size the VTurb scratch for the largest grid
     return RP_OK;
 }
 

@@ -109,7 +109,7 @@ __device__ __forceinline__ void adj_particle(const DGrid* __restrict__ grids, co
         const RhoCellT<D3> c0 = rho_cell(g, X, Y);
         const SurfaceT<D3> s0 = surface_at<F>(g, A, c0);
         const ScrT<D3> S{(D3*)A.scratch + tid, T};
-        TurbV = vturb<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
+        TurbV = vturb<F>(g, A, (NESTED && A.np_mixed) ? g.np : A.np, c0, s0, Zpar, dv, S, A.capped);
     }
     const StepOut<D3> o = finish_math<F, NESTED>(grids, A, gid, X, Y, Z, newX, newY, AdvectZ, TurbV);
     const double lam[3] = {J.lx[q], J.ly[q], J.lz[q]};

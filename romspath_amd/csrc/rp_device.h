@@ -48,6 +48,7 @@ struct DGrid {
     // that four levels of a corner arrive in one aligned 16-byte load)
     const float *czeta, *cu, *cv, *cw, *caks;
     int nwc;
+    int np;                              // VTurb resampling points, int(s_w(ng) * sub)
     double kh;                           // ConstantHTurb(ng)
     double scl[kMaxGrid][2], off[kMaxGrid][2];
 };

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "rp_device.h"
+#include "rp_modal.h"
 
 namespace rp {
 
@@ -29,7 +30,8 @@ struct StepArgs {
     int vturb_on, hturb_on, scheme, nsb;
     double vertdist;
     int open_boundary, ngrid;
-    int np, loop;
+    int np, loop;                // np: int(s_w * sub) of the grid with the most levels
+    int np_mixed;                // the grids' np differ: each lane takes its grid's np
     int64_t q0, n;               // stored particles q0 .. q0+n-1 take this launch
     int64_t nmid;                // row stride of mid (>= n); row r holds particle q0 + r
     int64_t step_index;
@@ -56,6 +58,9 @@ struct StepArgs {
     // r < *act_n; null: every particle q0 .. q0+n-1, inactive ones skipped
     const uint32_t* act;
     const unsigned long long* act_n;
+    // eigenbasis tables of the FP32 form's vertical walk (every grid has the
+    // same resampling size), or null
+    const ModalTab* modal;
 };
 
 // NESTED=false: one grid, all lanes read grid 0 (uniform, scalar loads).
@@ -67,6 +72,11 @@ template <typename F, bool NESTED>
 __global__ void advect_kernel(const DGrid* __restrict__ grids, StepArgs A);
 template <typename F, bool NESTED>
 __global__ void vwalk_kernel(const DGrid* __restrict__ grids, StepArgs A);
+// the FP32-field vertical walk with the secant search in the eigenbasis of
+// chol1d's pencil (A.modal set): static LDS for the two basis tables, no
+// dynamic LDS
+template <bool NESTED>
+__global__ void vwalk_modal_kernel(const DGrid* __restrict__ grids, StepArgs A);
 
 // The active set of a launch (the particles that pass the skip tests of
 // ROMSPath.f90:672-690), in particle order: compact_count writes each

@@ -93,6 +93,235 @@ __device__ __forceinline__ void vwalk_body(const DGrid* __restrict__ grids, cons
     count_active(A, nact);
 }
 
+// ---------------------------------------------------------------------------
+// The FP32-field vertical walk with smooth's secant search in the eigenbasis
+// of chol1d's matrix pencil (rp_modal.h).  Same persistent wave-granular claiming as vwalk_body, but
+// each 64-particle claim runs in phases so that the wave is converged where
+// the matrix instructions run:
+//   1. per lane: hand-over rows, the column walk (y and qty work rows);
+//   2. wave: c = S qty (lanes without a column contribute zeros);
+//   3. per lane: the secant search on the modes in registers, then the
+//      modes of the accepted solution;
+//   4. wave: u = S vh into the u work rows;
+//   5. per lane: the random-displacement sub-steps and the rest of the step.
+// ---------------------------------------------------------------------------
+
+// a double in two FP32 work rows (its bits, unchanged)
+__device__ __forceinline__ void st_dbl(const Scr32& S, int row, double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    S(row) = __builtin_bit_cast(float, (uint32_t)u);
+    S(row + 1) = __builtin_bit_cast(float, (uint32_t)(u >> 32));
+}
+__device__ __forceinline__ double ld_dbl(const Scr32& S, int row)
+{
+    const uint32_t lo = __builtin_bit_cast(uint32_t, (float)S(row));
+    const uint32_t hi = __builtin_bit_cast(uint32_t, (float)S(row + 1));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// The accepted solution in the eigenbasis: mode k is d_k R scale with
+// R = 1/(lambda_k + t), or, at p = 1, d_k scale.
+struct ModalSol {
+    double t, scale;
+    bool p1;
+};
+
+// smooth's search for p (pppack.f90:5236-5298) on the modal sums: sfq and
+// utru as in rp_modal.h.  Returns p and the accepted solution's form.
+__device__ __forceinline__ double modal_search(const ModalTab* __restrict__ MT, float (&co)[kMh],
+                                               float (&ce)[kMh], double h, const StepArgs& A,
+                                               ModalSol& sol)
+{
+#pragma clang fp contract(fast)
+    const double serr = A.serr, s = A.smth;
+    const double r2 = (serr / h) * (serr / h);
+    double p = 0.0, t = 0.0;
+    ModalSums Sm;
+    if (s <= 0.0) {
+        p = 1.0;
+    } else {
+        // one loop body for every evaluation (the first at p = 0), so that
+        // the mode loop is compiled once and never with t known
+        double alpha = 6.0 * r2, q = 0.0, prevq = 0.0, prevsf = 0.0, oosf = 0.0, ooss = 0.0;
+        int it = -1;
+        for (;;) {
+            modal_sums(MT, co, ce, t, Sm);
+            const double g = Sm.g, gp = -Sm.gp2;
+            if (it < 0) {
+                const double sfq = 36.0 * r2 * g / (alpha * alpha);
+                if (!(s < sfq)) break;                      // p = 0
+                const double utru = -h * gp / (2.0 * alpha * alpha);
+                ooss = 1.0 / sqrt(s);
+                oosf = 1.0 / sqrt(sfq);
+                q = -(oosf - ooss) * sfq / (6.0 * utru * oosf);
+                prevq = 0.0;
+                prevsf = oosf;
+                it = 0;
+            } else {
+                const double sfq = 36.0 * r2 * (g + t * gp) / (alpha * alpha * ((1.0 + q) * (1.0 + q)));
+                if (fabs(sfq - s) <= 0.01 * s) {
+                    p = q / (1.0 + q);
+                    break;
+                }
+                if (++it >= kSecantCap) {
+                    atomicAdd(A.capped, 1u);
+                    p = q / (1.0 + q);
+                    break;
+                }
+                oosf = 1.0 / sqrt(sfq);
+                const double change = (q - prevq) / (oosf - prevsf) * (oosf - ooss);
+                prevq = q;
+                q = q - change;
+                prevsf = oosf;
+            }
+            const double pq = q / (1.0 + q);
+            alpha = 6.0 * (1.0 - pq) * r2;
+            t = pq * h / alpha;
+        }
+    }
+    // the last evaluation was at the accepted p
+    sol.t = t;
+    sol.p1 = p >= 1.0;
+    sol.scale = sol.p1 ? 1.0 / h : 1.0 / (6.0 * (1.0 - p) * r2);
+    return p;
+}
+
+template <bool NESTED>
+__device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids, const StepArgs& A)
+{
+    __shared__ float tab[2 * kMh * kMStride];
+    const ModalTab* __restrict__ MT = A.modal;
+    for (int i = threadIdx.x; i < 2 * kMh * kMh; i += blockDim.x) {
+        const int pm = i / kMh, j = i % kMh;      // pm = par * kMh + m
+        tab[pm * kMStride + j] = (&MT->basis[0][0][0])[i];
+    }
+    __syncthreads();
+    const float* tab_e = tab + kMh * kMStride;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
+    const int n0 = __builtin_amdgcn_readfirstlane(MT->n);
+    unsigned long long nact = 0;
+    const int64_t ne = launch_count(A);
+    for (;;) {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(A.work, 64ull);
+        // lane 0's claim, as a wave-uniform (scalar) value
+        b = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+        if (b >= (unsigned long long)ne) break;
+        // The hundreds of work-row offsets below are invariant across the
+        // claims; hoisted out of this loop they overflow the scalar
+        // registers.  Opaque copies of n and the row stride keep them
+        // computed where they are used.
+        int n = n0;
+        uint32_t Tu = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)T);
+        asm volatile("" : "+s"(n), "+s"(Tu));
+        const int half = n / 2;
+        const FastLayout Lo(n + 2);
+        const Scr32 S((float*)A.scratch, A.scratch_bytes, (uint32_t)tid, Tu);
+        const int64_t r = (int64_t)b + lane;
+        // 1. the column.  Only what the next phases use stays in registers
+        // (the 128 mode values need the room): h; z1, 1/h and the surface
+        // go to spare work rows, and the particle's state is read again for
+        // the sub-steps.
+        bool go = false;
+        double h = 0.0;
+        if (r < ne) {
+            const int64_t q = launch_particle(A, ne, r);
+            go = !skips(A, q);
+            if (go) {
+                const int gid = NESTED ? (int)A.gid[q] : 1;
+                const DGrid& g = grids[NESTED ? gid - 1 : 0];
+                const RhoCell c0 = rho_cell(g, A.x[q], A.y[q]);
+                const Surface s0 = surface_at<Cur>(g, A, c0);
+                const VtCol col = vt_column_fast<Cur, false>(g, A, n + 2, c0, s0, S);
+                st_dbl(S, Lo.w + 0, col.z1);
+                st_dbl(S, Lo.w + 2, col.rh);
+                st_dbl(S, Lo.w + 4, s0.tdepth);
+                st_dbl(S, Lo.w + 6, s0.zeta);
+                h = col.h;
+            }
+        }
+        // 2. modes of qty: odd from q+ = q_j + q_{n-1-j}, even from q- (the
+        // rows are read once per parity, which keeps one folded set live)
+        // Every lane reads its own column (n >= kMh: rows j and n-1-j lie in
+        // 0..n); rows past n/2 read qty row n, which holds 0, and lanes
+        // without a column are zeroed after the load, so the reads need no
+        // branch and go out together.
+        float co[kMh], ce[kMh];
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+            // a fence: the second parity's reads would otherwise be hoisted
+            // into the first parity's transform, with 128 more live values
+            asm volatile("" ::: "memory");
+            float qp[kMh], qm[kMh];
+#pragma unroll
+            for (int j = 0; j < kMh; j++) {
+                qp[j] = S(Lo.q + (j < half ? j : n));
+                qm[j] = S(Lo.q + (j < half ? n - 1 - j : n));
+            }
+            // the folded rows: q+ for the odd modes, q- for the even ones
+            // (the rows are read once per parity, so one folded set is live)
+#pragma unroll
+            for (int j = 0; j < kMh; j++) qp[j] = go ? (par == 0 ? qp[j] + qm[j] : qp[j] - qm[j]) : 0.0f;
+            if (par == 0) modal_gemm<false>(tab, [&](int j) { return qp[j]; }, co);
+            else modal_gemm<false>(tab_e, [&](int j) { return qp[j]; }, ce);
+        }
+        asm volatile("" ::: "memory");
+        // 3. the search and the accepted solution's modes
+        double p = 0.0;
+        ModalSol sol{0.0, 0.0, true};
+        if (go) p = modal_search(MT, co, ce, h, A, sol);
+        // 4. u = S vh: rows j and n-1-j from the odd part +/- the even part
+        // (plain stores: the sub-steps read them soon).  Every lane writes
+        // its own column; the pairs past n/2 go to qty rows, which are no
+        // longer read.
+        {
+            // scalar reads of the constants (as in modal_sums)
+            const __attribute__((address_space(4))) ModalTab* T =
+                (const __attribute__((address_space(4))) ModalTab*)MT;
+            asm volatile("" : "+s"(T));
+            // mode m of the solution, formed where the transform takes it;
+            // lanes without a column give zeros
+            auto vh = [&](int par, float c, int m) -> float {
+                const double R = sol.p1 ? 1.0 : modal_rcp(T->lam[par][m] + sol.t);
+                const double v = (double)c * R * sol.scale;
+                return go ? (float)v : 0.0f;
+            };
+            float uo[kMh], ue[kMh];
+            modal_gemm<true>(tab, [&](int m) { return vh(0, co[m], m); }, uo);
+            modal_gemm<true>(tab_e, [&](int m) { return vh(1, ce[m], m); }, ue);
+#pragma unroll
+            for (int j = 0; j < kMh; j++) {
+                const bool in = j < half;
+                S(in ? Lo.u + 1 + j : Lo.q + (j - half)) = uo[j] + ue[j];
+                S(in ? Lo.u + (n - j) : Lo.q + (j - half)) = uo[j] - ue[j];
+            }
+            S(Lo.u + 0) = 0.0f;
+            S(Lo.u + n + 1) = 0.0f;
+        }
+        // 5. sub-steps and the rest of the step
+        if (go) {
+            const int64_t q = launch_particle(A, ne, r);
+            VtCol col;
+            col.z1 = ld_dbl(S, Lo.w + 0);
+            col.rh = ld_dbl(S, Lo.w + 2);
+            col.h = h;
+            Surface s0;
+            s0.tdepth = ld_dbl(S, Lo.w + 4);
+            s0.zeta = ld_dbl(S, Lo.w + 6);
+            DevSrc dv = dev_of(A, q);
+            const double TurbV = vt_substeps_fast(A, Lo, S, col, s0, A.mid[3 * A.nmid + r], p, dv);
+            finish<Cur, NESTED>(grids, A, q, NESTED ? (int)A.gid[q] : 1, A.z[q], A.mid[r],
+                                A.mid[A.nmid + r], A.mid[2 * A.nmid + r], TurbV);
+            nact++;
+        }
+    }
+    count_active(A, nact);
+}
+
 // Occupancy targets (waves per SIMD) of the three step kernels; 0 leaves the
 // choice to the compiler.
 #ifndef RP_WAVES_ALL
@@ -147,6 +376,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(vwalk_
 {
     vwalk_body<F, NESTED>(grids, A);
 }
+
+// Eigenbasis form: the mode registers (128 FP32 per lane) and the
+// transforms' tiles need 2 waves per SIMD.
+template <bool NESTED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    vwalk_modal_kernel(const DGrid* __restrict__ grids, StepArgs A)
+{
+    vwalk_modal_body<NESTED>(grids, A);
+}
+template __global__ void vwalk_modal_kernel<false>(const DGrid*, StepArgs);
+template __global__ void vwalk_modal_kernel<true>(const DGrid*, StepArgs);
 
 #define RP_INST(F, N)                                                              \
     template __global__ void step_kernel<F, N>(const DGrid*, StepArgs);            \

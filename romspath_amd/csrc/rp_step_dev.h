@@ -617,12 +617,11 @@ __device__ __forceinline__ R ppvalu_dev(const VtLayout& Lo, const ScrT<R>& S, Pi
 }
 
 template <typename F, class R>
-__device__ __forceinline__ R vturb(const DGrid& g, const StepArgs& A, const RhoCellT<R>& c,
+__device__ __forceinline__ R vturb(const DGrid& g, const StepArgs& A, int np, const RhoCellT<R>& c,
                                    const SurfaceT<R>& sf, R Zpar, DevSrc& dv, const ScrT<R>& S,
                                    unsigned* capped)
 {
     const int nw = g.N + 1;
-    const int np = A.np;
     const VtLayout Lo(np, nw);
     const R tdepth = sf.tdepth, zeta = sf.zeta;
     const R h = -1.0 * tdepth;
@@ -1186,14 +1185,20 @@ __device__ __forceinline__ void column_level(const DGrid& g, const StepArgs& A, 
     }
 }
 
-template <typename F>
-__device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, const RhoCell& c,
-                                             const Surface& sf, double Zpar, DevSrc& dv,
-                                             const Scr32& S, unsigned* capped)
+// The AKs column at the particle resampled on np uniform points: y rows and
+// qty rows (qty row n = np-2 holds 0) into the work rows.  NT_Y: the y rows
+// are stored non-temporal (the band sweeps re-read qty eight times and keep
+// it cached; the sine-basis kernel reads every row once, soon).
+struct VtCol {
+    double z1, h, rh;
+};
+
+template <typename F, bool NT_Y = true>
+__device__ __forceinline__ VtCol vt_column_fast(const DGrid& g, const StepArgs& A, int np,
+                                                const RhoCell& c, const Surface& sf, const Scr32& S)
 {
 #pragma clang fp contract(fast)
     const int nw = g.N + 1;
-    const int np = A.np;
     const FastLayout Lo(np);
     const double tdepth = sf.tdepth, zeta = sf.zeta;
     const double hh = -1.0 * tdepth;
@@ -1204,15 +1209,26 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     // point fz uses the segment of the largest level j<=nw-1 with z(j)<=fz)
     double z1, k1, zn, kn;
     constexpr int kLv = 4;
-    // levels kb..kb+kLv-1 of the column (those in 2..nw are used)
-    auto batch = [&](int kb, double zk[kLv], double kk[kLv]) {
+    // the four corners' AKs of levels kb..kb+kLv-1 (kb-1 is a multiple of
+    // four: one aligned 16-byte load per corner, within the padded column)
+    struct Quad {
+        float4 a[4];
+    };
+    auto load_quad = [&](int kb) {
+        Quad Q;
         if constexpr (is_cur<F>()) {
-            // kb-1 is a multiple of four: one aligned 16-byte load per corner
             const int nwc = g.nwc;
-            const float4 a0 = *reinterpret_cast<const float4*>(g.caks + c.o[0] * nwc + (kb - 1));
-            const float4 a1 = *reinterpret_cast<const float4*>(g.caks + c.o[1] * nwc + (kb - 1));
-            const float4 a2 = *reinterpret_cast<const float4*>(g.caks + c.o[2] * nwc + (kb - 1));
-            const float4 a3 = *reinterpret_cast<const float4*>(g.caks + c.o[3] * nwc + (kb - 1));
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                Q.a[q] = *reinterpret_cast<const float4*>(g.caks + c.o[q] * nwc + (kb - 1));
+        }
+        return Q;
+    };
+    // levels kb..kb+kLv-1 of the column (those in 2..nw are used); Q: their
+    // AKs corners, loaded one batch ahead
+    auto batch = [&](int kb, const Quad& Q, double zk[kLv], double kk[kLv]) {
+        if constexpr (is_cur<F>()) {
+            const float4 a0 = Q.a[0], a1 = Q.a[1], a2 = Q.a[2], a3 = Q.a[3];
             const float v0[kLv] = {a0.x, a0.y, a0.z, a0.w}, v1[kLv] = {a1.x, a1.y, a1.z, a1.w};
             const float v2[kLv] = {a2.x, a2.y, a2.z, a2.w}, v3[kLv] = {a3.x, a3.y, a3.z, a3.w};
 #pragma unroll
@@ -1240,11 +1256,15 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     {
         double zlo = z1, klo = k1;
         double ym1 = 0.0, y0 = k1;
-        S.st_nt(Lo.y + 0, (float)k1);
+        if constexpr (NT_Y) S.st_nt(Lo.y + 0, (float)k1);
+        else S(Lo.y + 0) = (float)k1;
         int i = 2;                                   // next point to emit
+        Quad Qn = load_quad(1);
         for (int kb = 1; kb <= nw; kb += kLv) {
+            const Quad Q = Qn;
+            if (kb + kLv <= nw) Qn = load_quad(kb + kLv);
             double zk[kLv], kk[kLv];
-            batch(kb, zk, kk);
+            batch(kb, Q, zk, kk);
 #pragma unroll
             for (int t = 0; t < kLv; t++) {
                 const int k = kb + t;
@@ -1261,7 +1281,8 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
                             if (k < nw && !(fz < zhi)) break;
                             yi = m * fz + b;
                         }
-                        S.st_nt_at(Lo.y + i - 1, (float)yi);
+                        if constexpr (NT_Y) S.st_nt_at(Lo.y + i - 1, (float)yi);
+                        else S.at(Lo.y + i - 1) = (float)yi;
                         // qty row i-3 needs y(i-2): defined for i >= 3 (rows 0..np-3)
                         if (i >= 3) S.at(Lo.q + i - 3) = (float)(((yi - y0) - (y0 - ym1)) * rh);
                         ym1 = y0;
@@ -1273,10 +1294,59 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
             }
         }
     }
+    S(Lo.q + (np - 2)) = 0.0f;   // read as the next row's qty by the last row of a sweep
+    return VtCol{z1, h, rh};
+}
+
+// the random-displacement sub-steps (ver_turb_module.f90:240-293) on the
+// spline of p whose u and y rows are in the work rows
+__device__ __forceinline__ double vt_substeps_fast(const StepArgs& A, const FastLayout& Lo,
+                                                   const Scr32& S, const VtCol& col,
+                                                   const Surface& sf, double Zpar, double p,
+                                                   DevSrc& dv)
+{
+#pragma clang fp contract(fast)
+    const double tdepth = sf.tdepth, zeta = sf.zeta;
+    const double h = col.h, rh = col.rh, z1 = col.z1;
+    const double k2 = 6.0 * (1.0 - p) * (A.serr * A.serr);
+    PieceCache pc;
+    pc.i = -1;
+    const double deltat = A.deltat;
+    double ParZc = Zpar;
+    for (int i = 1; i <= A.loop; i++) {
+        double Kprimec;
+        if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
+        else Kprimec = ppvalu_fast(Lo, S, pc, ParZc, 1, p, k2, h, rh, z1);
+        const double KprimeZc = Kprimec * deltat;
+        const double Z3rdc = ParZc + 0.5 * KprimeZc;
+        double KH3rdc;
+        if (Z3rdc < tdepth || Z3rdc > zeta) {
+            KH3rdc = A.aksback;
+        } else {
+            KH3rdc = ppvalu_fast(Lo, S, pc, Z3rdc, 0, p, k2, h, rh, z1);
+            if (KH3rdc < A.aksback) KH3rdc = A.aksback;
+        }
+        const double DEV = dv.get(1 + i);
+        const double dTz = KprimeZc + DEV * sqrt(2.0 * KH3rdc * deltat);
+        ParZc = ParZc + dTz;
+        if (ParZc <= tdepth) ParZc = tdepth + fabs(ParZc - tdepth);
+        if (ParZc >= zeta) ParZc = zeta - fabs(ParZc - zeta);
+    }
+    return ParZc - Zpar;
+}
+
+template <typename F>
+__device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, int np,
+                                             const RhoCell& c, const Surface& sf, double Zpar,
+                                             DevSrc& dv, const Scr32& S, unsigned* capped)
+{
+#pragma clang fp contract(fast)
+    const FastLayout Lo(np);
+    const VtCol col = vt_column_fast<F>(g, A, np, c, sf, S);
+    const double h = col.h;
     // secant search for p (pppack.f90:5236-5298), sfq from G and G'
     const double serr = A.serr, s = A.smth;
     const int n = np - 2;
-    S(Lo.q + n) = 0.0f;   // read as the next row's qty by the last row of a sweep
     // the column walk wrote every qty row to global memory; the first kLdsQ
     // go to LDS too, where the sweeps read them
     for (int j0 = 0; j0 < kLdsQ; j0 += 5) {
@@ -1328,31 +1398,7 @@ __device__ __forceinline__ double vturb_fast(const DGrid& g, const StepArgs& A, 
     }
     if (stored) back_u(band_of(p, h, serr), Lo, S, tail);
     else solve_u(band_of(p, h, serr), Lo, S);
-    const double k2 = 6.0 * (1.0 - p) * (serr * serr);
-    PieceCache pc;
-    pc.i = -1;
-    const double deltat = A.deltat;
-    double ParZc = Zpar;
-    for (int i = 1; i <= A.loop; i++) {
-        double Kprimec;
-        if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
-        else Kprimec = ppvalu_fast(Lo, S, pc, ParZc, 1, p, k2, h, rh, z1);
-        const double KprimeZc = Kprimec * deltat;
-        const double Z3rdc = ParZc + 0.5 * KprimeZc;
-        double KH3rdc;
-        if (Z3rdc < tdepth || Z3rdc > zeta) {
-            KH3rdc = A.aksback;
-        } else {
-            KH3rdc = ppvalu_fast(Lo, S, pc, Z3rdc, 0, p, k2, h, rh, z1);
-            if (KH3rdc < A.aksback) KH3rdc = A.aksback;
-        }
-        const double DEV = dv.get(1 + i);
-        const double dTz = KprimeZc + DEV * sqrt(2.0 * KH3rdc * deltat);
-        ParZc = ParZc + dTz;
-        if (ParZc <= tdepth) ParZc = tdepth + fabs(ParZc - tdepth);
-        if (ParZc >= zeta) ParZc = zeta - fabs(ParZc - zeta);
-    }
-    return ParZc - Zpar;
+    return vt_substeps_fast(A, Lo, S, col, sf, Zpar, p, dv);
 }
 
 // boundary_module.f90:43-143 with the 3-point inside() of
@@ -1654,12 +1700,17 @@ __device__ __forceinline__ bool step_one(const DGrid* __restrict__ grids, const 
         if (A.vturb_on) {
             const RhoCell c0 = rho_cell(g, X0, Y0);
             const Surface s0 = surface_at<F>(g, A, c0);
+            // np: int(s_w(ng) * sub) of the particle's grid.  It is the
+            // launch's (uniform) value unless the grids' N differ, in which
+            // case a second copy of the walk takes it per lane.
             if constexpr (is_fast<F>() && !FAITHFUL) {
                 const Scr32 S((float*)A.scratch, A.scratch_bytes, (uint32_t)tid, (uint32_t)T);
-                TurbV = vturb_fast<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
+                if (NESTED && A.np_mixed) TurbV = vturb_fast<F>(g, A, g.np, c0, s0, Zpar, dv, S, A.capped);
+                else TurbV = vturb_fast<F>(g, A, A.np, c0, s0, Zpar, dv, S, A.capped);
             } else {
                 const Scr S{A.scratch + tid, T};
-                TurbV = vturb<F>(g, A, c0, s0, Zpar, dv, S, A.capped);
+                if (NESTED && A.np_mixed) TurbV = vturb<F>(g, A, g.np, c0, s0, Zpar, dv, S, A.capped);
+                else TurbV = vturb<F>(g, A, A.np, c0, s0, Zpar, dv, S, A.capped);
             }
         }
         finish<F, NESTED>(grids, A, q, gid, Z0, newX, newY, AdvectZ, TurbV);

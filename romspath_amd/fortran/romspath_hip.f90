@@ -54,7 +54,8 @@ module romspath_hip
 
   public :: rp_create, rp_set_grid, rp_set_hydro_slot, rp_set_particles, rp_step, &
             rp_get_particles, rp_set_deviates, rp_sample_deviates, rp_active_steps, &
-            rp_secant_capped, rp_synchronize, rp_kernel_timing, rp_phase_timing, rp_last_error, &
+            rp_secant_capped, rp_synchronize, rp_kernel_timing, rp_phase_timing, rp_modal_basis, &
+            rp_last_error, &
             rp_destroy, rp_set_tuning, rp_set_scenarios, rp_step_scenario, rp_step_batch, &
             rp_set_perturbation, rp_adjoint_record, rp_adjoint, &
             rp_set_lonlat, rp_sample_lonlat, rp_nc_create, rp_nc_write, rp_nc_close, &
@@ -168,6 +169,13 @@ module romspath_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: avg_ms
       integer(c_int64_t), intent(out) :: launches
+    end function
+
+    integer(c_int) function rp_modal_basis(n, basis, lam) bind(C, name="rp_modal_basis")
+      import :: c_int, c_int32_t, c_float, c_double
+      integer(c_int32_t), value :: n
+      real(c_float), intent(out) :: basis(64, 64, 2)
+      real(c_double), intent(out) :: lam(64, 2)
     end function
 
     integer(c_int) function rp_phase_timing(ctx, advect_ms, walk_ms) bind(C, name="rp_phase_timing")
