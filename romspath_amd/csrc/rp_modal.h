@@ -149,7 +149,9 @@ __device__ __forceinline__ void modal_sums(const ModalTab* __restrict__ T0, floa
     for (int par = 0; par < 2; par++) {
 #pragma unroll
         for (int m = 0; m < kMh; m++) {
-            const double R = modal_rcp(T->lam[par][m] + t);
+            // the hardware reciprocal alone (4.6e-8 relative): the sums then
+            // carry about the FP32 rounding of the modes themselves
+            const double R = __builtin_amdgcn_rcp(T->lam[par][m] + t);
             float& cm = par == 0 ? co[m] : ce[m];
             asm("" : "+v"(cm));
             const double d = (double)cm;

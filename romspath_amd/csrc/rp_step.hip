@@ -95,14 +95,13 @@ __device__ __forceinline__ void vwalk_body(const DGrid* __restrict__ grids, cons
 
 // ---------------------------------------------------------------------------
 // The FP32-field vertical walk with smooth's secant search in the eigenbasis
-// of chol1d's matrix pencil (rp_modal.h).  Same persistent wave-granular claiming as vwalk_body, but
-// each 64-particle claim runs in phases so that the wave is converged where
-// the matrix instructions run:
-//   1. per lane: hand-over rows, the column walk (y and qty work rows);
-//   2. wave: c = S qty (lanes without a column contribute zeros);
-//   3. per lane: the secant search on the modes in registers, then the
-//      modes of the accepted solution;
-//   4. wave: u = S vh into the u work rows;
+// of chol1d's matrix pencil (rp_modal.h).  Same persistent wave-granular
+// claiming as vwalk_body, but each 64-particle claim runs in phases so that
+// the wave is converged where the matrix instructions run:
+//   1. per lane: the column walk (y and qty work rows);
+//   2. wave: d = Phi' qty (lanes without a column contribute zeros);
+//   3. per lane: the secant search on the modes in registers;
+//   4. wave: u = Phi vh into the u work rows, vh formed from d on the way;
 //   5. per lane: the random-displacement sub-steps and the rest of the step.
 // ---------------------------------------------------------------------------
 
@@ -232,9 +231,10 @@ __device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids
             const int64_t q = launch_particle(A, ne, r);
             go = !skips(A, q);
             if (go) {
+                const double X0 = A.x[q], Y0 = A.y[q];
                 const int gid = NESTED ? (int)A.gid[q] : 1;
                 const DGrid& g = grids[NESTED ? gid - 1 : 0];
-                const RhoCell c0 = rho_cell(g, A.x[q], A.y[q]);
+                const RhoCell c0 = rho_cell(g, X0, Y0);
                 const Surface s0 = surface_at<Cur>(g, A, c0);
                 const VtCol col = vt_column_fast<Cur, false>(g, A, n + 2, c0, s0, S);
                 st_dbl(S, Lo.w + 0, col.z1);
@@ -244,8 +244,9 @@ __device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids
                 h = col.h;
             }
         }
-        // 2. modes of qty: odd from q+ = q_j + q_{n-1-j}, even from q- (the
-        // rows are read once per parity, which keeps one folded set live)
+        // 2. modes of qty: symmetric ones from q+ = q_j + q_{n-1-j},
+        // antisymmetric ones from q- (the rows are read once per parity,
+        // which keeps one folded set live)
         // Every lane reads its own column (n >= kMh: rows j and n-1-j lie in
         // 0..n); rows past n/2 read qty row n, which holds 0, and lanes
         // without a column are zeroed after the load, so the reads need no
@@ -274,7 +275,8 @@ __device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids
         double p = 0.0;
         ModalSol sol{0.0, 0.0, true};
         if (go) p = modal_search(MT, co, ce, h, A, sol);
-        // 4. u = S vh: rows j and n-1-j from the odd part +/- the even part
+        // 4. u = Phi vh: rows j and n-1-j from the symmetric part +/- the
+        // antisymmetric part
         // (plain stores: the sub-steps read them soon).  Every lane writes
         // its own column; the pairs past n/2 go to qty rows, which are no
         // longer read.
@@ -313,9 +315,11 @@ __device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids
             s0.tdepth = ld_dbl(S, Lo.w + 4);
             s0.zeta = ld_dbl(S, Lo.w + 6);
             DevSrc dv = dev_of(A, q);
-            const double TurbV = vt_substeps_fast(A, Lo, S, col, s0, A.mid[3 * A.nmid + r], p, dv);
-            finish<Cur, NESTED>(grids, A, q, NESTED ? (int)A.gid[q] : 1, A.z[q], A.mid[r],
-                                A.mid[A.nmid + r], A.mid[2 * A.nmid + r], TurbV);
+            const double newX = A.mid[r], newY = A.mid[A.nmid + r];
+            const double AdvectZ = A.mid[2 * A.nmid + r], Zpar = A.mid[3 * A.nmid + r];
+            const double TurbV = vt_substeps_fast(A, Lo, S, col, s0, Zpar, p, dv);
+            finish<Cur, NESTED>(grids, A, q, NESTED ? (int)A.gid[q] : 1, A.z[q], newX, newY, AdvectZ,
+                                TurbV);
             nact++;
         }
     }

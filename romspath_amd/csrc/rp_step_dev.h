@@ -1105,17 +1105,39 @@ __device__ __forceinline__ void back_u(const Band& B, const FastLayout& Lo, cons
     S.st_nt(Lo.u + Lo.np - 1, 0.0f);
 }
 
-__device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr32& S, int i, double p,
-                                           double k2, double h, double rh, double z1,
-                                           PieceCache& pc)
+// The sub-steps' spline pieces from a window of the u and y rows: four
+// intervals w..w+3 (u(w-1)..u(w+5), y(w)..y(w+4)) arrive in one round of
+// loads, and a piece inside the window is formed from registers.  The
+// coefficients of smooth's final loop (pppack.f90:5305-5314), formed from
+// u, h and y.
+struct PieceWindow {
+    int w;           // first interval of the window (0: none loaded)
+    float u[7], y[5];
+};
+
+__device__ __forceinline__ void window_load(const FastLayout& Lo, const Scr32& S, int i, PieceWindow& W)
+{
+    const int w = i <= 1 ? 1 : (i - 1 > Lo.np - 4 ? Lo.np - 4 : i - 1);
+    W.w = w;
+#pragma unroll
+    for (int k = 0; k < 7; k++) W.u[k] = S.at(Lo.u + w - 2 + k);   // u(w-1+k); u(0), u(np+1) unused
+#pragma unroll
+    for (int k = 0; k < 5; k++) W.y[k] = S.at(Lo.y + w - 1 + k);   // y(w+k)
+}
+
+__device__ __forceinline__ void piece_window(const FastLayout& Lo, const PieceWindow& W, int i, double p,
+                                             double k2, double h, double rh, double z1, PieceCache& pc)
 {
 #pragma clang fp contract(fast)
     const int np = Lo.np;
-    // u(i-1..i+2), y(i..i+1), issued together
-    const double um1 = (i >= 2) ? (double)S.at(Lo.u + i - 2) : 0.0;
-    const double u0 = (float)S.at(Lo.u + i - 1), u1 = (float)S.at(Lo.u + i);
-    const double u2 = (i + 2 <= np) ? (double)S.at(Lo.u + i + 1) : 0.0;
-    const double y0 = (float)S.at(Lo.y + i - 1), y1 = (float)S.at(Lo.y + i);
+    const int k = i - W.w;   // 0..3
+    auto sel = [&](const float* a, int o) {   // a[k + o] for k in 0..3
+        return k == 0 ? a[o] : (k == 1 ? a[o + 1] : (k == 2 ? a[o + 2] : a[o + 3]));
+    };
+    const double um1 = (i >= 2) ? (double)sel(W.u, 0) : 0.0;
+    const double u0 = (float)sel(W.u, 1), u1 = (float)sel(W.u, 2);
+    const double u2 = (i + 2 <= np) ? (double)sel(W.u, 3) : 0.0;
+    const double y0 = (float)sel(W.y, 0), y1 = (float)sel(W.y, 1);
     const double r_i = (i >= 2) ? (u0 - um1) * rh : 0.0;
     const double r_i1 = (u1 - u0) * rh;
     const double qu_i = r_i1 - r_i;
@@ -1131,9 +1153,9 @@ __device__ __forceinline__ void piece_fast(const FastLayout& Lo, const Scr32& S,
     pc.brk = z1 + (double)(i - 1) * h;
 }
 
-__device__ __forceinline__ double ppvalu_fast(const FastLayout& Lo, const Scr32& S, PieceCache& pc,
-                                              double x, int jderiv, double p, double k2, double h,
-                                              double rh, double z1)
+__device__ __forceinline__ double ppvalu_window(const FastLayout& Lo, const Scr32& S, PieceWindow& W,
+                                               PieceCache& pc, double x, int jderiv, double p,
+                                               double k2, double h, double rh, double z1)
 {
     const int np = Lo.np;
     const double gss = (x - z1) * rh;
@@ -1141,9 +1163,13 @@ __device__ __forceinline__ double ppvalu_fast(const FastLayout& Lo, const Scr32&
     if (!(gss >= 0.0)) i = 1;
     else if (gss >= (double)(np - 1)) i = np - 1;
     else i = (int)gss + 1;
-    if (i != pc.i) piece_fast(Lo, S, i, p, k2, h, rh, z1, pc);
+    if (i != pc.i) {
+        if (W.w == 0 || i < W.w || i > W.w + 3) window_load(Lo, S, i, W);
+        piece_window(Lo, W, i, p, k2, h, rh, z1, pc);
+    }
     return horner(pc, x - pc.brk, jderiv);
 }
+
 
 // AKs at w level k of the particle's column, time-interpolated, and the
 // level's depth (getInterpAKs + getWlevel + polintd, ver_turb_module.f90:100-124)
@@ -1311,19 +1337,21 @@ __device__ __forceinline__ double vt_substeps_fast(const StepArgs& A, const Fast
     const double k2 = 6.0 * (1.0 - p) * (A.serr * A.serr);
     PieceCache pc;
     pc.i = -1;
+    PieceWindow W;
+    W.w = 0;
     const double deltat = A.deltat;
     double ParZc = Zpar;
     for (int i = 1; i <= A.loop; i++) {
         double Kprimec;
         if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
-        else Kprimec = ppvalu_fast(Lo, S, pc, ParZc, 1, p, k2, h, rh, z1);
+        else Kprimec = ppvalu_window(Lo, S, W, pc, ParZc, 1, p, k2, h, rh, z1);
         const double KprimeZc = Kprimec * deltat;
         const double Z3rdc = ParZc + 0.5 * KprimeZc;
         double KH3rdc;
         if (Z3rdc < tdepth || Z3rdc > zeta) {
             KH3rdc = A.aksback;
         } else {
-            KH3rdc = ppvalu_fast(Lo, S, pc, Z3rdc, 0, p, k2, h, rh, z1);
+            KH3rdc = ppvalu_window(Lo, S, W, pc, Z3rdc, 0, p, k2, h, rh, z1);
             if (KH3rdc < A.aksback) KH3rdc = A.aksback;
         }
         const double DEV = dv.get(1 + i);
