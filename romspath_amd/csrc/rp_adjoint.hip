@@ -106,10 +106,14 @@ __device__ __forceinline__ void adj_particle(const DGrid* __restrict__ grids, co
     StageRec<D3> rec;
     advect<F, NSLOT>(g, A, dv, X, Y, Zpar, newX, newY, AdvectZ, &rec);
     if (A.vturb_on) {
-        const RhoCellT<D3> c0 = rho_cell(g, X, Y);
-        const SurfaceT<D3> s0 = surface_at<F>(g, A, c0);
-        const ScrT<D3> S{(D3*)A.scratch + tid, T};
-        TurbV = vturb<F>(g, A, (NESTED && A.np_mixed) ? g.np : A.np, c0, s0, Zpar, dv, S, A.capped);
+        // the column and the spline fit depend on the start X, Y only: they
+        // run in two-direction duals, the sub-steps in three
+        const Dual<2> X2 = Dual<2>::seed(X0, 0), Y2 = Dual<2>::seed(Y0, 1);
+        const RhoCellT<Dual<2>> c0 = rho_cell(g, X2, Y2);
+        const SurfaceT<Dual<2>> s0 = surface_at<F>(g, A, c0);
+        const ScrT<Dual<2>> S{(Dual<2>*)A.scratch + tid, T};
+        TurbV = vturb<F, Dual<2>, D3>(g, A, (NESTED && A.np_mixed) ? g.np : A.np, c0, s0, Zpar, dv, S,
+                                      A.capped);
     }
     const StepOut<D3> o = finish_math<F, NESTED>(grids, A, gid, X, Y, Z, newX, newY, AdvectZ, TurbV);
     const double lam[3] = {J.lx[q], J.ly[q], J.lz[q]};

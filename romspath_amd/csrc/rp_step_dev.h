@@ -546,16 +546,17 @@ __device__ __forceinline__ R chol1d_dev(R p, const VtLayout& Lo, const ScrT<R>& 
 // guess from the mean spacing, corrected against the stored breakpoints, so
 // the result is the reference's: the largest i<=np-1 with fitz(i)<=x, else 1.
 template <class R>
-__device__ __forceinline__ int interv_dev(const VtLayout& Lo, const ScrT<R>& S, R x, R f1, R dz)
+__device__ __forceinline__ int interv_dev(const VtLayout& Lo, const ScrT<R>& S, double x, double f1,
+                                          double dz)
 {
     const int np = Lo.np;
-    const R gss = (x - f1) / dz;
+    const double gss = (x - f1) / dz;
     int i;
     if (!(gss >= 0.0)) i = 1;
     else if (gss >= (double)(np - 1)) i = np - 1;
-    else i = (int)val(gss) + 1;
-    while (i > 1 && S(Lo.fitz + i - 1) > x) i--;
-    while (i < np - 1 && S(Lo.fitz + i) <= x) i++;
+    else i = (int)gss + 1;
+    while (i > 1 && val(S(Lo.fitz + i - 1)) > x) i--;
+    while (i < np - 1 && val(S(Lo.fitz + i)) <= x) i++;
     return i;
 }
 
@@ -605,21 +606,32 @@ __device__ __forceinline__ R horner(const PieceCacheT<R>& pc, R h, int jderiv)
     return (pc.c[3] / 2.0 * h + pc.c[2]) * h + pc.c[1];
 }
 
-// ppvalu (pppack.f90:4246-4362), k=4
-template <class R>
-__device__ __forceinline__ R ppvalu_dev(const VtLayout& Lo, const ScrT<R>& S, PieceCacheT<R>& pc,
-                                        R x, int jderiv, R p, R k2, R f1, R dz)
+// ppvalu (pppack.f90:4246-4362), k=4.  The pieces are formed in R (the
+// spline's type) and evaluated in RZ (the argument's): in the adjoint the
+// spline depends on the step's start X, Y only and the argument on Z too,
+// so the fit runs in two-direction duals and only the sub-steps in three.
+template <class R, class RZ>
+__device__ __forceinline__ RZ ppvalu_dev(const VtLayout& Lo, const ScrT<R>& S, PieceCacheT<RZ>& pc,
+                                         RZ x, int jderiv, R p, R k2, R f1, R dz)
 {
-    const int i = interv_dev(Lo, S, x, f1, dz);
-    if (i != pc.i) spline_piece(Lo, S, i, p, k2, pc);
-    const R h = x - pc.brk;
+    const int i = interv_dev(Lo, S, val(x), val(f1), val(dz));
+    if (i != pc.i) {
+        PieceCacheT<R> pr;
+        spline_piece(Lo, S, i, p, k2, pr);
+        pc.i = i;
+#pragma unroll
+        for (int k = 0; k < 4; k++) pc.c[k] = Conv<RZ>::from(pr.c[k]);
+        pc.brk = Conv<RZ>::from(pr.brk);
+    }
+    const RZ h = x - pc.brk;
     return horner(pc, h, jderiv);
 }
 
-template <typename F, class R>
-__device__ __forceinline__ R vturb(const DGrid& g, const StepArgs& A, int np, const RhoCellT<R>& c,
-                                   const SurfaceT<R>& sf, R Zpar, DevSrc& dv, const ScrT<R>& S,
-                                   unsigned* capped)
+// R: the type of the column and the spline fit; RZ: of the sub-steps
+template <typename F, class R, class RZ = R>
+__device__ __forceinline__ RZ vturb(const DGrid& g, const StepArgs& A, int np, const RhoCellT<R>& c,
+                                    const SurfaceT<R>& sf, RZ Zpar, DevSrc& dv, const ScrT<R>& S,
+                                    unsigned* capped)
 {
     const int nw = g.N + 1;
     const VtLayout Lo(np, nw);
@@ -722,18 +734,19 @@ __device__ __forceinline__ R vturb(const DGrid& g, const StepArgs& A, int np, co
     }
     const R k2 = 6.0 * (1.0 - p) * dy2;
     // RDM sub-steps (ver_turb_module.f90:231-293)
-    PieceCacheT<R> pc;
+    PieceCacheT<RZ> pc;
     pc.i = -1;
     const double deltat = A.deltat;
-    R ParZc = Zpar;
+    const RZ td = Conv<RZ>::from(tdepth), ze = Conv<RZ>::from(zeta);
+    RZ ParZc = Zpar;
     for (int i = 1; i <= A.loop; i++) {
-        R Kprimec;
-        if (ParZc <= tdepth || ParZc >= zeta) Kprimec = 0.0;
+        RZ Kprimec;
+        if (ParZc <= td || ParZc >= ze) Kprimec = 0.0;
         else Kprimec = ppvalu_dev(Lo, S, pc, ParZc, 1, p, k2, z1, dz);
-        const R KprimeZc = Kprimec * deltat;
-        const R Z3rdc = ParZc + 0.5 * KprimeZc;
-        R KH3rdc;
-        if (Z3rdc < tdepth || Z3rdc > zeta) {
+        const RZ KprimeZc = Kprimec * deltat;
+        const RZ Z3rdc = ParZc + 0.5 * KprimeZc;
+        RZ KH3rdc;
+        if (Z3rdc < td || Z3rdc > ze) {
             KH3rdc = A.aksback;
         } else {
             KH3rdc = ppvalu_dev(Lo, S, pc, Z3rdc, 0, p, k2, z1, dz);
@@ -741,10 +754,10 @@ __device__ __forceinline__ R vturb(const DGrid& g, const StepArgs& A, int np, co
         }
         const double DEV = dv.get(1 + i);
         const double r = 1.0;
-        const R dTz = KprimeZc + DEV * sqrt(2.0 / r * KH3rdc * deltat);
+        const RZ dTz = KprimeZc + DEV * sqrt(2.0 / r * KH3rdc * deltat);
         ParZc = ParZc + dTz;
-        if (ParZc <= tdepth) ParZc = tdepth + fabs(ParZc - tdepth);
-        if (ParZc >= zeta) ParZc = zeta - fabs(ParZc - zeta);
+        if (ParZc <= td) ParZc = td + fabs(ParZc - td);
+        if (ParZc >= ze) ParZc = ze - fabs(ParZc - ze);
     }
     return ParZc - Zpar;
 }
