@@ -40,6 +40,7 @@ from romspath_amd.schedule import schedule  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_VALU_TFLOPS = 78.6          # MI355X FP64 vector, FMA counted as 2
+PEAK_FP32_MFMA_TFLOPS = 157.3         # MI355X FP32 matrix, dense
 # VALU lane-instructions per second: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 PEAK_VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9
 # minimal stencil bytes per active particle-step, no reuse between particles:
@@ -443,13 +444,20 @@ def main():
         flop = sum(va["fp64_flop_per_particle_step"].values())
         instr = sum(va["valu_lane_instr_per_particle_step"].values())
         ach = flop * per_launch / ks / 1e12
+        mfma = sum(va.get("mfma_per_particle_step", {}).values()) * va.get("mfma_flop", 0)
+        mach = mfma * per_launch / ks / 1e12
         compute = {"bound": "fp64 valu", "achieved": ach, "peak": PEAK_FP64_VALU_TFLOPS,
                    "unit": "TFLOP/s", "frac": ach / PEAK_FP64_VALU_TFLOPS,
                    "fp64_flop_per_particle_step": flop,
                    "valu_lane_instr_per_particle_step": instr,
                    "valu_issue_frac": instr * per_launch / ks / PEAK_VALU_LANE_OPS,
-                   "source": "rocprofv3 SQ_INSTS_VALU(_FMA/_MUL/_ADD_F64) per dispatch, "
-                             "profiles/valu_c3_fp32.md"}
+                   "fp32_matrix": {"achieved": mach, "peak": PEAK_FP32_MFMA_TFLOPS,
+                                   "frac": mach / PEAK_FP32_MFMA_TFLOPS,
+                                   "flop_per_particle_step": mfma},
+                   "source": "rocprofv3 SQ_INSTS_VALU(_FMA/_MUL/_ADD_F64), SQ_INSTS_MFMA per "
+                             "dispatch of this command, profiles/valu_c3_fp32.md"}
+        if mfma:
+            log(f"  FP32 matrix: {mach:.1f} TFLOP/s ({mach / PEAK_FP32_MFMA_TFLOPS:.1%} of peak)")
         log(f"  FP64 VALU: {ach:.1f} TFLOP/s ({ach / PEAK_FP64_VALU_TFLOPS:.1%} of peak), "
             f"VALU issue {compute['valu_issue_frac']:.1%}")
 
@@ -528,7 +536,8 @@ def main():
                                             "step kernels, profiles/hbm_traffic_c3_fp32.md")
                      if l2_hit is not None else None,
                      "measured_traffic_gbs": measured_gbs,
-                     "kernel": "one step = advect_kernel + vwalk_kernel",
+                     "kernel": "one step = current_fields + advect_kernel + vertical walk "
+                               "(vwalk_modal_kernel)",
                      "b_alg_bytes_per_particle_step": balg,
                      "avg_kernel_ms": res["avg_kernel_ms"],
                      "advect_kernel_ms": adv_ms, "vwalk_kernel_ms": walk_ms,
