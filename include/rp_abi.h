@@ -150,6 +150,21 @@ int rp_get_particles(void *ctx, int64_t n, double *x, double *y, double *z, doub
 int rp_set_lonlat(void *ctx, int32_t ng, const double *lon, const double *lat);
 int rp_sample_lonlat(void *ctx, int64_t n, double *lon, double *lat);
 
+/* The reference built with -DWETDRY (makefile:12-13; off by default):
+ * updateHydro reads wetdry_mask_rho (xi_rho, eta_rho), wetdry_mask_u
+ * (xi_u, eta_u) and wetdry_mask_v (xi_v, eta_v) of each record into its slot
+ * (hydrodynamic_module.f90:510-570), and getInterp3d's u, v, w, salt, temp
+ * and getInterpStr's bustr, bvstr use the slot's masks in place of the grid's
+ * static masks (interpolation_module.f90:628-923, 1022-1140); getInterpAKs
+ * keeps the static mask.  rp_set_hydro_wetdry stores the three masks (FP64,
+ * each value 0 or 1, else RP_E_ARG) of time slot 1..3 of grid ng, after
+ * waiting for the steps already queued; from the first call on, the grid's
+ * gathers use the masks, and slots not given masks are dry (the reference
+ * allocates them as 0).  FP64 fields only: RP_E_STATE in the FP32-field form,
+ * which combines the three slots of each point before its gathers. */
+int rp_set_hydro_wetdry(void *ctx, int32_t ng, int32_t slot, const double *mask_rho,
+                        const double *mask_u, const double *mask_v);
+
 /* SaltTempOn (ROMSPath.f90:958-981): rp_set_hydro_tracers stores salt and
  * temp (xi_rho, eta_rho, s_rho) of time slot 1..3, as updateHydro reads them
  * (hydrodynamic_module.f90:663-745); asynchronous like rp_set_hydro_slot.

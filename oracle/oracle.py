@@ -30,7 +30,8 @@ class OgGrid(C.Structure):
                 ("mask_rho", PI32), ("mask_u", PI32), ("mask_v", PI32),
                 ("scl", (C.c_double * 2) * MAXGRID), ("off", (C.c_double * 2) * MAXGRID),
                 ("zeta", PD * 3), ("u", PD * 3), ("v", PD * 3), ("w", PD * 3), ("aks", PD * 3),
-                ("salt", PD * 3), ("temp", PD * 3), ("bustr", PD * 3), ("bvstr", PD * 3)]
+                ("salt", PD * 3), ("temp", PD * 3), ("bustr", PD * 3), ("bvstr", PD * 3),
+                ("wd_rho", PI32 * 3), ("wd_u", PI32 * 3), ("wd_v", PI32 * 3)]
 
 
 class OgParams(C.Structure):
@@ -169,6 +170,16 @@ class OracleGrid:
                 a = np.ascontiguousarray(rec[name], dtype=np.float64)
                 keep[name] = a
                 getattr(self.s, name)[slot - 1] = dptr(a)
+        # the reference built with -DWETDRY: the record's wet/dry masks
+        # (hydrodynamic_module.f90:510-570); without them the static masks
+        for name, field in (("wetdry_mask_rho", "wd_rho"), ("wetdry_mask_u", "wd_u"),
+                            ("wetdry_mask_v", "wd_v")):
+            if name in rec:
+                a = np.ascontiguousarray(rec[name], dtype=np.int32)
+                keep[name] = a
+                getattr(self.s, field)[slot - 1] = iptr(a)
+            else:
+                getattr(self.s, field)[slot - 1] = PI32()
         self.slots[slot - 1] = keep
 
 

@@ -172,6 +172,13 @@ double og_interp3d(const og_grid *g, int var, double X, double Y, double Z, int 
     int nz, itop, ibot, i;
     double zlev[2] = {0.0, 0.0}, zcff[2], Zr;
     const int s = slot - 1;
+    /* -DWETDRY: this slot's wet/dry masks replace the static ones
+     * (interpolation_module.f90:628-665, 710-747, 774-923) */
+    const int32_t *mk_u = g->wd_u[s] ? g->wd_u[s] : g->mask_u;
+    const int32_t *mk_v = g->wd_v[s] ? g->wd_v[s] : g->mask_v;
+    const int32_t *mk_r = g->wd_rho[s] ? g->wd_rho[s] : g->mask_rho;
+#define MKU(i, j) (mk_u[((i) - 1) + (int64_t)(g->L - 1) * ((j) - 1)])
+#define MKV(i, j) (mk_v[((i) - 1) + (int64_t)g->L * ((j) - 1)])
     if (ztype == 1) {
         nz = g->N;
         itop = nz;
@@ -249,15 +256,15 @@ double og_interp3d(const og_grid *g, int var, double X, double Y, double Z, int 
         int ia, ib;
         if (Inode >= xiu) {
             ia = ib = xiu;
-            m[0] = MU(xiu, Jnode); m[1] = 0; m[2] = 0; m[3] = MU(xiu, Jnode + 1);
+            m[0] = MKU(xiu, Jnode); m[1] = 0; m[2] = 0; m[3] = MKU(xiu, Jnode + 1);
         } else if (Inode < 1) {
             ia = ib = 1;
-            m[0] = 0; m[1] = MU(1, Jnode); m[2] = MU(1, Jnode + 1); m[3] = 0;
+            m[0] = 0; m[1] = MKU(1, Jnode); m[2] = MKU(1, Jnode + 1); m[3] = 0;
         } else {
             ia = Inode;
             ib = Inode + 1;
-            m[0] = MU(Inode, Jnode); m[1] = MU(Inode + 1, Jnode);
-            m[2] = MU(Inode + 1, Jnode + 1); m[3] = MU(Inode, Jnode + 1);
+            m[0] = MKU(Inode, Jnode); m[1] = MKU(Inode + 1, Jnode);
+            m[2] = MKU(Inode + 1, Jnode + 1); m[3] = MKU(Inode, Jnode + 1);
         }
         const double *f = g->u[s];
         vb[0] = U3(f, ia, Jnode, ibot);     vt[0] = U3(f, ia, Jnode, itop);
@@ -285,15 +292,15 @@ double og_interp3d(const og_grid *g, int var, double X, double Y, double Z, int 
         int ja, jb;
         if (Jnode >= etav) {
             ja = jb = etav;
-            m[0] = MV(Inode, etav); m[1] = MV(Inode + 1, etav); m[2] = 0; m[3] = 0;
+            m[0] = MKV(Inode, etav); m[1] = MKV(Inode + 1, etav); m[2] = 0; m[3] = 0;
         } else if (Jnode < 1) {
             ja = jb = 1;
-            m[0] = 0; m[1] = 0; m[2] = MV(Inode + 1, 1); m[3] = MV(Inode, 1);
+            m[0] = 0; m[1] = 0; m[2] = MKV(Inode + 1, 1); m[3] = MKV(Inode, 1);
         } else {
             ja = Jnode;
             jb = Jnode + 1;
-            m[0] = MV(Inode, Jnode); m[1] = MV(Inode + 1, Jnode);
-            m[2] = MV(Inode + 1, Jnode + 1); m[3] = MV(Inode, Jnode + 1);
+            m[0] = MKV(Inode, Jnode); m[1] = MKV(Inode + 1, Jnode);
+            m[2] = MKV(Inode + 1, Jnode + 1); m[3] = MKV(Inode, Jnode + 1);
         }
         const double *f = g->v[s];
         vb[0] = V3(f, Inode, ja, ibot);     vt[0] = V3(f, Inode, ja, itop);
@@ -315,10 +322,10 @@ double og_interp3d(const og_grid *g, int var, double X, double Y, double Z, int 
         vb[1] = W3(f, Inode + 1, Jnode, ibot);     vt[1] = W3(f, Inode + 1, Jnode, itop);
         vb[2] = W3(f, Inode + 1, Jnode + 1, ibot); vt[2] = W3(f, Inode + 1, Jnode + 1, itop);
         vb[3] = W3(f, Inode, Jnode + 1, ibot);     vt[3] = W3(f, Inode, Jnode + 1, itop);
-        m[0] = R2(g->mask_rho, Inode, Jnode);
-        m[1] = R2(g->mask_rho, Inode + 1, Jnode);
-        m[2] = R2(g->mask_rho, Inode + 1, Jnode + 1);
-        m[3] = R2(g->mask_rho, Inode, Jnode + 1);
+        m[0] = R2(mk_r, Inode, Jnode);
+        m[1] = R2(mk_r, Inode + 1, Jnode);
+        m[2] = R2(mk_r, Inode + 1, Jnode + 1);
+        m[3] = R2(mk_r, Inode, Jnode + 1);
         double nwater = m[0] + m[1] + m[2] + m[3];
         if (nwater < 4.0)
             og_getcoeff(x, y, m, cff);
@@ -328,14 +335,16 @@ double og_interp3d(const og_grid *g, int var, double X, double Y, double Z, int 
         vb[1] = W3(f, Inode + 1, Jnode, ibot);     vt[1] = W3(f, Inode + 1, Jnode, itop);
         vb[2] = W3(f, Inode + 1, Jnode + 1, ibot); vt[2] = W3(f, Inode + 1, Jnode + 1, itop);
         vb[3] = W3(f, Inode, Jnode + 1, ibot);     vt[3] = W3(f, Inode, Jnode + 1, itop);
-        m[0] = R2(g->mask_rho, Inode, Jnode);
-        m[1] = R2(g->mask_rho, Inode + 1, Jnode);
-        m[2] = R2(g->mask_rho, Inode + 1, Jnode + 1);
-        m[3] = R2(g->mask_rho, Inode, Jnode + 1);
+        m[0] = R2(mk_r, Inode, Jnode);
+        m[1] = R2(mk_r, Inode + 1, Jnode);
+        m[2] = R2(mk_r, Inode + 1, Jnode + 1);
+        m[3] = R2(mk_r, Inode, Jnode + 1);
         double nwater = m[0] + m[1] + m[2] + m[3];
         if (nwater < 4.0)
             og_getcoeff(x, y, m, cff);
     }
+#undef MKU
+#undef MKV
     double zb = cff[0] * vb[0] + cff[1] * vb[1] + cff[2] * vb[2] + cff[3] * vb[3];
     double zt = cff[0] * vt[0] + cff[1] * vt[1] + cff[2] * vt[2] + cff[3] * vt[3];
     return zcff[0] * zb + zcff[1] * zt;
@@ -618,6 +627,9 @@ static int og_clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi 
 double og_interp_str(const og_grid *g, int var, int slot, double Ipar, double Jpar)
 {
     const int s = slot - 1;
+    /* -DWETDRY: this slot's wet/dry masks (interpolation_module.f90:1022-1060, 1103-1140) */
+    const int32_t *mk_u = g->wd_u[s] ? g->wd_u[s] : g->mask_u;
+    const int32_t *mk_v = g->wd_v[s] ? g->wd_v[s] : g->mask_v;
     int Inode = (int)floor(Ipar), Jnode = (int)floor(Jpar);
     double X = Ipar - Inode, Y = Jpar - Jnode;
     double cff[4], v[4], m[4];
@@ -632,16 +644,16 @@ double og_interp_str(const og_grid *g, int var, int slot, double Ipar, double Jp
         if (Inode >= xiu) {
             v[0] = BU(f, xiu, j0); v[1] = BU(f, xiu, j0);
             v[2] = BU(f, xiu, j1); v[3] = BU(f, xiu, j1);
-            m[0] = BU(g->mask_u, xiu, j0); m[1] = 0; m[2] = 0; m[3] = BU(g->mask_u, xiu, j1);
+            m[0] = BU(mk_u, xiu, j0); m[1] = 0; m[2] = 0; m[3] = BU(mk_u, xiu, j1);
         } else if (Inode < 1) {
             v[0] = BU(f, 1, j0); v[1] = BU(f, 1, j0);
             v[2] = BU(f, 1, j1); v[3] = BU(f, 1, j1);
-            m[0] = 0; m[1] = BU(g->mask_u, 1, j0); m[2] = BU(g->mask_u, 1, j1); m[3] = 0;
+            m[0] = 0; m[1] = BU(mk_u, 1, j0); m[2] = BU(mk_u, 1, j1); m[3] = 0;
         } else {
             v[0] = BU(f, Inode, j0); v[1] = BU(f, Inode + 1, j0);
             v[2] = BU(f, Inode + 1, j1); v[3] = BU(f, Inode, j1);
-            m[0] = BU(g->mask_u, Inode, j0); m[1] = BU(g->mask_u, Inode + 1, j0);
-            m[2] = BU(g->mask_u, Inode + 1, j1); m[3] = BU(g->mask_u, Inode, j1);
+            m[0] = BU(mk_u, Inode, j0); m[1] = BU(mk_u, Inode + 1, j0);
+            m[2] = BU(mk_u, Inode + 1, j1); m[3] = BU(mk_u, Inode, j1);
         }
 #undef BU
     } else {
@@ -657,12 +669,12 @@ double og_interp_str(const og_grid *g, int var, int slot, double Ipar, double Jp
         if (Jnode >= etav) {
             v[0] = BV(f, i0, etav); v[1] = BV(f, i1, etav);
             v[2] = BV(f, i1, etav); v[3] = BV(f, i0, etav);
-            m[0] = BV(g->mask_v, i0, etav); m[1] = BV(g->mask_v, i1, etav); m[2] = 0; m[3] = 0;
+            m[0] = BV(mk_v, i0, etav); m[1] = BV(mk_v, i1, etav); m[2] = 0; m[3] = 0;
         } else {
             v[0] = BV(f, i0, Jnode); v[1] = BV(f, i1, Jnode);
             v[2] = BV(f, i1, Jnode + 1); v[3] = BV(f, i0, Jnode + 1);
-            m[0] = BV(g->mask_v, i0, Jnode); m[1] = BV(g->mask_v, i1, Jnode);
-            m[2] = BV(g->mask_v, i1, Jnode + 1); m[3] = BV(g->mask_v, i0, Jnode + 1);
+            m[0] = BV(mk_v, i0, Jnode); m[1] = BV(mk_v, i1, Jnode);
+            m[2] = BV(mk_v, i1, Jnode + 1); m[3] = BV(mk_v, i0, Jnode + 1);
         }
 #undef BV
     }

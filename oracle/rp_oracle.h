@@ -44,6 +44,13 @@ typedef struct {
     const double *temp[3];       /* L*M*N */
     const double *bustr[3];      /* (L-1)*M, may be NULL (WriteBottom off) */
     const double *bvstr[3];      /* L*(M-1) */
+    /* The reference built with -DWETDRY: wet/dry masks of each slot
+     * (hydrodynamic_module.f90:510-570) in place of the static masks in
+     * getInterp3d's u, v, w, salt, temp and getInterpStr's bustr, bvstr.
+     * NULL: the static masks (the default build). */
+    const int32_t *wd_rho[3];    /* L*M */
+    const int32_t *wd_u[3];      /* (L-1)*M */
+    const int32_t *wd_v[3];      /* L*(M-1) */
 } og_grid;
 
 typedef struct {

@@ -35,7 +35,7 @@ EXPORTS = ["rp_create", "rp_set_grid", "rp_set_hydro_slot", "rp_set_particles", 
            "rp_set_perturbation", "rp_adjoint_record", "rp_adjoint",
            "rp_set_lonlat", "rp_sample_lonlat", "rp_set_grid_angle", "rp_locate_particles",
            "rp_set_hydro_tracers", "rp_sample_tracers",
-           "rp_set_hydro_bstress", "rp_bottom_begin", "rp_bottom_update", "rp_bottom_get",
+           "rp_set_hydro_bstress", "rp_set_hydro_wetdry", "rp_bottom_begin", "rp_bottom_update", "rp_bottom_get",
            "rp_tracers_begin", "rp_tracers_update", "rp_tracers_get",
            "rp_nc_create", "rp_nc_write", "rp_nc_records", "rp_nc_error", "rp_nc_close",
            "rp_ncin_open", "rp_ncin_dim", "rp_ncin_var", "rp_ncin_get", "rp_ncin_att_text",
@@ -125,6 +125,7 @@ def lib():
         L.rp_tracers_update.argtypes = [vp, PD, PD, C.c_int32, C.c_int32, C.c_int32]
         L.rp_tracers_get.argtypes = [vp, C.c_int64, PD, PD, C.c_int32]
         L.rp_set_hydro_bstress.argtypes = [vp, C.c_int32, C.c_int32, PD, PD]
+        L.rp_set_hydro_wetdry.argtypes = [vp, C.c_int32, C.c_int32, PD, PD, PD]
         L.rp_bottom_begin.argtypes = [vp]
         L.rp_bottom_update.argtypes = [vp, PD, PD, C.c_int32, C.c_int32, C.c_int32]
         L.rp_bottom_get.argtypes = [vp, C.c_int64, PD, PD, PD, C.c_int32]
@@ -369,6 +370,13 @@ class Context:
     def set_hydro_bstress(self, ng, slot, bustr, bvstr):
         bu, bv = _f64(bustr), _f64(bvstr)
         self._chk("rp_set_hydro_bstress", self._L.rp_set_hydro_bstress(self.h, ng, slot, _d(bu), _d(bv)))
+
+    def set_hydro_wetdry(self, ng, slot, mask_rho, mask_u, mask_v):
+        """The reference built with -DWETDRY: the record's wetdry_mask_rho/u/v
+        (0/1) into slot 1..3 (FP64 fields only)."""
+        mr, mu, mv = _f64(mask_rho), _f64(mask_u), _f64(mask_v)
+        self._chk("rp_set_hydro_wetdry",
+                  self._L.rp_set_hydro_wetdry(self.h, ng, slot, _d(mr), _d(mu), _d(mv)))
 
     def bottom_begin(self):
         self._chk("rp_bottom_begin", self._L.rp_bottom_begin(self.h))

@@ -74,6 +74,9 @@ struct GridDev {
     bool tracer_set[3] = {false, false, false};
     void *bustr = nullptr, *bvstr = nullptr;  // WriteBottom fields, slot-interleaved (L-1, M), (L, M-1)
     bool bstress_set[3] = {false, false, false};
+    // wet/dry masks (rp_set_hydro_wetdry): per slot, rho then u then v
+    // points, one byte each; null until the first call
+    uint8_t* wd = nullptr;
     double scl[kMaxGrid][2], off[kMaxGrid][2];
     bool set = false;
     bool slot_set[3] = {false, false, false};
@@ -384,6 +387,12 @@ int build_modal_tab(int n, ModalTab& T)
     return rp_modal_basis(n, &T.basis[0][0][0], &T.lam[0][0]);
 }
 
+// bytes of one slot's wet/dry masks: rho, u and v points
+int64_t wd_plane(const GridDev& g)
+{
+    return (int64_t)g.L * g.M + (int64_t)(g.L - 1) * g.M + (int64_t)g.L * (g.M - 1);
+}
+
 void free_grid(GridDev& g)
 {
     dfree(g.s_rho); dfree(g.cs_r); dfree(g.s_w); dfree(g.cs_w);
@@ -397,6 +406,7 @@ void free_grid(GridDev& g)
     for (auto& t : g.tracer_set) t = false;
     dfree(g.bustr); dfree(g.bvstr);
     for (auto& t : g.bstress_set) t = false;
+    dfree(g.wd);
     g.set = false;
 }
 
@@ -596,6 +606,13 @@ void fill_tab(const Ctx* c, DGrid tab[kMaxGrid])
         d.nwc = cur_column(g);
         d.np = np_of(c, g.N);
         d.kh = c->cfg.kh[i];
+        if (g.wd) {
+            const int64_t nr = (int64_t)g.L * g.M, nu = (int64_t)(g.L - 1) * g.M;
+            d.wdr = g.wd;
+            d.wdu = g.wd + nr;
+            d.wdv = g.wd + nr + nu;
+            d.wdp = wd_plane(g);
+        }
         for (int a = 0; a < kMaxGrid; a++)
             for (int k = 0; k < 2; k++) {
                 d.scl[a][k] = g.scl[a][k];
@@ -1708,6 +1725,43 @@ int rp_set_hydro_tracers(void* ctx, int32_t ng, int32_t slot, const double* salt
         (rc = upload_field(c, gi, 6, g.temp, temp, nullptr, g.L, g.M, g.N, s)))
         return rc;
     g.tracer_set[s] = true;
+    return RP_OK;
+}
+
+int rp_set_hydro_wetdry(void* ctx, int32_t ng, int32_t slot, const double* mask_rho, const double* mask_u,
+                        const double* mask_v)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return RP_E_ARG;
+    if (ng < 1 || ng > c->cfg.ngrid || !c->grids[ng - 1].set)
+        return fail(c, RP_E_STATE, "rp_set_hydro_wetdry: grid not set");
+    if (slot < 1 || slot > 3) return fail(c, RP_E_ARG, "slot must be 1..3");
+    if (!mask_rho || !mask_u || !mask_v) return fail(c, RP_E_ARG, "null mask");
+    if (c->fbytes != 8)
+        return fail(c, RP_E_STATE,
+                    "rp_set_hydro_wetdry: wet/dry masks need FP64 fields (the FP32-field form combines "
+                    "the three time slots of each point before its gathers, which per-slot masks forbid)");
+    GridDev& g = c->grids[ng - 1];
+    const int64_t cnt[3] = {(int64_t)g.L * g.M, (int64_t)(g.L - 1) * g.M, (int64_t)g.L * (g.M - 1)};
+    const double* src[3] = {mask_rho, mask_u, mask_v};
+    const int64_t plane = wd_plane(g);
+    std::vector<uint8_t> b((size_t)plane);
+    int64_t o = 0;
+    for (int f = 0; f < 3; f++)
+        for (int64_t i = 0; i < cnt[f]; i++, o++) {
+            const double m = src[f][i];
+            if (!(m == 0.0 || m == 1.0)) return fail(c, RP_E_ARG, "rp_set_hydro_wetdry: masks must be 0 or 1");
+            b[(size_t)o] = m == 1.0 ? 1 : 0;
+        }
+    int rc;
+    if (!g.wd) {
+        if ((rc = dalloc(c, (char**)&g.wd, (size_t)(3 * plane)))) return rc;
+        // slots not given masks yet are dry, as the reference allocates them
+        HIPCHK(c, hipMemsetAsync(g.wd, 0, (size_t)(3 * plane), c->stream));
+    }
+    // steps already queued may still read this slot's previous masks
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(g.wd + (slot - 1) * plane, b.data(), (size_t)plane, hipMemcpyHostToDevice));
     return RP_OK;
 }
 

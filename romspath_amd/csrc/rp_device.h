@@ -51,6 +51,12 @@ struct DGrid {
     int np;                              // VTurb resampling points, int(s_w(ng) * sub)
     double kh;                           // ConstantHTurb(ng)
     double scl[kMaxGrid][2], off[kMaxGrid][2];
+    // the reference built with -DWETDRY: wet/dry masks (0/1 bytes) of each
+    // slot s at wdr/wdu/wdv[s * wdp + point] for rho, u and v points, in
+    // place of mrho/mu/mv in the u, v, w, salt, temp and bottom-stress
+    // gathers; null: the static masks
+    const uint8_t *wdr, *wdu, *wdv;
+    int64_t wdp;
 };
 
 // Field kinds of the step code: double and float read the slot-interleaved

@@ -219,16 +219,19 @@ __global__ void __launch_bounds__(256) tracer_kernel(const DGrid* __restrict__ g
         zeta3<F>(g, c, A.slots, zt);
         double cff[4];
         rho_weights(g, c, cff);
-        const bool zero[4] = {false, false, false, false};
         double es[3], et[3];
         for (int t = 0; t < 3; t++) {
             const VBracket vb = bracket_rho<false>(g, Z, zt[t], tdepth);
             SlotSel S;
             S.s[0] = S.s[1] = S.s[2] = A.slots.s[t];
+            // -DWETDRY: the slot's wet/dry mask in place of the static one
+            // (interpolation_module.f90:774-830)
+            double cw[4] = {cff[0], cff[1], cff[2], cff[3]};
+            if (g.wdr) rho_weights_wet(g, c, A.slots.s[t], cw);
             double o[3];
-            blend3<F, 1>(A.salt[gi], nullptr, c.o, g.N, cff, zero, vb, S, o);
+            blend3<F, 1>(A.salt[gi], nullptr, c.o, g.N, cw, 0u, vb, S, o);
             es[t] = o[0];
-            blend3<F, 1>(A.temp[gi], nullptr, c.o, g.N, cff, zero, vb, S, o);
+            blend3<F, 1>(A.temp[gi], nullptr, c.o, g.N, cw, 0u, vb, S, o);
             et[t] = o[0];
         }
         const double ps = polintd(A.ex, es[0], es[1], es[2], A.ix2);
@@ -256,7 +259,9 @@ __global__ void __launch_bounds__(256) tracer_kernel(const DGrid* __restrict__ g
 template __global__ void tracer_kernel<float>(const DGrid*, TracerArgs);
 template __global__ void tracer_kernel<double>(const DGrid*, TracerArgs);
 
-// getInterpStr (interpolation_module.f90:964-1174, without WETDRY) at slot s:
+// getInterpStr (interpolation_module.f90:964-1174) at slot s, with the
+// slot's wet/dry masks in place of the static ones when the grid has them
+// (-DWETDRY, :1022-1060, 1103-1140):
 // var 0 bustr on u points (I shifted by 1/2), var 1 bvstr on v points (J
 // shifted, Jnode clamped to >= 1, and the reference's Inode = eta_v when
 // Jnode > eta_v kept).  Masked corners give 0, an all-land cell 0.  Node
@@ -276,7 +281,10 @@ __device__ double interp_str(const DGrid& g, const F* __restrict__ f, int var, i
         X = uvI - Inode;
         const int j0 = min(max(Jnode, 1), g.M), j1 = min(max(Jnode + 1, 1), g.M);
         auto F_ = [&](int i, int j) { return (double)f[4 * ((int64_t)(i - 1) + (int64_t)xiu * (j - 1)) + s]; };
-        auto M_ = [&](int i, int j) { return (double)g.mu[(int64_t)(i - 1) + (int64_t)xiu * (j - 1)]; };
+        auto M_ = [&](int i, int j) {
+            const int64_t p = (int64_t)(i - 1) + (int64_t)xiu * (j - 1);
+            return (double)(g.wdu ? g.wdu[(int64_t)s * g.wdp + p] : g.mu[p]);
+        };
         if (Inode >= xiu) {
             v[0] = v[1] = F_(xiu, j0);
             v[2] = v[3] = F_(xiu, j1);
@@ -300,7 +308,10 @@ __device__ double interp_str(const DGrid& g, const F* __restrict__ f, int var, i
         Y = uvJ - Jnode;
         const int i0 = min(max(Inode, 1), g.L), i1 = min(max(Inode + 1, 1), g.L);
         auto F_ = [&](int i, int j) { return (double)f[4 * ((int64_t)(i - 1) + (int64_t)g.L * (j - 1)) + s]; };
-        auto M_ = [&](int i, int j) { return (double)g.mv[(int64_t)(i - 1) + (int64_t)g.L * (j - 1)]; };
+        auto M_ = [&](int i, int j) {
+            const int64_t p = (int64_t)(i - 1) + (int64_t)g.L * (j - 1);
+            return (double)(g.wdv ? g.wdv[(int64_t)s * g.wdp + p] : g.mv[p]);
+        };
         if (Jnode >= etav) {
             v[0] = v[3] = F_(i0, etav);
             v[1] = v[2] = F_(i1, etav);

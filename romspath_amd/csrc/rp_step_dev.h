@@ -96,17 +96,21 @@ __device__ __forceinline__ void zeta3(const DGrid& g, const RhoCellT<R>& c, cons
 // The vertical bracket is shared by every slot: the reference recomputes it
 // per call from the same (z, zeta, depth), so the values are identical.
 // ---------------------------------------------------------------------------
-template <typename F, int NSLOT, class R>
+// zero: bit 4t+k set where corner k of slot t (S.s[t]) reads as 0 (a dry
+// or masked corner); cff: the corner weights, the same for every slot
+// (CS = 0) or slot t's at cff + 4t (CS = 4, per-slot wet/dry masks)
+template <typename F, int NSLOT, int CS = 0, class R>
 __device__ __forceinline__ void blend3(const void* fv, const float* fc, const int64_t col[4], int nk,
-                                       const R cff[4], const bool zero[4], const VBracketT<R>& vb,
+                                       const R* cff, uint32_t zero, const VBracketT<R>& vb,
                                        const SlotSel& S, R out[3])
 {
     if constexpr (is_cur<F>()) {
         // one value per point: the current field fc
         R zb = 0.0, zt = 0.0;
         for (int k = 0; k < 4; k++) {
-            const double b = zero[k] ? 0.0 : (double)fc[col[k] * nk + (vb.ibot - 1)];
-            const double a = zero[k] ? 0.0 : (double)fc[col[k] * nk + (vb.itop - 1)];
+            const bool z = (zero >> k) & 1u;
+            const double b = z ? 0.0 : (double)fc[col[k] * nk + (vb.ibot - 1)];
+            const double a = z ? 0.0 : (double)fc[col[k] * nk + (vb.itop - 1)];
             zb = zb + cff[k] * b;
             zt = zt + cff[k] * a;
         }
@@ -122,20 +126,48 @@ __device__ __forceinline__ void blend3(const void* fv, const float* fc, const in
 #pragma unroll
         for (int t = 0; t < NSLOT; t++) {
             const int s = S.s[t];
-            const double b = zero[k] ? 0.0 : pick(lb, s);
-            const double a = zero[k] ? 0.0 : pick(lt, s);
+            const R* c = cff + CS * t;
+            const bool z = (zero >> (4 * t + k)) & 1u;
+            const double b = z ? 0.0 : pick(lb, s);
+            const double a = z ? 0.0 : pick(lt, s);
             if (k == 0) {
-                zb[t] = cff[0] * b;
-                zt[t] = cff[0] * a;
+                zb[t] = c[0] * b;
+                zt[t] = c[0] * a;
             } else {
-                zb[t] = zb[t] + cff[k] * b;
-                zt[t] = zt[t] + cff[k] * a;
+                zb[t] = zb[t] + c[k] * b;
+                zt[t] = zt[t] + c[k] * a;
             }
         }
     }
 #pragma unroll
     for (int t = 0; t < NSLOT; t++) out[t] = vb.c0 * zb[t] + vb.c1 * zt[t];
     }
+}
+
+// the same corner flags for every slot
+__device__ __forceinline__ uint32_t zero_all_slots(const bool zero[4])
+{
+    const uint32_t z = (uint32_t)zero[0] | (uint32_t)zero[1] << 1 | (uint32_t)zero[2] << 2 | (uint32_t)zero[3] << 3;
+    return z | z << 4 | z << 8;
+}
+
+// The reference built with -DWETDRY (interpolation_module.f90:628-665,
+// 710-747, 1022-1140): each slot's wet/dry mask in place of the static one.
+// Corner k of slot t reads as 0 where the slot's mask at point pt[k] is 0,
+// or where pt[k] < 0 (the corners the edge rows zero whatever the masks).
+// Zeroing the dry corners with the weights kept is the reference's rule for
+// 0 < nwater < 4, and gives 0 for nwater = 0 as its cff = 0 does.
+template <int NSLOT>
+__device__ __forceinline__ uint32_t zero_wet(const uint8_t* wd, int64_t wdp, const SlotSel& S,
+                                             const int64_t pt[4])
+{
+    uint32_t z = 0;
+#pragma unroll
+    for (int t = 0; t < NSLOT; t++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (pt[k] < 0 || wd[(int64_t)S.s[t] * wdp + pt[k]] == 0) z |= 1u << (4 * t + k);
+    return z;
 }
 
 template <typename F, int NSLOT, class R>
@@ -178,7 +210,19 @@ __device__ __forceinline__ void interp_u(const DGrid& g, R X, R Y, const VBracke
     col[1] = (int64_t)(Jnode - 1) * xiu + (ib - 1);
     col[2] = (int64_t)(Jnode) * xiu + (ib - 1);
     col[3] = (int64_t)(Jnode) * xiu + (ia - 1);
-    blend3<F, NSLOT>(g.u, g.cu, col, g.N, cff, zero, vb, S, out);
+    if constexpr (!is_cur<F>()) {
+        if (g.wdu) {
+            // the slots' masks alone (the static ones take no part)
+            const bool east = Inode >= xiu, west = Inode < 1;
+            const int64_t pt[4] = {west ? -1 : col[0], east ? -1 : col[1], east ? -1 : col[2],
+                                   west ? -1 : col[3]};
+            R cw[4];
+            bilin(x, y, cw);
+            blend3<F, NSLOT>(g.u, g.cu, col, g.N, cw, zero_wet<NSLOT>(g.wdu, g.wdp, S, pt), vb, S, out);
+            return;
+        }
+    }
+    blend3<F, NSLOT>(g.u, g.cu, col, g.N, cff, zero_all_slots(zero), vb, S, out);
 }
 
 template <typename F, int NSLOT, class R>
@@ -220,7 +264,17 @@ __device__ __forceinline__ void interp_v(const DGrid& g, R X, R Y, const VBracke
     col[1] = (int64_t)(ja - 1) * g.L + (Inode);
     col[2] = (int64_t)(jb - 1) * g.L + (Inode);
     col[3] = (int64_t)(jb - 1) * g.L + (Inode - 1);
-    blend3<F, NSLOT>(g.v, g.cv, col, g.N, cff, zero, vb, S, out);
+    if constexpr (!is_cur<F>()) {
+        if (g.wdv) {
+            const bool north = Jnode >= etav;
+            const int64_t pt[4] = {col[0], col[1], north ? -1 : col[2], north ? -1 : col[3]};
+            R cw[4];
+            bilin(x, y, cw);
+            blend3<F, NSLOT>(g.v, g.cv, col, g.N, cw, zero_wet<NSLOT>(g.wdv, g.wdp, S, pt), vb, S, out);
+            return;
+        }
+    }
+    blend3<F, NSLOT>(g.v, g.cv, col, g.N, cff, zero_all_slots(zero), vb, S, out);
 }
 
 // masked rho-point weights for w and AKs: bilinear, or IDW when a corner is dry
@@ -233,14 +287,33 @@ __device__ __forceinline__ void rho_weights(const DGrid& g, const RhoCellT<R>& c
     if (m[0] + m[1] + m[2] + m[3] < 4.0) getcoeff(c.x, c.y, m, cff);
 }
 
+// the same weights from slot s's wet/dry mask of the rho points (-DWETDRY,
+// interpolation_module.f90:837-923)
+template <class R>
+__device__ __forceinline__ void rho_weights_wet(const DGrid& g, const RhoCellT<R>& c, int s, R cff[4])
+{
+    double m[4];
+    for (int k = 0; k < 4; k++) m[k] = (double)g.wdr[(int64_t)s * g.wdp + c.o[k]];
+    bilin(c.x, c.y, cff);
+    if (m[0] + m[1] + m[2] + m[3] < 4.0) getcoeff(c.x, c.y, m, cff);
+}
+
 template <typename F, int NSLOT, class R>
 __device__ __forceinline__ void interp_w(const DGrid& g, const RhoCellT<R>& c,
                                         const VBracketT<R>& vb, const SlotSel& S, R out[3])
 {
+    if constexpr (!is_cur<F>()) {
+        if (g.wdr) {
+            R cw[4 * NSLOT];
+#pragma unroll
+            for (int t = 0; t < NSLOT; t++) rho_weights_wet(g, c, S.s[t], cw + 4 * t);
+            blend3<F, NSLOT, 4>(g.w, g.cw, c.o, g.N + 1, cw, 0u, vb, S, out);
+            return;
+        }
+    }
     R cff[4];
     rho_weights(g, c, cff);
-    const bool zero[4] = {false, false, false, false};
-    blend3<F, NSLOT>(g.w, g.cw, c.o, g.N + 1, cff, zero, vb, S, out);
+    blend3<F, NSLOT>(g.w, g.cw, c.o, g.N + 1, cff, 0u, vb, S, out);
 }
 
 // ---------------------------------------------------------------------------

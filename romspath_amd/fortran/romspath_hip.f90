@@ -60,7 +60,7 @@ module romspath_hip
             rp_set_perturbation, rp_adjoint_record, rp_adjoint, &
             rp_set_lonlat, rp_sample_lonlat, rp_nc_create, rp_nc_write, rp_nc_close, &
             rp_set_grid_angle, rp_locate_particles, rp_set_hydro_tracers, rp_sample_tracers, &
-            rp_set_hydro_bstress, rp_bottom_begin, rp_bottom_update, rp_bottom_get, &
+            rp_set_hydro_bstress, rp_set_hydro_wetdry, rp_bottom_begin, rp_bottom_update, rp_bottom_get, &
             rp_tracers_begin, rp_tracers_update, rp_tracers_get, &
             rp_ncin_open, rp_ncin_dim, rp_ncin_var, rp_ncin_get, rp_ncin_close, &
             rp_ncin_att_text, rp_ncin_error, rp_nc_records, rp_nc_error, &
@@ -414,6 +414,16 @@ module romspath_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: ng, slot
       real(c_double), intent(in) :: bustr(*), bvstr(*)
+    end function
+
+    ! -DWETDRY: wetdry_mask_rho (xi_rho, eta_rho), wetdry_mask_u (xi_u, eta_u),
+    ! wetdry_mask_v (xi_v, eta_v) of time slot 1..3 (hydrodynamic_module.f90:510-570)
+    integer(c_int) function rp_set_hydro_wetdry(ctx, ng, slot, mask_rho, mask_u, mask_v) &
+        bind(C, name="rp_set_hydro_wetdry")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: ng, slot
+      real(c_double), intent(in) :: mask_rho(*), mask_u(*), mask_v(*)
     end function
 
     integer(c_int) function rp_bottom_begin(ctx) bind(C, name="rp_bottom_begin")

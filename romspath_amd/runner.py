@@ -59,11 +59,19 @@ class GpuRun:
         """Records 1, 2, 3 into slots 1, 2, 3, as ini_ROMSPath does (ROMSPath.f90:307-309)."""
         for slot in (1, 2, 3):
             for g in range(1, self.cfg.ngrid + 1):
-                self.ctx.set_hydro_slot(g, slot, self.records.get(g, slot))
+                rec = self.records.get(g, slot)
+                self.ctx.set_hydro_slot(g, slot, rec)
+                self._set_wetdry(g, slot, rec)
+
+    def _set_wetdry(self, g, slot, rec):
+        if "wetdry_mask_rho" in rec:
+            self.ctx.set_hydro_wetdry(g, slot, rec["wetdry_mask_rho"], rec["wetdry_mask_u"],
+                                      rec["wetdry_mask_v"])
 
     def load(self, st):
         """updateHydro for the step's loads: the slot's hydro fields, and salt/temp
-        (SaltTempOn) and bustr/bvstr (WriteBottom) when the record has them."""
+        (SaltTempOn), bustr/bvstr (WriteBottom) and the wet/dry masks (-DWETDRY)
+        when the record has them."""
         for ld in st.loads:
             for g in range(1, self.cfg.ngrid + 1):
                 rec = self.records.get(g, ld.record, ld.filenum)
@@ -72,6 +80,7 @@ class GpuRun:
                     self.ctx.set_hydro_tracers(g, ld.slot, rec["salt"], rec["temp"])
                 if "bustr" in rec and "bvstr" in rec:
                     self.ctx.set_hydro_bstress(g, ld.slot, rec["bustr"], rec["bvstr"])
+                self._set_wetdry(g, ld.slot, rec)
 
     def run(self, max_steps=None, inject=False, on_step=None, diag_every_step=False,
             mode="step", counts=None, use_graph=True, output=None, salttemp_mean=False,

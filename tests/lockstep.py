@@ -46,6 +46,9 @@ class Lockstep:
                     self.ctx.set_hydro_tracers(gi + 1, ld.slot, rec["salt"], rec["temp"])
                 if "bustr" in rec:
                     self.ctx.set_hydro_bstress(gi + 1, ld.slot, rec["bustr"], rec["bvstr"])
+                if "wetdry_mask_rho" in rec:
+                    self.ctx.set_hydro_wetdry(gi + 1, ld.slot, rec["wetdry_mask_rho"],
+                                              rec["wetdry_mask_u"], rec["wetdry_mask_v"])
                 self.ogs[gi].set_slot(ld.slot, rec)
 
     def step(self, st, want_diag=False):

@@ -1,0 +1,119 @@
+"""The reference built with -DWETDRY on the GPU (rp_set_hydro_wetdry) against
+the oracle: each record carries wet/dry masks (hydrodynamic_module.f90:
+510-570) that dry a shore band around the land, widening and narrowing from
+record to record, and a patch of open water on odd records, so the three
+slots a step reads disagree near the shore.  The GPU and the oracle step
+side by side with the same injected deviates (FP64 fields): every step's
+state agrees bit for bit (status, grid, isIn; positions within 1e-9 cell),
+and so do the bottom-stress sampler after every step and the salt/temp
+sample at two points of the run.  The masks must change trajectories (the
+same run without them differs), and the FP32-field form refuses them.
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from oracle import oracle as O
+from romspath_amd import synth
+from romspath_amd.abi import RpError
+from romspath_amd.runner import GpuRun
+from romspath_amd.schedule import schedule
+
+from lockstep import Lockstep, check_same_state
+from test_bottom import stress_fields
+
+pytestmark = pytest.mark.gpu
+
+
+def wet_masks(g, rec_no):
+    """Wet/dry masks of record rec_no: the static mask less a shore band of
+    1..3 cells that breathes with the record, less a patch on odd records;
+    u and v points are wet where both rho neighbours are, as ROMS sets them."""
+    static = np.asarray(g.mask_rho, dtype=np.int32)
+    dist = ndimage.distance_transform_edt(static) if not static.all() else np.full(static.shape, 1e9)
+    width = 1.0 + 2.0 * (0.5 + 0.5 * np.sin(2.0 * np.pi * rec_no / 5.0))
+    wet = static * (dist > width)
+    if rec_no % 2:
+        M, L = static.shape
+        wet[M // 3:M // 3 + 4, L // 4:L // 4 + 5] = 0
+    wet = np.ascontiguousarray(wet, dtype=np.int32)
+    return dict(wetdry_mask_rho=wet,
+                wetdry_mask_u=np.ascontiguousarray(wet[:, :-1] * wet[:, 1:], dtype=np.int32),
+                wetdry_mask_v=np.ascontiguousarray(wet[:-1, :] * wet[1:, :], dtype=np.int32))
+
+
+def _extra(cfg):
+    def extra(gi, rec_no):
+        g = cfg.grids[gi]
+        rng = np.random.default_rng(1000 * gi + rec_no)
+        d = wet_masks(g, rec_no)
+        bu, bv = stress_fields(g, 3600.0 * rec_no, rng)
+        d.update(bustr=bu, bvstr=bv, salt=34.0 + rng.standard_normal((g.N, g.M, g.L)),
+                 temp=10.0 + 3.0 * rng.standard_normal((g.N, g.M, g.L)))
+        return d
+    return extra
+
+
+@pytest.mark.parametrize("case", ["mini", "nested"])
+def test_wetdry_steps_and_samplers_match_oracle(case):
+    cfg = (synth.config_mini(n_particles=600, days=2 / 24.0) if case == "mini"
+           else synth.config_mini_nested(n_particles=600))
+    prm = cfg.params
+    steps = list(schedule(prm.dt, prm.idt, prm.days, prm.iprint, 48))
+    print(f"case wetdry {case} fp64, {len(steps)} steps", flush=True)
+    ls = Lockstep(cfg, precision="fp64", extra=_extra(cfg))
+    n = cfg.particles.n
+    la = np.ascontiguousarray(cfg.particles.age, dtype=np.float64).copy()
+    hob, bu, bv = np.full(n, 9999.0), np.zeros(n), np.zeros(n)
+    sampled = 0
+    try:
+        ls.ctx.bottom_begin()
+        for k, st in enumerate(steps):
+            ls.load(st)
+            T = ls.step(st)
+            ls.ctx.bottom_update(st.ex, st.ix, st.tb, st.tc, st.tf)
+            O.bottom_update(ls.ogs, T, ls.state, la, hob, bu, bv)
+            g = ls.ctx.get_particles()
+            check_same_state(g, ls.state, f"{case} step {k}")
+            if k in (20, len(steps) - 1):
+                gh, gu, gv = ls.ctx.bottom_get()
+                assert np.array_equal(gh, hob) and np.array_equal(gu, bu) and np.array_equal(gv, bv)
+                salt, temp = ls.ctx.sample_tracers(st.ex, st.ix, st.tb, st.tc, st.tf)
+                os_, ot = O.sample_tracers(ls.ogs, st.ex, st.ix, st.tb, st.tc, st.tf, ls.state.x,
+                                           ls.state.y, ls.state.z, ls.state.gid)
+                assert np.array_equal(salt, os_) and np.array_equal(temp, ot)
+                sampled += 1
+        final = ls.ctx.get_particles()
+    finally:
+        ls.close()
+    assert sampled == 2
+    # the masks matter: the same run without them goes elsewhere
+    plain = O.run_oracle(cfg, max_steps=len(steps), inject=True)
+    moved = np.abs(plain.x - final["x"]) + np.abs(plain.y - final["y"]) > 1e-6
+    print(f"{case}: {moved.sum()} of {n} particles differ from the run without masks", flush=True)
+    assert moved.sum() >= 5
+
+
+def test_wetdry_refused_in_fp32_form_and_checked():
+    cfg = synth.config_mini(n_particles=50, days=2 / 24.0)
+    m = wet_masks(cfg.grids[0], 1)
+    run = GpuRun(cfg, precision="fp32")
+    try:
+        with pytest.raises(RpError) as e:
+            run.ctx.set_hydro_wetdry(1, 1, m["wetdry_mask_rho"], m["wetdry_mask_u"], m["wetdry_mask_v"])
+        assert e.value.code == -2                           # RP_E_STATE
+    finally:
+        run.close()
+    run = GpuRun(cfg, precision="fp64")
+    try:
+        bad = m["wetdry_mask_rho"].astype(np.float64)
+        bad[3, 3] = 0.5
+        with pytest.raises(RpError) as e:
+            run.ctx.set_hydro_wetdry(1, 1, bad, m["wetdry_mask_u"], m["wetdry_mask_v"])
+        assert e.value.code == -1                           # RP_E_ARG
+        with pytest.raises(RpError) as e:
+            run.ctx.set_hydro_wetdry(1, 4, m["wetdry_mask_rho"], m["wetdry_mask_u"], m["wetdry_mask_v"])
+        assert e.value.code == -1
+        run.ctx.set_hydro_wetdry(1, 2, m["wetdry_mask_rho"], m["wetdry_mask_u"], m["wetdry_mask_v"])
+    finally:
+        run.close()
