@@ -18,10 +18,13 @@
 ! roll-over, and a record whenever printdt reaches iprint; model_time is
 ! time + reftime of the last nest (ROMSPath.f90:1931).
 !
-! usage: rp_driver [-n] [--stokes] [--fp32] <namelist file> [<final-state file>]
+! usage: rp_driver [-n] [--stokes] [--wetdry] [--fp32] <namelist file> [<final-state file>]
 !   -n         read the namelist, print what was read, stop (touches no GPU)
 !   --stokes   add the Stokes drift of stokesprefix(ng) to u, v: the reference
 !              built with -DSTOKES (hydrodynamic_module.f90:573-660)
+!   --wetdry   read each record's wetdry_mask_rho/_u/_v into its slot: the
+!              reference built with -DWETDRY (hydrodynamic_module.f90:510-570;
+!              FP64 fields only)
 !   --fp32     the FP32-field form of the step (default: FP64, the reference's)
 ! The final-state file, when named, gets "X Y Z status GID age" per particle.
 !
@@ -125,7 +128,7 @@ program rp_driver
   double precision :: sclv(2, MAXNgrid, MAXNgrid), offv(2, MAXNgrid, MAXNgrid)
   double precision :: reftime
   character(len=100) :: time_units
-  logical :: from_files, use_stokes, check_only, fp32
+  logical :: from_files, use_stokes, use_wetdry, check_only, fp32
   ! analytic records (prefix(1) blank)
   double precision, allocatable :: gz(:, :), gu(:, :, :), gv(:, :, :), gw(:, :, :), gaks(:, :, :)
 
@@ -150,6 +153,7 @@ program rp_driver
   ! ---- command line ----
   check_only = .false.
   use_stokes = .false.
+  use_wetdry = .false.
   fp32 = .false.
   nmlfile = ' '
   statefile = ' '
@@ -161,6 +165,8 @@ program rp_driver
       check_only = .true.
     case ('--stokes')
       use_stokes = .true.
+    case ('--wetdry')
+      use_wetdry = .true.
     case ('--fp32')
       fp32 = .true.
     case default
@@ -170,7 +176,7 @@ program rp_driver
     end select
   end do
   if (len_trim(nmlfile) == 0) then
-    write (*, '(A)') 'usage: rp_driver [-n] [--stokes] [--fp32] <namelist file> [<final-state file>]'
+    write (*, '(A)') 'usage: rp_driver [-n] [--stokes] [--wetdry] [--fp32] <namelist file> [<final-state file>]'
     stop 1
   end if
 
@@ -186,6 +192,7 @@ program rp_driver
   from_files = len_trim(prefix(1)) > 0
   if (.not. from_files .and. Ngrid /= 1) call fatal('the analytic double gyre is one nest: give prefix(ng)')
   if (use_stokes .and. .not. from_files) call fatal('--stokes needs history files')
+  if (use_wetdry .and. .not. from_files) call fatal('--wetdry needs history files')
 
   ! ---- InitGrid for every nest, then the nest maps ----
   allocate (nests(Ngrid))
@@ -912,7 +919,7 @@ contains
     integer, intent(in) :: slot, rec, fnum
     integer :: ng_, L, M, N
     double precision, allocatable, target :: zeta(:), u(:), v(:), w(:), aks(:), su(:), sv(:), &
-      salt(:), temp(:), bu(:), bv(:)
+      salt(:), temp(:), bu(:), bv(:), wdr(:), wdu(:), wdv(:)
     type(c_ptr) :: f, sf
     do ng_ = 1, Ngrid
       L = nests(ng_)%L; M = nests(ng_)%M; N = nests(ng_)%N
@@ -970,6 +977,15 @@ contains
       if (WriteBottom) then
         rc = rp_set_hydro_bstress(ctx, int(ng_, c_int32_t), int(slot, c_int32_t), bu, bv)
         if (rc /= 0) call die('rp_set_hydro_bstress', rc)
+      end if
+      if (use_wetdry) then
+        allocate (wdr(L * M), wdu((L - 1) * M), wdv(L * (M - 1)))
+        call getr(f, 'wetdry_mask_rho', rec, wdr, size(wdr))
+        call getr(f, 'wetdry_mask_u', rec, wdu, size(wdu))
+        call getr(f, 'wetdry_mask_v', rec, wdv, size(wdv))
+        rc = rp_set_hydro_wetdry(ctx, int(ng_, c_int32_t), int(slot, c_int32_t), wdr, wdu, wdv)
+        if (rc /= 0) call die('rp_set_hydro_wetdry', rc)
+        deallocate (wdr, wdu, wdv)
       end if
       deallocate (zeta, u, v, w, aks, salt, temp, bu, bv)
     end do

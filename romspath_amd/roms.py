@@ -8,8 +8,9 @@ the library's netCDF classic reader (include/rp_ncin.h).
   reftime and time_units as InitGrid derives them from the time variable.
 * `HydroFiles`: hydro records by (grid, record in file, file number), the
   fields updateHydro reads (zeta, u, v, w, AKs; salt and temp with
-  SaltTempOn; bustr and bvstr with WriteBottom; constants where a read flag
-  is off), and the file roll-over the time schedule needs (tdim, multifile).
+  SaltTempOn; bustr and bvstr with WriteBottom; the wet/dry masks of the
+  -DWETDRY build; constants where a read flag is off), and the file roll-over
+  the time schedule needs (tdim, multifile).
 """
 from __future__ import annotations
 
@@ -226,8 +227,12 @@ class HydroFiles:
     against (ROMSPath.f90:473-480)."""
 
     def __init__(self, prefixes, filenum=1, numdigits=4, suffix=".nc", multifile=False,
-                 flags=None, time_dname="ocean_time", cache=True, stokes_prefixes=None):
+                 flags=None, time_dname="ocean_time", cache=True, stokes_prefixes=None,
+                 wetdry=False):
         self.prefixes = list(prefixes)
+        # -DWETDRY: wetdry_mask_rho / _u / _v of every record
+        # (hydrodynamic_module.f90:510-570), for rp_set_hydro_wetdry
+        self.wetdry = bool(wetdry)
         # -DSTOKES: ustokes / vstokes of the same record from a second file
         # series (stokesprefix, hydrodynamic_module.f90:573-660), handed on
         # as su / sv for the library to add to u / v
@@ -292,6 +297,9 @@ class HydroFiles:
         if fl.WriteBottom:
             rec["bustr"] = f.read("bustr", r)
             rec["bvstr"] = f.read("bvstr", r)
+        if self.wetdry:
+            for name in ("wetdry_mask_rho", "wetdry_mask_u", "wetdry_mask_v"):
+                rec[name] = f.read(name, r)
         if self.stokes_prefixes is not None:
             fs = self._file(ng, filenum, stokes=True)
             rec["su"] = fs.read("ustokes", r)

@@ -361,3 +361,46 @@ def test_fortran_driver_nested_files_samplers_equal_harness(tmp_path):
     finally:
         fa.close()
         fb.close()
+
+
+@pytest.mark.gpu
+def test_fortran_driver_wetdry_masks_from_files(tmp_path):
+    """--wetdry: the driver reads each record's wetdry_mask_rho/_u/_v from
+    the history file into its slot (the reference built with -DWETDRY,
+    hydrodynamic_module.f90:510-570); the final state equals the oracle's on
+    the same records and masks (a shore band that dries on odd records, on
+    the island grid), and the masks change the result."""
+    from test_ncin import write_history
+    from test_gpu_wetdry import wet_masks
+    from romspath_amd.roms import HydroFiles
+    from romspath_amd.runner import RecordSource
+    n, days = 300, 4 / 24.0
+    cfg = synth.config_mini(n_particles=n, days=days, vturb=False, hturb=False, dob_spread=0.0)
+    recs = RecordSource(cfg)
+    g = cfg.grids[0]
+    rows = []
+    for r in range(1, 8):
+        rec = dict(recs.get(1, r))
+        rec.update(wet_masks(g, r))
+        rows.append(rec)
+    write_history(tmp_path / "his_0001.nc", g, rows, [(r - 1) * 3600.0 for r in range(1, 8)])
+    lon, lat, z, dob = _release_file(tmp_path / "par.txt", cfg)
+    p = cfg.params
+    (tmp_path / "run.nml").write_text(namelist(
+        numpar=n, days=days, dt=p.dt, idt=p.idt, iprint=p.iprint, parfile="par.txt",
+        prefix=[str(tmp_path / "his_")], multifile=True, outpath=str(tmp_path) + "/"))
+    print("case fortran driver --wetdry mini", n, "particles", flush=True)
+    r = _run(["--wetdry", "run.nml", "state.txt"], tmp_path)
+    assert r.returncode == 0
+    got = _state(tmp_path / "state.txt")
+    hf = HydroFiles([str(tmp_path / "his_")], filenum=1, multifile=True, wetdry=True)
+    _, o = _oracle_from_release(cfg, lon, lat, z, dob, records=hf)
+    hf.close()
+    _check_vs_oracle(got, o)
+    hf = HydroFiles([str(tmp_path / "his_")], filenum=1, multifile=True)
+    _, plain = _oracle_from_release(cfg, lon, lat, z, dob, records=hf)
+    hf.close()
+    assert (np.abs(plain.x - got["x"]) + np.abs(plain.y - got["y"]) > 1e-6).sum() >= 5
+    # the FP32-field form refuses the masks with a message
+    r = _run(["--wetdry", "--fp32", "run.nml", "state32.txt"], tmp_path)
+    assert r.returncode != 0 and "FP64" in (r.stdout + r.stderr)

@@ -22,7 +22,8 @@ from romspath_amd.schedule import schedule
 def write_history(path, g, recs, times, version=2, units="seconds since 2000-01-01 00:00:00",
                   ftype="d"):
     """A ROMS-like history file: grid variables and one record per entry of
-    recs (dicts of zeta, u, v, w, aks and optional salt, temp, bustr, bvstr)."""
+    recs (dicts of zeta, u, v, w, aks and optional salt, temp, bustr, bvstr and
+    the -DWETDRY masks wetdry_mask_rho/_u/_v)."""
     L, M, N = g.L, g.M, g.N
     f = netcdf_file(str(path), "w", version=version)
     for name, n in (("ocean_time", None), ("xi_rho", L), ("eta_rho", M), ("xi_u", L - 1),
@@ -59,8 +60,10 @@ def write_history(path, g, recs, times, version=2, units="seconds since 2000-01-
             "v": ("ocean_time", "s_rho", "eta_v", "xi_v"), "w": ("ocean_time", "s_w") + rho,
             "AKs": ("ocean_time", "s_w") + rho, "salt": ("ocean_time", "s_rho") + rho,
             "temp": ("ocean_time", "s_rho") + rho, "bustr": ("ocean_time", "eta_u", "xi_u"),
-            "bvstr": ("ocean_time", "eta_v", "xi_v")}
-    keys = [k for k in ("zeta", "u", "v", "w", "aks", "salt", "temp", "bustr", "bvstr") if k in recs[0]]
+            "bvstr": ("ocean_time", "eta_v", "xi_v"), "wetdry_mask_rho": ("ocean_time",) + rho,
+            "wetdry_mask_u": ("ocean_time", "eta_u", "xi_u"), "wetdry_mask_v": ("ocean_time", "eta_v", "xi_v")}
+    keys = [k for k in ("zeta", "u", "v", "w", "aks", "salt", "temp", "bustr", "bvstr", "wetdry_mask_rho",
+                        "wetdry_mask_u", "wetdry_mask_v") if k in recs[0]]
     vars_ = {k: f.createVariable("AKs" if k == "aks" else k, ftype, dims["AKs" if k == "aks" else k])
              for k in keys}
     for r, (rec, tt) in enumerate(zip(recs, times)):
@@ -402,4 +405,47 @@ def test_nested_oracle_run_from_files_equals_in_memory(tmp_path):
     hf.close()
     assert set(np.unique(a.gid)) == {1, 2}
     for k in ("x", "y", "z", "status", "gid"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+def test_wetdry_masks_from_files_drive_the_oracle_as_in_memory(tmp_path):
+    """-DWETDRY: the wet/dry masks of every record read back from a history
+    file (hydrodynamic_module.f90:510-570) equal what was written, and the
+    oracle run from the file equals the in-memory run with the same masks,
+    bit for bit."""
+    cfg = synth.config_mini(n_particles=300)
+    g = cfg.grids[0]
+    recs = RecordSource(cfg)
+    static = np.asarray(g.mask_rho, dtype=np.int32)
+    nrec = 4
+
+    def masks(r):
+        wet = static.copy()
+        wet[:, : 3 + r] = 0 if r % 2 else wet[:, : 3 + r]
+        return dict(wetdry_mask_rho=wet, wetdry_mask_u=wet[:, :-1] * wet[:, 1:],
+                    wetdry_mask_v=wet[:-1, :] * wet[1:, :])
+
+    rows = []
+    for r in range(1, nrec + 1):
+        rec = dict(recs.get(1, r))
+        rec.update(masks(r))
+        rows.append(rec)
+    write_history(tmp_path / "his_0001.nc", g, rows, [(r - 1) * cfg.params.dt for r in range(1, nrec + 1)])
+    hf = HydroFiles([str(tmp_path / "his_")], filenum=1, multifile=True, wetdry=True)
+    for r in range(1, nrec + 1):
+        got = hf.get(1, r)
+        for k, v in masks(r).items():
+            assert np.array_equal(got[k], v), (r, k)
+
+    class Mem:
+        def get(self, ng, record, filenum=None):
+            rec = dict(recs.get(ng, record))
+            rec.update(masks(record))
+            return rec
+
+    steps = int(3 * cfg.params.dt / cfg.params.idt)
+    a = O.run_oracle(cfg, max_steps=steps, records=Mem())
+    b = O.run_oracle(cfg, max_steps=steps, records=hf)
+    hf.close()
+    for k in ("x", "y", "z", "status"):
         assert np.array_equal(getattr(a, k), getattr(b, k)), k
