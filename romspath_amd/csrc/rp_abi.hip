@@ -190,6 +190,7 @@ struct Ctx {
     // adjoint recording (rp_adjoint_record)
     struct AdjSnap {
         void* f[kMaxGrid][5];         // copies of zeta, u, v, w, aks of every grid
+        uint8_t* wd[kMaxGrid];        // and of its wet/dry masks, when it has them
         DGrid* dtab;                  // device grid table pointing at them
     };
     struct AdjRec {
@@ -915,8 +916,10 @@ void free_recording(Ctx* c)
 {
     Ctx::AdjRec& R = c->adj;
     for (auto& sn : R.snaps) {
-        for (int g = 0; g < kMaxGrid; g++)
+        for (int g = 0; g < kMaxGrid; g++) {
             for (int f = 0; f < 5; f++) dfree(sn.f[g][f]);
+            dfree(sn.wd[g]);
+        }
         dfree(sn.dtab);
     }
     for (auto& p : R.ckpt) dfree(p);
@@ -946,6 +949,14 @@ int take_snapshot(Ctx* c)
         }
         tab[g].zeta = sn.f[g][0]; tab[g].u = sn.f[g][1]; tab[g].v = sn.f[g][2];
         tab[g].w = sn.f[g][3]; tab[g].aks = sn.f[g][4];
+        if (G.wd) {
+            const int64_t plane = wd_plane(G), nr = (int64_t)G.L * G.M, nu = (int64_t)(G.L - 1) * G.M;
+            if (dalloc(c, (char**)&sn.wd[g], (size_t)(3 * plane))) return RP_E_ALLOC;
+            HIPCHK(c, hipMemcpyAsync(sn.wd[g], G.wd, (size_t)(3 * plane), hipMemcpyDeviceToDevice, c->stream));
+            tab[g].wdr = sn.wd[g];
+            tab[g].wdu = sn.wd[g] + nr;
+            tab[g].wdv = sn.wd[g] + nr + nu;
+        }
     }
     if (dalloc(c, &sn.dtab, kMaxGrid)) return RP_E_ALLOC;
     HIPCHK(c, hipMemcpyAsync(sn.dtab, tab, sizeof(tab), hipMemcpyHostToDevice, c->stream));
@@ -1762,6 +1773,7 @@ int rp_set_hydro_wetdry(void* ctx, int32_t ng, int32_t slot, const double* mask_
     // steps already queued may still read this slot's previous masks
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(g.wd + (slot - 1) * plane, b.data(), (size_t)plane, hipMemcpyHostToDevice));
+    c->adj.dirty = true;   // a recorded run snapshots the masks with the slots
     return RP_OK;
 }
 

@@ -7,7 +7,9 @@ side by side with the same injected deviates (FP64 fields): every step's
 state agrees bit for bit (status, grid, isIn; positions within 1e-9 cell),
 and so do the bottom-stress sampler after every step and the salt/temp
 sample at two points of the run.  The masks must change trajectories (the
-same run without them differs), and the FP32-field form refuses them.
+same run without them differs), and the FP32-field form refuses them.  A
+recorded run keeps the masks with its hydro snapshots, so the adjoint's
+recomputed segments see the masks of their own steps.
 """
 import numpy as np
 import pytest
@@ -117,3 +119,60 @@ def test_wetdry_refused_in_fp32_form_and_checked():
         run.ctx.set_hydro_wetdry(1, 2, m["wetdry_mask_rho"], m["wetdry_mask_u"], m["wetdry_mask_v"])
     finally:
         run.close()
+
+
+class _WetSource:
+    """The configuration's records with wet/dry masks added."""
+
+    def __init__(self, cfg):
+        from romspath_amd.runner import RecordSource
+        self.cfg = cfg
+        self.base = RecordSource(cfg)
+
+    def get(self, ng, record, filenum=None):
+        rec = dict(self.base.get(ng, record, filenum))
+        rec.update(wet_masks(self.cfg.grids[ng - 1], record))
+        return rec
+
+
+def test_wetdry_adjoint_records_the_masks():
+    """A recorded run (rp_adjoint_record) with masks that change at the
+    record loads inside it equals the plain run bit for bit, and the
+    adjoint's release gradient, whose segments are recomputed from the
+    snapshots, matches central differences (the masks are snapshotted with
+    the hydro slots)."""
+    from test_gpu_adjoint import _check
+    steps, ckpt = 70, 16
+    cfg = synth.config_mini(n_particles=256, days=steps * 120 / 86400.0, dob_spread=0.0, seed=11)
+    n = cfg.particles.n
+    w = np.random.default_rng(3).standard_normal((n, 3))
+    print(f"case wetdry adjoint fp64, {steps} steps, checkpoints every {ckpt}", flush=True)
+
+    def forward(X, record=0):
+        p = cfg.particles
+        parts = synth.Particles(X.copy(), p.y.copy(), p.z.copy(), p.dob.copy())
+        c2 = synth.Config(cfg.name, cfg.grids, cfg.hydro, cfg.params, parts, cfg.refine)
+        run = GpuRun(c2, precision="fp64", records=_WetSource(cfg))
+        if record:
+            run.ctx.adjoint_record(record)
+        return run, run.run(max_steps=steps)
+
+    r0, a = forward(cfg.particles.x)
+    r0.close()
+    r1, b = forward(cfg.particles.x, record=ckpt)
+    try:
+        for k in ("x", "y", "z", "status", "gid"):
+            assert np.array_equal(a[k], b[k]), k
+        (gx, _, _), _ = r1.ctx.adjoint((w[:, 0], w[:, 1], w[:, 2]))
+    finally:
+        r1.close()
+    fds, ok = [], np.ones(n, dtype=bool)
+    for eps in (1e-4, 1e-5):
+        vals = []
+        for sgn in (1.0, -1.0):
+            r, o = forward(cfg.particles.x + sgn * eps)
+            r.close()
+            ok &= o["status"] == a["status"]
+            vals.append(w[:, 0] * o["x"] + w[:, 1] * o["y"] + w[:, 2] * o["z"])
+        fds.append((vals[0] - vals[1]) / (2 * eps))
+    _check(gx, fds[0], fds[1], ok, "wetdry d/dx0")
