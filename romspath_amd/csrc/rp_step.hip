@@ -236,7 +236,7 @@ __device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids
                 const DGrid& g = grids[NESTED ? gid - 1 : 0];
                 const RhoCell c0 = rho_cell(g, X0, Y0);
                 const Surface s0 = surface_at<Cur>(g, A, c0);
-                const VtCol col = vt_column_fast<Cur, false>(g, A, n + 2, c0, s0, S);
+                const VtCol col = vt_column_fast<Cur, false, 8>(g, A, n + 2, c0, s0, S);
                 st_dbl(S, Lo.w + 0, col.z1);
                 st_dbl(S, Lo.w + 2, col.rh);
                 st_dbl(S, Lo.w + 4, s0.tdepth);

@@ -1305,7 +1305,9 @@ struct VtCol {
     double z1, h, rh;
 };
 
-template <typename F, bool NT_Y = true>
+// KLV: levels per batch of loads (8 where the kernel has registers to spare
+// in this phase, 4 under the band form's tighter budget)
+template <typename F, bool NT_Y = true, int KLV = 4>
 __device__ __forceinline__ VtCol vt_column_fast(const DGrid& g, const StepArgs& A, int np,
                                                 const RhoCell& c, const Surface& sf, const Scr32& S)
 {
@@ -1320,19 +1322,24 @@ __device__ __forceinline__ VtCol vt_column_fast(const DGrid& g, const StepArgs& 
     // walked once, kLv levels per batch of loads, emitting y (linint: the
     // point fz uses the segment of the largest level j<=nw-1 with z(j)<=fz)
     double z1, k1, zn, kn;
-    constexpr int kLv = 4;
+    constexpr int kLv = KLV;
+    constexpr int kQ = kLv / 4;
     // the four corners' AKs of levels kb..kb+kLv-1 (kb-1 is a multiple of
-    // four: one aligned 16-byte load per corner, within the padded column)
+    // four: kQ aligned 16-byte loads per corner; past the padded column, zeros)
     struct Quad {
-        float4 a[4];
+        float4 a[4][kQ];
     };
     auto load_quad = [&](int kb) {
         Quad Q;
         if constexpr (is_cur<F>()) {
             const int nwc = g.nwc;
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                Q.a[q] = *reinterpret_cast<const float4*>(g.caks + c.o[q] * nwc + (kb - 1));
+            for (int j = 0; j < kQ; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    Q.a[q][j] = (kb - 1 + 4 * j < nwc)
+                                    ? *reinterpret_cast<const float4*>(g.caks + c.o[q] * nwc + (kb - 1 + 4 * j))
+                                    : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         return Q;
     };
@@ -1340,17 +1347,22 @@ __device__ __forceinline__ VtCol vt_column_fast(const DGrid& g, const StepArgs& 
     // AKs corners, loaded one batch ahead
     auto batch = [&](int kb, const Quad& Q, double zk[kLv], double kk[kLv]) {
         if constexpr (is_cur<F>()) {
-            const float4 a0 = Q.a[0], a1 = Q.a[1], a2 = Q.a[2], a3 = Q.a[3];
-            const float v0[kLv] = {a0.x, a0.y, a0.z, a0.w}, v1[kLv] = {a1.x, a1.y, a1.z, a1.w};
-            const float v2[kLv] = {a2.x, a2.y, a2.z, a2.w}, v3[kLv] = {a3.x, a3.y, a3.z, a3.w};
+            float v[4][kLv];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int j = 0; j < kQ; j++) {
+                    v[q][4 * j] = Q.a[q][j].x; v[q][4 * j + 1] = Q.a[q][j].y;
+                    v[q][4 * j + 2] = Q.a[q][j].z; v[q][4 * j + 3] = Q.a[q][j].w;
+                }
 #pragma unroll
             for (int t = 0; t < kLv; t++) {
                 const int k = kb + t;
                 if (k >= 2 && k <= nw) {
                     const double Sk = level_S(g, g.s_w, g.cs_w, k, hh);
                     zk[t] = level_z(g, g.s_w, g.cs_w, k, zeta, hh, Sk);
-                    kk[t] = cff[0] * (double)v0[t] + cff[1] * (double)v1[t] + cff[2] * (double)v2[t] +
-                            cff[3] * (double)v3[t];
+                    kk[t] = cff[0] * (double)v[0][t] + cff[1] * (double)v[1][t] + cff[2] * (double)v[2][t] +
+                            cff[3] * (double)v[3][t];
                 }
             }
         } else {
