@@ -58,6 +58,82 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# TCC counters of the in-run counter pass: 32-byte memory requests (read,
+# write) and L2 hits and misses; one pass holds all four
+PMC = ["TCC_HIT_sum", "TCC_MISS_sum", "TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum"]
+STEP_KERNELS = ("vwalk_modal_kernel", "vwalk_kernel", "advect_kernel", "step_kernel", "current_fields")
+
+
+def counter_pass(args, limit_s=360):
+    """HBM bytes and L2 hit rate of the step kernels of this command, counted
+    in a short child run of the same workload under rocprofv3 --pmc (the
+    counters cannot be read from inside a run).  Run before this process
+    touches the GPU.  Bytes = 32 x the 32-byte request counters, which read
+    exactly 1 GiB on a 1 GiB read-only kernel and on a 1 GiB copy
+    (--copy-calibration).  Returns None without rocprofv3 or when the child
+    run fails; the caller then falls back to the stored profile."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        log("counter pass: rocprofv3 not found")
+        return None
+    out = tempfile.mkdtemp(prefix="rp_pmc_", dir="/tmp")
+    child = [sys.executable, str(ROOT / "bench.py"), "--steps", "4", "--warmup", "1", "--no-batched",
+             "--no-adjoint", "--no-cpu-baseline", "--no-fp64-form", "--no-counters",
+             "--workload", args.workload, "--precision", args.precision,
+             "--sort-interval", str(args.sort_interval)]
+    if args.particles:
+        child += ["--particles", str(args.particles)]
+    cmd = ["timeout", "-s", "KILL", str(limit_s), exe, "--pmc", *PMC, "--output-format", "csv",
+           "-d", out, "-o", "pmc", "--", *child]
+    log("counter pass: the same workload, 4 timed steps, under rocprofv3 --pmc " + " ".join(PMC))
+    try:
+        r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                           stdout=subprocess.PIPE, text=True)
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not files:
+            log(f"counter pass failed (exit {r.returncode}); using the stored profile")
+            return None
+        child_line = json.loads(r.stdout.strip().splitlines()[-1])
+        import csv
+        tot = {}
+        disp = {}
+        with open(files[0], newline="") as fh:
+            for row in csv.DictReader(fh):
+                name = row["Kernel_Name"]
+                kind = next((k for k in STEP_KERNELS if k in name), None)
+                if kind is None:
+                    continue
+                key = (kind, row["Counter_Name"])
+                tot[key] = tot.get(key, 0.0) + float(row["Counter_Value"])
+                disp.setdefault(kind, set()).add(row["Dispatch_Id"])
+    except (OSError, ValueError, KeyError, IndexError) as e:
+        log(f"counter pass failed ({e}); using the stored profile")
+        return None
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+    kinds = sorted(disp)
+    by_kernel = {}
+    hit = miss = 0.0
+    for k in kinds:
+        n = len(disp[k])
+        rd = 32.0 * tot.get((k, PMC[2]), 0.0) / n
+        wr = 32.0 * tot.get((k, PMC[3]), 0.0) / n
+        by_kernel[k] = {"read_bytes_per_dispatch": rd, "write_bytes_per_dispatch": wr, "dispatches": n}
+        hit += tot.get((k, PMC[0]), 0.0)
+        miss += tot.get((k, PMC[1]), 0.0)
+    launch = sum(v["read_bytes_per_dispatch"] + v["write_bytes_per_dispatch"] for v in by_kernel.values())
+    per_launch = child_line["roofline"]["active_particles_per_launch"]
+    log(f"counter pass: {launch / 1e9:.2f} GB per step ({launch / per_launch:.0f} B per particle-step), "
+        f"L2 hit rate {hit / max(hit + miss, 1.0):.3f}")
+    return {"bytes_per_launch": launch, "bytes_per_particle_step": launch / per_launch,
+            "l2_hit_rate": hit / max(hit + miss, 1.0), "by_kernel": by_kernel,
+            "child_particles_per_launch": per_launch}
+
+
 def window_start(prm, steps, warmup):
     """First timed step: at least `warmup` untimed steps, and late enough that
     the timed window holds a hydro record load after the initial ones (the
@@ -357,6 +433,8 @@ def main():
     ap.add_argument("--c4-tune-steps", type=int, default=64,
                     help="steps each tuning candidate replays (>= 2 x the largest sort interval)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the C5 adjoint measurement")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter pass (HBM bytes and L2 hit rate of the step)")
     ap.add_argument("--c5-particles", type=int, default=262_144)
     ap.add_argument("--c5-steps", type=int, default=128)
     ap.add_argument("--c5-ckpt", type=int, default=64)
@@ -383,6 +461,12 @@ def main():
               "(the particles would shard by release id, one process per GPU); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    # the counter pass runs first: a child process under the profiler, while
+    # this process has not touched the GPU
+    counters = None
+    if not args.no_counters:
+        counters = counter_pass(args)
 
     if args.workload == "c3":
         npart = args.particles or 4_000_000
@@ -421,13 +505,28 @@ def main():
     per_launch = res["active"] / max(res["launches"], 1)
     achieved = balg * per_launch / (res["avg_kernel_ms"] * 1e-3) / 1e9 if res["avg_kernel_ms"] else 0.0
     adv_ms, walk_ms = res["phase_ms"]
-    # HBM bytes and L2 hit rate of this command's step kernels, from the
-    # stored rocprofv3 profile of the same command (32-byte request counters,
-    # calibrated on a 1 GiB read-only kernel and a 1 GiB copy): the counters
-    # cannot be read from inside the run itself
+    # HBM bytes and L2 hit rate of this command's step kernels: from the
+    # counter pass (a child run under rocprofv3, before this process took the
+    # GPU), else from the stored rocprofv3 profile of the same command
     traffic = l2_hit = measured_gbs = None
+    tr_src = l2_src = None
     tr_file = ROOT / "profiles" / "hbm_traffic_c3_fp32.json"
-    if args.workload == "c3" and args.precision == "fp32" and tr_file.exists():
+    if counters is not None:
+        # this run's own counts: bytes per step of the child run's step kernels
+        traffic = counters["bytes_per_particle_step"] * per_launch
+        l2_hit = counters["l2_hit_rate"]
+        if res["avg_kernel_ms"]:
+            measured_gbs = traffic / (res["avg_kernel_ms"] * 1e-3) / 1e9
+        tr_src = ("counted in this run: 32 x rocprofv3 TCC_EA0_RDREQ_DRAM_32B + "
+                  "TCC_EA0_WRREQ_WRITE_DRAM_32B of the step kernels, a child run of the same "
+                  "workload (4 timed steps, every dispatch averaged), per particle-step x this "
+                  "run's particles per launch")
+        l2_src = "counted in this run: TCC_HIT / (TCC_HIT + TCC_MISS) of the step kernels"
+    elif args.workload == "c3" and args.precision == "fp32" and tr_file.exists():
+        tr_src = ("stored profile of this command: 32 x rocprofv3 TCC_EA0_RDREQ_DRAM_32B + "
+                  "TCC_EA0_WRREQ_WRITE_DRAM_32B per particle-step x this run's particles per "
+                  "launch, profiles/hbm_traffic_c3_fp32.md")
+        l2_src = "stored profile: TCC_HIT / (TCC_HIT + TCC_MISS) of the step kernels, profiles/hbm_traffic_c3_fp32.md"
         tr = json.loads(tr_file.read_text())
         traffic = (tr["read_bytes_per_particle_step"] + tr["write_bytes_per_particle_step"]) * per_launch
         l2_hit = tr.get("l2_hit_rate")
@@ -527,14 +626,10 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "achieved_is": "algorithmic bytes (b_alg x active particles per launch) "
                                     "per second of step kernel time",
-                     "traffic_source": ("stored profile of this command: 32 x rocprofv3 "
-                                        "TCC_EA0_RDREQ_DRAM_32B + TCC_EA0_WRREQ_WRITE_DRAM_32B per "
-                                        "particle-step x this run's particles per launch, "
-                                        "profiles/hbm_traffic_c3_fp32.md") if traffic else None,
+                     "traffic_source": tr_src,
+                     "traffic_by_kernel": counters["by_kernel"] if counters else None,
                      "l2_hit_rate": l2_hit,
-                     "l2_hit_rate_source": ("stored profile: TCC_HIT / (TCC_HIT + TCC_MISS) of the "
-                                            "step kernels, profiles/hbm_traffic_c3_fp32.md")
-                     if l2_hit is not None else None,
+                     "l2_hit_rate_source": l2_src,
                      "measured_traffic_gbs": measured_gbs,
                      "kernel": "one step = current_fields + advect_kernel + vertical walk "
                                "(vwalk_modal_kernel)",
