@@ -58,9 +58,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# TCC counters of the in-run counter pass: 32-byte memory requests (read,
-# write) and L2 hits and misses; one pass holds all four
-PMC = ["TCC_HIT_sum", "TCC_MISS_sum", "TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum"]
+# counters of the in-run counter pass, within one pass's limits (4 TCC, 8 SQ):
+# L2 hits and misses and 32-byte memory requests (read, write); VALU wave
+# instructions, the FP64 FMA/MUL/ADD among them, and matrix instructions
+PMC = ["TCC_HIT_sum", "TCC_MISS_sum", "TCC_EA0_RDREQ_DRAM_32B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum",
+       "SQ_INSTS_VALU", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64",
+       "SQ_INSTS_MFMA"]
+MFMA_FLOP = 4096          # one v_mfma_f32_32x32x2_f32: 32 x 32 x 2 multiply-adds
 STEP_KERNELS = ("vwalk_modal_kernel", "vwalk_kernel", "advect_kernel", "step_kernel", "current_fields")
 
 
@@ -118,19 +122,34 @@ def counter_pass(args, limit_s=360):
     kinds = sorted(disp)
     by_kernel = {}
     hit = miss = 0.0
+    per_launch = child_line["roofline"]["active_particles_per_launch"]
+    valu = flop = mfma = 0.0
     for k in kinds:
         n = len(disp[k])
-        rd = 32.0 * tot.get((k, PMC[2]), 0.0) / n
-        wr = 32.0 * tot.get((k, PMC[3]), 0.0) / n
-        by_kernel[k] = {"read_bytes_per_dispatch": rd, "write_bytes_per_dispatch": wr, "dispatches": n}
+        c = {name: tot.get((k, name), 0.0) / n for name in PMC}
+        rd = 32.0 * c[PMC[2]]
+        wr = 32.0 * c[PMC[3]]
+        # wave instructions x 64 lanes per particle-step
+        kv = 64.0 * c["SQ_INSTS_VALU"] / per_launch
+        kf = 64.0 * (2.0 * c["SQ_INSTS_VALU_FMA_F64"] + c["SQ_INSTS_VALU_MUL_F64"]
+                     + c["SQ_INSTS_VALU_ADD_F64"]) / per_launch
+        km = c["SQ_INSTS_MFMA"] / per_launch
+        by_kernel[k] = {"read_bytes_per_dispatch": rd, "write_bytes_per_dispatch": wr, "dispatches": n,
+                        "valu_lane_instr_per_particle_step": kv, "fp64_flop_per_particle_step": kf,
+                        "mfma_per_particle_step": km}
         hit += tot.get((k, PMC[0]), 0.0)
         miss += tot.get((k, PMC[1]), 0.0)
+        valu += kv
+        flop += kf
+        mfma += km
     launch = sum(v["read_bytes_per_dispatch"] + v["write_bytes_per_dispatch"] for v in by_kernel.values())
-    per_launch = child_line["roofline"]["active_particles_per_launch"]
     log(f"counter pass: {launch / 1e9:.2f} GB per step ({launch / per_launch:.0f} B per particle-step), "
-        f"L2 hit rate {hit / max(hit + miss, 1.0):.3f}")
+        f"L2 hit rate {hit / max(hit + miss, 1.0):.3f}; per particle-step {valu:.0f} VALU "
+        f"lane-instructions, {flop:.0f} FP64 FLOP, {mfma:.2f} matrix instructions")
     return {"bytes_per_launch": launch, "bytes_per_particle_step": launch / per_launch,
             "l2_hit_rate": hit / max(hit + miss, 1.0), "by_kernel": by_kernel,
+            "valu_lane_instr_per_particle_step": valu, "fp64_flop_per_particle_step": flop,
+            "mfma_flop_per_particle_step": mfma * MFMA_FLOP,
             "child_particles_per_launch": per_launch}
 
 
@@ -537,13 +556,23 @@ def main():
     # particle-step from rocprofv3 SQ counters of this workload
     compute = None
     va_file = ROOT / "profiles" / "valu_c3_fp32.json"
-    if args.workload == "c3" and args.precision == "fp32" and va_file.exists() and res["avg_kernel_ms"]:
+    csrc = None
+    if counters is not None and res["avg_kernel_ms"]:
+        flop = counters["fp64_flop_per_particle_step"]
+        instr = counters["valu_lane_instr_per_particle_step"]
+        mfma = counters["mfma_flop_per_particle_step"]
+        csrc = ("counted in this run: rocprofv3 SQ_INSTS_VALU(_FMA/_MUL/_ADD_F64), SQ_INSTS_MFMA of "
+                "the step kernels in the counter pass's child run, per particle-step")
+    elif args.workload == "c3" and args.precision == "fp32" and va_file.exists() and res["avg_kernel_ms"]:
         va = json.loads(va_file.read_text())
-        ks = res["avg_kernel_ms"] * 1e-3
         flop = sum(va["fp64_flop_per_particle_step"].values())
         instr = sum(va["valu_lane_instr_per_particle_step"].values())
-        ach = flop * per_launch / ks / 1e12
         mfma = sum(va.get("mfma_per_particle_step", {}).values()) * va.get("mfma_flop", 0)
+        csrc = ("stored profile: rocprofv3 SQ_INSTS_VALU(_FMA/_MUL/_ADD_F64), SQ_INSTS_MFMA per "
+                "dispatch of this command, profiles/valu_c3_fp32.md")
+    if csrc is not None:
+        ks = res["avg_kernel_ms"] * 1e-3
+        ach = flop * per_launch / ks / 1e12
         mach = mfma * per_launch / ks / 1e12
         compute = {"bound": "fp64 valu", "achieved": ach, "peak": PEAK_FP64_VALU_TFLOPS,
                    "unit": "TFLOP/s", "frac": ach / PEAK_FP64_VALU_TFLOPS,
@@ -553,8 +582,7 @@ def main():
                    "fp32_matrix": {"achieved": mach, "peak": PEAK_FP32_MFMA_TFLOPS,
                                    "frac": mach / PEAK_FP32_MFMA_TFLOPS,
                                    "flop_per_particle_step": mfma},
-                   "source": "rocprofv3 SQ_INSTS_VALU(_FMA/_MUL/_ADD_F64), SQ_INSTS_MFMA per "
-                             "dispatch of this command, profiles/valu_c3_fp32.md"}
+                   "source": csrc}
         if mfma:
             log(f"  FP32 matrix: {mach:.1f} TFLOP/s ({mach / PEAK_FP32_MFMA_TFLOPS:.1%} of peak)")
         log(f"  FP64 VALU: {ach:.1f} TFLOP/s ({ach / PEAK_FP64_VALU_TFLOPS:.1%} of peak), "
