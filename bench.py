@@ -102,50 +102,59 @@ def counter_pass(args, limit_s=360):
             log(f"counter pass failed (exit {r.returncode}); using the stored profile")
             return None
         child_line = json.loads(r.stdout.strip().splitlines()[-1])
-        import csv
-        tot = {}
-        disp = {}
-        with open(files[0], newline="") as fh:
-            for row in csv.DictReader(fh):
-                name = row["Kernel_Name"]
-                kind = next((k for k in STEP_KERNELS if k in name), None)
-                if kind is None:
-                    continue
-                key = (kind, row["Counter_Name"])
-                tot[key] = tot.get(key, 0.0) + float(row["Counter_Value"])
-                disp.setdefault(kind, set()).add(row["Dispatch_Id"])
-    except (OSError, ValueError, KeyError, IndexError) as e:
+        got = summarize_counters(files[0], child_line["roofline"]["active_particles_per_launch"])
+    except (OSError, ValueError, KeyError, IndexError, ZeroDivisionError) as e:
         log(f"counter pass failed ({e}); using the stored profile")
         return None
     finally:
         shutil.rmtree(out, ignore_errors=True)
-    kinds = sorted(disp)
+    log(f"counter pass: {got['bytes_per_launch'] / 1e9:.2f} GB per step "
+        f"({got['bytes_per_particle_step']:.0f} B per particle-step), L2 hit rate {got['l2_hit_rate']:.3f}; "
+        f"per particle-step {got['valu_lane_instr_per_particle_step']:.0f} VALU lane-instructions, "
+        f"{got['fp64_flop_per_particle_step']:.0f} FP64 FLOP, "
+        f"{got['mfma_flop_per_particle_step'] / MFMA_FLOP:.2f} matrix instructions")
+    return got
+
+
+def summarize_counters(path, per_launch):
+    """The step kernels' counters of a rocprofv3 counter-collection CSV (one
+    row per dispatch and counter), averaged over each kernel's dispatches:
+    HBM bytes per dispatch (32 x the 32-byte request counts), the L2 hit rate
+    over all of them, and per particle-step (per_launch active particles a
+    dispatch) VALU lane-instructions (wave instructions x 64), FP64 FLOP
+    (2 FMA + MUL + ADD, x 64) and matrix FLOP."""
+    import csv
+    tot = {}
+    disp = {}
+    with open(path, newline="") as fh:
+        for row in csv.DictReader(fh):
+            kind = next((k for k in STEP_KERNELS if k in row["Kernel_Name"]), None)
+            if kind is None:
+                continue
+            key = (kind, row["Counter_Name"])
+            tot[key] = tot.get(key, 0.0) + float(row["Counter_Value"])
+            disp.setdefault(kind, set()).add(row["Dispatch_Id"])
+    if not disp:
+        raise ValueError("no step-kernel dispatches in the counter file")
     by_kernel = {}
-    hit = miss = 0.0
-    per_launch = child_line["roofline"]["active_particles_per_launch"]
-    valu = flop = mfma = 0.0
-    for k in kinds:
+    hit = miss = valu = flop = mfma = 0.0
+    for k in sorted(disp):
         n = len(disp[k])
         c = {name: tot.get((k, name), 0.0) / n for name in PMC}
-        rd = 32.0 * c[PMC[2]]
-        wr = 32.0 * c[PMC[3]]
-        # wave instructions x 64 lanes per particle-step
         kv = 64.0 * c["SQ_INSTS_VALU"] / per_launch
         kf = 64.0 * (2.0 * c["SQ_INSTS_VALU_FMA_F64"] + c["SQ_INSTS_VALU_MUL_F64"]
                      + c["SQ_INSTS_VALU_ADD_F64"]) / per_launch
         km = c["SQ_INSTS_MFMA"] / per_launch
-        by_kernel[k] = {"read_bytes_per_dispatch": rd, "write_bytes_per_dispatch": wr, "dispatches": n,
-                        "valu_lane_instr_per_particle_step": kv, "fp64_flop_per_particle_step": kf,
-                        "mfma_per_particle_step": km}
-        hit += tot.get((k, PMC[0]), 0.0)
-        miss += tot.get((k, PMC[1]), 0.0)
+        by_kernel[k] = {"read_bytes_per_dispatch": 32.0 * c["TCC_EA0_RDREQ_DRAM_32B_sum"],
+                        "write_bytes_per_dispatch": 32.0 * c["TCC_EA0_WRREQ_WRITE_DRAM_32B_sum"],
+                        "dispatches": n, "valu_lane_instr_per_particle_step": kv,
+                        "fp64_flop_per_particle_step": kf, "mfma_per_particle_step": km}
+        hit += tot.get((k, "TCC_HIT_sum"), 0.0)
+        miss += tot.get((k, "TCC_MISS_sum"), 0.0)
         valu += kv
         flop += kf
         mfma += km
     launch = sum(v["read_bytes_per_dispatch"] + v["write_bytes_per_dispatch"] for v in by_kernel.values())
-    log(f"counter pass: {launch / 1e9:.2f} GB per step ({launch / per_launch:.0f} B per particle-step), "
-        f"L2 hit rate {hit / max(hit + miss, 1.0):.3f}; per particle-step {valu:.0f} VALU "
-        f"lane-instructions, {flop:.0f} FP64 FLOP, {mfma:.2f} matrix instructions")
     return {"bytes_per_launch": launch, "bytes_per_particle_step": launch / per_launch,
             "l2_hit_rate": hit / max(hit + miss, 1.0), "by_kernel": by_kernel,
             "valu_lane_instr_per_particle_step": valu, "fp64_flop_per_particle_step": flop,
