@@ -1422,6 +1422,93 @@ __device__ __forceinline__ VtCol vt_column_fast(const DGrid& g, const StepArgs& 
     return VtCol{z1, h, rh};
 }
 
+// Below this t = p h^3 / (6 (1-p) serr^2) the knot values are rebuilt from
+// their second differences (knots_from_curvature).  Smoothing weights of
+// order 1 give t of 1e-2 and above on metre-spaced columns, where the direct
+// form is the more accurate; t falls to 1e-4 and below as the weight grows.
+#ifndef RP_KNOT_T0
+#define RP_KNOT_T0 1.0e-3
+#endif
+
+// The spline's knot values f = y - 6(1-p) serr^2 Q u (smooth's final loop,
+// pppack.f90:5305-5314) tend to the least-squares line of y as p -> 0, and
+// the difference of the two nearly equal vectors y and 6(1-p) serr^2 Q u is
+// lost in the FP32 rows of u.  From the system smooth solves,
+// (6(1-p) Q'D^2 Q + p h tridiag(1,4,1)) u = Q'y, the second differences of f
+// are Q'f = p h tridiag(1,4,1) u, small when p is, and f's least-squares line
+// is y's (the columns of Q are orthogonal to linear functions).  So f is
+// built in FP64 from its second differences, g(j+1) = 2 g(j) - g(j-1) +
+// p h^2 T(j) with T(j) = u(j-1) + 4 u(j) + u(j+1) from g(1) = g(2) = 0, that
+// is g(i) = p h^2 sum_j (i - j) T(j) over j < i, and given y's line.  The
+// first pass sums y's line and, in closed form, g's; the second runs the
+// recurrence and writes f over the y rows.  The pieces are then formed with
+// k2 = 0 (a1 = the row).  u(1) = u(np) = 0.
+__device__ __forceinline__ void knots_from_curvature(const FastLayout& Lo, const Scr32& S, double h, double p)
+{
+    constexpr int kB = 8;                            // rows loaded per batch
+    const int np = Lo.np, last = np - 1;
+    const double c = 0.5 * (double)(np + 1);
+    // y's line about the centre knot, and with m = np - j the sums
+    // sum_i g(i) = w sum_j T(j) m(m+1)/2 and
+    // sum_i (i - c) g(i) = w sum_j T(j) [(j - c) m(m+1)/2 + m(m+1)(2m+1)/6]
+    const double y1 = S.at(Lo.y + 0), yn = S.at(Lo.y + np - 1);
+    double sy = y1 + yn, sjy = (1.0 - c) * (y1 - yn);
+    double a0 = 0.0, a1 = 0.0;
+    double um = 0.0, u0 = S.at(Lo.u + 1);            // u(1) = 0, u(2)
+    for (int j0 = 2; j0 <= last; j0 += kB) {
+        float yb[kB], ub[kB];
+#pragma unroll
+        for (int t = 0; t < kB; t++) {               // rows past the last are clamped, not used
+            const int j = min(j0 + t, last);
+            yb[t] = S.at(Lo.y + j - 1);
+            ub[t] = S.at(Lo.u + j);                  // u(j+1)
+        }
+#pragma unroll
+        for (int t = 0; t < kB; t++) {
+            const int j = j0 + t;
+            if (j <= last) {
+                const double y = yb[t], up = ub[t];
+                const double dj = (double)j - c;
+                sy += y;
+                sjy += dj * y;
+                const double T = um + 4.0 * u0 + up;
+                const double m = (double)(np - j);
+                const double m2 = 0.5 * m * (m + 1.0);
+                a0 += T * m2;
+                a1 += T * (dj * m2 + m2 * (2.0 * m + 1.0) * (1.0 / 3.0));
+                um = u0;
+                u0 = up;
+            }
+        }
+    }
+    const double w = p * h * h;
+    const double sjj = (double)np * ((double)np * (double)np - 1.0) / 12.0;
+    const double da = (sy - w * a0) / (double)np, db = (sjy - w * a1) / sjj;
+    S.at(Lo.y + 0) = (float)(da + db * (1.0 - c));
+    S.at(Lo.y + 1) = (float)(da + db * (2.0 - c));
+    double gm = 0.0, g0 = 0.0;
+    um = 0.0;
+    u0 = S.at(Lo.u + 1);
+    for (int j0 = 2; j0 <= last; j0 += kB) {
+        float ub[kB];
+#pragma unroll
+        for (int t = 0; t < kB; t++) ub[t] = S.at(Lo.u + min(j0 + t, last));
+#pragma unroll
+        for (int t = 0; t < kB; t++) {
+            const int j = j0 + t;
+            if (j <= last) {
+                const double up = ub[t];
+                const double g1 = 2.0 * g0 - gm + w * (um + 4.0 * u0 + up);
+                S.at(Lo.y + j) = (float)(g1 + da + db * ((double)j + 1.0 - c));
+                gm = g0;
+                g0 = g1;
+                um = u0;
+                u0 = up;
+            }
+        }
+    }
+}
+
 // the random-displacement sub-steps (ver_turb_module.f90:240-293) on the
 // spline of p whose u and y rows are in the work rows
 __device__ __forceinline__ double vt_substeps_fast(const StepArgs& A, const FastLayout& Lo,
@@ -1432,7 +1519,11 @@ __device__ __forceinline__ double vt_substeps_fast(const StepArgs& A, const Fast
 #pragma clang fp contract(fast)
     const double tdepth = sf.tdepth, zeta = sf.zeta;
     const double h = col.h, rh = col.rh, z1 = col.z1;
-    const double k2 = 6.0 * (1.0 - p) * (A.serr * A.serr);
+    double k2 = 6.0 * (1.0 - p) * (A.serr * A.serr);
+    if (p < 1.0 && p * h * h * h < RP_KNOT_T0 * k2) {
+        knots_from_curvature(Lo, S, h, p);
+        k2 = 0.0;
+    }
     PieceCache pc;
     pc.i = -1;
     PieceWindow W;

@@ -4,7 +4,11 @@ elsewhere (and under RP_VWALK_FORM=band).  np = int((N+1) * sub)
 (ver_turb_module.f90:52): N 16 gives n 66 (eigenbasis, 31 padded modes per
 symmetry half), N 31 gives n 126 (eigenbasis, one padded mode), N 15 gives
 n 62 and N 32 gives n 130 (band), N 30 with sub 4.1 gives n 125 (odd: band)
-and with sub 3.3 gives n 100 (eigenbasis).
+and with sub 3.3 gives n 100 (eigenbasis).  Three more cases on n 122 take
+the smoothing weight's ends: smth 0 (smooth's p = 1 branch, the
+interpolating spline), smth 30 (p near 1e-10: the knot values rebuilt from
+their second differences) and smth 1e6 (the p = 0 branch: the least-squares
+line of the column, pppack.f90:5236-5298).
 
 Each case steps 256 particles on the island grid with the vertical walk and
 injected deviates for 30 steps and is compared with the FP64 oracle: the
@@ -12,7 +16,8 @@ largest position error within 1e-4 cell over the particles whose status
 agrees (at least 99% of them).  The forced band form must give the same
 results as the default where the eigenbasis does not apply, and different
 (also within 1e-4 cell of the oracle) where it does, which shows that the
-eigenbasis form ran.
+eigenbasis form ran.  At p = 0 the knots are the line whatever u holds, so
+there the two forms must agree exactly.
 """
 import dataclasses
 import os
@@ -61,13 +66,17 @@ def _check(g, o, cfg, what):
     assert ex[same].max() <= 1e-4 and ey[same].max() <= 1e-4 and ez[same].max() <= 1e-4
 
 
-@pytest.mark.parametrize("N,sub,eigen", CASES)
-def test_vertical_walk_forms_match_oracle(N, sub, eigen):
+@pytest.mark.parametrize("N,sub,eigen,smth", [c + (0.6,) for c in CASES] + [
+    (30, 4.0, True, 0.0), (30, 4.0, True, 30.0), (30, 4.0, False, 1e6)])
+def test_vertical_walk_forms_match_oracle(N, sub, eigen, smth):
+    """eigen: the two forms' results must differ (the eigenbasis form ran and
+    its u enters the knots)."""
     cfg = synth.config_mini(n_particles=256, N=N, days=1 / 24.0, dob_spread=600.0)
-    cfg = dataclasses.replace(cfg, params=dataclasses.replace(cfg.params, sub=sub))
+    cfg = dataclasses.replace(cfg, params=dataclasses.replace(cfg.params, sub=sub, smth=smth))
     np_ = int((N + 1) * sub)
     steps = 30
-    print(f"case vwalk forms N={N} sub={sub} np={np_} n={np_ - 2} eigenbasis={eigen}", flush=True)
+    print(f"case vwalk forms N={N} sub={sub} np={np_} n={np_ - 2} smth={smth} eigenbasis={eigen}",
+          flush=True)
     o = O.run_oracle(cfg, max_steps=steps, inject=True)
     g, capped = _gpu(cfg, steps)
     assert capped == 0
