@@ -160,8 +160,10 @@ int rp_sample_lonlat(void *ctx, int64_t n, double *lon, double *lat);
  * each value 0 or 1, else RP_E_ARG) of time slot 1..3 of grid ng, after
  * waiting for the steps already queued; from the first call on, the grid's
  * gathers use the masks, and slots not given masks are dry (the reference
- * allocates them as 0).  FP64 fields only: RP_E_STATE in the FP32-field form,
- * which combines the three slots of each point before its gathers. */
+ * allocates them as 0).  In the FP32-field form the current u and v fields
+ * take each slot's value only where that slot's mask is wet.  w uses the
+ * current field where the step's slots agree on a cell's four corners, and
+ * weighs the FP32 slot fields slot by slot where they differ. */
 int rp_set_hydro_wetdry(void *ctx, int32_t ng, int32_t slot, const double *mask_rho,
                         const double *mask_u, const double *mask_v);
 

@@ -373,7 +373,7 @@ class Context:
 
     def set_hydro_wetdry(self, ng, slot, mask_rho, mask_u, mask_v):
         """The reference built with -DWETDRY: the record's wetdry_mask_rho/u/v
-        (0/1) into slot 1..3 (FP64 fields only)."""
+        (0/1) into slot 1..3."""
         mr, mu, mv = _f64(mask_rho), _f64(mask_u), _f64(mask_v)
         self._chk("rp_set_hydro_wetdry",
                   self._L.rp_set_hydro_wetdry(self.h, ng, slot, _d(mr), _d(mu), _d(mv)))

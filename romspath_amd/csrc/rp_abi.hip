@@ -709,7 +709,16 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
     P.grid_all = (cnt + block - 1) / block;
     const bool nested = c->cfg.ngrid > 1;
     const bool f32 = c->fbytes == 4;
-    if (f32 && !nested) {
+    // the FP32 form's kernels with the wet/dry mask code where a grid has masks
+    bool wet = false;
+    for (int i = 0; i < c->cfg.ngrid; i++) wet = wet || c->grids[i].wd;
+    if (f32 && wet && !nested) {
+        P.k[0] = step_kernel<CurWet, false>; P.k[1] = advect_kernel<CurWet, false>;
+        P.k[2] = vwalk_kernel<CurWet, false>;
+    } else if (f32 && wet) {
+        P.k[0] = step_kernel<CurWet, true>; P.k[1] = advect_kernel<CurWet, true>;
+        P.k[2] = vwalk_kernel<CurWet, true>;
+    } else if (f32 && !nested) {
         P.k[0] = step_kernel<Cur, false>; P.k[1] = advect_kernel<Cur, false>; P.k[2] = vwalk_kernel<Cur, false>;
     } else if (f32) {
         P.k[0] = step_kernel<Cur, true>; P.k[1] = advect_kernel<Cur, true>; P.k[2] = vwalk_kernel<Cur, true>;
@@ -802,6 +811,13 @@ int plan_step(Ctx* c, const double ex[3], const double ix[3], int ets, int t_b, 
                 J.kin[j] = kin;
                 J.kout[j] = kout;
                 const bool first = ets == 1 && f >= 1 && f <= 3;
+                // u and v under wet/dry masks: each slot's dry points
+                // contribute 0 (the masks' u then v sections follow the
+                // rho section of each slot's plane)
+                const int64_t nr = (int64_t)g.L * g.M, nu = (int64_t)(g.L - 1) * g.M;
+                J.wd[j] = (g.wd && (f == 1 || f == 2)) ? g.wd + nr + (f == 2 ? nu : 0) : nullptr;
+                J.wdp[j] = wd_plane(g);
+                J.nk[j] = g.N;
                 J.slot[j][0] = A.slots.s[0];
                 J.slot[j][1] = first ? A.slots.s[0] : A.slots.s[1];
                 J.slot[j][2] = first ? A.slots.s[1] : A.slots.s[2];
@@ -1748,10 +1764,6 @@ int rp_set_hydro_wetdry(void* ctx, int32_t ng, int32_t slot, const double* mask_
         return fail(c, RP_E_STATE, "rp_set_hydro_wetdry: grid not set");
     if (slot < 1 || slot > 3) return fail(c, RP_E_ARG, "slot must be 1..3");
     if (!mask_rho || !mask_u || !mask_v) return fail(c, RP_E_ARG, "null mask");
-    if (c->fbytes != 8)
-        return fail(c, RP_E_STATE,
-                    "rp_set_hydro_wetdry: wet/dry masks need FP64 fields (the FP32-field form combines "
-                    "the three time slots of each point before its gathers, which per-slot masks forbid)");
     GridDev& g = c->grids[ng - 1];
     const int64_t cnt[3] = {(int64_t)g.L * g.M, (int64_t)(g.L - 1) * g.M, (int64_t)g.L * (g.M - 1)};
     const double* src[3] = {mask_rho, mask_u, mask_v};
@@ -1774,6 +1786,7 @@ int rp_set_hydro_wetdry(void* ctx, int32_t ng, int32_t slot, const double* mask_
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(g.wd + (slot - 1) * plane, b.data(), (size_t)plane, hipMemcpyHostToDevice));
     c->adj.dirty = true;   // a recorded run snapshots the masks with the slots
+    c->field_gen++;        // the FP32 form's current u and v fields carry the masks
     return RP_OK;
 }
 

@@ -66,10 +66,20 @@ struct DGrid {
 // interpolation is linear in the field values, so combining the slots once
 // per point and step is the same interpolation in another order.
 struct Cur {};
+// the same current fields under the -DWETDRY build's wet/dry masks: u and v
+// carry each slot's mask (current_fields), w weighs the corners by them
+struct CurWet {};
 template <typename F>
 constexpr bool is_cur()
 {
-    return __is_same(F, Cur);
+    return __is_same(F, Cur) || __is_same(F, CurWet);
+}
+// the gathers take the per-slot wet/dry masks where the grid has them (the
+// slot fields' kinds, and CurWet; plain Cur kernels carry no mask code)
+template <typename F>
+constexpr bool wet_code()
+{
+    return !__is_same(F, Cur);
 }
 // the FP32-field form's arithmetic economies (Lagrange time weights, one
 // reciprocal per column, the streaming spline) apply to float and Cur

@@ -175,7 +175,9 @@ int upload_p0_tables();
 // stored FP32; one job per (grid, field), blockIdx.y selects the job.  A job
 // copies columns of kin levels into columns of kout >= kin levels: output e
 // = col kout + k takes source point e' = col kin + k for k < kin and is 0
-// in the padding.  n counts output points.
+// in the padding.  n counts output points.  With wet/dry masks (u and v
+// under -DWETDRY), slot t's term is taken only where that slot's mask at
+// the point's column, wd[slot[t] * wdp + e' / nk], is wet.
 constexpr int kMaxCurJobs = 5 * kMaxGrid;
 struct CurJobs {
     const float* src[kMaxCurJobs];
@@ -183,6 +185,9 @@ struct CurJobs {
     int64_t n[kMaxCurJobs];
     int kin[kMaxCurJobs], kout[kMaxCurJobs];
     int slot[kMaxCurJobs][3];
+    const uint8_t* wd[kMaxCurJobs];   // null: no masks
+    int64_t wdp[kMaxCurJobs];
+    int nk[kMaxCurJobs];
     double pw[3];
     int njobs;
 };

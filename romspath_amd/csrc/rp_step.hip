@@ -396,7 +396,8 @@ template __global__ void vwalk_modal_kernel<true>(const DGrid*, StepArgs);
     template __global__ void step_kernel<F, N>(const DGrid*, StepArgs);            \
     template __global__ void advect_kernel<F, N>(const DGrid*, StepArgs);          \
     template __global__ void vwalk_kernel<F, N>(const DGrid*, StepArgs);
-RP_INST(Cur, false) RP_INST(Cur, true) RP_INST(double, false) RP_INST(double, true)
+RP_INST(Cur, false) RP_INST(Cur, true) RP_INST(CurWet, false) RP_INST(CurWet, true)
+RP_INST(double, false) RP_INST(double, true)
 #undef RP_INST
 
 // ---------------------------------------------------------------------------
@@ -490,6 +491,9 @@ __global__ void __launch_bounds__(256) current_fields(CurJobs J)
     const int s0 = J.slot[j][0], s1 = J.slot[j][1], s2 = J.slot[j][2];
     const double p0 = J.pw[0], p1 = J.pw[1], p2 = J.pw[2];
     const int kin = J.kin[j], kout = J.kout[j];
+    const uint8_t* wd = J.wd[j];
+    const int64_t wdp = J.wdp[j];
+    const int nk = J.nk[j];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < J.n[j];
          e += (int64_t)gridDim.x * blockDim.x) {
         int64_t si = e;
@@ -504,7 +508,16 @@ __global__ void __launch_bounds__(256) current_fields(CurJobs J)
         }
         const float4 v = reinterpret_cast<const float4*>(src)[si];
         auto pk = [&](int t) { return (double)(t == 0 ? v.x : (t == 1 ? v.y : (t == 2 ? v.z : v.w))); };
-        dst[e] = (float)(p0 * pk(s0) + p1 * pk(s1) + p2 * pk(s2));
+        if (wd) {
+            // each slot's value where that slot's mask is wet, 0 where dry
+            const int64_t pt = si / nk;
+            const double a0 = wd[s0 * wdp + pt] ? pk(s0) : 0.0;
+            const double a1 = wd[s1 * wdp + pt] ? pk(s1) : 0.0;
+            const double a2 = wd[s2 * wdp + pt] ? pk(s2) : 0.0;
+            dst[e] = (float)(p0 * a0 + p1 * a1 + p2 * a2);
+        } else {
+            dst[e] = (float)(p0 * pk(s0) + p1 * pk(s1) + p2 * pk(s2));
+        }
     }
 }
 

@@ -210,17 +210,23 @@ __device__ __forceinline__ void interp_u(const DGrid& g, R X, R Y, const VBracke
     col[1] = (int64_t)(Jnode - 1) * xiu + (ib - 1);
     col[2] = (int64_t)(Jnode) * xiu + (ib - 1);
     col[3] = (int64_t)(Jnode) * xiu + (ia - 1);
-    if constexpr (!is_cur<F>()) {
-        if (g.wdu) {
-            // the slots' masks alone (the static ones take no part)
-            const bool east = Inode >= xiu, west = Inode < 1;
+    if (wet_code<F>() && g.wdu) {
+        // the slots' masks alone (the static ones take no part)
+        const bool east = Inode >= xiu, west = Inode < 1;
+        R cw[4];
+        bilin(x, y, cw);
+        if constexpr (is_cur<F>()) {
+            // the current field holds each slot's value where its mask is
+            // wet and 0 where dry (zeroing a slot's dry corners is linear in
+            // its values); the edge corners are zeroed here
+            const bool edge[4] = {west, east, east, west};
+            blend3<F, NSLOT>(g.u, g.cu, col, g.N, cw, zero_all_slots(edge), vb, S, out);
+        } else {
             const int64_t pt[4] = {west ? -1 : col[0], east ? -1 : col[1], east ? -1 : col[2],
                                    west ? -1 : col[3]};
-            R cw[4];
-            bilin(x, y, cw);
             blend3<F, NSLOT>(g.u, g.cu, col, g.N, cw, zero_wet<NSLOT>(g.wdu, g.wdp, S, pt), vb, S, out);
-            return;
         }
+        return;
     }
     blend3<F, NSLOT>(g.u, g.cu, col, g.N, cff, zero_all_slots(zero), vb, S, out);
 }
@@ -264,15 +270,19 @@ __device__ __forceinline__ void interp_v(const DGrid& g, R X, R Y, const VBracke
     col[1] = (int64_t)(ja - 1) * g.L + (Inode);
     col[2] = (int64_t)(jb - 1) * g.L + (Inode);
     col[3] = (int64_t)(jb - 1) * g.L + (Inode - 1);
-    if constexpr (!is_cur<F>()) {
-        if (g.wdv) {
-            const bool north = Jnode >= etav;
+    if (wet_code<F>() && g.wdv) {
+        const bool north = Jnode >= etav;
+        R cw[4];
+        bilin(x, y, cw);
+        if constexpr (is_cur<F>()) {
+            // as in interp_u: dry corners are 0 in the current field
+            const bool edge[4] = {false, false, north, north};
+            blend3<F, NSLOT>(g.v, g.cv, col, g.N, cw, zero_all_slots(edge), vb, S, out);
+        } else {
             const int64_t pt[4] = {col[0], col[1], north ? -1 : col[2], north ? -1 : col[3]};
-            R cw[4];
-            bilin(x, y, cw);
             blend3<F, NSLOT>(g.v, g.cv, col, g.N, cw, zero_wet<NSLOT>(g.wdv, g.wdp, S, pt), vb, S, out);
-            return;
         }
+        return;
     }
     blend3<F, NSLOT>(g.v, g.cv, col, g.N, cff, zero_all_slots(zero), vb, S, out);
 }
@@ -298,18 +308,62 @@ __device__ __forceinline__ void rho_weights_wet(const DGrid& g, const RhoCellT<R
     if (m[0] + m[1] + m[2] + m[3] < 4.0) getcoeff(c.x, c.y, m, cff);
 }
 
+// w in the FP32 current-field form where the slots' wet/dry masks differ at
+// the cell's corners: each slot weighed with its own mask from the FP32 slot
+// fields, the slots combined with the current fields' time weights pw
+// (slots (b, b, c) when NSLOT is 2).  One slot at a time, in a loop kept
+// rolled: such cells are few, and the plain path's registers stay its own.
+template <int NSLOT>
+__device__ __forceinline__ double w_wet_slots(const DGrid& g, const RhoCell& c, const VBracketT<double>& vb,
+                                              const SlotSel& S, const double* pw)
+{
+    double v = 0.0;
+#pragma unroll 1
+    for (int t = 0; t < NSLOT; t++) {
+        const int s = t == 0 ? S.s[0] : (t == 1 ? S.s[1] : S.s[2]);
+        const double wt = NSLOT == 2 ? (t == 0 ? pw[0] + pw[1] : pw[2]) : (t == 0 ? pw[0] : (t == 1 ? pw[1] : pw[2]));
+        double cw[4], o[3];
+        rho_weights_wet(g, c, s, cw);
+        const SlotSel one{{s, s, s}};
+        blend3<float, 1>(g.w, g.cw, c.o, g.N + 1, cw, 0u, vb, one, o);
+        v += wt * o[0];
+    }
+    return v;
+}
+
+// pw: the time weights of the current fields (w_wet_slots)
 template <typename F, int NSLOT, class R>
 __device__ __forceinline__ void interp_w(const DGrid& g, const RhoCellT<R>& c,
-                                        const VBracketT<R>& vb, const SlotSel& S, R out[3])
+                                        const VBracketT<R>& vb, const SlotSel& S, const double pw[3],
+                                        R out[3])
 {
-    if constexpr (!is_cur<F>()) {
-        if (g.wdr) {
+    if (wet_code<F>() && g.wdr) {
+        if constexpr (is_cur<F>()) {
+            // Where the slots' masks agree at the four corners every slot
+            // has the same weights, and the current field (the slots
+            // combined) gives the weighted sum of the slots; elsewhere
+            // w_wet_slots weighs each slot with its own mask.
+            bool same = true;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint8_t m0 = g.wdr[(int64_t)S.s[0] * g.wdp + c.o[k]];
+#pragma unroll
+                for (int t = 1; t < NSLOT; t++) same = same && g.wdr[(int64_t)S.s[t] * g.wdp + c.o[k]] == m0;
+            }
+            if (same) {
+                R cw[4];
+                rho_weights_wet(g, c, S.s[0], cw);
+                blend3<F, NSLOT>(g.w, g.cw, c.o, g.N + 1, cw, 0u, vb, S, out);
+            } else {
+                out[0] = out[1] = out[2] = w_wet_slots<NSLOT>(g, c, vb, S, pw);
+            }
+        } else {
             R cw[4 * NSLOT];
 #pragma unroll
             for (int t = 0; t < NSLOT; t++) rho_weights_wet(g, c, S.s[t], cw + 4 * t);
             blend3<F, NSLOT, 4>(g.w, g.cw, c.o, g.N + 1, cw, 0u, vb, S, out);
-            return;
         }
+        return;
     }
     R cff[4];
     rho_weights(g, c, cff);
@@ -411,7 +465,7 @@ __device__ __forceinline__ void find_currents(const DGrid& g, const StepArgs& A,
     U = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
     interp_v<F, NSLOT>(g, X, Y, br, A.slots, q);
     V = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
-    interp_w<F, NSLOT>(g, c, bw, A.slots, q);
+    interp_w<F, NSLOT>(g, c, bw, A.slots, A.pw, q);
     W = NSLOT == 2 ? tinterp<F>(A, q[0], q[0], q[1]) : tinterp<F>(A, q[0], q[1], q[2]);
     if (A.pert) add_perturbation(g, A, X, Y, U, V);
 }

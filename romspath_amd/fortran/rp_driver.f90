@@ -23,8 +23,7 @@
 !   --stokes   add the Stokes drift of stokesprefix(ng) to u, v: the reference
 !              built with -DSTOKES (hydrodynamic_module.f90:573-660)
 !   --wetdry   read each record's wetdry_mask_rho/_u/_v into its slot: the
-!              reference built with -DWETDRY (hydrodynamic_module.f90:510-570;
-!              FP64 fields only)
+!              reference built with -DWETDRY (hydrodynamic_module.f90:510-570)
 !   --fp32     the FP32-field form of the step (default: FP64, the reference's)
 ! The final-state file, when named, gets "X Y Z status GID age" per particle.
 !
