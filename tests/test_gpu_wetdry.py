@@ -200,24 +200,27 @@ class _WetSource:
         return rec
 
 
-def test_wetdry_adjoint_records_the_masks():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_wetdry_adjoint_records_the_masks(precision):
     """A recorded run (rp_adjoint_record) with masks that change at the
-    record loads inside it equals the plain run bit for bit, and the
-    adjoint's release gradient, whose segments are recomputed from the
-    snapshots, matches central differences (the masks are snapshotted with
-    the hydro slots)."""
+    record loads inside it equals the plain run (bit for bit with FP64
+    fields; within 1e-4 cell with FP32 fields, whose recorded run takes the
+    slot fields' reference-order path), and the adjoint's release gradient,
+    whose segments are recomputed from the snapshots, matches central
+    differences of recorded runs (the masks are snapshotted with the hydro
+    slots)."""
     from test_gpu_adjoint import _check
     steps, ckpt = 70, 16
     cfg = synth.config_mini(n_particles=256, days=steps * 120 / 86400.0, dob_spread=0.0, seed=11)
     n = cfg.particles.n
     w = np.random.default_rng(3).standard_normal((n, 3))
-    print(f"case wetdry adjoint fp64, {steps} steps, checkpoints every {ckpt}", flush=True)
+    print(f"case wetdry adjoint {precision}, {steps} steps, checkpoints every {ckpt}", flush=True)
 
     def forward(X, record=0):
         p = cfg.particles
         parts = synth.Particles(X.copy(), p.y.copy(), p.z.copy(), p.dob.copy())
         c2 = synth.Config(cfg.name, cfg.grids, cfg.hydro, cfg.params, parts, cfg.refine)
-        run = GpuRun(c2, precision="fp64", records=_WetSource(cfg))
+        run = GpuRun(c2, precision=precision, records=_WetSource(cfg))
         if record:
             run.ctx.adjoint_record(record)
         return run, run.run(max_steps=steps)
@@ -226,18 +229,24 @@ def test_wetdry_adjoint_records_the_masks():
     r0.close()
     r1, b = forward(cfg.particles.x, record=ckpt)
     try:
-        for k in ("x", "y", "z", "status", "gid"):
-            assert np.array_equal(a[k], b[k]), k
+        if precision == "fp64":
+            for k in ("x", "y", "z", "status", "gid"):
+                assert np.array_equal(a[k], b[k]), k
+        else:
+            same = a["status"] == b["status"]
+            assert same.mean() >= 0.99
+            assert max(np.abs(a["x"] - b["x"])[same].max(), np.abs(a["y"] - b["y"])[same].max()) <= 1e-4
         (gx, _, _), _ = r1.ctx.adjoint((w[:, 0], w[:, 1], w[:, 2]))
     finally:
         r1.close()
+    # the differences of recorded runs: the map the adjoint differentiates
     fds, ok = [], np.ones(n, dtype=bool)
     for eps in (1e-4, 1e-5):
         vals = []
         for sgn in (1.0, -1.0):
-            r, o = forward(cfg.particles.x + sgn * eps)
+            r, o = forward(cfg.particles.x + sgn * eps, record=ckpt)
             r.close()
-            ok &= o["status"] == a["status"]
+            ok &= o["status"] == b["status"]
             vals.append(w[:, 0] * o["x"] + w[:, 1] * o["y"] + w[:, 2] * o["z"])
         fds.append((vals[0] - vals[1]) / (2 * eps))
     _check(gx, fds[0], fds[1], ok, "wetdry d/dx0")
