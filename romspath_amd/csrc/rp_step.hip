@@ -346,18 +346,21 @@ __global__ void __launch_bounds__(256) RP_WAVES(RP_WAVES_ALL)
     step_body<F, NESTED, 0>(grids, A);
 }
 
-// The FP32-field advection kernel is held to 3 waves per SIMD (168 VGPRs and
-// a small spill), the FP64 one to 2: their RK stages wait on field gathers,
-// and the extra waves hide them better than the spill costs (measured on C3:
-// FP64 form 2.47 ms at 2 waves against 3.63 at 1 and 2.66 at 3).
-template <typename F>
+// Waves per SIMD of the advection kernels: their RK stages wait on field
+// gathers, which more waves hide, as long as the stages' state fits their
+// registers.  The FP32-field form fits 3 waves (168 VGPRs) with a spill of
+// 20 B a lane on one grid, 140 B with nesting.  Measured: FP64 form on C3
+// 2.47 ms at 2 waves against 3.63 at 1 and 2.66 at 3; FP32 form on C3
+// (nested) 1.41 ms advection phase at 2 against 1.51 at 3 and 1.76 at 4; on
+// C4 (one grid) 17.55 ms per batched step at 3 against 18.10 at 2.
+template <typename F, bool NESTED>
 constexpr int advect_waves()
 {
-    return RP_WAVES_ADVECT > 0 ? RP_WAVES_ADVECT : (is_fast<F>() ? 3 : 2);
+    return RP_WAVES_ADVECT > 0 ? RP_WAVES_ADVECT : (is_fast<F>() && !NESTED ? 3 : 2);
 }
 
 template <typename F, bool NESTED>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(advect_waves<F>())))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(advect_waves<F, NESTED>())))
     advect_kernel(const DGrid* __restrict__ grids, StepArgs A)
 {
     step_body<F, NESTED, 1>(grids, A);
