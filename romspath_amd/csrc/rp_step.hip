@@ -230,10 +230,11 @@ __device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids
         // (the 128 mode values need the room): h; z1, 1/h and the surface
         // go to spare work rows, and the particle's state is read again for
         // the sub-steps.
-        // Wave priority: the phases that wait on memory (the column walk
-        // and the sub-steps) issue ahead of the other wave's transforms and
-        // search, whose arithmetic fills the gaps (C3 vertical walk 5.29 ->
-        // 5.14 ms; the column walk alone 5.17, the sub-steps alone 5.25).
+        // Wave priority: the stretches that wait on memory (the column walk,
+        // each parity's qty row reads and the sub-steps) issue ahead of the
+        // other wave's transforms and search, whose arithmetic fills the
+        // gaps (C3 vertical walk 5.29 -> 5.14 ms for the column walk and the
+        // sub-steps, 4.73 ms with the row reads too).
         __builtin_amdgcn_s_setprio(RP_PRIO_MEM);
         bool go = false;
         double h = 0.0;
@@ -269,6 +270,7 @@ __device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids
             // into the first parity's transform, with 128 more live values
             asm volatile("" ::: "memory");
             float qp[kMh], qm[kMh];
+            __builtin_amdgcn_s_setprio(RP_PRIO_MEM);   // the row reads, as in the column walk
 #pragma unroll
             for (int j = 0; j < kMh; j++) {
                 qp[j] = S(Lo.q + (j < half ? j : n));
@@ -278,6 +280,7 @@ __device__ __forceinline__ void vwalk_modal_body(const DGrid* __restrict__ grids
             // (the rows are read once per parity, so one folded set is live)
 #pragma unroll
             for (int j = 0; j < kMh; j++) qp[j] = go ? (par == 0 ? qp[j] + qm[j] : qp[j] - qm[j]) : 0.0f;
+            __builtin_amdgcn_s_setprio(0);
             if (par == 0) modal_gemm<false>(tab, [&](int j) { return qp[j]; }, co);
             else modal_gemm<false>(tab_e, [&](int j) { return qp[j]; }, ce);
         }
